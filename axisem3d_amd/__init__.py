@@ -1,0 +1,41 @@
+"""MI355X-native time-stepping path of AxiSEM3D (stiffness loop + Newmark).
+
+The product is the C-ABI library ``_build/libaxg.so`` (HIP kernels for gfx950
+plus the host-side preloop); this package only loads it and mirrors the
+reference's Domain / Newmark interface on top of it.  Loading fails loudly when
+the library has not been built: there is no CPU fallback.
+"""
+import ctypes as _C
+import os as _os
+
+from . import _abi
+
+LIB_DIR = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "_build")
+LIB_PATH = _os.path.join(LIB_DIR, "libaxg.so")
+
+_lib = None
+
+
+def lib():
+    """Load libaxg.so once; raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not _os.path.exists(LIB_PATH):
+            raise ImportError(
+                "axisem3d_amd: %s is missing; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+        l = _C.CDLL(LIB_PATH, mode=_C.RTLD_GLOBAL)
+        _abi.declare_mesh_api(l)
+        _abi.declare_domain_api(l)
+        _lib = l
+    return _lib
+
+
+def last_error():
+    return lib().axg_last_error().decode()
+
+
+from .mesh import ShellMesh, shell_config  # noqa: E402
+from .domain import Domain, Newmark  # noqa: E402
+
+__all__ = ["lib", "last_error", "ShellMesh", "shell_config", "Domain", "Newmark"]

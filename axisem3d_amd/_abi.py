@@ -1,0 +1,158 @@
+"""ctypes mirror of include/axg.h (structures and prototypes).
+
+Only plain pointers and sizes cross this boundary; numpy is used to own and
+view host arrays.  The layout below must match include/axg.h field by field.
+"""
+import ctypes as C
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("npoint", C.c_int32),
+        ("point_nr", i32p),
+        ("point_flags", i32p),
+        ("point_sz", f64p),
+        ("mass_solid_off", i64p),
+        ("inv_mass_solid", f32p),
+        ("mass_fluid_off", i64p),
+        ("inv_mass_fluid", f32p),
+        ("nelem", C.c_int32),
+        ("elem_points", i32p),
+        ("elem_flags", i32p),
+        ("elem_material", i32p),
+        ("mat_off", i64p),
+        ("mat_pool", f32p),
+        ("elem_geom", f32p),
+        ("elem_tiso", f32p),
+        ("elem_att", i32p),
+        ("att_off", i64p),
+        ("att_pool", f32p),
+        ("nsls", C.c_int32),
+        ("sls_alpha", f32p),
+        ("sls_beta", f32p),
+        ("sls_gamma", f32p),
+        ("att_do_kappa", C.c_int32),
+        ("nsf", C.c_int32),
+        ("sf_point", i32p),
+        ("sf_off", i64p),
+        ("sf_normal", f32p),
+        ("src_elem", C.c_int32),
+        ("src_force", f32p),
+        ("nstep", C.c_int32),
+        ("stf", f32p),
+        ("dt", C.c_double),
+        ("t_shift", C.c_double),
+        ("nrec", C.c_int32),
+        ("rec_elem", i32p),
+        ("rec_weights", f32p),
+        ("rec_phi", f32p),
+        ("rec_theta", f32p),
+        ("rec_components", C.c_int32),
+    ]
+
+
+class ShellLayer(C.Structure):
+    _fields_ = [
+        ("r_bottom", C.c_double), ("r_top", C.c_double),
+        ("nelem_radial", C.c_int32), ("fluid", C.c_int32),
+        ("vp_bottom", C.c_double), ("vp_top", C.c_double),
+        ("vs_bottom", C.c_double), ("vs_top", C.c_double),
+        ("rho_bottom", C.c_double), ("rho_top", C.c_double),
+        ("q_mu", C.c_double), ("q_kappa", C.c_double),
+    ]
+
+
+class ShellConfig(C.Structure):
+    _fields_ = [
+        ("nelem_theta", C.c_int32),
+        ("nlayer", C.c_int32),
+        ("layers", C.POINTER(ShellLayer)),
+        ("material", C.c_int32),
+        ("tiso_xi", C.c_double), ("tiso_phi", C.c_double), ("tiso_eta", C.c_double),
+        ("aniso_perturb", C.c_double),
+        ("dv_amp", C.c_double),
+        ("dv_corr_len", C.c_double),
+        ("dv_fluid", C.c_int32),
+        ("nu_mode", C.c_int32),
+        ("nu_const", C.c_int32), ("nu_min", C.c_int32), ("nu_max", C.c_int32),
+        ("nu_median", C.c_double), ("nu_sigma", C.c_double),
+        ("lucky", C.c_int32),
+        ("attenuation", C.c_int32),
+        ("nsls", C.c_int32),
+        ("q_min", C.c_double), ("q_max", C.c_double),
+        ("f_min", C.c_double), ("f_max", C.c_double), ("f_ref", C.c_double),
+        ("do_kappa", C.c_int32),
+        ("courant", C.c_double),
+        ("nstep", C.c_int32),
+        ("period", C.c_double),
+        ("src_depth", C.c_double),
+        ("moment", C.c_double * 6),
+        ("nrec", C.c_int32),
+        ("rec_depth", C.c_double),
+        ("rec_components", C.c_int32),
+        ("seed", C.c_uint64),
+        ("nthread", C.c_int32),
+    ]
+
+
+# constants of axg.h
+PT_SOLID, PT_FLUID, PT_AXIAL, PT_FLUID_SURFACE = 1, 2, 4, 8
+EL_FLUID, EL_AXIAL, EL_3D, EL_TISO = 1, 2, 4, 8
+MAT_ISOTROPIC, MAT_TRANSVERSELY_ISO, MAT_ANISOTROPIC, MAT_ACOUSTIC = 0, 1, 2, 3
+ATT_NONE, ATT_FULL, ATT_CG4 = 0, 1, 2
+FIELD_SOLID_DISPL, FIELD_SOLID_VELOC, FIELD_SOLID_ACCEL, FIELD_SOLID_STIFF = 0, 1, 2, 3
+FIELD_FLUID_DISPL, FIELD_FLUID_VELOC, FIELD_FLUID_ACCEL, FIELD_FLUID_STIFF = 4, 5, 6, 7
+E_OK, E_INVAL, E_UNSUPPORTED, E_HIP, E_UNSTABLE, E_NOMEM = 0, 1, 2, 3, 4, 5
+
+
+def declare_mesh_api(lib):
+    """Prototypes of the preloop (mesh) entry points."""
+    lib.axg_shell_config_default.argtypes = [C.c_int, C.POINTER(ShellConfig)]
+    lib.axg_shell_config_default.restype = C.c_int
+    lib.axg_mesh_build_shell.argtypes = [C.POINTER(ShellConfig), C.POINTER(C.c_void_p)]
+    lib.axg_mesh_build_shell.restype = C.c_int
+    lib.axg_mesh_get_desc.argtypes = [C.c_void_p]
+    lib.axg_mesh_get_desc.restype = C.POINTER(MeshDesc)
+    lib.axg_mesh_stats.argtypes = [C.c_void_p, i64p, C.c_int]
+    lib.axg_mesh_stats.restype = C.c_int
+    lib.axg_mesh_free.argtypes = [C.c_void_p]
+    lib.axg_mesh_free.restype = None
+    lib.axg_last_error.argtypes = []
+    lib.axg_last_error.restype = C.c_char_p
+    lib.axg_field_size.argtypes = [C.POINTER(MeshDesc), C.c_int32]
+    lib.axg_field_size.restype = C.c_int64
+
+
+def declare_domain_api(lib):
+    """Prototypes of the GPU domain entry points."""
+    vp = C.c_void_p
+    sig = {
+        "axg_create": ([C.POINTER(MeshDesc), C.c_int, C.POINTER(vp)], C.c_int),
+        "axg_destroy": ([vp], C.c_int),
+        "axg_reset_zero": ([vp], C.c_int),
+        "axg_update_newmark": ([vp, C.c_double], C.c_int),
+        "axg_apply_source": ([vp, C.c_int32], C.c_int),
+        "axg_compute_stiff": ([vp], C.c_int),
+        "axg_couple_solid_fluid": ([vp], C.c_int),
+        "axg_assemble_stiff": ([vp, C.c_int32], C.c_int),
+        "axg_record": ([vp, C.c_int32, C.c_double], C.c_int),
+        "axg_check_stability": ([vp, i64p], C.c_int),
+        "axg_learn_wisdom": ([vp, C.c_float], C.c_int),
+        "axg_get_wisdom": ([vp, i32p], C.c_int),
+        "axg_run": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "axg_get_seismograms": ([vp, f32p, C.c_int64], C.c_int),
+        "axg_get_field": ([vp, C.c_int32, f32p, C.c_int64], C.c_int),
+        "axg_set_field": ([vp, C.c_int32, f32p, C.c_int64], C.c_int),
+        "axg_last_run_timing": ([vp, f64p, f64p, C.c_int32], C.c_int),
+        "axg_step_cost": ([vp, f64p, f64p], C.c_int),
+        "axg_class_bytes": ([vp, f64p, C.c_int32], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res

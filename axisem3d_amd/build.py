@@ -1,0 +1,50 @@
+"""Build recipe of libaxg.so (hipcc for gfx950, g++ for the host preloop).
+
+``python -m axisem3d_amd.build`` compiles every source every time.  Outputs go
+to axisem3d_amd/_build/ (git-ignored; travels with the tree to the GPU box).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "_build")
+CSRC = os.path.join(HERE, "csrc")
+INC = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-munsafe-fp-atomics"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+
+HOST_SOURCES = ["preloop/shell_mesh.cpp", "host/errors.cpp"]
+HIP_SOURCES = ["axg_domain.hip"]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        run(["g++"] + CXX_FLAGS + INC + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        run([HIPCC] + HIP_FLAGS + INC + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    lib = os.path.join(BUILD, "libaxg.so")
+    tmp = lib + ".tmp"
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, lib)
+    return lib
+
+
+if __name__ == "__main__":
+    sys.exit(0 if build() else 1)

@@ -1,0 +1,866 @@
+// axg_domain.hip -- the Domain on one MI355X: device buffers, kernel launches
+// and the C ABI of include/axg.h.
+//
+// Layout in HBM (structure of arrays, mode-major inside a point):
+//   solid fields  displ/veloc/accel/stiff : [row][3] float2, row = first_row(point) + alpha
+//   fluid fields                          : [row]    float2
+//   element constants                     : geometry [e][5][25], TIso trig [e][4][25],
+//                                           material pool as in the descriptor
+//   element -> point map                  : [e][25] first row + last valid alpha
+// Elements are bucketed by (phase, material, attenuation, 1-D/3-D); each bucket
+// is one kernel launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "axg.h"
+#include "kernels/common.hpp"
+#include "kernels/elem1d.hpp"
+#include "kernels/elem3d.hpp"
+#include "kernels/points.hpp"
+#include "preloop/spectral.hpp"
+
+namespace axg {
+__constant__ float cG[2][25];
+void set_last_error(const std::string &msg);
+}  // namespace axg
+
+using namespace axg;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t err__ = (x);                                                                       \
+        if (err__ != hipSuccess)                                                                      \
+            throw HipError(std::string("HIP || ") + #x + ": " + hipGetErrorString(err__));            \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        free();
+        n = count;
+        if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) HIPCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    void zero(hipStream_t s) {
+        if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+};
+
+struct Bucket1D {
+    int phase = 0;      // 0 solid, 1 fluid
+    int material = 0;
+    bool tiso = false;
+    int att = 0;
+    DevBuf<int32_t> item_elem, item_alpha;
+    int32_t nitem = 0;
+};
+
+struct Phase {
+    int32_t npt = 0, nrow = 0;
+    int ncomp = 3;
+    std::vector<int32_t> compact;   // point -> compact (-1)
+    std::vector<int32_t> point_of;  // compact -> point
+    std::vector<int32_t> pt_row, pt_nu, pt_flags;
+    std::vector<float> pt_invmass;
+    DevBuf<int32_t> d_row_point, d_pt_row, d_pt_nu, d_pt_flags, d_point_of;
+    DevBuf<float> d_invmass;
+    DevBuf<float2> displ, veloc, accel, stiff;
+    PhasePoints view() const {
+        PhasePoints P;
+        P.row_point = d_row_point.p;
+        P.pt_row = d_pt_row.p;
+        P.pt_nu = d_pt_nu.p;
+        P.pt_flags = d_pt_flags.p;
+        P.pt_invmass = d_invmass.p;
+        P.nrow = nrow;
+        P.npt = npt;
+        return P;
+    }
+};
+
+}  // namespace
+
+struct axg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int32_t npoint = 0, nelem = 0, nstep = 0, nrec = 0;
+    double dt = 0.;
+    Phase S, F;
+    // elements
+    std::vector<int32_t> elem_flags, elem_nr;
+    DevBuf<int32_t> d_elem_flags, d_elem_nr, d_ep_row, d_ep_amax;
+    DevBuf<float> d_geom, d_tiso, d_mat_pool, d_att_pool;
+    DevBuf<int64_t> d_mat_off, d_att_off, d_att_mem_off;
+    DevBuf<float2> d_att_memvar, d_att_stressR;
+    int64_t att_mem_stride = 0;
+    int32_t nsls = 0, do_kappa = 0;
+    float sls_a[8] = {0}, sls_b[8] = {0}, sls_g[8] = {0};
+    std::vector<std::unique_ptr<Bucket1D>> buckets1d;
+    std::vector<std::unique_ptr<Bucket3D>> buckets3d;
+    DevBuf<float> d_fft_twiddle;
+    // solid-fluid coupling
+    int32_t nsf = 0, nsf_row = 0;
+    DevBuf<int32_t> d_row_sf, d_sf_srow, d_sf_frow, d_sf_row0;
+    DevBuf<float> d_sf_normal;
+    // source
+    int32_t src_elem = -1;
+    DevBuf<float> d_src_force, d_stf;
+    DevBuf<int32_t> d_src_row, d_src_nu;
+    // receivers
+    DevBuf<int32_t> d_rec_elem;
+    DevBuf<float> d_rec_w, d_rec_phi, d_rec_cost, d_rec_sint, d_seis;
+    int32_t rec_rtz = 1;
+    // stability
+    DevBuf<int> d_bad;
+    // cost model
+    double bytes_step = 0., updates_step = 0.;
+    double class_bytes[3] = {0., 0., 0.};
+    double last_ms = 0.;
+    double class_ms[4] = {0., 0., 0., 0.};
+    // graph
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    DevBuf<int32_t> d_tstep;
+};
+
+namespace {
+
+void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int phase, int mat, int tiso, int att,
+                     const std::vector<int32_t> &elems) {
+    (void)c; (void)d; (void)phase; (void)mat; (void)tiso; (void)att; (void)elems;
+    throw std::runtime_error("FieldFFT::transformF2P || azimuth-dependent (3-D) elements are not supported by this build yet.");
+}
+void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
+    (void)c; (void)b;
+}
+
+int fail(int code, const std::string &msg) {
+    set_last_error(msg);
+    return code;
+}
+
+#define API_GUARD(body)                                                          \
+    try {                                                                        \
+        body;                                                                    \
+        return AXG_OK;                                                           \
+    } catch (const HipError &e) {                                                \
+        return fail(AXG_EHIP, e.what());                                         \
+    } catch (const std::exception &e) {                                          \
+        return fail(AXG_EINVAL, e.what());                                       \
+    }
+
+void buildPhase(const axg_mesh_desc *d, Phase &P, bool solid) {
+    const int flagBit = solid ? AXG_PT_SOLID : AXG_PT_FLUID;
+    P.ncomp = solid ? 3 : 1;
+    P.compact.assign(d->npoint, -1);
+    P.point_of.clear();
+    P.pt_row.clear();
+    P.pt_nu.clear();
+    P.pt_flags.clear();
+    P.pt_invmass.clear();
+    int32_t row = 0;
+    const int64_t *moff = solid ? d->mass_solid_off : d->mass_fluid_off;
+    const float *minv = solid ? d->inv_mass_solid : d->inv_mass_fluid;
+    for (int p = 0; p < d->npoint; p++) {
+        if (!(d->point_flags[p] & flagBit)) continue;
+        int nr = d->point_nr[p];
+        int nu = nr / 2;
+        if (moff[p + 1] - moff[p] != 1)
+            throw std::runtime_error("Mass3D::computeAccel || azimuth-dependent mass is not supported by this build.");
+        P.compact[p] = (int32_t)P.point_of.size();
+        P.point_of.push_back(p);
+        P.pt_row.push_back(row);
+        P.pt_nu.push_back(nu);
+        int f = 0;
+        if (d->point_flags[p] & AXG_PT_AXIAL) f |= 1;
+        if (nr % 2 == 0) f |= 2;
+        if (!solid && (d->point_flags[p] & AXG_PT_FLUID_SURFACE)) f |= 4;
+        P.pt_flags.push_back(f);
+        P.pt_invmass.push_back(minv[moff[p]]);
+        if ((int64_t)row + nu + 1 > INT32_MAX / 3) throw std::runtime_error("axg_create || mesh too large.");
+        row += nu + 1;
+    }
+    P.npt = (int32_t)P.point_of.size();
+    P.nrow = row;
+}
+
+void uploadPhase(Phase &P, hipStream_t s) {
+    std::vector<int32_t> row_point(P.nrow);
+    for (int q = 0; q < P.npt; q++)
+        for (int a = 0; a <= P.pt_nu[q]; a++) row_point[P.pt_row[q] + a] = q;
+    P.d_row_point.upload(row_point, s);
+    P.d_pt_row.upload(P.pt_row, s);
+    P.d_pt_nu.upload(P.pt_nu, s);
+    P.d_pt_flags.upload(P.pt_flags, s);
+    P.d_invmass.upload(P.pt_invmass, s);
+    P.d_point_of.upload(P.point_of, s);
+    size_t n = (size_t)P.nrow * P.ncomp;
+    P.displ.alloc(n);
+    P.veloc.alloc(n);
+    P.accel.alloc(n);
+    P.stiff.alloc(n);
+    P.displ.zero(s);
+    P.veloc.zero(s);
+    P.accel.zero(s);
+    P.stiff.zero(s);
+    // keep row_point alive until the copy finished
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+Elem1DArgs makeArgs1D(axg_ctx *c, const Bucket1D &b) {
+    Elem1DArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.item_elem = b.item_elem.p;
+    a.item_alpha = b.item_alpha.p;
+    a.nitem = b.nitem;
+    a.elem_flags = c->d_elem_flags.p;
+    a.geom = c->d_geom.p;
+    a.tiso = c->d_tiso.p;
+    a.mat_pool = c->d_mat_pool.p;
+    a.mat_off = c->d_mat_off.p;
+    a.att_pool = c->d_att_pool.p;
+    a.att_off = c->d_att_off.p;
+    a.ep_row = c->d_ep_row.p;
+    a.ep_amax = c->d_ep_amax.p;
+    Phase &P = b.phase == 0 ? c->S : c->F;
+    a.displ = P.displ.p;
+    a.stiff = reinterpret_cast<float *>(P.stiff.p);
+    a.att_memvar = c->d_att_memvar.p;
+    a.att_stressR = c->d_att_stressR.p;
+    a.att_mem_off = c->d_att_mem_off.p;
+    a.nsls = c->nsls;
+    for (int i = 0; i < 8; i++) {
+        a.sls_alpha[i] = c->sls_a[i];
+        a.sls_beta[i] = c->sls_b[i];
+        a.sls_gamma[i] = c->sls_g[i];
+    }
+    a.do_kappa = c->do_kappa;
+    a.att_mem_stride = c->att_mem_stride;
+    return a;
+}
+
+template <int MAT, bool TISO>
+void launchSolid1DAtt(const Elem1DArgs &a, int att, hipStream_t s) {
+    dim3 grid((a.nitem + E1D_ITEMS - 1) / E1D_ITEMS);
+    if (att == AXG_ATT_NONE) hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 0>), grid, dim3(256), 0, s, a);
+    else if (att == AXG_ATT_FULL) hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 1>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 2>), grid, dim3(256), 0, s, a);
+}
+
+void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
+    if (b.nitem == 0) return;
+    Elem1DArgs a = makeArgs1D(c, b);
+    hipStream_t s = c->stream;
+    if (b.phase == 1) {
+        hipLaunchKernelGGL(fluid1d_kernel, dim3((a.nitem + E1D_ITEMS - 1) / E1D_ITEMS), dim3(256), 0, s, a);
+        return;
+    }
+    if (b.material == AXG_MAT_ISOTROPIC) {
+        if (b.tiso) launchSolid1DAtt<0, true>(a, b.att, s);
+        else launchSolid1DAtt<0, false>(a, b.att, s);
+    } else if (b.material == AXG_MAT_TRANSVERSELY_ISO) {
+        launchSolid1DAtt<1, true>(a, b.att, s);
+    } else {
+        launchSolid1DAtt<2, true>(a, b.att, s);
+    }
+}
+
+void doNewmark(axg_ctx *c, double dt) {
+    double half_dt = 0.5 * dt;
+    double half_dt_dt = half_dt * dt;
+    const int B = 256;
+    if (c->S.nrow)
+        hipLaunchKernelGGL(newmark_solid_kernel, dim3((c->S.nrow + B - 1) / B), dim3(B), 0, c->stream, c->S.view(),
+                           c->S.displ.p, c->S.veloc.p, c->S.accel.p, c->S.stiff.p, (float)dt, (float)half_dt,
+                           (float)half_dt_dt);
+    if (c->F.nrow)
+        hipLaunchKernelGGL(newmark_fluid_kernel, dim3((c->F.nrow + B - 1) / B), dim3(B), 0, c->stream, c->F.view(),
+                           c->F.displ.p, c->F.veloc.p, c->F.accel.p, c->F.stiff.p, (float)dt, (float)half_dt,
+                           (float)half_dt_dt);
+}
+
+void doSource(axg_ctx *c, int tstep) {
+    if (c->src_elem < 0) return;
+    if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::applySource || time step out of range.");
+    hipLaunchKernelGGL(source_kernel, dim3(1), dim3(256), 0, c->stream, c->d_src_force.p, c->d_src_row.p,
+                       c->d_src_nu.p, c->d_stf.p, tstep, c->S.stiff.p);
+}
+
+void doStiff(axg_ctx *c) {
+    for (auto &b : c->buckets1d) launchBucket1D(c, *b);
+    for (auto &b : c->buckets3d) launchBucket3D(c, *b);
+}
+
+void doCouple(axg_ctx *c) {
+    if (c->nsf_row == 0) return;
+    const int B = 256;
+    hipLaunchKernelGGL(couple1d_kernel, dim3((c->nsf_row + B - 1) / B), dim3(B), 0, c->stream, c->nsf_row,
+                       c->d_row_sf.p, c->d_sf_srow.p, c->d_sf_frow.p, c->d_sf_row0.p, c->d_sf_normal.p,
+                       c->S.displ.p, c->S.stiff.p, c->F.stiff.p);
+}
+
+void doRecord(axg_ctx *c, int tstep) {
+    if (c->nrec == 0) return;
+    if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::record || time step out of range.");
+    hipLaunchKernelGGL(record_solid_kernel, dim3(c->nrec), dim3(256), 0, c->stream, c->nrec, c->d_rec_elem.p,
+                       c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p, c->d_rec_sint.p, c->d_ep_row.p,
+                       c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz,
+                       c->d_seis.p + (int64_t)tstep * c->nrec * 3);
+}
+
+int64_t doStability(axg_ctx *c) {
+    int big = INT32_MAX;
+    HIPCHECK(hipMemcpyAsync(c->d_bad.p, &big, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int B = 256;
+    if (c->S.nrow)
+        hipLaunchKernelGGL(stability_kernel, dim3((c->S.nrow + B - 1) / B), dim3(B), 0, c->stream, c->S.nrow,
+                           c->S.d_row_point.p, c->S.d_point_of.p, c->S.displ.p, 3, c->d_bad.p);
+    if (c->F.nrow)
+        hipLaunchKernelGGL(stability_kernel, dim3((c->F.nrow + B - 1) / B), dim3(B), 0, c->stream, c->F.nrow,
+                           c->F.d_row_point.p, c->F.d_point_of.p, c->F.displ.p, 1, c->d_bad.p);
+    int bad = 0;
+    HIPCHECK(hipMemcpyAsync(&bad, c->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return bad == INT32_MAX ? -1 : bad;
+}
+
+void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
+    if (!d || d->npoint <= 0 || d->nelem <= 0) throw std::runtime_error("axg_create || empty mesh descriptor.");
+    c->device = device;
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&c->ev0));
+    HIPCHECK(hipEventCreate(&c->ev1));
+    hipStream_t s = c->stream;
+    c->npoint = d->npoint;
+    c->nelem = d->nelem;
+    c->nstep = d->nstep;
+    c->dt = d->dt;
+    // derivative matrices
+    {
+        float G[2][25];
+        const auto &gl = spectral::gll();
+        const auto &gj = spectral::glj();
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) {
+                G[0][i * 5 + j] = (float)gl.G[i][j];
+                G[1][i * 5 + j] = (float)gj.G[i][j];
+            }
+        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cG), G, sizeof(G)));
+    }
+    buildPhase(d, c->S, true);
+    buildPhase(d, c->F, false);
+    uploadPhase(c->S, s);
+    uploadPhase(c->F, s);
+
+    // ---- elements ----
+    const int NE = d->nelem;
+    c->elem_flags.assign(d->elem_flags, d->elem_flags + NE);
+    c->elem_nr.resize(NE);
+    std::vector<int32_t> ep_row((size_t)NE * 25), ep_amax((size_t)NE * 25);
+    for (int e = 0; e < NE; e++) {
+        bool fluid = (d->elem_flags[e] & AXG_EL_FLUID) != 0;
+        Phase &P = fluid ? c->F : c->S;
+        int nr = 0;
+        for (int k = 0; k < 25; k++) {
+            int p = d->elem_points[(size_t)e * 25 + k];
+            if (p < 0 || p >= d->npoint) throw std::runtime_error("axg_create || element point out of range.");
+            int q = P.compact[p];
+            if (q < 0) throw std::runtime_error("Point::scatterDisplToElement || Incompatible point type.");
+            int pnr = d->point_nr[p];
+            nr = std::max(nr, pnr);
+            ep_row[(size_t)e * 25 + k] = P.pt_row[q];
+            ep_amax[(size_t)e * 25 + k] = pnr / 2 - ((pnr % 2 == 0) ? 1 : 0);
+        }
+        c->elem_nr[e] = nr;
+    }
+    c->d_elem_flags.upload(c->elem_flags, s);
+    c->d_elem_nr.upload(c->elem_nr, s);
+    c->d_ep_row.upload(ep_row, s);
+    c->d_ep_amax.upload(ep_amax, s);
+    c->d_geom.upload(d->elem_geom, (size_t)NE * 125, s);
+    c->d_tiso.upload(d->elem_tiso, (size_t)NE * 100, s);
+    c->d_mat_pool.upload(d->mat_pool, (size_t)d->mat_off[NE], s);
+    c->d_mat_off.upload(d->mat_off, (size_t)NE + 1, s);
+    // attenuation
+    c->nsls = d->nsls;
+    c->do_kappa = d->att_do_kappa;
+    if (c->nsls > 8) throw std::runtime_error("axg_create || at most 8 standard linear solids.");
+    for (int i = 0; i < c->nsls; i++) {
+        c->sls_a[i] = d->sls_alpha[i];
+        c->sls_b[i] = d->sls_beta[i];
+        c->sls_g[i] = d->sls_gamma[i];
+    }
+    {
+        std::vector<int64_t> memoff(NE, 0);
+        int64_t tot = 0;
+        for (int e = 0; e < NE; e++) {
+            int att = d->elem_att[e];
+            memoff[e] = tot;
+            if (att == AXG_ATT_NONE) continue;
+            int np = (att == AXG_ATT_CG4) ? 4 : 25;
+            bool is3d = (d->elem_flags[e] & AXG_EL_3D) != 0;
+            int nr = c->elem_nr[e];
+            // 1-D: complex [alpha][6][np]; 3-D: real [row][6][np] stored in float2 pairs
+            int64_t cnt = is3d ? ((int64_t)nr * 6 * np + 1) / 2 : (int64_t)(nr / 2 + 1) * 6 * np;
+            tot += cnt;
+        }
+        c->att_mem_stride = tot;
+        c->d_att_mem_off.upload(memoff, s);
+        c->d_att_pool.upload(d->att_pool, (size_t)d->att_off[NE], s);
+        c->d_att_off.upload(d->att_off, (size_t)NE + 1, s);
+        if (tot > 0) {
+            c->d_att_memvar.alloc((size_t)tot * c->nsls);
+            c->d_att_stressR.alloc((size_t)tot);
+            c->d_att_memvar.zero(s);
+            c->d_att_stressR.zero(s);
+        }
+    }
+    // buckets
+    {
+        std::vector<std::vector<int32_t>> ie(64), ia(64);
+        std::vector<std::vector<int32_t>> e3(64);
+        for (int e = 0; e < NE; e++) {
+            int fl = d->elem_flags[e];
+            bool fluid = fl & AXG_EL_FLUID;
+            int mat = d->elem_material[e];
+            if (fluid != (mat == AXG_MAT_ACOUSTIC))
+                throw std::runtime_error("Quad::release || element phase and material disagree.");
+            bool tiso = (fl & AXG_EL_TISO) != 0;
+            if ((mat == AXG_MAT_TRANSVERSELY_ISO || mat == AXG_MAT_ANISOTROPIC) && !tiso)
+                throw std::runtime_error("SolidElement::SolidElement || material needs the TIso rotation.");
+            int att = d->elem_att[e];
+            int key = (fluid ? 1 : 0) * 32 + mat * 8 + (tiso ? 4 : 0) + att;
+            if (fl & AXG_EL_3D) {
+                e3[key].push_back(e);
+                continue;
+            }
+            int nr = c->elem_nr[e];
+            int nu = nr / 2;
+            int amaxe = nu - ((nr % 2 == 0) ? 1 : 0);
+            for (int a = 0; a <= amaxe; a++) {
+                ie[key].push_back(e);
+                ia[key].push_back(a);
+            }
+        }
+        for (int key = 0; key < 64; key++) {
+            if (!ie[key].empty()) {
+                auto b = std::make_unique<Bucket1D>();
+                b->phase = key / 32;
+                b->material = (key / 8) % 4;
+                b->tiso = (key / 4) % 2;
+                b->att = key % 4;
+                b->nitem = (int32_t)ie[key].size();
+                b->item_elem.upload(ie[key], s);
+                b->item_alpha.upload(ia[key], s);
+                HIPCHECK(hipStreamSynchronize(s));
+                c->buckets1d.push_back(std::move(b));
+            }
+            if (!e3[key].empty()) {
+                createBuckets3D(c, d, key / 32, (key / 8) % 4, (key / 4) % 2, key % 4, e3[key]);
+            }
+        }
+    }
+
+    // ---- solid-fluid coupling ----
+    c->nsf = d->nsf;
+    if (c->nsf > 0) {
+        std::vector<int32_t> row_sf, srow, frow, row0;
+        std::vector<float> nrm;
+        int32_t r = 0;
+        for (int q = 0; q < d->nsf; q++) {
+            int p = d->sf_point[q];
+            if (d->sf_off[q + 1] - d->sf_off[q] != 1)
+                throw std::runtime_error("SFCoupling3D::coupleSolidFluid || azimuth-dependent coupling is not supported by this build.");
+            int qs = c->S.compact[p], qf = c->F.compact[p];
+            if (qs < 0 || qf < 0) throw std::runtime_error("SolidFluidPoint::SolidFluidPoint || Incompatible point.");
+            srow.push_back(c->S.pt_row[qs]);
+            frow.push_back(c->F.pt_row[qf]);
+            row0.push_back(r);
+            int nu = d->point_nr[p] / 2;
+            for (int a = 0; a <= nu; a++) row_sf.push_back(q);
+            r += nu + 1;
+            for (int t = 0; t < 6; t++) nrm.push_back(d->sf_normal[d->sf_off[q] * 6 + t]);
+        }
+        c->nsf_row = r;
+        c->d_row_sf.upload(row_sf, s);
+        c->d_sf_srow.upload(srow, s);
+        c->d_sf_frow.upload(frow, s);
+        c->d_sf_row0.upload(row0, s);
+        c->d_sf_normal.upload(nrm, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+
+    // ---- source ----
+    c->src_elem = d->src_elem;
+    if (c->src_elem >= 0) {
+        if (c->src_elem >= NE || (d->elem_flags[c->src_elem] & AXG_EL_FLUID))
+            throw std::runtime_error("Source::release || source element must be a solid element.");
+        std::vector<int32_t> srow(25), snu(25);
+        for (int k = 0; k < 25; k++) {
+            int p = d->elem_points[(size_t)c->src_elem * 25 + k];
+            srow[k] = c->S.pt_row[c->S.compact[p]];
+            snu[k] = std::min(2, d->point_nr[p] / 2);
+        }
+        c->d_src_row.upload(srow, s);
+        c->d_src_nu.upload(snu, s);
+        c->d_src_force.upload(d->src_force, 450, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    c->d_stf.upload(d->stf, (size_t)std::max(1, d->nstep), s);
+
+    // ---- receivers ----
+    c->nrec = d->nrec;
+    c->rec_rtz = d->rec_components == 0;
+    if (c->nrec > 0) {
+        std::vector<float> cost(c->nrec), sint(c->nrec);
+        for (int i = 0; i < c->nrec; i++) {
+            int e = d->rec_elem[i];
+            if (e < 0 || e >= NE) throw std::runtime_error("PointwiseRecorder::addReceiver || bad element.");
+            if (d->elem_flags[e] & AXG_EL_FLUID)
+                throw std::runtime_error("FluidElement::computeGroundMotion || fluid receivers are not supported by this build.");
+            cost[i] = (float)std::cos((double)d->rec_theta[i]);
+            sint[i] = (float)std::sin((double)d->rec_theta[i]);
+        }
+        c->d_rec_elem.upload(d->rec_elem, c->nrec, s);
+        c->d_rec_w.upload(d->rec_weights, (size_t)c->nrec * 25, s);
+        c->d_rec_phi.upload(d->rec_phi, c->nrec, s);
+        c->d_rec_cost.upload(cost, s);
+        c->d_rec_sint.upload(sint, s);
+        c->d_seis.alloc((size_t)std::max(1, c->nstep) * c->nrec * 3);
+        c->d_seis.zero(s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    c->d_bad.alloc(1);
+    c->d_tstep.alloc(1);
+
+    // ---- algorithmic bytes / updates per step ----
+    {
+        double by = 0., up = 0., bel = 0., bnm = 0.;
+        for (int q = 0; q < c->S.npt; q++) {
+            by += 240. * (c->S.pt_nu[q] + 1) + 4.;
+            bel += 48. * (c->S.pt_nu[q] + 1);
+            bnm += 192. * (c->S.pt_nu[q] + 1) + 4.;
+            up += c->S.pt_nu[q] + 1;
+        }
+        for (int q = 0; q < c->F.npt; q++) {
+            by += 80. * (c->F.pt_nu[q] + 1) + 4.;
+            bel += 16. * (c->F.pt_nu[q] + 1);
+            bnm += 64. * (c->F.pt_nu[q] + 1) + 4.;
+            up += c->F.pt_nu[q] + 1;
+        }
+        double bconst0 = by;
+        for (int e = 0; e < NE; e++) {
+            int fl = d->elem_flags[e];
+            bool is3d = fl & AXG_EL_3D;
+            double nr = c->elem_nr[e];
+            by += 500.;
+            if (fl & AXG_EL_TISO) by += 400.;
+            by += 4. * (double)(d->mat_off[e + 1] - d->mat_off[e]);
+            int att = d->elem_att[e];
+            if (att != AXG_ATT_NONE) {
+                int np = att == AXG_ATT_CG4 ? 4 : 25;
+                double per = (2. * (c->nsls + 1) * 6 * np + 2. * np);
+                by += is3d ? per * 4. * nr : per * 8. * (double)(c->elem_nr[e] / 2 + 1);
+            }
+        }
+        c->bytes_step = by;
+        c->updates_step = up;
+        c->class_bytes[0] = bel + (by - bconst0);
+        c->class_bytes[1] = bnm;
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+// Kernel classes timed by the profiling mode of axg_run
+enum { KC_ELEMENT = 0, KC_NEWMARK = 1, KC_OTHER = 2, KC_N = 3 };
+
+void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
+    std::vector<hipEvent_t> ev;
+    if (profile) {
+        ev.resize((size_t)count * 4);
+        for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    }
+    for (int tstep = first; tstep < first + count; tstep++) {
+        size_t k = (size_t)(tstep - first) * 4;
+        if (profile) HIPCHECK(hipEventRecord(ev[k + 0], c->stream));
+        doNewmark(c, c->dt);
+        if (profile) HIPCHECK(hipEventRecord(ev[k + 1], c->stream));
+        doSource(c, tstep - 1);
+        doStiff(c);
+        if (profile) HIPCHECK(hipEventRecord(ev[k + 2], c->stream));
+        doCouple(c);
+        doRecord(c, tstep - 1);
+        if (profile) HIPCHECK(hipEventRecord(ev[k + 3], c->stream));
+        if (stab > 0 && tstep % stab == 0) {
+            int64_t bad = doStability(c);
+            if (bad >= 0)
+                throw std::runtime_error("Domain::checkStability || Simulation blew up at point " +
+                                         std::to_string(bad) + ", step " + std::to_string(tstep) + ".");
+        }
+    }
+    for (int i = 0; i < KC_N; i++) c->class_ms[i] = 0.;
+    if (profile) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < count; i++) {
+            float a = 0.f, b = 0.f, d = 0.f;
+            HIPCHECK(hipEventElapsedTime(&a, ev[i * 4 + 0], ev[i * 4 + 1]));
+            HIPCHECK(hipEventElapsedTime(&b, ev[i * 4 + 1], ev[i * 4 + 2]));
+            HIPCHECK(hipEventElapsedTime(&d, ev[i * 4 + 2], ev[i * 4 + 3]));
+            c->class_ms[KC_NEWMARK] += a;
+            c->class_ms[KC_ELEMENT] += b;
+            c->class_ms[KC_OTHER] += d;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int64_t axg_field_size(const axg_mesh_desc *d, int32_t which) {
+    if (!d) return -1;
+    bool solid = which < 4;
+    int64_t n = 0;
+    for (int p = 0; p < d->npoint; p++) {
+        if (!(d->point_flags[p] & (solid ? AXG_PT_SOLID : AXG_PT_FLUID))) continue;
+        n += (int64_t)(d->point_nr[p] / 2 + 1) * (solid ? 6 : 2);
+    }
+    return n;
+}
+
+int axg_create(const axg_mesh_desc *desc, int device, axg_ctx **ctx) {
+    if (!ctx) return fail(AXG_EINVAL, "axg_create || null context pointer.");
+    *ctx = nullptr;
+    std::unique_ptr<axg_ctx> c(new axg_ctx());
+    try {
+        createCtx(desc, device, c.get());
+    } catch (const HipError &e) {
+        return fail(AXG_EHIP, e.what());
+    } catch (const std::exception &e) {
+        std::string m = e.what();
+        bool uns = m.find("not supported") != std::string::npos;
+        return fail(uns ? AXG_EUNSUPPORTED : AXG_EINVAL, m);
+    }
+    *ctx = c.release();
+    return AXG_OK;
+}
+
+int axg_destroy(axg_ctx *c) {
+    if (!c) return AXG_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+    return AXG_OK;
+}
+
+#define CTX_CHECK(c)                                                          \
+    if (!(c)) return fail(AXG_EINVAL, "axg || null context.");               \
+    (void)hipSetDevice((c)->device);
+
+int axg_reset_zero(axg_ctx *c) {
+    CTX_CHECK(c);
+    API_GUARD({
+        for (Phase *P : {&c->S, &c->F}) {
+            P->displ.zero(c->stream);
+            P->veloc.zero(c->stream);
+            P->accel.zero(c->stream);
+            P->stiff.zero(c->stream);
+        }
+        c->d_att_memvar.zero(c->stream);
+        c->d_att_stressR.zero(c->stream);
+        c->d_seis.zero(c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    })
+}
+
+int axg_update_newmark(axg_ctx *c, double dt) {
+    CTX_CHECK(c);
+    API_GUARD({
+        doNewmark(c, dt);
+        HIPCHECK(hipGetLastError());
+    })
+}
+
+int axg_apply_source(axg_ctx *c, int32_t tstep) {
+    CTX_CHECK(c);
+    API_GUARD({
+        doSource(c, tstep);
+        HIPCHECK(hipGetLastError());
+    })
+}
+
+int axg_compute_stiff(axg_ctx *c) {
+    CTX_CHECK(c);
+    API_GUARD({
+        doStiff(c);
+        HIPCHECK(hipGetLastError());
+    })
+}
+
+int axg_couple_solid_fluid(axg_ctx *c) {
+    CTX_CHECK(c);
+    API_GUARD({
+        doCouple(c);
+        HIPCHECK(hipGetLastError());
+    })
+}
+
+int axg_assemble_stiff(axg_ctx *c, int32_t phase) {
+    CTX_CHECK(c);
+    (void)phase;
+    return AXG_OK;
+}
+
+int axg_record(axg_ctx *c, int32_t tstep, double t) {
+    CTX_CHECK(c);
+    (void)t;
+    API_GUARD({
+        doRecord(c, tstep);
+        HIPCHECK(hipGetLastError());
+    })
+}
+
+int axg_check_stability(axg_ctx *c, int64_t *bad_point) {
+    CTX_CHECK(c);
+    try {
+        int64_t bad = doStability(c);
+        if (bad_point) *bad_point = bad;
+        if (bad >= 0) return fail(AXG_EUNSTABLE, "Domain::checkStability || Simulation blew up.");
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        return fail(AXG_EHIP, e.what());
+    }
+}
+
+int axg_learn_wisdom(axg_ctx *c, float cutoff) {
+    CTX_CHECK(c);
+    (void)cutoff;
+    return fail(AXG_EUNSUPPORTED, "Domain::learnWisdom || not supported by this build yet.");
+}
+
+int axg_get_wisdom(axg_ctx *c, int32_t *nu) {
+    CTX_CHECK(c);
+    (void)nu;
+    return fail(AXG_EUNSUPPORTED, "Domain::dumpWisdom || not supported by this build yet.");
+}
+
+int axg_run(axg_ctx *c, int32_t first, int32_t count, int32_t stab_interval, int32_t flags) {
+    CTX_CHECK(c);
+    bool profile = (flags & AXG_RUN_PROFILE) != 0;
+    if (first < 1 || count < 0 || first + count - 1 > c->nstep)
+        return fail(AXG_EINVAL, "Newmark::solve || step range outside the source-time function.");
+    try {
+        HIPCHECK(hipEventRecord(c->ev0, c->stream));
+        runSteps(c, first, count, stab_interval, profile);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(c->ev1, c->stream));
+        HIPCHECK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+        return AXG_OK;
+    } catch (const HipError &e) {
+        return fail(AXG_EHIP, e.what());
+    } catch (const std::exception &e) {
+        std::string m = e.what();
+        return fail(m.find("blew up") != std::string::npos ? AXG_EUNSTABLE : AXG_EINVAL, m);
+    }
+}
+
+int axg_get_seismograms(axg_ctx *c, float *out, int64_t nfloat) {
+    CTX_CHECK(c);
+    int64_t need = (int64_t)c->nstep * c->nrec * 3;
+    if (!out || nfloat < need) return fail(AXG_EINVAL, "axg_get_seismograms || output buffer too small.");
+    API_GUARD({
+        if (need) HIPCHECK(hipMemcpyAsync(out, c->d_seis.p, need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    })
+}
+
+static DevBuf<float2> *fieldBuf(axg_ctx *c, int which) {
+    Phase &P = which < 4 ? c->S : c->F;
+    switch (which % 4) {
+        case 0: return &P.displ;
+        case 1: return &P.veloc;
+        case 2: return &P.accel;
+        default: return &P.stiff;
+    }
+}
+
+int axg_get_field(axg_ctx *c, int32_t which, float *out, int64_t nfloat) {
+    CTX_CHECK(c);
+    if (which < 0 || which > 7) return fail(AXG_EINVAL, "axg_get_field || bad field.");
+    DevBuf<float2> *b = fieldBuf(c, which);
+    if (!out || nfloat != (int64_t)b->n * 2) return fail(AXG_EINVAL, "axg_get_field || size mismatch.");
+    API_GUARD({
+        if (b->n) HIPCHECK(hipMemcpyAsync(out, b->p, b->n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    })
+}
+
+int axg_set_field(axg_ctx *c, int32_t which, const float *in, int64_t nfloat) {
+    CTX_CHECK(c);
+    if (which < 0 || which > 7) return fail(AXG_EINVAL, "axg_set_field || bad field.");
+    DevBuf<float2> *b = fieldBuf(c, which);
+    if (!in || nfloat != (int64_t)b->n * 2) return fail(AXG_EINVAL, "axg_set_field || size mismatch.");
+    API_GUARD({
+        if (b->n) HIPCHECK(hipMemcpyAsync(b->p, in, b->n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    })
+}
+
+int axg_last_run_timing(axg_ctx *c, double *ms_total, double *ms_per_class, int32_t nclass) {
+    CTX_CHECK(c);
+    if (ms_total) *ms_total = c->last_ms;
+    for (int i = 0; i < nclass; i++) ms_per_class[i] = i < KC_N ? c->class_ms[i] : 0.;
+    return AXG_OK;
+}
+
+int axg_class_bytes(axg_ctx *c, double *b, int32_t n) {
+    CTX_CHECK(c);
+    for (int i = 0; i < n; i++) b[i] = i < 3 ? c->class_bytes[i] : 0.;
+    return AXG_OK;
+}
+
+int axg_step_cost(axg_ctx *c, double *bytes_step, double *updates_step) {
+    CTX_CHECK(c);
+    if (bytes_step) *bytes_step = c->bytes_step;
+    if (updates_step) *updates_step = c->updates_step;
+    return AXG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// common.hpp -- device helpers shared by the time-loop kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace axg {
+
+// derivative matrices, row-major G[i][j] = l_i'(x_j); [0] = GLL, [1] = GLJ(0,1)
+// (Gradient::setGMat, SOLVER/src/core/element/grad/Gradient.cpp:371-380)
+extern __constant__ float cG[2][25];
+
+struct cf {
+    float x, y;
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+// i * alpha * z
+__device__ __forceinline__ float2 cimul(float alpha, float2 z) { return make_float2(-alpha * z.y, alpha * z.x); }
+__device__ __forceinline__ float2 czero() { return make_float2(0.f, 0.f); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// geometry block of an element: q = 0 dsdxii, 1 dsdeta, 2 dzdxii, 3 dzdeta, 4 inv_s
+struct PointGeom {
+    float dsdxii, dsdeta, dzdxii, dzdeta, inv_s;
+};
+
+__device__ __forceinline__ PointGeom load_geom(const float *__restrict__ geom, int64_t e, int k) {
+    const float *g = geom + e * 125;
+    PointGeom p;
+    p.dsdxii = g[0 * 25 + k];
+    p.dsdeta = g[1 * 25 + k];
+    p.dzdxii = g[2 * 25 + k];
+    p.dzdeta = g[3 * 25 + k];
+    p.inv_s = g[4 * 25 + k];
+    return p;
+}
+
+// CrdTransTIsoSolid::transformSPZ_RTZ on one GLL point of one mode
+// (SOLVER/src/core/element/crd/CrdTransTIsoSolid.cpp:14-27)
+__device__ __forceinline__ void tiso_spz_rtz(float2 u[6], float s1, float c1, float s2, float c2) {
+    float2 sum02 = cadd(u[0], u[2]);
+    float2 dif02 = csub(u[0], u[2]);
+    float2 u3 = u[3];
+    float2 u4 = u[4];
+    float2 u5 = u[5];
+    float2 n0 = cscale(0.5f, csub(cadd(sum02, cscale(c2, dif02)), cscale(s2, u4)));
+    u[0] = n0;
+    u[2] = csub(sum02, n0);
+    u[4] = cadd(cscale(c2, u4), cscale(s2, dif02));
+    u[3] = cadd(cscale(c1, u3), cscale(s1, u5));
+    u[5] = csub(cscale(c1, u5), cscale(s1, u3));
+}
+
+// CrdTransTIsoSolid::transformRTZ_SPZ (CrdTransTIsoSolid.cpp:29-42)
+__device__ __forceinline__ void tiso_rtz_spz(float2 u[6], float s1, float c1, float s2, float c2) {
+    float2 sum02 = cadd(u[0], u[2]);
+    float2 dif02 = cscale(0.5f, csub(u[0], u[2]));
+    float2 u3 = u[3];
+    float2 u4 = u[4];
+    float2 u5 = u[5];
+    float2 n0 = cadd(cadd(cscale(0.5f, sum02), cscale(c2, dif02)), cscale(s2, u4));
+    u[0] = n0;
+    u[2] = csub(sum02, n0);
+    u[4] = csub(cscale(c2, u4), cscale(s2, dif02));
+    u[3] = csub(cscale(c1, u3), cscale(s1, u5));
+    u[5] = cadd(cscale(c1, u5), cscale(s1, u3));
+}
+
+// same rotations on real physical samples (3-D path)
+__device__ __forceinline__ void tiso_spz_rtz_r(float u[6], float s1, float c1, float s2, float c2) {
+    float sum02 = u[0] + u[2], dif02 = u[0] - u[2], u3 = u[3], u4 = u[4], u5 = u[5];
+    float n0 = 0.5f * (sum02 + c2 * dif02 - s2 * u4);
+    u[0] = n0;
+    u[2] = sum02 - n0;
+    u[4] = c2 * u4 + s2 * dif02;
+    u[3] = c1 * u3 + s1 * u5;
+    u[5] = c1 * u5 - s1 * u3;
+}
+
+}  // namespace axg

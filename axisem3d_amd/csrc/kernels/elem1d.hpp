@@ -1,0 +1,403 @@
+// elem1d.hpp -- stiffness of elements whose material does not vary in azimuth
+// (the reference's "1D" / Fourier-space path): every Fourier mode is
+// independent, so one work item is one (element, alpha) pair and one thread is
+// one GLL point of it.  A 256-thread block holds 10 items (250 lanes); the
+// 5x5 GLL sum-factorised contractions go through LDS.
+//
+// Restates, for one (element, alpha):
+//   Point::scatterDisplToElement   SOLVER/src/core/point/SolidPoint.cpp:175-195
+//   Gradient::computeGrad6         SOLVER/src/core/element/grad/Gradient.cpp:206-265
+//   CrdTransTIsoSolid              SOLVER/src/core/element/crd/CrdTransTIsoSolid.cpp:14-42
+//   Isotropic1D / TransverselyIsotropic1D / Anisotropic1D::strainToStress
+//                                  SOLVER/src/core/element/material/elastic/1D/*.cpp
+//   Attenuation1D_Full / _CG4      SOLVER/src/core/element/material/attenuation/1D/*.cpp
+//   Gradient::computeQuad6         Gradient.cpp:267-317
+//   Point::gatherStiffFromElement  SolidPoint.cpp:197-209 (assembly by atomic add)
+// and the fluid equivalents (Gradient::computeGrad/computeQuad, Gradient.cpp:26-82,
+// Acoustic1D.cpp:8-16, FluidPoint.cpp:169-195).
+//
+// The alpha = 0 row uses the same expressions as alpha > 0: with alpha = 0 the
+// i*alpha terms vanish and what remains is exactly the reference's hard-coded
+// alpha = 0 branch (its imaginary part is never used: maskField zeroes it).
+#pragma once
+#include "common.hpp"
+
+namespace axg {
+
+constexpr int E1D_ITEMS = 10;  // (element, alpha) items per 256-thread block
+
+struct Elem1DArgs {
+    const int32_t *item_elem;
+    const int32_t *item_alpha;
+    int32_t nitem;
+    const int32_t *elem_flags;
+    const float *geom;        // [nelem*125]
+    const float *tiso;        // [nelem*100]
+    const float *mat_pool;
+    const int64_t *mat_off;   // [nelem+1]
+    const float *att_pool;
+    const int64_t *att_off;
+    const int32_t *ep_row;    // [nelem*25] first row of the point in its phase's field arrays
+    const int32_t *ep_amax;   // [nelem*25] Nu_p - nyquist_p
+    const float2 *displ;      // solid: rows x 3, fluid: rows x 1
+    float *stiff;             // same layout, as floats (atomic accumulation)
+    // attenuation (memory variables per element, per mode)
+    float2 *att_memvar;       // [nsls][...]
+    float2 *att_stressR;
+    const int64_t *att_mem_off;  // [nelem] complex offset of (alpha=0) block
+    int32_t nsls;
+    float sls_alpha[8], sls_beta[8], sls_gamma[8];
+    int32_t do_kappa;
+    int64_t att_mem_stride;   // complex entries per SLS
+};
+
+// strain of one GLL point (i, j) of mode alpha, from the LDS displacement
+// U[c][m*5+n] of the item.  Gradient::computeGrad6 with the axial row-0 terms.
+__device__ __forceinline__ void strain6_at(const float2 (*U)[25], int i, int j, float alpha, bool axial,
+                                           const PointGeom &g, float2 e[6]) {
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    float2 GU[3], UG[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float2 a = czero(), b = czero();
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            float gx = Gx[m * 5 + i];  // G^T_xi (i, m) = G_xi(m, i)
+            float ge = Ge[m * 5 + j];  // G_eta (m, j)
+            float2 um = U[c][m * 5 + j];
+            float2 un = U[c][i * 5 + m];
+            a.x += gx * um.x;
+            a.y += gx * um.y;
+            b.x += un.x * ge;
+            b.y += un.y * ge;
+        }
+        GU[c] = a;
+        UG[c] = b;
+    }
+    float2 u0 = U[0][i * 5 + j], u1 = U[1][i * 5 + j], u2 = U[2][i * 5 + j];
+    float2 v0 = cadd(u0, cimul(alpha, u1));
+    float2 v1 = csub(cimul(alpha, u0), u1);
+    float2 v2 = cimul(alpha, u2);
+    e[0] = cadd(cscale(g.dzdeta, GU[0]), cscale(g.dzdxii, UG[0]));
+    e[1] = cscale(g.inv_s, v0);
+    e[2] = cadd(cscale(g.dsdeta, GU[2]), cscale(g.dsdxii, UG[2]));
+    e[3] = cadd(cadd(cscale(g.dsdeta, GU[1]), cscale(g.dsdxii, UG[1])), cscale(g.inv_s, v2));
+    e[4] = cadd(cadd(cscale(g.dsdeta, GU[0]), cscale(g.dsdxii, UG[0])),
+                cadd(cscale(g.dzdeta, GU[2]), cscale(g.dzdxii, UG[2])));
+    e[5] = cadd(cadd(cscale(g.dzdeta, GU[1]), cscale(g.dzdxii, UG[1])), cscale(g.inv_s, v1));
+    if (axial && i == 0) {
+        // d/dxi of v0, v1, v2 at (0, j) with the GLJ operator
+        float2 dv0 = cadd(GU[0], cimul(alpha, GU[1]));
+        float2 dv1 = csub(cimul(alpha, GU[0]), GU[1]);
+        float2 dv2 = cimul(alpha, GU[2]);
+        e[1] = cadd(e[1], cscale(g.dzdeta, dv0));
+        e[5] = cadd(e[5], cscale(g.dzdeta, dv1));
+        e[3] = cadd(e[3], cscale(g.dzdeta, dv2));
+        if (alpha == 1.f) {
+            float2 ev0 = cadd(UG[0], cimul(alpha, UG[1]));
+            float2 ev1 = csub(cimul(alpha, UG[0]), UG[1]);
+            e[1] = cadd(e[1], cscale(g.dzdxii, ev0));
+            e[5] = cadd(e[5], cscale(g.dzdxii, ev1));
+        }
+    }
+}
+
+template <int MAT>
+__device__ __forceinline__ void stress1d(const float *__restrict__ mp, int k, const float2 e[6], float2 s[6]) {
+    if (MAT == 0) {
+        // Isotropic1D::strainToStress (Isotropic1D.cpp:9-26)
+        float lam = mp[k], mu = mp[25 + k];
+        float mu2 = 2.f * mu;
+        float2 sii = cscale(lam, cadd(cadd(e[0], e[1]), e[2]));
+        s[0] = cadd(sii, cscale(mu2, e[0]));
+        s[1] = cadd(sii, cscale(mu2, e[1]));
+        s[2] = cadd(sii, cscale(mu2, e[2]));
+        s[3] = cscale(mu, e[3]);
+        s[4] = cscale(mu, e[4]);
+        s[5] = cscale(mu, e[5]);
+    } else if (MAT == 1) {
+        // TransverselyIsotropic1D::strainToStress (TransverselyIsotropic1D.cpp:9-27)
+        float A = mp[k], Cc = mp[25 + k], F = mp[50 + k], L = mp[75 + k], N = mp[100 + k];
+        float N2 = 2.f * N;
+        float2 e01 = cadd(e[0], e[1]);
+        float2 tmp = cadd(cscale(A, e01), cscale(F, e[2]));
+        s[0] = csub(tmp, cscale(N2, e[1]));
+        s[1] = csub(tmp, cscale(N2, e[0]));
+        s[2] = cadd(cscale(Cc, e[2]), cscale(F, e01));
+        s[3] = cscale(L, e[3]);
+        s[4] = cscale(L, e[4]);
+        s[5] = cscale(N, e[5]);
+    } else {
+        // Anisotropic1D::strainToStress (Anisotropic1D.cpp:9-54), packed upper triangle
+        float Cm[21];
+#pragma unroll
+        for (int t = 0; t < 21; t++) Cm[t] = mp[t * 25 + k];
+        const int idx[6][6] = {{0, 1, 2, 3, 4, 5},     {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
+                               {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            float2 acc = czero();
+#pragma unroll
+            for (int b = 0; b < 6; b++) acc = cadd(acc, cscale(Cm[idx[a][b]], e[b]));
+            s[a] = acc;
+        }
+    }
+}
+
+// Attenuation1D_Full / _CG4: applyToStress then updateMemoryVariables on one
+// (point, alpha).  Memory layout per element: [sls][alpha][6][25 or 4] complex,
+// stressR [alpha][6][25 or 4].
+template <int ATT>
+__device__ __forceinline__ void attenuation1d(const Elem1DArgs &a, int64_t e, int alpha, int k, const float2 strain[6],
+                                              float2 stress[6]) {
+    int cgk = -1;
+    if (ATT == 2) {
+        // coarse-grid points (1,1), (1,3), (3,1), (3,3)
+        cgk = (k == 6) ? 0 : (k == 8) ? 1 : (k == 16) ? 2 : (k == 18) ? 3 : -1;
+        if (cgk < 0) return;
+    }
+    const int np = (ATT == 2) ? 4 : 25;
+    const int kk = (ATT == 2) ? cgk : k;
+    const float *ap = a.att_pool + a.att_off[e];
+    float dkappa = ap[kk];
+    float dmu = ap[np + kk];
+    float dkappa3 = 3.f * dkappa, dmu2 = 2.f * dmu;
+    int64_t base = a.att_mem_off[e] + (int64_t)alpha * 6 * np + kk;
+    float2 *R = a.att_memvar;
+    float2 *SR = a.att_stressR;
+    // applyToStress: stress -= sum_s R_s (old)
+    for (int s = 0; s < a.nsls; s++) {
+        const float2 *Rs = R + (int64_t)s * a.att_mem_stride + base;
+#pragma unroll
+        for (int c = 0; c < 6; c++) stress[c] = csub(stress[c], Rs[c * np]);
+    }
+    // updateMemoryVariables: R = a R + b stressR(old); stressR = f(strain); R += g stressR
+    float2 sr[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) sr[c] = SR[base + c * np];
+    const float third = (float)(1.0 / 3.0);
+    float2 eii3 = cscale(third, cadd(cadd(strain[0], strain[1]), strain[2]));
+    float2 nsr[6];
+    if (a.do_kappa) {
+        float2 sii3 = cscale(dkappa3, eii3);
+        nsr[0] = cadd(sii3, cscale(dmu2, csub(strain[0], eii3)));
+        nsr[1] = cadd(sii3, cscale(dmu2, csub(strain[1], eii3)));
+        nsr[2] = cadd(sii3, cscale(dmu2, csub(strain[2], eii3)));
+    } else {
+        nsr[0] = cscale(dmu2, csub(strain[0], eii3));
+        nsr[1] = cscale(dmu2, csub(strain[1], eii3));
+        nsr[2] = make_float2(-(nsr[0].x + nsr[1].x), -(nsr[0].y + nsr[1].y));
+    }
+    nsr[3] = cscale(dmu, strain[3]);
+    nsr[4] = cscale(dmu, strain[4]);
+    nsr[5] = cscale(dmu, strain[5]);
+    for (int s = 0; s < a.nsls; s++) {
+        float2 *Rs = R + (int64_t)s * a.att_mem_stride + base;
+        float al = a.sls_alpha[s], be = a.sls_beta[s], ga = a.sls_gamma[s];
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            float2 r = Rs[c * np];
+            r = cadd(cscale(al, r), cscale(be, sr[c]));
+            r = cadd(r, cscale(ga, nsr[c]));
+            Rs[c * np] = r;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) SR[base + c * np] = nsr[c];
+}
+
+template <int MAT, bool TISO, int ATT>
+__global__ __launch_bounds__(256) void solid1d_kernel(Elem1DArgs a) {
+    __shared__ float2 sU[E1D_ITEMS][3][25];
+    __shared__ float2 sX[E1D_ITEMS][3][25];
+    __shared__ float2 sY[E1D_ITEMS][3][25];
+    __shared__ float2 sW[E1D_ITEMS][3][5];
+    __shared__ float2 sV[E1D_ITEMS][2][5];
+    const int t = threadIdx.x;
+    const int it = t / 25;
+    const int k = t - it * 25;
+    const int item = blockIdx.x * E1D_ITEMS + it;
+    const bool active = (it < E1D_ITEMS) && (item < a.nitem);
+    int64_t e = 0;
+    int alpha = 0, row = 0, amax = -1;
+    bool axial = false;
+    if (active) {
+        e = a.item_elem[item];
+        alpha = a.item_alpha[item];
+        row = a.ep_row[e * 25 + k];
+        amax = a.ep_amax[e * 25 + k];
+        axial = (a.elem_flags[e] & 2) != 0;
+        // scatterDisplToElement: Nyquist and alpha > Nu_p read as zero
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            sU[it][c][k] = (alpha <= amax) ? a.displ[((int64_t)row + alpha) * 3 + c] : czero();
+    }
+    __syncthreads();
+    const int i = k / 5, j = k - (k / 5) * 5;
+    float2 f[3];
+    float2 g[3];
+    PointGeom gm;
+    if (active) {
+        gm = load_geom(a.geom, e, k);
+        float fa = (float)alpha;
+        float2 e6[6], s6[6];
+        strain6_at(sU[it], i, j, fa, axial, gm, e6);
+        float ts1 = 0.f, tc1 = 0.f, ts2 = 0.f, tc2 = 0.f;
+        if (TISO) {
+            const float *tp = a.tiso + e * 100;
+            ts1 = tp[k];
+            tc1 = tp[25 + k];
+            ts2 = tp[50 + k];
+            tc2 = tp[75 + k];
+            tiso_spz_rtz(e6, ts1, tc1, ts2, tc2);
+        }
+        stress1d<MAT>(a.mat_pool + a.mat_off[e], k, e6, s6);
+        if (ATT != 0) attenuation1d<ATT>(a, e, alpha, k, e6, s6);
+        if (TISO) tiso_rtz_spz(s6, ts1, tc1, ts2, tc2);
+        // Gradient::computeQuad6 operands at this point
+        sX[it][0][k] = cadd(cscale(gm.dzdeta, s6[0]), cscale(gm.dsdeta, s6[4]));
+        sX[it][1][k] = cadd(cscale(gm.dzdeta, s6[5]), cscale(gm.dsdeta, s6[3]));
+        sX[it][2][k] = cadd(cscale(gm.dzdeta, s6[4]), cscale(gm.dsdeta, s6[2]));
+        sY[it][0][k] = cadd(cscale(gm.dzdxii, s6[0]), cscale(gm.dsdxii, s6[4]));
+        sY[it][1][k] = cadd(cscale(gm.dzdxii, s6[5]), cscale(gm.dsdxii, s6[3]));
+        sY[it][2][k] = cadd(cscale(gm.dzdxii, s6[4]), cscale(gm.dsdxii, s6[2]));
+        // g = (s1 - i a s5, -i a s1 - s5, -i a s3)
+        float nb = -fa;
+        g[0] = cadd(s6[1], cimul(nb, s6[5]));
+        g[1] = csub(cimul(nb, s6[1]), s6[5]);
+        g[2] = cimul(nb, s6[3]);
+        if (axial && i == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) sW[it][c][j] = cscale(gm.dzdeta, g[c]);
+            sV[it][0][j] = cscale(gm.dzdxii, g[0]);
+            sV[it][1][j] = cscale(gm.dzdxii, g[1]);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float2 acc = cscale(gm.inv_s, g[c]);
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            float gx = Gx[i * 5 + m];  // G_xi (i, m)
+            float ge = Ge[j * 5 + m];  // G^T_eta (m, j) = G_eta (j, m)
+            float2 xm = sX[it][c][m * 5 + j];
+            float2 ym = sY[it][c][i * 5 + m];
+            acc.x += gx * xm.x + ym.x * ge;
+            acc.y += gx * xm.y + ym.y * ge;
+        }
+        if (axial) {
+            float g0 = Gx[i * 5 + 0];
+            float2 w = sW[it][c][j];
+            acc.x += g0 * w.x;
+            acc.y += g0 * w.y;
+            if (alpha == 1 && i == 0 && c < 2) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    float ge = Ge[j * 5 + m];
+                    float2 v = sV[it][c][m];
+                    acc.x += v.x * ge;
+                    acc.y += v.y * ge;
+                }
+            }
+        }
+        f[c] = acc;
+    }
+    // gatherStiffFromElement: stiff -= f on rows alpha <= Nu_p - nyquist_p
+    if (alpha <= amax) {
+        float *sp = a.stiff + ((int64_t)row + alpha) * 6;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            atomicAdd(sp + 2 * c, -f[c].x);
+            if (alpha > 0) atomicAdd(sp + 2 * c + 1, -f[c].y);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void fluid1d_kernel(Elem1DArgs a) {
+    __shared__ float2 sU[E1D_ITEMS][25];
+    __shared__ float2 sX[E1D_ITEMS][25];
+    __shared__ float2 sY[E1D_ITEMS][25];
+    __shared__ float2 sW[E1D_ITEMS][5];
+    const int t = threadIdx.x;
+    const int it = t / 25;
+    const int k = t - it * 25;
+    const int item = blockIdx.x * E1D_ITEMS + it;
+    const bool active = (it < E1D_ITEMS) && (item < a.nitem);
+    int64_t e = 0;
+    int alpha = 0, row = 0, amax = -1;
+    bool axial = false;
+    if (active) {
+        e = a.item_elem[item];
+        alpha = a.item_alpha[item];
+        row = a.ep_row[e * 25 + k];
+        amax = a.ep_amax[e * 25 + k];
+        axial = (a.elem_flags[e] & 2) != 0;
+        sU[it][k] = (alpha <= amax) ? a.displ[(int64_t)row + alpha] : czero();
+    }
+    __syncthreads();
+    const int i = k / 5, j = k - (k / 5) * 5;
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    float2 gg = czero();
+    PointGeom gm;
+    if (active) {
+        gm = load_geom(a.geom, e, k);
+        float fa = (float)alpha;
+        // Gradient::computeGrad (Gradient.cpp:26-56)
+        float2 GU = czero(), UG = czero();
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            float gx = Gx[m * 5 + i];
+            float ge = Ge[m * 5 + j];
+            float2 um = sU[it][m * 5 + j];
+            float2 un = sU[it][i * 5 + m];
+            GU.x += gx * um.x;
+            GU.y += gx * um.y;
+            UG.x += un.x * ge;
+            UG.y += un.y * ge;
+        }
+        float2 v = cimul(fa, sU[it][k]);
+        float2 e0 = cadd(cscale(gm.dzdeta, GU), cscale(gm.dzdxii, UG));
+        float2 e1 = cscale(gm.inv_s, v);
+        float2 e2 = cadd(cscale(gm.dsdeta, GU), cscale(gm.dsdxii, UG));
+        if (axial && i == 0) e1 = cadd(e1, cscale(gm.dzdeta, cimul(fa, GU)));
+        // Acoustic1D::strainToStress (Acoustic1D.cpp:8-16)
+        float K = a.mat_pool[a.mat_off[e] + k];
+        float2 s0 = cscale(K, e0), s1 = cscale(K, e1), s2 = cscale(K, e2);
+        // Gradient::computeQuad (Gradient.cpp:58-82)
+        sX[it][k] = cadd(cscale(gm.dzdeta, s0), cscale(gm.dsdeta, s2));
+        sY[it][k] = cadd(cscale(gm.dzdxii, s0), cscale(gm.dsdxii, s2));
+        gg = cimul(-fa, s1);
+        if (axial && i == 0) sW[it][j] = cscale(gm.dzdeta, gg);
+    }
+    __syncthreads();
+    if (!active) return;
+    float2 acc = cscale(gm.inv_s, gg);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        float gx = Gx[i * 5 + m];
+        float ge = Ge[j * 5 + m];
+        float2 xm = sX[it][m * 5 + j];
+        float2 ym = sY[it][i * 5 + m];
+        acc.x += gx * xm.x + ym.x * ge;
+        acc.y += gx * xm.y + ym.y * ge;
+    }
+    if (axial) {
+        float g0 = Gx[i * 5 + 0];
+        float2 w = sW[it][j];
+        acc.x += g0 * w.x;
+        acc.y += g0 * w.y;
+    }
+    if (alpha <= amax) {
+        float *sp = a.stiff + ((int64_t)row + alpha) * 2;
+        atomicAdd(sp, -acc.x);
+        if (alpha > 0) atomicAdd(sp + 1, -acc.y);
+    }
+}
+
+}  // namespace axg

@@ -1,0 +1,231 @@
+// points.hpp -- per-point kernels: Newmark update with mass inverse and masks,
+// source injection, solid-fluid coupling, stability check, receivers.
+#pragma once
+#include "common.hpp"
+
+namespace axg {
+
+// per-point metadata of one phase (solid or fluid), indexed by compact index
+struct PhasePoints {
+    const int32_t *row_point;   // [nrow] compact point of each row
+    const int32_t *pt_row;      // [npt] first row
+    const int32_t *pt_nu;       // [npt]
+    const int32_t *pt_flags;    // [npt] bit0 axial, bit1 nyquist (Nr even), bit2 fluid surface
+    const float *pt_invmass;    // [npt] Mass1D
+    int32_t nrow;
+    int32_t npt;
+};
+
+// SolidPoint::maskField (SOLVER/src/core/point/SolidPoint.cpp:216-238) on row alpha
+__device__ __forceinline__ void mask_solid_row(float2 s[3], int alpha, int nu, int flags) {
+    if (alpha == 0) {
+        s[0].y = 0.f;
+        s[1].y = 0.f;
+        s[2].y = 0.f;
+    }
+    if (flags & 1) {
+        if (alpha == 0) {
+            s[0] = czero();
+            s[1] = czero();
+        } else if (alpha == 1) {
+            float2 s0 = s[0], s1 = s[1];
+            // (1/2 (s0 - i s1), 1/2 (s1 + i s0))
+            s[0] = make_float2(0.5f * (s0.x + s1.y), 0.5f * (s0.y - s1.x));
+            s[1] = make_float2(0.5f * (s1.x - s0.y), 0.5f * (s1.y + s0.x));
+            s[2] = czero();
+        } else {
+            s[0] = s[1] = s[2] = czero();
+        }
+    }
+    if ((flags & 2) && alpha == nu) s[0] = s[1] = s[2] = czero();
+}
+
+// SolidPoint::updateNewmark (SolidPoint.cpp:23-38) with Mass1D (Mass1D.cpp:12-14)
+__global__ __launch_bounds__(256) void newmark_solid_kernel(PhasePoints P, float2 *__restrict__ displ,
+                                                            float2 *__restrict__ veloc, float2 *__restrict__ accel,
+                                                            float2 *__restrict__ stiff, float dt, float half_dt,
+                                                            float half_dt_dt) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.nrow) return;
+    int p = P.row_point[r];
+    int alpha = r - P.pt_row[p];
+    int nu = P.pt_nu[p];
+    int flags = P.pt_flags[p];
+    float im = P.pt_invmass[p];
+    int64_t b = (int64_t)r * 3;
+    float2 s[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) s[c] = stiff[b + c];
+    mask_solid_row(s, alpha, nu, flags);
+#pragma unroll
+    for (int c = 0; c < 3; c++) s[c] = cscale(im, s[c]);
+    mask_solid_row(s, alpha, nu, flags);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float2 a = accel[b + c];
+        float2 v = veloc[b + c];
+        float2 d = displ[b + c];
+        v.x += half_dt * (a.x + s[c].x);
+        v.y += half_dt * (a.y + s[c].y);
+        a = s[c];
+        d.x += dt * v.x + half_dt_dt * a.x;
+        d.y += dt * v.y + half_dt_dt * a.y;
+        accel[b + c] = a;
+        veloc[b + c] = v;
+        displ[b + c] = d;
+        stiff[b + c] = czero();
+    }
+}
+
+// FluidPoint::updateNewmark / maskField (FluidPoint.cpp:23-44, 197-207)
+__global__ __launch_bounds__(256) void newmark_fluid_kernel(PhasePoints P, float2 *__restrict__ displ,
+                                                            float2 *__restrict__ veloc, float2 *__restrict__ accel,
+                                                            float2 *__restrict__ stiff, float dt, float half_dt,
+                                                            float half_dt_dt) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.nrow) return;
+    int p = P.row_point[r];
+    int alpha = r - P.pt_row[p];
+    int nu = P.pt_nu[p];
+    int flags = P.pt_flags[p];
+    if (flags & 4) {
+        // fluid point on the free surface: resetZero
+        displ[r] = veloc[r] = accel[r] = stiff[r] = czero();
+        return;
+    }
+    float2 s = stiff[r];
+    bool zero = ((flags & 1) && alpha >= 1) || ((flags & 2) && alpha == nu);
+    if (alpha == 0) s.y = 0.f;
+    if (zero) s = czero();
+    s = cscale(P.pt_invmass[p], s);
+    if (alpha == 0) s.y = 0.f;
+    if (zero) s = czero();
+    float2 a = accel[r], v = veloc[r], d = displ[r];
+    v.x += half_dt * (a.x + s.x);
+    v.y += half_dt * (a.y + s.y);
+    a = s;
+    d.x += dt * v.x + half_dt_dt * a.x;
+    d.y += dt * v.y + half_dt_dt * a.y;
+    accel[r] = a;
+    veloc[r] = v;
+    displ[r] = d;
+    stiff[r] = czero();
+}
+
+// SourceTerm::apply + Element::addSourceTerm (SourceTerm.cpp:29-35, Element.cpp:31-35)
+__global__ void source_kernel(const float *__restrict__ force, const int32_t *__restrict__ src_row,
+                              const int32_t *__restrict__ src_nu, const float *__restrict__ stf, int tstep,
+                              float2 *__restrict__ stiff) {
+    int t = threadIdx.x;  // 25 points x 3 rows x 3 components
+    if (t >= 225) return;
+    int k = t / 9, a = (t / 3) % 3, c = t % 3;
+    if (a > src_nu[k]) return;
+    float f = stf[tstep];
+    float2 F = make_float2(force[t * 2] * f, force[t * 2 + 1] * f);
+    float2 *sp = stiff + ((int64_t)src_row[k] + a) * 3 + c;
+    *sp = cadd(*sp, F);
+}
+
+// SolidFluidPoint::coupleSolidFluid with SFCoupling1D (SolidFluidPoint.cpp:106-110,
+// SFCoupling1D.cpp:9-17): fluid stiff += n.u_solid, then solid stiff -= n/m_f * fluid stiff
+__global__ __launch_bounds__(256) void couple1d_kernel(int32_t nrow, const int32_t *__restrict__ row_sf,
+                                                       const int32_t *__restrict__ sf_srow,
+                                                       const int32_t *__restrict__ sf_frow,
+                                                       const int32_t *__restrict__ sf_row0,
+                                                       const float *__restrict__ sf_normal,
+                                                       const float2 *__restrict__ sdispl,
+                                                       float2 *__restrict__ sstiff, float2 *__restrict__ fstiff) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    int q = row_sf[r];
+    int alpha = r - sf_row0[q];
+    const float *n = sf_normal + (int64_t)q * 6;
+    int64_t sr = ((int64_t)sf_srow[q] + alpha) * 3;
+    int64_t fr = (int64_t)sf_frow[q] + alpha;
+    float2 us = sdispl[sr + 0], uz = sdispl[sr + 2];
+    float2 fs = fstiff[fr];
+    fs = cadd(fs, cadd(cscale(n[0], us), cscale(n[2], uz)));
+    fstiff[fr] = fs;
+    sstiff[sr + 0] = csub(sstiff[sr + 0], cscale(n[3], fs));
+    sstiff[sr + 2] = csub(sstiff[sr + 2], cscale(n[5], fs));
+}
+
+// Domain::checkStability: smallest point index with a non-finite displacement
+__global__ __launch_bounds__(256) void stability_kernel(int32_t nrow, const int32_t *__restrict__ row_point,
+                                                        const int32_t *__restrict__ compact_to_point,
+                                                        const float2 *__restrict__ displ, int ncomp,
+                                                        int *__restrict__ bad) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    bool ok = true;
+    for (int c = 0; c < ncomp; c++) {
+        float2 d = displ[(int64_t)r * ncomp + c];
+        ok = ok && isfinite(d.x) && isfinite(d.y);
+    }
+    if (!ok) atomicMin(bad, compact_to_point[row_point[r]]);
+}
+
+// SolidElement::computeGroundMotion (SolidElement.cpp:189-216) + RTZ rotation
+// (PointwiseRecorder.cpp:75-93): one block per receiver.
+__global__ __launch_bounds__(256) void record_solid_kernel(int32_t nrec, const int32_t *__restrict__ rec_elem,
+                                                           const float *__restrict__ rec_w,
+                                                           const float *__restrict__ rec_phi,
+                                                           const float *__restrict__ rec_cost,
+                                                           const float *__restrict__ rec_sint,
+                                                           const int32_t *__restrict__ ep_row,
+                                                           const int32_t *__restrict__ ep_amax,
+                                                           const int32_t *__restrict__ elem_nr,
+                                                           const float2 *__restrict__ displ, int rtz,
+                                                           float *__restrict__ out) {
+    __shared__ float red[3][256];
+    int irec = blockIdx.x;
+    if (irec >= nrec) return;
+    int64_t e = rec_elem[irec];
+    int nr = elem_nr[e];
+    int nu = nr / 2;
+    int amaxe = nu - ((nr % 2 == 0) ? 1 : 0);
+    float phi = rec_phi[irec];
+    float acc[3] = {0.f, 0.f, 0.f};
+    // work: 25 points x (amaxe + 1) modes
+    int nwork = 25 * (amaxe + 1);
+    for (int w = threadIdx.x; w < nwork; w += blockDim.x) {
+        int k = w % 25, alpha = w / 25;
+        float wt = rec_w[irec * 25 + k];
+        if (fabsf(wt) < 1e-10f) continue;
+        if (alpha > ep_amax[e * 25 + k]) continue;
+        const float2 *u = displ + ((int64_t)ep_row[e * 25 + k] + alpha) * 3;
+        if (alpha == 0) {
+            for (int c = 0; c < 3; c++) acc[c] += wt * u[c].x;
+        } else {
+            float sn, cs;
+            sincosf((float)alpha * phi, &sn, &cs);
+            // Re(2 e^{i a phi} u)
+            for (int c = 0; c < 3; c++) acc[c] += wt * (2.f * (cs * u[c].x - sn * u[c].y));
+        }
+    }
+    for (int c = 0; c < 3; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float us = red[0][0], up = red[1][0], uz = red[2][0];
+        float *o = out + (int64_t)irec * 3;
+        if (rtz) {
+            float cost = rec_cost[irec], sint = rec_sint[irec];
+            float ur = us * sint + uz * cost;
+            float ut = us * cost - uz * sint;
+            o[0] = ut;
+            o[1] = up;
+            o[2] = ur;
+        } else {
+            o[0] = us;
+            o[1] = up;
+            o[2] = uz;
+        }
+    }
+}
+
+}  // namespace axg

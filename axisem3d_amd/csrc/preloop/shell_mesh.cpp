@@ -1,0 +1,1464 @@
+// shell_mesh.cpp -- synthetic 2-D meridian spherical-shell meshes for the
+// time loop, produced in the form the reference's objects hold after
+// "release" (Mesh::release, SOLVER/src/preloop/mesh/Mesh.cpp:177-208).
+//
+// This is preloop code: it runs once on the host, in double precision, and
+// casts to float at the end exactly where the reference casts to Real.  What it
+// restates, formula by formula:
+//   spherical mapping + Jacobian     SphericalMapping.cpp:9-78 (Q2 rotation, Mapping.cpp:60-65)
+//   inverse mapping (Newton)         Mapping.cpp:20-31
+//   gradient operators, inv_s        Quad::createGraident, Quad.cpp:509-525
+//   integral factor                  Quad::formIntegralFactor, Quad.cpp:527-547
+//   Nr field (cap, odd, lucky)       Quad::formNrField, Quad.cpp:549-616; PreloopFFTW.cpp:59-109
+//   average GLL spacing, vicinal     ExodusModel::formAuxiliary, ExodusModel.cpp:343-430
+//   moduli x integral factor         Material::createElastic/createElasticAniso/createAcoustic,
+//                                    Material.cpp:257-519
+//   elemental mass                   Material::computeElementalMass, Material.cpp:233-255
+//   attenuation factors              AttAxiSEM::computeAttFactors, AttAxiSEM.cpp:14-57;
+//                                    AttBuilder::createAttenuation{1D,3D}, AttBuilder.cpp:17-129;
+//                                    Quad::computeWeightsCG4, Quad.cpp:438-466
+//   solid-fluid normals              Quad::computeNormal, Quad.cpp:661-715; Quad::setupGLLPoints
+//                                    Quad.cpp:353-366; GLLPoint::release, GLLPoint.cpp:98-118
+//   time step                        Quad::getDeltaT, Quad.cpp:292-307; XMath::findClosestDist
+//   source                           Source::locate, Source.cpp:61-94;
+//                                    Earthquake::computeSourceFourier, Earthquake.cpp:20-88
+//   Gaussian STF                     GaussSTF.cpp:10-21 (decay 1.628, STF.cpp:28)
+//   receivers                        Receiver::locate/computeInterpFact, Receiver.cpp:57-117
+// The geometry (a structured shell, uniform in colatitude, layered in radius),
+// the seeded Gaussian-random-field perturbations and the seeded Nu / Q / moment
+// tensor draws are this project's own synthetic workloads.
+#include "axg.h"
+#include "spectral.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace axg {
+void set_last_error(const std::string &msg);
+}
+
+namespace {
+
+using namespace axg;
+using spectral::nP;
+using spectral::nPol;
+constexpr double pi = 3.141592653589793238463;
+constexpr double tinyDouble = 1e-10;
+constexpr double tinySingle = 1e-5;
+
+inline int period0123(int p) { return ((p % 4) + 4) % 4; }
+
+// Mapping::sOrthogQ2 (Mapping.cpp:60-65)
+const double kQ2[4][2][2] = {{{-1., 0.}, {0., -1.}},
+                             {{0., -1.}, {1., 0.}},
+                             {{1., 0.}, {0., 1.}},
+                             {{0., 1.}, {-1., 0.}}};
+
+// XMath::makeClose (XMath.cpp:9-16)
+inline void makeClose(double &a, double &b) {
+    if (a - b > pi) b += 2. * pi;
+    if (b - a > pi) a += 2. * pi;
+}
+
+struct QuadGeom {
+    double ns[4], nz[4];  // nodal (s, z)
+    int curvedOuter;
+};
+
+struct Rot {
+    double r0, r3, t0, t1, t2, t3, xi, eta;
+    const double (*Q)[2];
+};
+
+inline Rot rotate(const QuadGeom &q, double xi, double eta) {
+    const double(*Q)[2] = kQ2[q.curvedOuter];
+    double rr[4], tt[4];
+    for (int i = 0; i < 4; i++) {
+        double s2 = Q[0][0] * q.ns[i] + Q[0][1] * q.nz[i];
+        double z2 = Q[1][0] * q.ns[i] + Q[1][1] * q.nz[i];
+        rr[i] = std::sqrt(s2 * s2 + z2 * z2);
+        tt[i] = std::atan2(s2, z2);
+    }
+    int co = q.curvedOuter;
+    Rot r;
+    r.Q = Q;
+    r.r0 = rr[period0123(co - 2)];
+    r.r3 = rr[period0123(co + 1)];
+    r.t0 = tt[period0123(co - 2)];
+    r.t1 = tt[period0123(co - 1)];
+    r.t2 = tt[period0123(co - 0)];
+    r.t3 = tt[period0123(co + 1)];
+    r.xi = Q[0][0] * xi + Q[0][1] * eta;
+    r.eta = Q[1][0] * xi + Q[1][1] * eta;
+    makeClose(r.t2, r.t3);
+    makeClose(r.t0, r.t1);
+    return r;
+}
+
+// SphericalMapping::mapping (SphericalMapping.cpp:9-38)
+inline void sphMapping(const QuadGeom &q, double xi, double eta, double &s, double &z) {
+    Rot R = rotate(q, xi, eta);
+    double a3 = ((1. - R.xi) * R.t3 + (1. + R.xi) * R.t2) / 2.;
+    double a0 = ((1. - R.xi) * R.t0 + (1. + R.xi) * R.t1) / 2.;
+    double s2 = (1. + R.eta) * R.r3 / 2. * std::sin(a3) + (1. - R.eta) * R.r0 / 2. * std::sin(a0);
+    double z2 = (1. + R.eta) * R.r3 / 2. * std::cos(a3) + (1. - R.eta) * R.r0 / 2. * std::cos(a0);
+    // Q2^T * sz2
+    s = R.Q[0][0] * s2 + R.Q[1][0] * z2;
+    z = R.Q[0][1] * s2 + R.Q[1][1] * z2;
+}
+
+// SphericalMapping::jacobian (SphericalMapping.cpp:40-78): J = Q2^T J2 Q2
+inline void sphJacobian(const QuadGeom &q, double xi, double eta, double J[2][2]) {
+    Rot R = rotate(q, xi, eta);
+    double a3 = ((1. - R.xi) * R.t3 + (1. + R.xi) * R.t2) / 2.;
+    double a0 = ((1. - R.xi) * R.t0 + (1. + R.xi) * R.t1) / 2.;
+    double J2[2][2];
+    J2[0][0] = (1. + R.eta) * R.r3 * (R.t2 - R.t3) / 4. * std::cos(a3) +
+               (1. - R.eta) * R.r0 * (R.t1 - R.t0) / 4. * std::cos(a0);
+    J2[0][1] = .5 * (R.r3 * std::sin(a3) - R.r0 * std::sin(a0));
+    J2[1][0] = -(1. + R.eta) * R.r3 * (R.t2 - R.t3) / 4. * std::sin(a3) -
+               (1. - R.eta) * R.r0 * (R.t1 - R.t0) / 4. * std::sin(a0);
+    J2[1][1] = .5 * (R.r3 * std::cos(a3) - R.r0 * std::cos(a0));
+    double T[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) T[i][j] = J2[i][0] * R.Q[0][j] + J2[i][1] * R.Q[1][j];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) J[i][j] = R.Q[0][i] * T[0][j] + R.Q[1][i] * T[1][j];
+}
+
+inline double det2(const double J[2][2]) { return J[0][0] * J[1][1] - J[0][1] * J[1][0]; }
+
+// Mapping::invMapping (Mapping.cpp:20-31)
+bool invMapping(const QuadGeom &q, double s, double z, double &xi, double &eta) {
+    xi = eta = 0.;
+    for (int it = 1; it <= 10; it++) {
+        double ms, mz;
+        sphMapping(q, xi, eta, ms, mz);
+        double ds = s - ms, dz = z - mz;
+        if (std::sqrt(ds * ds + dz * dz) < 1e-7) return true;
+        double J[2][2];
+        sphJacobian(q, xi, eta, J);
+        double d = det2(J);
+        xi += (J[1][1] * ds - J[0][1] * dz) / d;
+        eta += (-J[1][0] * ds + J[0][0] * dz) / d;
+    }
+    return false;
+}
+
+// PreloopFFTW::isLuckyNumber / nextLuckyNumber (PreloopFFTW.cpp:59-109)
+bool isLuckyNumber(int n, bool forceOdd) {
+    int num = n;
+    if (forceOdd && num % 2 == 0) return false;
+    if (!forceOdd && num % 2 != 0 && num > 10) return false;
+    for (int i = 2; i <= num; i++) {
+        while (num % i == 0) {
+            num /= i;
+            if (i > 13) return false;
+        }
+    }
+    int e = 0, f = 0;
+    num = n;
+    while (num % 11 == 0) {
+        num /= 11;
+        e++;
+    }
+    num = n;
+    while (num % 13 == 0) {
+        num /= 13;
+        f++;
+    }
+    return e + f <= 1;
+}
+int nextLuckyNumber(int n, bool forceOdd) {
+    while (!isLuckyNumber(n, forceOdd)) n++;
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// seeded Gaussian random field: random-phase spectral sum with a Gaussian
+// covariance of correlation length ell, unit variance.
+struct GRF {
+    std::vector<std::array<double, 4>> k;  // kx, ky, kz, phase
+    double norm = 0.;
+    GRF() = default;
+    GRF(uint64_t seed, double ell, int nmode = 48) {
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<double> nd(0., 1. / ell);
+        std::uniform_real_distribution<double> ud(0., 2. * pi);
+        k.resize(nmode);
+        for (auto &m : k) {
+            m[0] = nd(rng);
+            m[1] = nd(rng);
+            m[2] = nd(rng);
+            m[3] = ud(rng);
+        }
+        norm = std::sqrt(2. / nmode);
+    }
+    double operator()(double x, double y, double z) const {
+        double f = 0.;
+        for (const auto &m : k) f += std::cos(m[0] * x + m[1] * y + m[2] * z + m[3]);
+        return f * norm;
+    }
+};
+
+// ---------------------------------------------------------------------------
+struct QuadInfo {
+    QuadGeom g;
+    int nodes[4];
+    int it, ir, layer;
+    bool axial, fluid, south;
+    int gll[nP * nP];          // global point index of (ipol, jpol)
+    int vicinal[4];            // ExodusModel::mVicinalAxis
+    double xi[nP], eta[nP];
+    double wxi[nP], weta[nP];
+    double crd[nP * nP][2];
+    double J[nP * nP][2][2];
+    double iFact[nP * nP];
+    int pointNr[nP * nP];
+    int nr;
+};
+
+struct Layer {
+    axg_shell_layer p;
+    double valAt(double rb, double rt, double vb, double vt, double r) const {
+        double f = (rt > rb) ? (r - rb) / (rt - rb) : 0.;
+        f = std::min(1., std::max(0., f));
+        return vb + (vt - vb) * f;
+    }
+    double vp(double r) const { return valAt(p.r_bottom, p.r_top, p.vp_bottom, p.vp_top, r); }
+    double vs(double r) const { return valAt(p.r_bottom, p.r_top, p.vs_bottom, p.vs_top, r); }
+    double rho(double r) const { return valAt(p.r_bottom, p.r_top, p.rho_bottom, p.rho_top, r); }
+};
+
+void parallelFor(int n, int nthread, const std::function<void(int, int)> &fn) {
+    if (nthread <= 1 || n < 64) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    int chunk = (n + nthread - 1) / nthread;
+    for (int t = 0; t < nthread; t++) {
+        int b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back(fn, b, e);
+    }
+    for (auto &x : th) x.join();
+}
+
+// Cholesky test of a 6x6 symmetric matrix
+bool isSPD6(const double C[6][6]) {
+    double L[6][6] = {{0.}};
+    for (int i = 0; i < 6; i++) {
+        for (int j = 0; j <= i; j++) {
+            double s = C[i][j];
+            for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+            if (i == j) {
+                if (s <= 0.) return false;
+                L[i][i] = std::sqrt(s);
+            } else {
+                L[i][j] = s / L[j][j];
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+// ===========================================================================
+struct axg_mesh {
+    axg_mesh_desc desc;
+    // backing storage
+    std::vector<int32_t> point_nr, point_flags;
+    std::vector<double> point_sz;
+    std::vector<int64_t> mass_solid_off, mass_fluid_off;
+    std::vector<float> inv_mass_solid, inv_mass_fluid;
+    std::vector<int32_t> elem_points, elem_flags, elem_material, elem_att;
+    std::vector<int64_t> mat_off, att_off;
+    std::vector<float> mat_pool, att_pool, elem_geom, elem_tiso;
+    std::vector<float> sls_alpha, sls_beta, sls_gamma;
+    std::vector<int32_t> sf_point;
+    std::vector<int64_t> sf_off;
+    std::vector<float> sf_normal;
+    std::vector<float> src_force, stf;
+    std::vector<int32_t> rec_elem;
+    std::vector<float> rec_weights, rec_phi, rec_theta;
+    // statistics
+    int64_t nsolid_elem = 0, nfluid_elem = 0, n3d_elem = 0, max_nr = 0, sum_nu1 = 0;
+    double dt = 0.;
+    double t_shift = 0.;
+
+    void bind() {
+        axg_mesh_desc &d = desc;
+        std::memset(&d, 0, sizeof(d));
+        d.npoint = (int32_t)point_nr.size();
+        d.point_nr = point_nr.data();
+        d.point_flags = point_flags.data();
+        d.point_sz = point_sz.data();
+        d.mass_solid_off = mass_solid_off.data();
+        d.inv_mass_solid = inv_mass_solid.data();
+        d.mass_fluid_off = mass_fluid_off.data();
+        d.inv_mass_fluid = inv_mass_fluid.data();
+        d.nelem = (int32_t)elem_flags.size();
+        d.elem_points = elem_points.data();
+        d.elem_flags = elem_flags.data();
+        d.elem_material = elem_material.data();
+        d.mat_off = mat_off.data();
+        d.mat_pool = mat_pool.data();
+        d.elem_geom = elem_geom.data();
+        d.elem_tiso = elem_tiso.data();
+        d.elem_att = elem_att.data();
+        d.att_off = att_off.data();
+        d.att_pool = att_pool.data();
+        d.nsls = (int32_t)sls_alpha.size();
+        d.sls_alpha = sls_alpha.data();
+        d.sls_beta = sls_beta.data();
+        d.sls_gamma = sls_gamma.data();
+        d.nsf = (int32_t)sf_point.size();
+        d.sf_point = sf_point.data();
+        d.sf_off = sf_off.data();
+        d.sf_normal = sf_normal.data();
+        d.src_force = src_force.data();
+        d.nstep = (int32_t)stf.size();
+        d.stf = stf.data();
+        d.dt = dt;
+        d.t_shift = t_shift;
+        d.nrec = (int32_t)rec_elem.size();
+        d.rec_elem = rec_elem.data();
+        d.rec_weights = rec_weights.data();
+        d.rec_phi = rec_phi.data();
+        d.rec_theta = rec_theta.data();
+    }
+};
+
+namespace {
+
+// AttAxiSEM::computeAttFactors (AttAxiSEM.cpp:14-57), per sample
+struct AttFactors {
+    std::vector<double> alpha, beta, gamma;
+    double ysum = 0., fact = 0., w0 = 0., w1 = 0.;
+    std::vector<double> ydsum;
+};
+
+AttFactors makeAttFactors(const std::vector<double> &w, const std::vector<double> &y, double fmin,
+                          double fmax, double fref, double dt) {
+    AttFactors a;
+    int n = (int)w.size();
+    a.ysum = 0.;
+    for (double v : y) a.ysum += v;
+    a.ydsum.resize(n);
+    for (int i = 0; i < n; i++) a.ydsum[i] = y[i] / a.ysum;
+    a.w0 = fref * (2 * pi);
+    a.w1 = std::sqrt(fmin * fmax) * (2. * pi);
+    a.fact = 0.;
+    for (int i = 0; i < n; i++) a.fact += a.ydsum[i] * w[i] * w[i] / (a.w1 * a.w1 + w[i] * w[i]);
+    a.alpha.resize(n);
+    a.beta.resize(n);
+    a.gamma.resize(n);
+    for (int i = 0; i < n; i++) {
+        a.alpha[i] = std::exp(-w[i] * dt);
+        a.beta[i] = ((1. - a.alpha[i]) / (w[i] * dt) - a.alpha[i]) * a.ydsum[i];
+        a.gamma[i] = ((a.alpha[i] - 1.) / (w[i] * dt) + 1.) * a.ydsum[i];
+    }
+    return a;
+}
+
+// per-sample factors of a Q value
+inline void attModulusFactors(const AttFactors &a, double Q, bool doKappaOrMu, double &dFact,
+                              double &factAtt, double &factNoAtt) {
+    if (!doKappaOrMu) {
+        dFact = 0.;
+        factAtt = 1.;
+        factNoAtt = 1.;
+        return;
+    }
+    factNoAtt = 1. + 2. * std::log(a.w1 / a.w0) / pi / Q;
+    dFact = factNoAtt / (1. / a.ysum * Q + (1. - a.fact));
+    factAtt = factNoAtt + dFact * a.fact;
+}
+
+// SLS relaxation frequencies log-spaced over [fmin, fmax] and weights y fitted
+// (non-negative least squares by projected iteration) to a flat 1/Q in band.
+void fitSLS(int nsls, double fmin, double fmax, std::vector<double> &w, std::vector<double> &y) {
+    w.resize(nsls);
+    y.assign(nsls, 1.);
+    for (int i = 0; i < nsls; i++) {
+        double f = (nsls == 1) ? std::sqrt(fmin * fmax)
+                               : fmin * std::pow(fmax / fmin, (double)i / (nsls - 1));
+        w[i] = 2. * pi * f;
+    }
+    const int nf = 64;
+    std::vector<std::vector<double>> A(nf, std::vector<double>(nsls));
+    for (int k = 0; k < nf; k++) {
+        double om = 2. * pi * fmin * std::pow(fmax / fmin, (double)k / (nf - 1));
+        for (int i = 0; i < nsls; i++) A[k][i] = om * w[i] / (w[i] * w[i] + om * om);
+    }
+    // normal equations solved by projected Gauss-Seidel, y >= 0
+    std::vector<std::vector<double>> M(nsls, std::vector<double>(nsls, 0.));
+    std::vector<double> b(nsls, 0.);
+    for (int k = 0; k < nf; k++)
+        for (int i = 0; i < nsls; i++) {
+            b[i] += A[k][i];
+            for (int j = 0; j < nsls; j++) M[i][j] += A[k][i] * A[k][j];
+        }
+    for (int it = 0; it < 2000; it++) {
+        for (int i = 0; i < nsls; i++) {
+            double r = b[i];
+            for (int j = 0; j < nsls; j++)
+                if (j != i) r -= M[i][j] * y[j];
+            y[i] = std::max(1e-6, r / M[i][i]);
+        }
+    }
+}
+
+struct Builder {
+    const axg_shell_config &c;
+    std::vector<Layer> layers;
+    int NT = 0, NR = 0;
+    std::vector<double> rnode;      // radial node radii, bottom to top
+    std::vector<int> rlayer;        // layer of each radial element
+    std::vector<QuadInfo> quads;
+    int ngt = 0, ngr = 0, npoint = 0;
+    std::vector<std::vector<std::pair<int, int>>> pointElems;  // (quad, ipnt)
+    std::vector<double> nodeSpacing;
+    std::vector<int> elemNu;
+    GRF grfVs, grfVp, grfAniso;
+    int nthread = 1;
+
+    explicit Builder(const axg_shell_config &cfg) : c(cfg) {}
+
+    int nodeTag(int it, int ir) const { return ir * (NT + 1) + it; }
+    int pointTag(int gt, int gr) const { return gr * ngt + gt; }
+
+    void buildTopology() {
+        if (c.nelem_theta < 2 || c.nelem_theta % 2 != 0)
+            throw std::runtime_error("ShellMesh::build || nelem_theta must be even and >= 2.");
+        if (c.nlayer < 1 || !c.layers) throw std::runtime_error("ShellMesh::build || No layers.");
+        NT = c.nelem_theta;
+        layers.resize(c.nlayer);
+        rnode.clear();
+        for (int l = 0; l < c.nlayer; l++) {
+            layers[l].p = c.layers[l];
+            const auto &L = c.layers[l];
+            if (L.nelem_radial < 1 || !(L.r_top > L.r_bottom))
+                throw std::runtime_error("ShellMesh::build || Invalid layer.");
+            if (l > 0 && std::abs(c.layers[l - 1].r_top - L.r_bottom) > 1e-6)
+                throw std::runtime_error("ShellMesh::build || Layers must be contiguous.");
+            if (l == 0) rnode.push_back(L.r_bottom);
+            for (int k = 1; k <= L.nelem_radial; k++) {
+                rnode.push_back(L.r_bottom + (L.r_top - L.r_bottom) * k / L.nelem_radial);
+                rlayer.push_back(l);
+            }
+        }
+        NR = (int)rlayer.size();
+        ngt = nPol * NT + 1;
+        ngr = nPol * NR + 1;
+        npoint = ngt * ngr;
+
+        const auto &GLL = spectral::gll();
+        const auto &GLJ = spectral::glj();
+        quads.resize((size_t)NT * NR);
+        for (int ir = 0; ir < NR; ir++) {
+            for (int it = 0; it < NT; it++) {
+                QuadInfo &q = quads[(size_t)ir * NT + it];
+                q.it = it;
+                q.ir = ir;
+                q.layer = rlayer[ir];
+                q.fluid = layers[q.layer].p.fluid != 0;
+                q.south = (it == NT - 1);
+                q.axial = (it == 0) || (it == NT - 1);
+                // corner nodes (theta index, radial index); side 3 (nodes 3-0) on the axis
+                int tn[4], rn[4];
+                if (!q.south) {
+                    tn[0] = it; rn[0] = ir;
+                    tn[1] = it + 1; rn[1] = ir;
+                    tn[2] = it + 1; rn[2] = ir + 1;
+                    tn[3] = it; rn[3] = ir + 1;
+                } else {
+                    tn[0] = it + 1; rn[0] = ir + 1;
+                    tn[1] = it; rn[1] = ir + 1;
+                    tn[2] = it; rn[2] = ir;
+                    tn[3] = it + 1; rn[3] = ir;
+                }
+                for (int k = 0; k < 4; k++) {
+                    double th = pi * tn[k] / NT;
+                    double r = rnode[rn[k]];
+                    q.g.ns[k] = (tn[k] == 0 || tn[k] == NT) ? 0. : r * std::sin(th);
+                    q.g.nz[k] = r * std::cos(th);
+                    if (tn[k] == NT / 2) q.g.nz[k] = 0.;
+                    q.nodes[k] = nodeTag(tn[k], rn[k]);
+                }
+                // Quad::Quad curved-outer detection (Quad.cpp:52-65)
+                double r0 = std::hypot(q.g.ns[0], q.g.nz[0]), r1 = std::hypot(q.g.ns[1], q.g.nz[1]);
+                double r2 = std::hypot(q.g.ns[2], q.g.nz[2]), r3 = std::hypot(q.g.ns[3], q.g.nz[3]);
+                double tol = 1e-6 * std::max(r0, r2);
+                if (std::abs(r0 - r1) < tol && std::abs(r2 - r3) < tol) {
+                    q.g.curvedOuter = r2 > r0 ? 2 : 0;
+                } else {
+                    throw std::runtime_error("Quad::Quad || Invalid spherical element shape.");
+                }
+                const auto &bx = q.axial ? GLJ : GLL;
+                for (int i = 0; i < nP; i++) {
+                    q.xi[i] = bx.x[i];
+                    q.wxi[i] = bx.w[i];
+                    q.eta[i] = GLL.x[i];
+                    q.weta[i] = GLL.w[i];
+                }
+                for (int ipol = 0; ipol < nP; ipol++) {
+                    for (int jpol = 0; jpol < nP; jpol++) {
+                        int gt, gr;
+                        if (!q.south) {
+                            gt = nPol * it + ipol;
+                            gr = nPol * ir + jpol;
+                        } else {
+                            gt = nPol * (it + 1) - ipol;
+                            gr = nPol * (ir + 1) - jpol;
+                        }
+                        q.gll[ipol * nP + jpol] = pointTag(gt, gr);
+                    }
+                }
+            }
+        }
+        // GLL spacing and vicinal flags (ExodusModel::formAuxiliary)
+        int nnode = (NT + 1) * (NR + 1);
+        std::vector<std::vector<int>> refElem(nnode);
+        for (size_t e = 0; e < quads.size(); e++)
+            for (int k = 0; k < 4; k++) refElem[quads[e].nodes[k]].push_back((int)e);
+        nodeSpacing.assign(nnode, 0.);
+        for (int n = 0; n < nnode; n++) {
+            for (int e : refElem[n]) {
+                const QuadGeom &g = quads[e].g;
+                double d = 0.;
+                for (int k = 0; k < 4; k++) {
+                    int k1 = (k + 1) % 4;
+                    d += std::hypot(g.ns[k] - g.ns[k1], g.nz[k] - g.nz[k1]);
+                }
+                nodeSpacing[n] += d / 4. / nPol / refElem[n].size();
+            }
+        }
+        std::vector<bool> nodeNearAxis(nnode, false);
+        for (auto &q : quads)
+            if (q.axial)
+                for (int k = 0; k < 4; k++) nodeNearAxis[q.nodes[k]] = true;
+        for (auto &q : quads) {
+            for (int k = 0; k < 4; k++) q.vicinal[k] = -1;
+            if (q.axial) continue;
+            for (int k = 0; k < 4; k++)
+                if (nodeNearAxis[q.nodes[k]]) q.vicinal[k] = k;
+        }
+        pointElems.assign(npoint, {});
+        for (size_t e = 0; e < quads.size(); e++)
+            for (int k = 0; k < nP * nP; k++) pointElems[quads[e].gll[k]].push_back({(int)e, k});
+    }
+
+    void buildGeometry() {
+        parallelFor((int)quads.size(), nthread, [&](int b, int e) {
+            for (int iq = b; iq < e; iq++) {
+                QuadInfo &q = quads[iq];
+                for (int ipol = 0; ipol < nP; ipol++) {
+                    for (int jpol = 0; jpol < nP; jpol++) {
+                        int k = ipol * nP + jpol;
+                        double xi = q.xi[ipol], eta = q.eta[jpol];
+                        sphMapping(q.g, xi, eta, q.crd[k][0], q.crd[k][1]);
+                        if (q.axial && ipol == 0) q.crd[k][0] = 0.;
+                        sphJacobian(q.g, xi, eta, q.J[k]);
+                        double detJ = det2(q.J[k]);
+                        double wxw = q.wxi[ipol] * q.weta[jpol];
+                        double s = q.crd[k][0];
+                        // Quad::formIntegralFactor (Quad.cpp:527-547)
+                        if (q.axial) {
+                            if (ipol == 0) {
+                                q.iFact[k] = wxw * q.J[k][0][0] * detJ;
+                            } else {
+                                q.iFact[k] = wxw * s / (1. + xi) * detJ;
+                            }
+                        } else {
+                            q.iFact[k] = wxw * s * detJ;
+                        }
+                    }
+                }
+            }
+        });
+    }
+
+    // Nr of each point: Quad::formNrField (Quad.cpp:549-616)
+    std::vector<int> buildNrField() {
+        std::mt19937_64 rng(c.seed * 7919ULL + 17ULL);
+        elemNu.assign(quads.size(), c.nu_const);
+        if (c.nu_mode == 1) {
+            std::uniform_int_distribution<int> ud(c.nu_min, c.nu_max);
+            for (auto &v : elemNu) v = ud(rng);
+        } else if (c.nu_mode == 2) {
+            std::lognormal_distribution<double> ln(std::log(c.nu_median), c.nu_sigma);
+            for (auto &v : elemNu) {
+                int nu = (int)std::lround(ln(rng));
+                v = std::max(c.nu_min, std::min(c.nu_max, nu));
+            }
+        }
+        // Nu of a point = max over the elements sharing it; Nr = 2 Nu + 1 (ConstNrField.cpp:13-16)
+        std::vector<int> baseNr(npoint, 0);
+        for (int p = 0; p < npoint; p++) {
+            int nu = 0;
+            for (auto &pe : pointElems[p]) nu = std::max(nu, elemNu[pe.first]);
+            baseNr[p] = 2 * nu + 1;
+        }
+        std::vector<int> pointNr(npoint, -1);
+        for (auto &q : quads) {
+            for (int ipol = 0; ipol < nP; ipol++) {
+                for (int jpol = 0; jpol < nP; jpol++) {
+                    int k = ipol * nP + jpol;
+                    int nr = baseNr[q.gll[k]];
+                    // upper limit from the average GLL spacing
+                    double xi = q.xi[ipol], eta = q.eta[jpol];
+                    double shp[4] = {(1. - xi) * (1. - eta) / 4., (1. + xi) * (1. - eta) / 4.,
+                                     (1. + xi) * (1. + eta) / 4., (1. - xi) * (1. + eta) / 4.};
+                    double spacing = 0.;
+                    for (int n = 0; n < 4; n++) spacing += shp[n] * nodeSpacing[q.nodes[n]];
+                    double circ = 2. * pi * q.crd[k][0];
+                    int upper = (int)(circ / spacing);
+                    if (upper < 3) upper = 3;
+                    nr = std::min(nr, upper);
+                    bool forceOdd = false;
+                    if (nr % 2 == 0) {
+                        if (q.axial) {
+                            nr++;
+                            forceOdd = true;
+                        } else if (*std::max_element(q.vicinal, q.vicinal + 4) >= 0) {
+                            for (int i = 0; i < 4; i++) {
+                                int n0 = q.vicinal[i];
+                                int n1 = q.vicinal[period0123(i + 1)];
+                                if (n0 >= 0 && n1 >= 0) {
+                                    bool onSide = (n0 == 0 && jpol == 0) || (n0 == 1 && ipol == nPol) ||
+                                                  (n0 == 2 && jpol == nPol) || (n0 == 3 && ipol == 0);
+                                    if (onSide) {
+                                        nr++;
+                                        forceOdd = true;
+                                        break;
+                                    }
+                                } else if (n0 >= 0 && n1 < 0) {
+                                    bool onPoint = (n0 == 0 && ipol == 0 && jpol == 0) ||
+                                                   (n0 == 1 && ipol == nPol && jpol == 0) ||
+                                                   (n0 == 2 && ipol == nPol && jpol == nPol) ||
+                                                   (n0 == 3 && ipol == 0 && jpol == nPol);
+                                    if (onPoint) {
+                                        nr++;
+                                        forceOdd = true;
+                                        break;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    if (c.lucky) nr = nextLuckyNumber(nr, forceOdd);
+                    q.pointNr[k] = nr;
+                    int &stored = pointNr[q.gll[k]];
+                    // GLLPoint::setup conflict check (GLLPoint.cpp:25-33)
+                    if (stored < 0) {
+                        stored = nr;
+                    } else if (stored != nr) {
+                        throw std::runtime_error("GLLPoint::setup || Conflict in GLL point setup.");
+                    }
+                }
+            }
+            q.nr = *std::max_element(q.pointNr, q.pointNr + nP * nP);
+        }
+        return pointNr;
+    }
+};
+
+}  // namespace
+
+// ===========================================================================
+static void buildShell(const axg_shell_config &c, axg_mesh &m) {
+    Builder B(c);
+    B.nthread = c.nthread > 0 ? c.nthread : (int)std::max(1u, std::thread::hardware_concurrency());
+    B.buildTopology();
+    B.buildGeometry();
+    // coordinates of the points, with the GLLPoint::setup distance check
+    std::vector<double> psz(2 * (size_t)B.npoint, 0.);
+    std::vector<char> pset(B.npoint, 0);
+    double distTol = 1e-6 * B.rnode.back();
+    for (auto &q : B.quads) {
+        for (int k = 0; k < nP * nP; k++) {
+            int p = q.gll[k];
+            if (!pset[p]) {
+                pset[p] = 1;
+                psz[2 * p] = q.crd[k][0];
+                psz[2 * p + 1] = q.crd[k][1];
+            } else if (std::hypot(psz[2 * p] - q.crd[k][0], psz[2 * p + 1] - q.crd[k][1]) > distTol) {
+                throw std::runtime_error("GLLPoint::setup || Conflict in GLL point setup.");
+            }
+        }
+    }
+    std::vector<int> pointNr = B.buildNrField();
+    const int NQ = (int)B.quads.size();
+
+    // ---------------- material sampling ----------------
+    const bool threeD = c.dv_amp > 0.;
+    if (threeD) {
+        B.grfVs = GRF(c.seed * 31ULL + 1ULL, c.dv_corr_len);
+        B.grfVp = GRF(c.seed * 31ULL + 2ULL, c.dv_corr_len);
+        B.grfAniso = GRF(c.seed * 31ULL + 3ULL, c.dv_corr_len);
+    }
+    const bool att = c.attenuation != AXG_ATT_NONE;
+    std::vector<double> slsW, slsY;
+    if (att) fitSLS(c.nsls, c.f_min, c.f_max, slsW, slsY);
+    // per-element random Q and anisotropy seeds
+    std::vector<double> elemQmu(NQ, 0.);
+    std::vector<std::array<double, 21>> elemAnisoDir(NQ);
+    {
+        std::mt19937_64 rng(c.seed * 104729ULL + 5ULL);
+        std::uniform_real_distribution<double> uq(c.q_min, c.q_max);
+        std::uniform_real_distribution<double> u11(-1., 1.);
+        for (int e = 0; e < NQ; e++) {
+            const auto &L = B.layers[B.quads[e].layer].p;
+            elemQmu[e] = L.q_mu > 0. ? L.q_mu : uq(rng);
+            for (auto &v : elemAnisoDir[e]) v = u11(rng);
+        }
+    }
+
+    // pass 1: dt needs vmax over slices; pass 2 builds the pool (needs dt for attenuation)
+    struct ElemOut {
+        int nr = 1;
+        bool is3D = false, tiso = false;
+        int material = 0;
+        std::vector<float> mat;    // pool block
+        std::vector<float> attv;   // attenuation block
+        double mass[nP * nP] = {0.};  // rho * iFact (solid) or iFact / (rho vp^2) (fluid)
+        double dtElem = DBL_MAX;
+        double rhoPt[nP * nP] = {0.};
+    };
+    std::vector<ElemOut> out(NQ);
+
+    // slice values of one element: vpv, vph, vsv, vsh, rho, eta at [row][ipnt]
+    auto sampleElem = [&](int e, std::vector<std::array<double, 6>> &v, int nrows) {
+        const QuadInfo &q = B.quads[e];
+        const Layer &L = B.layers[q.layer];
+        v.resize((size_t)nrows * nP * nP);
+        for (int k = 0; k < nP * nP; k++) {
+            double s = q.crd[k][0], z = q.crd[k][1];
+            double r = std::hypot(s, z);
+            double vp = L.vp(r), vs = L.vs(r), rho = L.rho(r);
+            for (int row = 0; row < nrows; row++) {
+                double dvs = 0., dvp = 0.;
+                if (threeD && (!q.fluid || c.dv_fluid)) {
+                    double phi = 2. * pi * row / nrows;
+                    double x = s * std::cos(phi), y = s * std::sin(phi);
+                    dvs = c.dv_amp * std::tanh(B.grfVs(x, y, z) / 2.);
+                    dvp = c.dv_amp * std::tanh(B.grfVp(x, y, z) / 2.);
+                }
+                double vpr = vp * (1. + dvp), vsr = q.fluid ? 0. : vs * (1. + dvs);
+                std::array<double, 6> a;
+                bool ti = (!q.fluid) && c.material != AXG_MAT_ISOTROPIC;
+                a[0] = ti ? vpr * std::sqrt(c.tiso_phi) : vpr;  // vpv
+                a[1] = vpr;                                     // vph
+                a[2] = vsr;                                     // vsv
+                a[3] = ti ? vsr * std::sqrt(c.tiso_xi) : vsr;   // vsh
+                a[4] = rho;
+                a[5] = ti ? c.tiso_eta : 1.;
+                v[(size_t)row * nP * nP + k] = a;
+            }
+        }
+    };
+    // XMath::equalRows over all parameters (Material::isSolidPar1D / isFluidPar1D)
+    auto rowsEqual = [&](const std::vector<std::array<double, 6>> &v, int nrows) {
+        for (int p = 0; p < 6; p++) {
+            double n0 = 0.;
+            for (int k = 0; k < nP * nP; k++) n0 += v[k][p] * v[k][p];
+            n0 = std::sqrt(n0);
+            for (int row = 1; row < nrows; row++) {
+                double d = 0.;
+                for (int k = 0; k < nP * nP; k++) {
+                    double x = v[(size_t)row * nP * nP + k][p] - v[k][p];
+                    d += x * x;
+                }
+                if (std::sqrt(d) > tinyDouble * n0) return false;
+            }
+        }
+        return true;
+    };
+
+    parallelFor(NQ, B.nthread, [&](int b, int eEnd) {
+        std::vector<std::array<double, 6>> v;
+        for (int e = b; e < eEnd; e++) {
+            const QuadInfo &q = B.quads[e];
+            ElemOut &o = out[e];
+            sampleElem(e, v, q.nr);
+            o.is3D = !rowsEqual(v, q.nr);
+            // Quad::getDeltaT with courant given (Quad.cpp:292-307)
+            double hmin = DBL_MAX;
+            for (int i = 0; i < nP * nP; i++)
+                for (int j = i + 1; j < nP * nP; j++)
+                    hmin = std::min(hmin, std::hypot(q.crd[i][0] - q.crd[j][0], q.crd[i][1] - q.crd[j][1]));
+            for (int row = 0; row < q.nr; row++) {
+                double vmax = 0.;
+                for (int k = 0; k < nP * nP; k++)
+                    vmax = std::max(vmax, std::max(v[(size_t)row * nP * nP + k][0], v[(size_t)row * nP * nP + k][1]));
+                o.dtElem = std::min(o.dtElem, c.courant * hmin / vmax);
+            }
+            // Material::computeElementalMass (rho and fluid vp taken from the first slice:
+            // the synthetic models never perturb rho, and fluid vp only with dv_fluid)
+            for (int k = 0; k < nP * nP; k++) {
+                double rho = v[k][4];
+                o.rhoPt[k] = rho;
+                o.mass[k] = q.fluid ? q.iFact[k] / (rho * v[k][0] * v[k][0]) : rho * q.iFact[k];
+            }
+        }
+    });
+    double dt = DBL_MAX;
+    for (auto &o : out) dt = std::min(dt, o.dtElem);
+    m.dt = dt;
+
+    AttFactors af;
+    if (att) af = makeAttFactors(slsW, slsY, c.f_min, c.f_max, c.f_ref, dt);
+
+    parallelFor(NQ, B.nthread, [&](int b, int eEnd) {
+        std::vector<std::array<double, 6>> v;
+        for (int e = b; e < eEnd; e++) {
+            const QuadInfo &q = B.quads[e];
+            ElemOut &o = out[e];
+            int nrows = o.is3D ? q.nr : 1;
+            o.nr = q.nr;
+            sampleElem(e, v, nrows);
+            const int NPE = nP * nP;
+            if (q.fluid) {
+                // Material::createAcoustic (Material.cpp:257-278): K = iFact / rho
+                o.material = AXG_MAT_ACOUSTIC;
+                o.mat.resize((size_t)nrows * NPE);
+                for (int row = 0; row < nrows; row++)
+                    for (int k = 0; k < NPE; k++)
+                        o.mat[(size_t)row * NPE + k] = (float)(q.iFact[k] / v[(size_t)row * NPE + k][4]);
+                continue;
+            }
+            o.material = c.material;
+            o.tiso = c.material != AXG_MAT_ISOTROPIC;
+            // A C F L N scaled by the integral factor (Material.cpp:313-330)
+            size_t n = (size_t)nrows * NPE;
+            std::vector<double> A(n), C(n), F(n), L(n), N(n), kp(n), mu(n);
+            for (int row = 0; row < nrows; row++) {
+                for (int k = 0; k < NPE; k++) {
+                    size_t i = (size_t)row * NPE + k;
+                    const auto &a = v[i];
+                    A[i] = a[4] * a[1] * a[1] * q.iFact[k];
+                    C[i] = a[4] * a[0] * a[0] * q.iFact[k];
+                    L[i] = a[4] * a[2] * a[2] * q.iFact[k];
+                    N[i] = a[4] * a[3] * a[3] * q.iFact[k];
+                    F[i] = a[5] * (A[i] - 2. * L[i]);
+                }
+            }
+            // full anisotropy: Voigt tensor in the (theta, phi, r) frame
+            std::vector<std::array<double, 21>> Cij;
+            if (c.material == AXG_MAT_ANISOTROPIC) {
+                Cij.resize(n);
+                for (size_t i = 0; i < n; i++) {
+                    double M[6][6] = {{0.}};
+                    M[0][0] = M[1][1] = A[i];
+                    M[2][2] = C[i];
+                    M[0][1] = M[1][0] = A[i] - 2. * N[i];
+                    M[0][2] = M[2][0] = M[1][2] = M[2][1] = F[i];
+                    M[3][3] = M[4][4] = L[i];
+                    M[5][5] = N[i];
+                    double scale = std::max(A[i], C[i]) * c.aniso_perturb;
+                    int row = (int)(i / NPE), k = (int)(i % NPE);
+                    double g = 1.;
+                    if (threeD) {
+                        double phi = 2. * pi * row / nrows;
+                        double s = q.crd[k][0], z = q.crd[k][1];
+                        g = std::tanh(B.grfAniso(s * std::cos(phi), s * std::sin(phi), z));
+                    }
+                    double P[6][6];
+                    int t = 0;
+                    for (int a = 0; a < 6; a++)
+                        for (int bb = a; bb < 6; bb++) {
+                            P[a][bb] = P[bb][a] = elemAnisoDir[e][t++] * scale * g;
+                        }
+                    double f = 1.;
+                    double Mt[6][6];
+                    for (int tries = 0; tries < 20; tries++) {
+                        for (int a = 0; a < 6; a++)
+                            for (int bb = 0; bb < 6; bb++) Mt[a][bb] = M[a][bb] + f * P[a][bb];
+                        if (isSPD6(Mt)) break;
+                        f *= 0.5;
+                    }
+                    t = 0;
+                    for (int a = 0; a < 6; a++)
+                        for (int bb = a; bb < 6; bb++) Cij[i][t++] = Mt[a][bb];
+                }
+            }
+            // attenuation: Voigt average, modulus factors (Material.cpp:336-363, 436-469)
+            std::vector<double> dkp, dmu;
+            if (att) {
+                for (size_t i = 0; i < n; i++) {
+                    if (c.material == AXG_MAT_ANISOTROPIC) {
+                        const auto &T = Cij[i];
+                        // indices in the packed upper triangle
+                        double C11 = T[0], C12 = T[1], C13 = T[2], C22 = T[6], C23 = T[7], C33 = T[11];
+                        double C44 = T[15], C55 = T[18], C66 = T[20];
+                        kp[i] = (C11 + C22 + C33 + 2. * (C12 + C23 + C13)) / 9.;
+                        mu[i] = (C11 + C22 + C33 - (C12 + C23 + C13) + 3. * (C44 + C55 + C66)) / 15.;
+                    } else {
+                        kp[i] = (4. * A[i] + C[i] + 4. * F[i] - 4. * N[i]) / 9.;
+                        mu[i] = (A[i] + C[i] - 2. * F[i] + 6. * L[i] + 5. * N[i]) / 15.;
+                    }
+                }
+                const auto &Lyr = B.layers[q.layer].p;
+                double Qmu = elemQmu[e];
+                double Qkp = Lyr.q_kappa > 0. ? Lyr.q_kappa : 57823.;
+                double dKpF, kpA, kpN, dMuF, muA, muN;
+                attModulusFactors(af, Qkp, c.do_kappa != 0, dKpF, kpA, kpN);
+                attModulusFactors(af, Qmu, true, dMuF, muA, muN);
+                std::vector<double> kpNew(kp), muNew(mu);
+                if (c.attenuation == AXG_ATT_CG4) {
+                    // Quad::computeWeightsCG4 on the integral factor (Quad.cpp:438-466)
+                    const double *f = q.iFact;
+                    auto I = [&](int a, int bb) { return f[a * nP + bb]; };
+                    double wcg[4];
+                    wcg[0] = (I(0, 0) + I(0, 1) + I(1, 0) + I(1, 1) + 0.5 * (I(0, 2) + I(1, 2) + I(2, 0) + I(2, 1)) + 0.25 * I(2, 2)) / I(1, 1);
+                    wcg[1] = (I(0, 3) + I(0, 4) + I(1, 3) + I(1, 4) + 0.5 * (I(0, 2) + I(1, 2) + I(2, 3) + I(2, 4)) + 0.25 * I(2, 2)) / I(1, 3);
+                    wcg[2] = (I(3, 0) + I(3, 1) + I(4, 0) + I(4, 1) + 0.5 * (I(2, 0) + I(2, 1) + I(3, 2) + I(4, 2)) + 0.25 * I(2, 2)) / I(3, 1);
+                    wcg[3] = (I(3, 3) + I(3, 4) + I(4, 3) + I(4, 4) + 0.5 * (I(2, 3) + I(2, 4) + I(3, 2) + I(4, 2)) + 0.25 * I(2, 2)) / I(3, 3);
+                    const int cgk[4] = {1 * nP + 1, 1 * nP + 3, 3 * nP + 1, 3 * nP + 3};
+                    dkp.resize((size_t)nrows * 4);
+                    dmu.resize((size_t)nrows * 4);
+                    for (int row = 0; row < nrows; row++)
+                        for (int i4 = 0; i4 < 4; i4++) {
+                            size_t i = (size_t)row * NPE + cgk[i4];
+                            dkp[(size_t)row * 4 + i4] = wcg[i4] * dKpF * kp[i];
+                            dmu[(size_t)row * 4 + i4] = wcg[i4] * dMuF * mu[i];
+                        }
+                    for (size_t i = 0; i < n; i++) {
+                        kpNew[i] = kp[i] * kpN;
+                        muNew[i] = mu[i] * muN;
+                    }
+                    for (int row = 0; row < nrows; row++)
+                        for (int i4 = 0; i4 < 4; i4++) {
+                            size_t i = (size_t)row * NPE + cgk[i4];
+                            kpNew[i] *= 1. + wcg[i4] * (kpA / kpN - 1.);
+                            muNew[i] *= 1. + wcg[i4] * (muA / muN - 1.);
+                        }
+                } else {
+                    dkp.resize(n);
+                    dmu.resize(n);
+                    for (size_t i = 0; i < n; i++) {
+                        dkp[i] = dKpF * kp[i];
+                        dmu[i] = dMuF * mu[i];
+                        kpNew[i] = kp[i] * kpA;
+                        muNew[i] = mu[i] * muA;
+                    }
+                }
+                // moduli: subtract old Voigt part, add the modified one
+                for (size_t i = 0; i < n; i++) {
+                    double dk = kpNew[i] - kp[i], dm = muNew[i] - mu[i];
+                    if (c.material == AXG_MAT_ANISOTROPIC) {
+                        auto &T = Cij[i];
+                        T[0] += dk + 4. / 3. * dm;
+                        T[6] += dk + 4. / 3. * dm;
+                        T[11] += dk + 4. / 3. * dm;
+                        T[1] += dk - 2. / 3. * dm;
+                        T[7] += dk - 2. / 3. * dm;
+                        T[2] += dk - 2. / 3. * dm;
+                        T[15] += dm;
+                        T[18] += dm;
+                        T[20] += dm;
+                    } else {
+                        A[i] += dk + 4. / 3. * dm;
+                        C[i] += dk + 4. / 3. * dm;
+                        F[i] += dk - 2. / 3. * dm;
+                        L[i] += dm;
+                        N[i] += dm;
+                    }
+                }
+                o.attv.resize(dkp.size() + dmu.size());
+                for (size_t i = 0; i < dkp.size(); i++) o.attv[i] = (float)dkp[i];
+                for (size_t i = 0; i < dmu.size(); i++) o.attv[dkp.size() + i] = (float)dmu[i];
+            }
+            if (c.material == AXG_MAT_ISOTROPIC) {
+                o.mat.resize(2 * n);
+                for (size_t i = 0; i < n; i++) {
+                    o.mat[i] = (float)F[i];      // lambda = F
+                    o.mat[n + i] = (float)L[i];  // mu = L
+                }
+            } else if (c.material == AXG_MAT_TRANSVERSELY_ISO) {
+                o.mat.resize(5 * n);
+                for (size_t i = 0; i < n; i++) {
+                    o.mat[i] = (float)A[i];
+                    o.mat[n + i] = (float)C[i];
+                    o.mat[2 * n + i] = (float)F[i];
+                    o.mat[3 * n + i] = (float)L[i];
+                    o.mat[4 * n + i] = (float)N[i];
+                }
+            } else {
+                o.mat.resize(21 * n);
+                for (size_t i = 0; i < n; i++)
+                    for (int t = 0; t < 21; t++) o.mat[t * n + i] = (float)Cij[i][t];
+            }
+        }
+    });
+
+    // ---------------- points ----------------
+    const int NPT = B.npoint;
+    std::vector<double> massS(NPT, 0.), massF(NPT, 0.);
+    std::vector<int> pflags(NPT, 0);
+    for (int e = 0; e < NQ; e++) {
+        const QuadInfo &q = B.quads[e];
+        for (int k = 0; k < nP * nP; k++) {
+            int p = q.gll[k];
+            if (q.fluid) {
+                massF[p] += out[e].mass[k];
+                pflags[p] |= AXG_PT_FLUID;
+            } else {
+                massS[p] += out[e].mass[k];
+                pflags[p] |= AXG_PT_SOLID;
+            }
+            int ipol = k / nP;
+            if (q.axial && ipol == 0) pflags[p] |= AXG_PT_AXIAL;
+        }
+    }
+    m.point_nr.assign(pointNr.begin(), pointNr.end());
+    m.point_flags.assign(pflags.begin(), pflags.end());
+    m.point_sz = psz;
+    m.mass_solid_off.assign(NPT + 1, 0);
+    m.mass_fluid_off.assign(NPT + 1, 0);
+    for (int p = 0; p < NPT; p++) {
+        // GLLPoint::release: Mass1D = 1 / mass (GLLPoint.cpp:74-79)
+        m.mass_solid_off[p + 1] = m.mass_solid_off[p] + ((pflags[p] & AXG_PT_SOLID) ? 1 : 0);
+        if (pflags[p] & AXG_PT_SOLID) m.inv_mass_solid.push_back((float)(1. / massS[p]));
+        m.mass_fluid_off[p + 1] = m.mass_fluid_off[p] + ((pflags[p] & AXG_PT_FLUID) ? 1 : 0);
+        if (pflags[p] & AXG_PT_FLUID) m.inv_mass_fluid.push_back((float)(1. / massF[p]));
+    }
+
+    // ---------------- elements ----------------
+    m.elem_points.resize((size_t)NQ * 25);
+    m.elem_flags.resize(NQ);
+    m.elem_material.resize(NQ);
+    m.elem_att.resize(NQ);
+    m.mat_off.assign(NQ + 1, 0);
+    m.att_off.assign(NQ + 1, 0);
+    m.elem_geom.resize((size_t)NQ * 125);
+    m.elem_tiso.resize((size_t)NQ * 100);
+    for (int e = 0; e < NQ; e++) {
+        const QuadInfo &q = B.quads[e];
+        const ElemOut &o = out[e];
+        for (int k = 0; k < 25; k++) m.elem_points[(size_t)e * 25 + k] = q.gll[k];
+        int f = 0;
+        if (q.fluid) f |= AXG_EL_FLUID;
+        if (q.axial) f |= AXG_EL_AXIAL;
+        if (o.is3D) f |= AXG_EL_3D;
+        if (o.tiso) f |= AXG_EL_TISO;
+        m.elem_flags[e] = f;
+        m.elem_material[e] = o.material;
+        m.elem_att[e] = (q.fluid || !att) ? AXG_ATT_NONE : c.attenuation;
+        m.mat_off[e + 1] = m.mat_off[e] + (int64_t)o.mat.size();
+        m.att_off[e + 1] = m.att_off[e] + (int64_t)(m.elem_att[e] ? o.attv.size() : 0);
+        // Quad::createGraident (Quad.cpp:509-525)
+        for (int k = 0; k < 25; k++) {
+            const double(*J)[2] = q.J[k];
+            double detJ = det2(J);
+            int ipol = k / nP;
+            float *g = &m.elem_geom[(size_t)e * 125];
+            g[0 * 25 + k] = (float)(J[0][0] / detJ);   // dsdxii
+            g[1 * 25 + k] = (float)(-J[0][1] / detJ);  // dsdeta
+            g[2 * 25 + k] = (float)(-J[1][0] / detJ);  // dzdxii
+            g[3 * 25 + k] = (float)(J[1][1] / detJ);   // dzdeta
+            g[4 * 25 + k] = (float)((q.axial && ipol == 0) ? 0. : 1. / q.crd[k][0]);
+            // CrdTransTIsoSolid ctor on Element::formThetaMat (Geodesy::theta)
+            double s = q.crd[k][0], z = q.crd[k][1];
+            double r = std::hypot(s, z);
+            double th = (r < tinyDouble) ? 0. : std::acos(z / r);
+            float *t = &m.elem_tiso[(size_t)e * 100];
+            t[0 * 25 + k] = (float)std::sin(th);
+            t[1 * 25 + k] = (float)std::cos(th);
+            t[2 * 25 + k] = (float)std::sin(2. * th);
+            t[3 * 25 + k] = (float)std::cos(2. * th);
+        }
+        if (q.fluid) m.nfluid_elem++;
+        else m.nsolid_elem++;
+        if (o.is3D) m.n3d_elem++;
+        m.max_nr = std::max<int64_t>(m.max_nr, q.nr);
+    }
+    m.mat_pool.resize(m.mat_off[NQ]);
+    m.att_pool.resize(m.att_off[NQ]);
+    for (int e = 0; e < NQ; e++) {
+        std::copy(out[e].mat.begin(), out[e].mat.end(), m.mat_pool.begin() + m.mat_off[e]);
+        if (m.elem_att[e]) std::copy(out[e].attv.begin(), out[e].attv.end(), m.att_pool.begin() + m.att_off[e]);
+    }
+    if (att) {
+        for (int i = 0; i < c.nsls; i++) {
+            m.sls_alpha.push_back((float)af.alpha[i]);
+            m.sls_beta.push_back((float)af.beta[i]);
+            m.sls_gamma.push_back((float)af.gamma[i]);
+        }
+    }
+    for (int p = 0; p < NPT; p++) m.sum_nu1 += pointNr[p] / 2 + 1;
+
+    // ---------------- solid-fluid boundary ----------------
+    {
+        std::vector<std::array<double, 3>> sfn(NPT, {0., 0., 0.});
+        std::vector<char> onSF(NPT, 0);
+        for (int e = 0; e < NQ; e++) {
+            const QuadInfo &q = B.quads[e];
+            // sides facing an element of the other phase: inner side and outer side
+            for (int updown = 0; updown < 2; updown++) {
+                int irn = q.ir + (updown == 0 ? -1 : 1);
+                if (irn < 0 || irn >= B.NR) continue;
+                bool otherFluid = B.layers[B.rlayer[irn]].p.fluid != 0;
+                if (otherFluid == q.fluid) continue;
+                // inner side: north side 0 (jpol 0), south side 2 (jpol nPol)
+                int side;
+                if (updown == 0) side = q.south ? 2 : 0;
+                else side = q.south ? 0 : 2;
+                int jside = (side == 0) ? 0 : nPol;
+                // Quad::computeNormal (Quad.cpp:661-715)
+                double r0 = std::hypot(q.g.ns[side], q.g.nz[side]);
+                double t0 = std::atan2(q.g.ns[side], q.g.nz[side]);
+                int s1 = period0123(side + 1);
+                double r1 = std::hypot(q.g.ns[s1], q.g.nz[s1]);
+                double t1 = std::atan2(q.g.ns[s1], q.g.nz[s1]);
+                double rsf = .5 * (r0 + r1);
+                double half_r_dtheta = .5 * rsf * std::abs(t1 - t0);
+                double half_r2_dtheta = .5 * rsf * rsf * std::abs(t1 - t0);
+                for (int ipol = 0; ipol < nP; ipol++) {
+                    int k = ipol * nP + jside;
+                    double s = q.crd[k][0], z = q.crd[k][1];
+                    double rr = std::hypot(s, z);
+                    double sint = s / rr, cost = z / rr;
+                    double nvec[3] = {sint, 0., cost};
+                    double wsf = q.wxi[ipol];  // side 0 / 2: xi weight
+                    double fac;
+                    if (q.axial) {
+                        if (ipol == 0) fac = wsf * q.J[k][0][0] * half_r_dtheta;
+                        else fac = wsf / (1. + q.xi[ipol]) * sint * half_r2_dtheta;
+                    } else {
+                        fac = wsf * sint * half_r2_dtheta;
+                    }
+                    if (side != q.g.curvedOuter) fac = -fac;
+                    if (!q.fluid) fac = -fac;  // inverse in solid domain
+                    int p = q.gll[k];
+                    for (int d = 0; d < 3; d++) sfn[p][d] += nvec[d] * fac * .5;
+                    onSF[p] = 1;
+                }
+            }
+        }
+        m.sf_off.push_back(0);
+        for (int p = 0; p < NPT; p++) {
+            if (!onSF[p] || (pflags[p] & 3) != 3) continue;
+            m.sf_point.push_back(p);
+            double mf = massF[p];
+            // SFCoupling1D: unassembled normal, then normal / fluid mass (GLLPoint.cpp:100-111)
+            m.sf_normal.push_back((float)sfn[p][0]);
+            m.sf_normal.push_back((float)sfn[p][1]);
+            m.sf_normal.push_back((float)sfn[p][2]);
+            m.sf_normal.push_back((float)(sfn[p][0] / mf));
+            m.sf_normal.push_back((float)(sfn[p][1] / mf));
+            m.sf_normal.push_back((float)(sfn[p][2] / mf));
+            m.sf_off.push_back(m.sf_off.back() + 1);
+        }
+    }
+
+    // ---------------- source ----------------
+    const double Rtop = B.rnode.back();
+    m.src_force.assign(25 * 3 * 3 * 2, 0.f);
+    m.desc.src_elem = -1;
+    int srcElem = -1;
+    {
+        double M[6];
+        bool given = false;
+        for (int i = 0; i < 6; i++) {
+            M[i] = c.moment[i];
+            if (M[i] != 0.) given = true;
+        }
+        if (!given) {
+            std::mt19937_64 rng(c.seed * 1299709ULL + 11ULL);
+            std::normal_distribution<double> nd(0., 1.);
+            for (int i = 0; i < 6; i++) M[i] = nd(rng);
+            // unit Frobenius norm of the symmetric tensor
+            double f = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + 2. * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
+            for (int i = 0; i < 6; i++) M[i] *= 1e20 / std::sqrt(f);
+        }
+        // Earthquake ctor: Mxx=Mtt, Myy=Mpp, Mzz=Mrr, Mxy=Mtp, Mxz=Mrt, Myz=Mrp
+        double Mrr = M[0], Mtt = M[1], Mpp = M[2], Mrt = M[3], Mrp = M[4], Mtp = M[5];
+        double Mxx = Mtt, Myy = Mpp, Mzz = Mrr, Mxy = Mtp, Mxz = Mrt, Myz = Mrp;
+        // Source::locate (Source.cpp:61-94)
+        double zs = Rtop - c.src_depth;
+        double etaSrc = 0.;
+        for (int e = 0; e < NQ && srcElem < 0; e++) {
+            const QuadInfo &q = B.quads[e];
+            if (!q.axial || q.fluid) continue;
+            double smin = *std::min_element(q.g.ns, q.g.ns + 4), smax = *std::max_element(q.g.ns, q.g.ns + 4);
+            double zmin = *std::min_element(q.g.nz, q.g.nz + 4), zmax = *std::max_element(q.g.nz, q.g.nz + 4);
+            if (0. > smax + tinySingle || 0. < smin - tinySingle || zs > zmax + tinySingle || zs < zmin - tinySingle) continue;
+            double xi, eta;
+            if (invMapping(q.g, 0., zs, xi, eta) && std::abs(eta) <= 1.000001) {
+                if (std::abs(xi + 1.) > tinySingle) throw std::runtime_error("Source::locate || Bad source location.");
+                srcElem = e;
+                etaSrc = eta;
+            }
+        }
+        if (srcElem < 0) throw std::runtime_error("Source::release || Error locating source.");
+        const QuadInfo &q = B.quads[srcElem];
+        std::array<double, nP> interpZ;
+        spectral::interpLagrange(etaSrc, spectral::gll().x, interpZ);
+        // Earthquake::computeSourceFourier (Earthquake.cpp:20-88), no relabelling
+        double axJ[nP][2][2];
+        for (int j = 0; j < nP; j++) {
+            sphJacobian(q.g, spectral::glj().x[0], spectral::gll().x[j], axJ[j]);
+            double d = det2(axJ[j]);
+            for (int a = 0; a < 2; a++)
+                for (int bb = 0; bb < 2; bb++) axJ[j][a][bb] /= d;
+        }
+        const auto &GJ = spectral::glj().G;
+        const auto &GL = spectral::gll().G;
+        typedef std::array<std::array<std::array<double, 2>, 3>, 3> F33;  // [alpha][comp][re/im]
+        for (int ipol = 0; ipol < nP; ipol++) {
+            for (int jpol = 0; jpol < nP; jpol++) {
+                int ipnt = ipol * nP + jpol;
+                F33 f{};
+                for (int js = 0; js < nP; js++) {
+                    double fact = interpZ[js];
+                    const double(*J)[2] = axJ[js];
+                    // GU = G_GLJ^T w, UG = w G_GLL evaluated at (0, js)
+                    double GU = (js == jpol) ? GJ[ipol][0] : 0.;
+                    double UG = (ipol == 0) ? GL[jpol][js] : 0.;
+                    double dwds = J[1][1] * GU - J[1][0] * UG;
+                    double dwdz = J[0][0] * UG - J[0][1] * GU;
+                    // monopole
+                    f[0][0][0] += fact * dwds * (Mxx + Myy) / (2. * pi);
+                    f[0][2][0] += fact * dwdz * Mzz / (2. * pi);
+                    // dipole: (Mxz - i Myz)
+                    double dre = fact * dwdz / (4. * pi);
+                    f[1][0][0] += dre * Mxz;
+                    f[1][0][1] += dre * (-Myz);
+                    // * i
+                    f[1][1][0] += dre * Myz;
+                    f[1][1][1] += dre * Mxz;
+                    f[1][2][0] += fact * dwds * Mxz / (2. * pi);
+                    f[1][2][1] += fact * dwds * (-Myz) / (2. * pi);
+                    // quadrupole: ((Mxx - Myy)/2 - i Mxy)
+                    double qre = fact * dwds / (2. * pi);
+                    f[2][0][0] += qre * (Mxx - Myy) / 2.;
+                    f[2][0][1] += qre * (-Mxy);
+                    f[2][1][0] += qre * Mxy;
+                    f[2][1][1] += qre * (Mxx - Myy) / 2.;
+                }
+                // SourceTerm: rows beyond the point's Nu are dropped (SourceTerm.cpp:16-24)
+                int nu = pointNr[q.gll[ipnt]] / 2;
+                for (int a = 0; a < 3; a++)
+                    for (int cc = 0; cc < 3; cc++)
+                        for (int ri = 0; ri < 2; ri++)
+                            m.src_force[((ipnt * 3 + a) * 3 + cc) * 2 + ri] = (a <= nu) ? (float)f[a][cc][ri] : 0.f;
+            }
+        }
+    }
+
+    // ---------------- STF (GaussSTF.cpp:10-21), truncated to nstep ----------------
+    {
+        double hdur = std::max(c.period / 2., 5. * dt);  // STF.cpp:25-28
+        double decay = 1.628;
+        int nBefore = (int)std::ceil(2.5 * hdur / dt);
+        double shift = nBefore * dt;
+        m.t_shift = shift;
+        m.stf.resize(c.nstep);
+        for (int i = 0; i < c.nstep; i++) {
+            double t = -shift + i * dt;
+            m.stf[i] = (float)(std::exp(-std::pow(decay / hdur * t, 2.)) * decay / (hdur * std::sqrt(pi)));
+        }
+    }
+
+    // ---------------- receivers ----------------
+    {
+        std::mt19937_64 rng(c.seed * 15485863ULL + 3ULL);
+        std::uniform_real_distribution<double> ut(std::cos(pi * 179. / 180.), std::cos(pi * 1. / 180.));
+        std::uniform_real_distribution<double> up(0., 2. * pi);
+        double rr = Rtop - c.rec_depth;
+        for (int irec = 0; irec < c.nrec; irec++) {
+            double theta = std::acos(ut(rng));  // uniform on the sphere
+            double phi = up(rng);
+            double s = rr * std::sin(theta), z = rr * std::cos(theta);
+            int found = -1;
+            double xi = 0., eta = 0.;
+            for (int e = 0; e < NQ && found < 0; e++) {
+                const QuadInfo &q = B.quads[e];
+                double smin = *std::min_element(q.g.ns, q.g.ns + 4), smax = *std::max_element(q.g.ns, q.g.ns + 4);
+                double zmin = *std::min_element(q.g.nz, q.g.nz + 4), zmax = *std::max_element(q.g.nz, q.g.nz + 4);
+                if (s > smax + tinySingle || s < smin - tinySingle || z > zmax + tinySingle || z < zmin - tinySingle) continue;
+                if (invMapping(q.g, s, z, xi, eta) && std::abs(xi) <= 1.000001 && std::abs(eta) <= 1.000001) found = e;
+            }
+            if (found < 0) throw std::runtime_error("ReceiverCollection::release || Error locating receiver.");
+            const QuadInfo &q = B.quads[found];
+            std::array<double, nP> ix, ie;
+            spectral::interpLagrange(xi, q.axial ? spectral::glj().x : spectral::gll().x, ix);
+            spectral::interpLagrange(eta, spectral::gll().x, ie);
+            m.rec_elem.push_back(found);
+            for (int ipol = 0; ipol < nP; ipol++)
+                for (int jpol = 0; jpol < nP; jpol++) {
+                    double w = ix[ipol] * ie[jpol];
+                    if (q.fluid) w /= q.iFact[ipol * nP + jpol];
+                    m.rec_weights.push_back((float)w);
+                }
+            m.rec_phi.push_back((float)phi);
+            m.rec_theta.push_back((float)theta);
+        }
+    }
+    m.bind();
+    m.desc.src_elem = srcElem;
+    m.desc.att_do_kappa = c.do_kappa;
+    m.desc.rec_components = c.rec_components;
+}
+
+// ===========================================================================
+namespace {
+std::vector<axg_shell_layer> &defaultLayers(int which) {
+    static std::vector<axg_shell_layer> L[4];
+    auto &v = L[which];
+    if (!v.empty()) return v;
+    auto lay = [&](double rb, double rt, int n, int fluid, double vpb, double vpt, double vsb, double vst,
+                   double rhob, double rhot) {
+        axg_shell_layer l;
+        l.r_bottom = rb * 1e3;
+        l.r_top = rt * 1e3;
+        l.nelem_radial = n;
+        l.fluid = fluid;
+        l.vp_bottom = vpb * 1e3;
+        l.vp_top = vpt * 1e3;
+        l.vs_bottom = vsb * 1e3;
+        l.vs_top = vst * 1e3;
+        l.rho_bottom = rhob * 1e3;
+        l.rho_top = rhot * 1e3;
+        l.q_mu = 0.;
+        l.q_kappa = 57823.;
+        v.push_back(l);
+    };
+    if (which == 0 || which == 2) {
+        // 3 mantle layers, vp 5.8-11, vs 3.2-6.2, rho 2.6-5.5 km/s, g/cc
+        int s = (which == 0) ? 1 : 6;  // 40 or 240 radial elements
+        lay(3480., 5701., 28 * s, 0, 11.0, 10.3, 6.2, 5.6, 5.5, 4.4);
+        lay(5701., 5971., 4 * s, 0, 10.2, 9.1, 5.5, 4.9, 4.0, 3.7);
+        lay(5971., 6371., 8 * s, 0, 8.9, 5.8, 4.8, 3.2, 3.5, 2.6);
+        if (which == 2) v.back().nelem_radial += 10;  // 250 radial elements
+    } else {
+        // thin solid inner-core shell, fluid outer core, 3 mantle layers: 100 radial elements
+        int k = (which == 1) ? 1 : 1;
+        lay(1000., 1221.5, 2 * k, 0, 11.3, 11.0, 3.7, 3.5, 13.1, 12.8);
+        lay(1221.5, 3480., 20 * k, 1, 10.3, 8.0, 0., 0., 12.2, 9.9);
+        lay(3480., 5701., 54 * k, 0, 11.0, 10.3, 6.2, 5.6, 5.5, 4.4);
+        lay(5701., 5971., 8 * k, 0, 10.2, 9.1, 5.5, 4.9, 4.0, 3.7);
+        lay(5971., 6371., 16 * k, 0, 8.9, 5.8, 4.8, 3.2, 3.5, 2.6);
+    }
+    return v;
+}
+}  // namespace
+
+extern "C" int axg_shell_config_default(int which, axg_shell_config *cfg) {
+    if (!cfg || which < 0 || which > 3) {
+        axg::set_last_error("axg_shell_config_default || which must be 0..3.");
+        return AXG_EINVAL;
+    }
+    axg_shell_config c;
+    std::memset(&c, 0, sizeof(c));
+    auto &L = defaultLayers(which);
+    c.nlayer = (int32_t)L.size();
+    c.layers = L.data();
+    c.material = AXG_MAT_ISOTROPIC;
+    c.tiso_xi = 1.05;
+    c.tiso_phi = 0.97;
+    c.tiso_eta = 0.95;
+    c.aniso_perturb = 0.03;
+    c.dv_amp = 0.;
+    c.dv_corr_len = 500e3;
+    c.nu_mode = 0;
+    c.nu_const = 4;
+    c.nu_min = 8;
+    c.nu_max = 128;
+    c.nu_median = 32.;
+    c.nu_sigma = 0.6;
+    c.lucky = 1;
+    c.attenuation = AXG_ATT_NONE;
+    c.nsls = 5;
+    c.q_min = 80.;
+    c.q_max = 600.;
+    c.f_min = 0.001;
+    c.f_max = 1.;
+    c.f_ref = 1.;
+    c.do_kappa = 1;
+    c.courant = 0.6;
+    c.period = 50.;
+    c.src_depth = 100e3;
+    c.nrec = 20;
+    c.rec_depth = 0.;
+    c.rec_components = 0;
+    c.seed = 20161;
+    c.nthread = 0;
+    switch (which) {
+        case 0:  // plain, small: 4000 solid quads, Nu = 4, isotropic 1-D, 2000 steps
+            c.nelem_theta = 100;
+            c.nstep = 2000;
+            break;
+        case 1:  // 3-D solid + fluid: 20000 quads (4000 fluid), TI, GRF +-5 %, Nu ~ U[8,128], 5000 steps
+            c.nelem_theta = 200;
+            c.material = AXG_MAT_TRANSVERSELY_ISO;
+            c.dv_amp = 0.05;
+            c.nu_mode = 1;
+            c.nstep = 5000;
+            break;
+        case 2:  // anisotropic attenuating: 100000 quads, 21 Cij, 5 SLS, Nu log-normal (median 32, cap 400)
+            c.nelem_theta = 400;
+            c.material = AXG_MAT_ANISOTROPIC;
+            c.dv_amp = 0.05;
+            c.nu_mode = 2;
+            c.nu_min = 1;
+            c.nu_max = 400;
+            c.attenuation = AXG_ATT_FULL;
+            c.nstep = 3000;
+            break;
+        case 3:  // one member of the batched set: 2000 quads
+            c.nelem_theta = 50;
+            c.material = AXG_MAT_TRANSVERSELY_ISO;
+            c.dv_amp = 0.05;
+            c.nu_mode = 1;
+            c.nu_min = 4;
+            c.nu_max = 200;
+            c.nstep = 2000;
+            break;
+    }
+    *cfg = c;
+    return AXG_OK;
+}
+
+extern "C" int axg_mesh_build_shell(const axg_shell_config *cfg, axg_mesh **mesh) {
+    if (!cfg || !mesh) {
+        axg::set_last_error("axg_mesh_build_shell || null argument.");
+        return AXG_EINVAL;
+    }
+    *mesh = nullptr;
+    try {
+        std::unique_ptr<axg_mesh> m(new axg_mesh());
+        buildShell(*cfg, *m);
+        *mesh = m.release();
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
+}
+
+extern "C" const axg_mesh_desc *axg_mesh_get_desc(const axg_mesh *mesh) { return mesh ? &mesh->desc : nullptr; }
+
+extern "C" int axg_mesh_stats(const axg_mesh *mesh, int64_t *stats, int nstats) {
+    if (!mesh || !stats) return AXG_EINVAL;
+    int64_t v[8] = {mesh->desc.npoint, mesh->desc.nelem, mesh->nsolid_elem, mesh->nfluid_elem,
+                    mesh->n3d_elem, mesh->max_nr, mesh->sum_nu1, (int64_t)mesh->sf_point.size()};
+    for (int i = 0; i < nstats && i < 8; i++) stats[i] = v[i];
+    return AXG_OK;
+}
+
+extern "C" void axg_mesh_free(axg_mesh *mesh) { delete mesh; }

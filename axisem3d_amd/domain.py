@@ -1,0 +1,158 @@
+"""Python mirror of the reference's Domain / Newmark stepping interface over
+the C ABI (include/axg.h).
+
+Method names, argument meaning and error behaviour follow
+SOLVER/src/core/domain/Domain.h:61-88 and SOLVER/src/core/newmark/Newmark.h:9-22:
+errors surface as ``RuntimeError("Class::method || message")``, the equivalent
+of the reference's ``std::runtime_error``.  Every call lands in HIP kernels on
+the GPU; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+def _check(rc):
+    from . import lib
+    if rc != 0:
+        raise RuntimeError(lib().axg_last_error().decode())
+
+
+class Domain:
+    """The whole mesh resident on one GPU (replaces Mesh::release -> Domain)."""
+
+    def __init__(self, mesh, device=0):
+        from . import lib
+        self._lib = lib()
+        self.mesh = mesh  # keeps the descriptor alive
+        h = C.c_void_p()
+        _check(self._lib.axg_create(mesh.desc, device, C.byref(h)))
+        self._h = h
+        d = mesh.d
+        self.nstep, self.nrec, self.dt, self.t_shift = d.nstep, d.nrec, d.dt, d.t_shift
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.axg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- the reference's per-step operators ----
+    def resetZero(self):
+        _check(self._lib.axg_reset_zero(self._h))
+
+    def updateNewmark(self, dt):
+        _check(self._lib.axg_update_newmark(self._h, dt))
+
+    def applySource(self, tstep):
+        _check(self._lib.axg_apply_source(self._h, tstep))
+
+    def computeStiff(self):
+        _check(self._lib.axg_compute_stiff(self._h))
+
+    def coupleSolidFluid(self):
+        _check(self._lib.axg_couple_solid_fluid(self._h))
+
+    def assembleStiff(self, phase=0):
+        _check(self._lib.axg_assemble_stiff(self._h, phase))
+
+    def record(self, tstep, t):
+        _check(self._lib.axg_record(self._h, tstep, t))
+
+    def checkStability(self, dt, tstep, t):
+        bad = C.c_int64(-1)
+        rc = self._lib.axg_check_stability(self._h, C.byref(bad))
+        if rc == _abi.E_UNSTABLE:
+            sz = self.mesh.array("point_sz").reshape(-1, 2)[bad.value] / 1e3
+            raise RuntimeError(
+                "Domain::checkStability || Simulation blew up. point %d, (s, z) / km = (%g, %g), "
+                "step %d, t = %g, DT = %g" % (bad.value, sz[0], sz[1], tstep, t, dt))
+        _check(rc)
+
+    def learnWisdom(self, cutoff):
+        _check(self._lib.axg_learn_wisdom(self._h, cutoff))
+
+    # ---- batched loop and data exchange ----
+    def run(self, first, count, stab_interval=0, use_graph=False):
+        """Newmark.cpp:47-93 for tstep = first .. first+count-1 on the device."""
+        _check(self._lib.axg_run(self._h, first, count, stab_interval, int(bool(use_graph))))
+
+    def run_profiled(self, first, count):
+        """Run with HIP-event timing of the kernel classes on the domain's stream."""
+        _check(self._lib.axg_run(self._h, first, count, 0, 2))
+        total = C.c_double(0.)
+        cls = (C.c_double * 3)()
+        _check(self._lib.axg_last_run_timing(self._h, C.byref(total), cls, 3))
+        return {"steps": count, "total_ms": total.value, "element_ms": cls[0], "newmark_ms": cls[1],
+                "other_ms": cls[2]}
+
+    def kernel_bytes(self):
+        """Algorithmic HBM bytes per step of the element-stiffness and Newmark kernel classes."""
+        b = (C.c_double * 3)()
+        _check(self._lib.axg_class_bytes(self._h, b, 3))
+        return b[0], b[1]
+
+    def last_run_ms(self):
+        ms = C.c_double(0.)
+        _check(self._lib.axg_last_run_timing(self._h, C.byref(ms), None, 0))
+        return ms.value
+
+    def step_cost(self):
+        b, u = C.c_double(0.), C.c_double(0.)
+        _check(self._lib.axg_step_cost(self._h, C.byref(b), C.byref(u)))
+        return b.value, u.value
+
+    def seismograms(self):
+        n = self.nstep * self.nrec * 3
+        out = np.zeros(max(n, 1), dtype=np.float32)
+        _check(self._lib.axg_get_seismograms(self._h, out.ctypes.data_as(_abi.f32p), out.size))
+        return out[:n].reshape(self.nstep, self.nrec, 3)
+
+    def get_field(self, which):
+        n = self.mesh.field_size(which)
+        out = np.zeros(n, dtype=np.float32)
+        _check(self._lib.axg_get_field(self._h, which, out.ctypes.data_as(_abi.f32p), n))
+        return out
+
+    def set_field(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        _check(self._lib.axg_set_field(self._h, which, a.ctypes.data_as(_abi.f32p), a.size))
+
+
+class Newmark:
+    """Newmark(domain, reportInterval, checkStabInterval).solve(verbose)
+    (SOLVER/src/core/newmark/Newmark.cpp:12-107)."""
+
+    def __init__(self, domain, reportInterval=0, checkStabInterval=0, randomDispl=False):
+        self.domain = domain
+        self.report = reportInterval if reportInterval > 0 else 100
+        self.stab = checkStabInterval if checkStabInterval > 0 else self.report
+        if randomDispl:
+            raise RuntimeError("Newmark::Newmark || tiny random initial displacement is not supported.")
+
+    def solve(self, verbose=0, per_operator=False):
+        d = self.domain
+        t = -d.t_shift
+        dt = d.dt
+        d.resetZero()
+        if not per_operator:
+            d.run(1, d.nstep, self.stab)
+            return
+        for tstep in range(1, d.nstep + 1):
+            d.updateNewmark(dt)
+            d.applySource(tstep - 1)
+            d.computeStiff()
+            d.coupleSolidFluid()
+            d.assembleStiff(-1)
+            d.record(tstep - 1, t)
+            t += dt
+            if tstep % self.stab == 0:
+                d.checkStability(dt, tstep, t)
+            d.assembleStiff(1)

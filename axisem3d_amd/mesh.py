@@ -1,0 +1,135 @@
+"""Synthetic spherical-shell meshes (preloop), wrapped from libaxg.
+
+``shell_config(which, **overrides)`` returns the default configuration of one of
+the four synthetic workloads (0: plain 4000-quad isotropic Nu=4; 1: 20000 quads
+solid+fluid TI with a 3-D Gaussian random field and per-element Nu in [8,128];
+2: 100000-quad anisotropic attenuating; 3: a 2000-quad member of the batched
+set) with fields overridden by keyword.  ``ShellMesh`` owns the built mesh and
+exposes the descriptor arrays as numpy views.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+def shell_config(which=0, layers=None, **overrides):
+    from . import lib
+    cfg = _abi.ShellConfig()
+    rc = lib().axg_shell_config_default(which, C.byref(cfg))
+    if rc != 0:
+        raise RuntimeError(lib().axg_last_error().decode())
+    keep = None
+    if layers is not None:
+        arr = (_abi.ShellLayer * len(layers))()
+        for i, l in enumerate(layers):
+            for k, v in l.items():
+                setattr(arr[i], k, v)
+        cfg.nlayer = len(layers)
+        cfg.layers = C.cast(arr, C.POINTER(_abi.ShellLayer))
+        keep = arr
+    for k, v in overrides.items():
+        if k == "moment":
+            for i in range(6):
+                cfg.moment[i] = v[i]
+        else:
+            setattr(cfg, k, v)
+    cfg._keep = keep  # keep the layer array alive with the config
+    return cfg
+
+
+def default_layers(which=0):
+    """The layer list of a default configuration, as dicts."""
+    cfg = shell_config(which)
+    out = []
+    for i in range(cfg.nlayer):
+        l = cfg.layers[i]
+        out.append({name: getattr(l, name) for name, _ in _abi.ShellLayer._fields_})
+    return out
+
+
+class ShellMesh:
+    """A built mesh; ``desc`` is the axg_mesh_desc pointer to hand to the C ABI."""
+
+    STAT_NAMES = ("npoint", "nelem", "nsolid_elem", "nfluid_elem", "n3d_elem",
+                  "max_nr", "sum_nu1", "nsf")
+
+    def __init__(self, cfg=None, which=0, **overrides):
+        from . import lib
+        if cfg is None:
+            cfg = shell_config(which, **overrides)
+        self.cfg = cfg
+        h = C.c_void_p()
+        rc = lib().axg_mesh_build_shell(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(lib().axg_last_error().decode())
+        self._h = h
+        self.desc = lib().axg_mesh_get_desc(h)
+        st = (C.c_int64 * 8)()
+        lib().axg_mesh_stats(h, st, 8)
+        self.stats = dict(zip(self.STAT_NAMES, list(st)))
+
+    def __del__(self):
+        try:
+            from . import lib
+            if getattr(self, "_h", None):
+                lib().axg_mesh_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ---- numpy views of the descriptor ----
+    def _view(self, name, n, dtype):
+        ptr = getattr(self.desc.contents, name)
+        if n == 0 or not ptr:
+            return np.zeros(0, dtype=dtype)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+    @property
+    def d(self):
+        return self.desc.contents
+
+    def array(self, name):
+        d = self.d
+        npt, ne = d.npoint, d.nelem
+        sizes = {
+            "point_nr": (npt, np.int32), "point_flags": (npt, np.int32),
+            "point_sz": (2 * npt, np.float64),
+            "mass_solid_off": (npt + 1, np.int64), "mass_fluid_off": (npt + 1, np.int64),
+            "elem_points": (25 * ne, np.int32), "elem_flags": (ne, np.int32),
+            "elem_material": (ne, np.int32), "mat_off": (ne + 1, np.int64),
+            "elem_geom": (125 * ne, np.float32), "elem_tiso": (100 * ne, np.float32),
+            "elem_att": (ne, np.int32), "att_off": (ne + 1, np.int64),
+            "sls_alpha": (d.nsls, np.float32), "sls_beta": (d.nsls, np.float32),
+            "sls_gamma": (d.nsls, np.float32),
+            "sf_point": (d.nsf, np.int32), "sf_off": (d.nsf + 1, np.int64),
+            "src_force": (450, np.float32), "stf": (d.nstep, np.float32),
+            "rec_elem": (d.nrec, np.int32), "rec_weights": (25 * d.nrec, np.float32),
+            "rec_phi": (d.nrec, np.float32), "rec_theta": (d.nrec, np.float32),
+        }
+        if name in sizes:
+            n, dt = sizes[name]
+            return self._view(name, n, dt)
+        if name == "inv_mass_solid":
+            return self._view(name, int(self.array("mass_solid_off")[-1]), np.float32)
+        if name == "inv_mass_fluid":
+            return self._view(name, int(self.array("mass_fluid_off")[-1]), np.float32)
+        if name == "mat_pool":
+            return self._view(name, int(self.array("mat_off")[-1]), np.float32)
+        if name == "att_pool":
+            return self._view(name, int(self.array("att_off")[-1]), np.float32)
+        if name == "sf_normal":
+            return self._view(name, 6 * int(self.array("sf_off")[-1]), np.float32)
+        raise KeyError(name)
+
+    def field_size(self, which):
+        from . import lib
+        return int(lib().axg_field_size(self.desc, which))
+
+    def updates_per_step(self):
+        """GLL-point x mode rows advanced per step (solid and fluid sides counted)."""
+        nr = self.array("point_nr").astype(np.int64)
+        fl = self.array("point_flags")
+        nu1 = nr // 2 + 1
+        return int(nu1[(fl & _abi.PT_SOLID) != 0].sum() + nu1[(fl & _abi.PT_FLUID) != 0].sum())

@@ -1,0 +1,279 @@
+/*
+ * axg.h -- C ABI of the MI355X-native AxiSEM3D time-stepping path.
+ *
+ * The reference drives its time loop through C++ virtuals on per-object state:
+ *   Newmark::solve                      SOLVER/src/core/newmark/Newmark.cpp:19-107
+ *   Domain::{computeStiff, applySource, assembleStiff, updateNewmark,
+ *            coupleSolidFluid, record, checkStability, learnWisdom}
+ *                                       SOLVER/src/core/domain/Domain.h:61-88
+ *   Element::computeStiff               SOLVER/src/core/element/Element.h:20
+ *   Point::updateNewmark                SOLVER/src/core/point/Point.h:17
+ * Per-element and per-point virtual calls are far too fine-grained for a GPU, so
+ * each Domain-level loop becomes ONE batched entry point below that acts on the
+ * whole mesh resident in HBM.  The objects the reference creates at its
+ * "release" boundary (Mesh::release -> GLLPoint::release / Quad::releaseSolid /
+ * Quad::releaseFluid, SOLVER/src/preloop/mesh/{GLLPoint.cpp:48-128,Quad.cpp:386-420})
+ * are described instead by one plain-array mesh descriptor (axg_mesh_desc).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a non-zero AXG_E* code on
+ *     failure; axg_last_error() then returns a message in the reference's
+ *     "Class::method || message" form (thread-local).
+ *   - A context is not thread-safe; it owns one HIP stream.  Several contexts
+ *     may share a device.
+ *   - Field arrays exchanged with the host use the canonical point layout:
+ *     solid points in point order, each (Nu_p+1) rows x 3 components of complex
+ *     (re, im) float; fluid points likewise with 1 component.  Nu_p = Nr_p / 2.
+ *   - No function allocates host memory that the caller must free, except the
+ *     mesh handle (axg_mesh_free) and the context (axg_destroy).
+ */
+#ifndef AXG_H
+#define AXG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXG_NPOL 4
+#define AXG_NPE_EDGE 5
+#define AXG_NPE 25
+
+/* return codes */
+#define AXG_OK 0
+#define AXG_EINVAL 1     /* bad argument or inconsistent descriptor */
+#define AXG_EUNSUPPORTED 2
+#define AXG_EHIP 3       /* HIP runtime failure */
+#define AXG_EUNSTABLE 4  /* Domain::checkStability found a non-finite value */
+#define AXG_ENOMEM 5
+
+/* point flags (GLLPoint::release decides solid / fluid / solid-fluid,
+ * SOLVER/src/preloop/mesh/GLLPoint.cpp:50-126) */
+#define AXG_PT_SOLID 1
+#define AXG_PT_FLUID 2
+#define AXG_PT_AXIAL 4
+#define AXG_PT_FLUID_SURFACE 8
+
+/* element flags (Quad::releaseSolid/releaseFluid, SolidElement ctor
+ * SOLVER/src/core/element/SolidElement.cpp:15-34) */
+#define AXG_EL_FLUID 1
+#define AXG_EL_AXIAL 2
+#define AXG_EL_3D 4      /* material varies in azimuth: FFT path (mElem3D) */
+#define AXG_EL_TISO 8    /* needs the (s,phi,z)<->(theta,phi,r) rotation (mInTIso) */
+
+/* constitutive relation of an element */
+#define AXG_MAT_ISOTROPIC 0            /* lambda, mu                     */
+#define AXG_MAT_TRANSVERSELY_ISO 1     /* A, C, F, L, N                  */
+#define AXG_MAT_ANISOTROPIC 2          /* C11 C12 .. C66 (21, upper tri) */
+#define AXG_MAT_ACOUSTIC 3             /* K = integral factor / rho      */
+
+/* attenuation of an element */
+#define AXG_ATT_NONE 0
+#define AXG_ATT_FULL 1   /* all 25 GLL points          */
+#define AXG_ATT_CG4 2    /* coarse grid (1,1),(1,3),(3,1),(3,3) */
+
+/*
+ * Mesh descriptor: everything the time loop consumes, already in the form the
+ * reference's objects hold after "release".  All arrays are owned by the caller
+ * and only read during axg_create (the context copies what it needs).
+ *
+ * Material pool layout of element e (floats, starting at mat_off[e]); every
+ * block is either 25 values (1-D, Fourier-space material) or nr_e x 25 values
+ * (3-D, row = azimuthal sample, column = GLL point ipol*5+jpol), nr_e being the
+ * element's Nr = max over its points:
+ *   ISOTROPIC           lambda, mu                  (Isotropic{1D,3D}.h)
+ *   TRANSVERSELY_ISO    A, C, F, L, N               (TransverselyIsotropic{1D,3D}.h)
+ *   ANISOTROPIC         C11 C12 C13 C14 C15 C16 C22 C23 C24 C25 C26
+ *                       C33 C34 C35 C36 C44 C45 C46 C55 C56 C66 (Anisotropic{1D,3D}.h)
+ *   ACOUSTIC            K                           (Acoustic{1D,3D}.h)
+ * Attenuation pool of element e (att_off[e]): dkappa then dmu, each 25 (FULL) or
+ * 4 (CG4) values per azimuthal row, rows = 1 (1-D) or nr_e (3-D).
+ */
+typedef struct axg_mesh_desc {
+    /* ---- GLL points ---- */
+    int32_t npoint;
+    const int32_t *point_nr;          /* [npoint] Nr of the point               */
+    const int32_t *point_flags;       /* [npoint] AXG_PT_*                      */
+    const double  *point_sz;          /* [2*npoint] (s, z) in metres            */
+    const int64_t *mass_solid_off;    /* [npoint+1] 0 (not solid), 1 (Mass1D) or nr values */
+    const float   *inv_mass_solid;
+    const int64_t *mass_fluid_off;    /* [npoint+1]                              */
+    const float   *inv_mass_fluid;
+
+    /* ---- elements ---- */
+    int32_t nelem;
+    const int32_t *elem_points;       /* [25*nelem] point index of (ipol,jpol) at ipol*5+jpol */
+    const int32_t *elem_flags;        /* [nelem] AXG_EL_*                        */
+    const int32_t *elem_material;     /* [nelem] AXG_MAT_*                       */
+    const int64_t *mat_off;           /* [nelem+1] offsets into mat_pool          */
+    const float   *mat_pool;
+    const float   *elem_geom;         /* [nelem*5*25] dsdxii, dsdeta, dzdxii, dzdeta, inv_s */
+    const float   *elem_tiso;         /* [nelem*4*25] sin t, cos t, sin 2t, cos 2t */
+    const int32_t *elem_att;          /* [nelem] AXG_ATT_*                       */
+    const int64_t *att_off;           /* [nelem+1] offsets into att_pool          */
+    const float   *att_pool;
+
+    /* ---- standard linear solids, shared by all attenuating elements ---- */
+    int32_t nsls;
+    const float *sls_alpha;           /* [nsls] */
+    const float *sls_beta;            /* [nsls] */
+    const float *sls_gamma;           /* [nsls] */
+    int32_t att_do_kappa;
+
+    /* ---- solid-fluid boundary points (SolidFluidPoint + SFCoupling) ---- */
+    int32_t nsf;
+    const int32_t *sf_point;          /* [nsf] point index                        */
+    const int64_t *sf_off;            /* [nsf+1] 1 (SFCoupling1D) or nr rows       */
+    const float   *sf_normal;         /* per row: ns, np, nz (unassembled) then ns/mf, np/mf, nz/mf */
+
+    /* ---- source (SourceTerm on one element) and source-time function ---- */
+    int32_t src_elem;                 /* -1: no source                            */
+    const float *src_force;           /* [25*3*3*2] point, alpha<=2, component, re/im */
+    int32_t nstep;                    /* length of the STF = number of time steps */
+    const float *stf;                 /* [nstep]                                  */
+    double dt;
+    double t_shift;                   /* t of step 1 is -t_shift                  */
+
+    /* ---- receivers (PointwiseRecorder) ---- */
+    int32_t nrec;
+    const int32_t *rec_elem;          /* [nrec] */
+    const float   *rec_weights;       /* [nrec*25] Lagrange weights (/ iFact in fluid) */
+    const float   *rec_phi;           /* [nrec] azimuth in the source frame        */
+    const float   *rec_theta;         /* [nrec] colatitude in the source frame     */
+    int32_t rec_components;           /* 0 = RTZ, 1 = SPZ                          */
+} axg_mesh_desc;
+
+/* ===================================================================== *
+ *  Synthetic spherical-shell meshes (preloop).  Restates, for an analytic
+ *  shell geometry, Quad/GLLPoint/Material/AttAxiSEM/Earthquake/Receiver
+ *  set-up; see axisem3d_amd/csrc/preloop/shell_mesh.cpp for citations.
+ * ===================================================================== */
+typedef struct axg_shell_layer {
+    double r_bottom, r_top;           /* metres */
+    int32_t nelem_radial;
+    int32_t fluid;
+    /* values at bottom / top, linear in radius (SI units) */
+    double vp_bottom, vp_top, vs_bottom, vs_top, rho_bottom, rho_top;
+    double q_mu, q_kappa;             /* used when attenuation is on; <=0: draw in [q_min,q_max] */
+} axg_shell_layer;
+
+typedef struct axg_shell_config {
+    int32_t nelem_theta;              /* elements along colatitude [0, pi]        */
+    int32_t nlayer;
+    const axg_shell_layer *layers;    /* bottom to top, contiguous                */
+    /* material */
+    int32_t material;                 /* AXG_MAT_ISOTROPIC / TRANSVERSELY_ISO / ANISOTROPIC for solids */
+    double tiso_xi, tiso_phi, tiso_eta; /* (vsh/vsv)^2, (vpv/vph)^2, eta used for TI */
+    double aniso_perturb;             /* max relative SPD perturbation of Cij      */
+    double dv_amp;                    /* 3-D dvs/dvp amplitude (0: 1-D model)      */
+    double dv_corr_len;               /* correlation length, metres               */
+    int32_t dv_fluid;                 /* perturb vp in fluid too                   */
+    /* Fourier order */
+    int32_t nu_mode;                  /* 0 const, 1 uniform per element, 2 log-normal per element */
+    int32_t nu_const, nu_min, nu_max;
+    double nu_median, nu_sigma;
+    int32_t lucky;                    /* FFTW_LUCKY_NUMBER                         */
+    /* attenuation */
+    int32_t attenuation;              /* AXG_ATT_*                                 */
+    int32_t nsls;
+    double q_min, q_max, f_min, f_max, f_ref;
+    int32_t do_kappa;
+    /* time */
+    double courant;
+    int32_t nstep;
+    double period;                    /* source half duration = period / 2         */
+    /* source */
+    double src_depth;                 /* metres                                   */
+    double moment[6];                 /* Mrr Mtt Mpp Mrt Mrp Mtp, N m; all zero: seeded random unit tensor * 1e20 */
+    /* receivers */
+    int32_t nrec;
+    double rec_depth;
+    int32_t rec_components;
+    uint64_t seed;
+    int32_t nthread;                  /* host threads for set-up (0 = all)         */
+} axg_shell_config;
+
+typedef struct axg_mesh axg_mesh;
+
+int axg_shell_config_default(int which, axg_shell_config *cfg); /* which = 0..3: the four synthetic workloads */
+int axg_mesh_build_shell(const axg_shell_config *cfg, axg_mesh **mesh);
+const axg_mesh_desc *axg_mesh_get_desc(const axg_mesh *mesh);
+int axg_mesh_stats(const axg_mesh *mesh, int64_t *stats, int nstats);
+void axg_mesh_free(axg_mesh *mesh);
+
+/* ===================================================================== *
+ *  Domain on one GPU.
+ * ===================================================================== */
+typedef struct axg_ctx axg_ctx;
+
+/* create: copy the descriptor into HBM (Mesh::release -> Domain) */
+int axg_create(const axg_mesh_desc *desc, int device, axg_ctx **ctx);
+int axg_destroy(axg_ctx *ctx);
+
+/* Domain::resetZero  (Domain.cpp:67-74) */
+int axg_reset_zero(axg_ctx *ctx);
+/* Domain::updateNewmark (Domain.cpp:165-177 -> SolidPoint.cpp:23-38, FluidPoint.cpp:23-44) */
+int axg_update_newmark(axg_ctx *ctx, double dt);
+/* Domain::applySource (Domain.cpp:96-109 -> SourceTerm.cpp:29-35) */
+int axg_apply_source(axg_ctx *ctx, int32_t tstep);
+/* Domain::computeStiff (Domain.cpp:82-94 -> SolidElement.cpp:43-65, FluidElement.cpp:43-65);
+ * includes the on-device assembly that replaces Domain::assembleStiff */
+int axg_compute_stiff(axg_ctx *ctx);
+/* Domain::coupleSolidFluid (Domain.cpp:179-191 -> SolidFluidPoint.cpp:106-110) */
+int axg_couple_solid_fluid(axg_ctx *ctx);
+/* Domain::assembleStiff (Domain.cpp:111-163): a no-op on one GPU, kept so the
+ * driver sequence of Newmark::solve is unchanged */
+int axg_assemble_stiff(axg_ctx *ctx, int32_t phase);
+/* Domain::record (Domain.cpp:207-220 -> PointwiseRecorder.cpp:62-144) */
+int axg_record(axg_ctx *ctx, int32_t tstep, double t);
+/* Domain::checkStability (Domain.cpp:237-275); bad_point = -1 when stable,
+ * otherwise the smallest unstable point index and AXG_EUNSTABLE is returned */
+int axg_check_stability(axg_ctx *ctx, int64_t *bad_point);
+/* Domain::learnWisdom (Domain.cpp:384-402 -> SolidPoint.cpp:240-266) */
+int axg_learn_wisdom(axg_ctx *ctx, float cutoff);
+/* Point::getNuWisdom for every point (Domain::dumpWisdom, Domain.cpp:404-440) */
+int axg_get_wisdom(axg_ctx *ctx, int32_t *nu_wisdom);
+
+/* The loop body of Newmark::solve (Newmark.cpp:47-93) for steps
+ * tstep = first .. first+count-1 (1-based), recorded every step; stability is
+ * checked every stab_interval steps (0: never).  flags: AXG_RUN_GRAPH replays
+ * the step from a captured HIP graph; AXG_RUN_PROFILE times the kernel classes
+ * (element stiffness, Newmark, other) with HIP events on the context's stream. */
+#define AXG_RUN_GRAPH 1
+#define AXG_RUN_PROFILE 2
+int axg_run(axg_ctx *ctx, int32_t first, int32_t count, int32_t stab_interval, int32_t flags);
+
+/* Seismograms recorded so far: [nstep][nrec][3] floats (RTZ or SPZ) */
+int axg_get_seismograms(axg_ctx *ctx, float *out, int64_t nfloat);
+
+/* canonical-layout field access; which = AXG_FIELD_* */
+#define AXG_FIELD_SOLID_DISPL 0
+#define AXG_FIELD_SOLID_VELOC 1
+#define AXG_FIELD_SOLID_ACCEL 2
+#define AXG_FIELD_SOLID_STIFF 3
+#define AXG_FIELD_FLUID_DISPL 4
+#define AXG_FIELD_FLUID_VELOC 5
+#define AXG_FIELD_FLUID_ACCEL 6
+#define AXG_FIELD_FLUID_STIFF 7
+int64_t axg_field_size(const axg_mesh_desc *desc, int32_t which); /* floats */
+int axg_get_field(axg_ctx *ctx, int32_t which, float *out, int64_t nfloat);
+int axg_set_field(axg_ctx *ctx, int32_t which, const float *in, int64_t nfloat);
+
+/* timing of the last axg_run: total ms and, after a profiled run, ms per
+ * kernel class [element stiffness, Newmark, source/coupling/record] */
+int axg_last_run_timing(axg_ctx *ctx, double *ms_total, double *ms_per_class, int32_t nclass);
+/* algorithmic HBM bytes of one step and updates (GLL-pt x mode) of one step */
+int axg_step_cost(axg_ctx *ctx, double *bytes_step, double *updates_step);
+/* algorithmic HBM bytes per step of each kernel class (same classes as
+ * axg_last_run_timing): element stiffness reads displ and accumulates stiff
+ * once per point-mode plus the element constants; Newmark streams the four
+ * point fields and the mass */
+int axg_class_bytes(axg_ctx *ctx, double *bytes_per_class, int32_t nclass);
+
+const char *axg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AXG_H */

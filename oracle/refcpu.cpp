@@ -1,0 +1,1288 @@
+// refcpu.cpp -- CPU oracle (test infrastructure only) of the AxiSEM3D time loop.
+//
+// Every function below restates one reference function; the file:line it
+// follows is given next to it (paths relative to SOLVER/src/).  The structure
+// is kept close to the reference on purpose (per-element mode loops over 5x5
+// matrices, a per-thread copy of the static element workspace, FFT columns of
+// the flat layout 25*component + 5*ipol + jpol) so that a reader can check it
+// line by line.  Arithmetic type Real = float or double.
+#include "refcpu.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include <xmmintrin.h>
+
+#include "../axisem3d_amd/csrc/preloop/spectral.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+// set_ftz (SOLVER/src/ftz.c:11-18): the reference flushes float denormals to
+// zero; MXCSR is per thread, so every worker sets it.
+inline void set_ftz() { _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON); }
+
+template <class Real>
+using Cx = std::complex<Real>;
+
+// ------------------------------------------------------------------ FFT
+// Mixed-radix complex DFT (recursive decimation in time, generic radix
+// butterflies); twiddles are computed in double and cast to Real.
+template <class Real>
+struct FFTPlan {
+    int n = 0;
+    std::vector<int> factors;
+    std::vector<Cx<Real>> tw;  // exp(-2 pi i t / n)
+    explicit FFTPlan(int nn) : n(nn) {
+        int m = n;
+        for (int p : {4, 2, 3, 5, 7, 11, 13}) {
+            while (m % p == 0) {
+                factors.push_back(p);
+                m /= p;
+            }
+        }
+        for (int p = 17; m > 1; p++) {
+            while (m % p == 0) {
+                factors.push_back(p);
+                m /= p;
+            }
+        }
+        tw.resize(n);
+        for (int t = 0; t < n; t++) {
+            double a = -2. * 3.141592653589793238463 * t / n;
+            tw[t] = Cx<Real>((Real)std::cos(a), (Real)std::sin(a));
+        }
+    }
+    // out[k] = sum_j in[j] exp(sign 2 pi i j k / n)
+    void rec(const Cx<Real> *in, int istride, Cx<Real> *out, int len, int fi, int sign, int twstride) const {
+        if (len == 1) {
+            out[0] = in[0];
+            return;
+        }
+        int p = factors[fi];
+        int m = len / p;
+        for (int j = 0; j < p; j++) rec(in + j * istride, istride * p, out + j * m, m, fi + 1, sign, twstride * p);
+        Cx<Real> y[32];
+        std::vector<Cx<Real>> yv;
+        Cx<Real> *yy = y;
+        if (p > 32) {
+            yv.resize(p);
+            yy = yv.data();
+        }
+        for (int k = 0; k < m; k++) {
+            for (int j = 0; j < p; j++) {
+                Cx<Real> w = tw[((int64_t)j * k * twstride) % n];
+                if (sign > 0) w = std::conj(w);
+                yy[j] = out[j * m + k] * w;
+            }
+            for (int q = 0; q < p; q++) {
+                Cx<Real> s(0, 0);
+                for (int j = 0; j < p; j++) {
+                    Cx<Real> w = tw[((int64_t)j * q * m * twstride) % n];
+                    if (sign > 0) w = std::conj(w);
+                    s += yy[j] * w;
+                }
+                out[k + q * m] = s;
+            }
+        }
+    }
+    void exec(const Cx<Real> *in, Cx<Real> *out, int sign) const { rec(in, 1, out, n, 0, sign, 1); }
+};
+
+template <class Real>
+const FFTPlan<Real> &planFor(int n) {
+    static thread_local std::map<int, std::unique_ptr<FFTPlan<Real>>> plans;
+    auto it = plans.find(n);
+    if (it != plans.end()) return *it->second;
+    auto *p = new FFTPlan<Real>(n);
+    plans[n].reset(p);
+    return *p;
+}
+
+// FFTW c2r semantics (SolverFFTW_N6::computeC2R, fftw/SolverFFTW_N6.cpp:51-53):
+// nc = nr/2+1 Hermitian rows -> nr reals, unscaled; imaginary parts of the DC
+// and (even nr) Nyquist rows are ignored.
+template <class Real>
+void c2r(int nr, const Cx<Real> *X, Real *x, std::vector<Cx<Real>> &wa, std::vector<Cx<Real>> &wb) {
+    wa.assign(nr, Cx<Real>(0, 0));
+    wb.resize(nr);
+    int nc = nr / 2 + 1;
+    wa[0] = Cx<Real>(X[0].real(), 0);
+    for (int k = 1; k < nc; k++) {
+        if (2 * k == nr) {
+            wa[k] = Cx<Real>(X[k].real(), 0);
+        } else {
+            wa[k] = X[k];
+            wa[nr - k] = std::conj(X[k]);
+        }
+    }
+    planFor<Real>(nr).exec(wa.data(), wb.data(), +1);
+    for (int i = 0; i < nr; i++) x[i] = wb[i].real();
+}
+
+// FFTW r2c then x 1/nr (SolverFFTW_N6::computeR2C, SolverFFTW_N6.cpp:45-49)
+template <class Real>
+void r2c(int nr, const Real *x, Cx<Real> *X, std::vector<Cx<Real>> &wa, std::vector<Cx<Real>> &wb) {
+    wa.resize(nr);
+    wb.resize(nr);
+    for (int i = 0; i < nr; i++) wa[i] = Cx<Real>(x[i], 0);
+    planFor<Real>(nr).exec(wa.data(), wb.data(), -1);
+    Real inv = (Real)1 / (Real)nr;
+    int nc = nr / 2 + 1;
+    for (int k = 0; k < nc; k++) X[k] = wb[k] * inv;
+}
+
+// ------------------------------------------------------------------ domain
+typedef std::array<double, 25> M25;
+
+template <class Real>
+struct Ref {
+    typedef Cx<Real> C;
+    typedef std::array<Real, 25> RMat;  // 5x5 row-major (ipol, jpol)
+    typedef std::array<C, 25> CMat;
+
+    const Real zero = 0, one = 1, two = 2, three = 3, half = (Real)0.5, third = (Real)(1.0 / 3.0);
+
+    // static G operators (Gradient::setGMat, element/grad/Gradient.cpp:371-380)
+    RMat G_GLL, G_GLJ, GT_GLL, GT_GLJ;
+
+    struct SolidPt {
+        int nr, nu, gidx;
+        bool axial;
+        Real invMass;
+        std::vector<C> displ, veloc, accel, stiff;  // [(nu+1)*3], alpha-major
+    };
+    struct FluidPt {
+        int nr, nu, gidx;
+        bool axial, surface;
+        Real invMass;
+        std::vector<C> displ, veloc, accel, stiff;
+    };
+    struct Elem {
+        bool fluid, axial, is3D, tiso;
+        int mat, att, nr, nu, maxNu;
+        int pts[25];  // compact solid or fluid point index
+        RMat dsdxii, dsdeta, dzdxii, dzdeta, inv_s;
+        RMat sin1t, cos1t, sin2t, cos2t;
+        std::vector<Real> matv;  // as in the descriptor (Real)
+        // attenuation
+        std::vector<Real> dkappa3, dmu, dmu2;
+        std::vector<C> stressR1;                // 1-D: [alpha][6][np]
+        std::vector<std::vector<C>> memvar1;    // [sls]
+        std::vector<Real> stressR3, strain4;    // 3-D: [row][6*np]
+        std::vector<std::vector<Real>> memvar3;
+    };
+
+    std::vector<SolidPt> S;
+    std::vector<FluidPt> F;
+    std::vector<int> sIndex, fIndex;  // global point -> compact
+    std::vector<Elem> E;
+    std::vector<std::vector<int>> colours;
+    int nthread = 1;
+
+    // SF coupling (SFCoupling1D)
+    struct SF {
+        int s, f;
+        Real ns, nz, ns_mf, nz_mf;
+    };
+    std::vector<SF> sfs;
+
+    // source
+    int srcElem = -1;
+    std::array<std::vector<C>, 25> srcForce;
+    std::vector<Real> stf;
+    double dt = 0, tshift = 0;
+    int nstep = 0;
+
+    // receivers
+    struct Rec {
+        int elem;
+        RMat w;
+        Real phi;
+        double theta;
+    };
+    std::vector<Rec> recs;
+    int rtz = 1;
+    std::vector<double> seis;
+
+    int nsls = 0;
+    std::vector<Real> slsA, slsB, slsG;
+    bool doKappa = true;
+    double cost[5] = {0, 0, 0, 0, 0};
+
+    // ---- per-thread workspace (SolidElement::sResponse / FluidElement::sResponse) ----
+    struct Work {
+        std::vector<std::array<CMat, 3>> displ, stiff;
+        std::vector<std::array<CMat, 6>> strain6, stress6;
+        std::vector<CMat> fdispl, fstiff;
+        std::vector<std::array<CMat, 3>> fstrain, fstress;
+        std::vector<C> flatC;     // [alpha][150]
+        std::vector<Real> flatR;  // [row][150]
+        std::vector<Real> flatS;  // [row][150]
+        std::vector<C> colC;
+        std::vector<Real> colR;
+        std::vector<C> wa, wb;
+    };
+    std::vector<Work> works;
+
+    void init(const axg_mesh_desc *d) {
+        const auto &gl = axg::spectral::gll();
+        const auto &gj = axg::spectral::glj();
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) {
+                G_GLL[i * 5 + j] = (Real)gl.G[i][j];
+                G_GLJ[i * 5 + j] = (Real)gj.G[i][j];
+                GT_GLL[j * 5 + i] = (Real)gl.G[i][j];
+                GT_GLJ[j * 5 + i] = (Real)gj.G[i][j];
+            }
+        sIndex.assign(d->npoint, -1);
+        fIndex.assign(d->npoint, -1);
+        for (int p = 0; p < d->npoint; p++) {
+            int nr = d->point_nr[p];
+            int fl = d->point_flags[p];
+            if (fl & AXG_PT_SOLID) {
+                if (d->mass_solid_off[p + 1] - d->mass_solid_off[p] != 1)
+                    throw std::runtime_error("refcpu || Mass3D not restated.");
+                SolidPt s;
+                s.nr = nr;
+                s.nu = nr / 2;
+                s.gidx = p;
+                s.axial = (fl & AXG_PT_AXIAL) != 0;
+                s.invMass = (Real)d->inv_mass_solid[d->mass_solid_off[p]];
+                s.displ.assign((s.nu + 1) * 3, C(0, 0));
+                s.veloc = s.accel = s.stiff = s.displ;
+                sIndex[p] = (int)S.size();
+                S.push_back(s);
+            }
+            if (fl & AXG_PT_FLUID) {
+                if (d->mass_fluid_off[p + 1] - d->mass_fluid_off[p] != 1)
+                    throw std::runtime_error("refcpu || Mass3D not restated.");
+                FluidPt f;
+                f.nr = nr;
+                f.nu = nr / 2;
+                f.gidx = p;
+                f.axial = (fl & AXG_PT_AXIAL) != 0;
+                f.surface = (fl & AXG_PT_FLUID_SURFACE) != 0;
+                f.invMass = (Real)d->inv_mass_fluid[d->mass_fluid_off[p]];
+                f.displ.assign(f.nu + 1, C(0, 0));
+                f.veloc = f.accel = f.stiff = f.displ;
+                fIndex[p] = (int)F.size();
+                F.push_back(f);
+            }
+        }
+        nsls = d->nsls;
+        doKappa = d->att_do_kappa != 0;
+        for (int i = 0; i < nsls; i++) {
+            slsA.push_back((Real)d->sls_alpha[i]);
+            slsB.push_back((Real)d->sls_beta[i]);
+            slsG.push_back((Real)d->sls_gamma[i]);
+        }
+        int maxNu = 0;
+        E.resize(d->nelem);
+        for (int e = 0; e < d->nelem; e++) {
+            Elem &el = E[e];
+            int fl = d->elem_flags[e];
+            el.fluid = fl & AXG_EL_FLUID;
+            el.axial = fl & AXG_EL_AXIAL;
+            el.is3D = fl & AXG_EL_3D;
+            el.tiso = fl & AXG_EL_TISO;
+            el.mat = d->elem_material[e];
+            el.att = d->elem_att[e];
+            // Element ctor: max Nr / Nu over the points (Element.cpp:14-18)
+            el.nr = 0;
+            el.maxNu = 0;
+            for (int k = 0; k < 25; k++) {
+                int p = d->elem_points[(size_t)e * 25 + k];
+                el.pts[k] = el.fluid ? fIndex[p] : sIndex[p];
+                if (el.pts[k] < 0) throw std::runtime_error("Point::scatterDisplToElement || Incompatible point type.");
+                el.nr = std::max(el.nr, d->point_nr[p]);
+                el.maxNu = std::max(el.maxNu, d->point_nr[p] / 2);
+            }
+            el.nu = el.nr / 2;
+            maxNu = std::max(maxNu, el.nu);
+            const float *g = d->elem_geom + (size_t)e * 125;
+            const float *t = d->elem_tiso + (size_t)e * 100;
+            for (int k = 0; k < 25; k++) {
+                el.dsdxii[k] = (Real)g[k];
+                el.dsdeta[k] = (Real)g[25 + k];
+                el.dzdxii[k] = (Real)g[50 + k];
+                el.dzdeta[k] = (Real)g[75 + k];
+                el.inv_s[k] = (Real)g[100 + k];
+                el.sin1t[k] = (Real)t[k];
+                el.cos1t[k] = (Real)t[25 + k];
+                el.sin2t[k] = (Real)t[50 + k];
+                el.cos2t[k] = (Real)t[75 + k];
+            }
+            el.matv.assign(d->mat_pool + d->mat_off[e], d->mat_pool + d->mat_off[e + 1]);
+            if (el.att != AXG_ATT_NONE) {
+                int np = el.att == AXG_ATT_CG4 ? 4 : 25;
+                int rows = el.is3D ? el.nr : 1;
+                const float *ap = d->att_pool + d->att_off[e];
+                int n = rows * np;
+                // Attenuation ctor: mDKappa3(three * dkappa), mDMu(dmu), mDMu2(two * dmu)
+                el.dkappa3.resize(n);
+                el.dmu.resize(n);
+                el.dmu2.resize(n);
+                for (int i = 0; i < n; i++) {
+                    el.dkappa3[i] = three * (Real)ap[i];
+                    el.dmu[i] = (Real)ap[n + i];
+                    el.dmu2[i] = two * (Real)ap[n + i];
+                }
+                if (el.is3D) {
+                    el.stressR3.assign((size_t)el.nr * 6 * np, zero);
+                    el.strain4 = el.stressR3;
+                    el.memvar3.assign(nsls, el.stressR3);
+                } else {
+                    el.stressR1.assign((size_t)(el.nu + 1) * 6 * np, C(0, 0));
+                    el.memvar1.assign(nsls, el.stressR1);
+                }
+            }
+        }
+        // greedy colouring: elements of one colour share no point
+        {
+            std::vector<int> colour(d->nelem, -1);
+            std::vector<std::vector<int>> pe(d->npoint);
+            for (int e = 0; e < d->nelem; e++)
+                for (int k = 0; k < 25; k++) pe[d->elem_points[(size_t)e * 25 + k]].push_back(e);
+            for (int e = 0; e < d->nelem; e++) {
+                std::vector<char> used(64, 0);
+                for (int k = 0; k < 25; k++)
+                    for (int o : pe[d->elem_points[(size_t)e * 25 + k]])
+                        if (colour[o] >= 0 && colour[o] < 64) used[colour[o]] = 1;
+                int cc = 0;
+                while (used[cc]) cc++;
+                colour[e] = cc;
+                if ((int)colours.size() <= cc) colours.resize(cc + 1);
+                colours[cc].push_back(e);
+            }
+        }
+        // SF points
+        for (int q = 0; q < d->nsf; q++) {
+            if (d->sf_off[q + 1] - d->sf_off[q] != 1) throw std::runtime_error("refcpu || SFCoupling3D not restated.");
+            const float *n = d->sf_normal + d->sf_off[q] * 6;
+            int p = d->sf_point[q];
+            sfs.push_back({sIndex[p], fIndex[p], (Real)n[0], (Real)n[2], (Real)n[3], (Real)n[5]});
+        }
+        // source
+        srcElem = d->src_elem;
+        if (srcElem >= 0) {
+            for (int k = 0; k < 25; k++) {
+                const SolidPt &sp = S[E[srcElem].pts[k]];
+                int rows = std::min(3, sp.nu + 1);  // SourceTerm ctor truncation
+                srcForce[k].resize(rows * 3);
+                for (int a = 0; a < rows; a++)
+                    for (int c = 0; c < 3; c++) {
+                        const float *f = d->src_force + ((k * 3 + a) * 3 + c) * 2;
+                        srcForce[k][a * 3 + c] = C((Real)f[0], (Real)f[1]);
+                    }
+            }
+        }
+        nstep = d->nstep;
+        dt = d->dt;
+        tshift = d->t_shift;
+        stf.resize(nstep);
+        for (int i = 0; i < nstep; i++) stf[i] = (Real)d->stf[i];
+        rtz = d->rec_components == 0;
+        for (int i = 0; i < d->nrec; i++) {
+            Rec r;
+            r.elem = d->rec_elem[i];
+            for (int k = 0; k < 25; k++) r.w[k] = (Real)d->rec_weights[i * 25 + k];
+            r.phi = (Real)d->rec_phi[i];
+            r.theta = (double)d->rec_theta[i];
+            recs.push_back(r);
+        }
+        seis.assign((size_t)nstep * recs.size() * 3, 0.);
+        setThreads(1);
+        (void)maxNu;
+    }
+
+    void setThreads(int n) {
+        nthread = std::max(1, n);
+        works.assign(nthread, Work());
+    }
+
+    // ---------------- points ----------------
+    // SolidPoint::maskField (core/point/SolidPoint.cpp:216-238)
+    void maskSolid(SolidPt &p, std::vector<C> &f) {
+        for (int c = 0; c < 3; c++) f[c] = C(f[c].real(), zero);
+        if (p.axial) {
+            f[0] = C(0, 0);
+            f[1] = C(0, 0);
+            if (p.nu >= 1) {
+                C s0 = f[3 + 0], s1 = f[3 + 1];
+                C ii(zero, one);
+                f[3 + 0] = half * (s0 - ii * s1);
+                f[3 + 1] = half * (s1 + ii * s0);
+                f[3 + 2] = C(0, 0);
+                for (int a = 2; a <= p.nu; a++)
+                    for (int c = 0; c < 3; c++) f[a * 3 + c] = C(0, 0);
+            }
+        }
+        if (p.nr % 2 == 0)
+            for (int c = 0; c < 3; c++) f[p.nu * 3 + c] = C(0, 0);
+    }
+    // FluidPoint::maskField (core/point/FluidPoint.cpp:197-207)
+    void maskFluid(FluidPt &p, std::vector<C> &f) {
+        f[0] = C(f[0].real(), zero);
+        if (p.axial)
+            for (int a = 1; a <= p.nu; a++) f[a] = C(0, 0);
+        if (p.nr % 2 == 0) f[p.nu] = C(0, 0);
+    }
+
+    // SolidPoint::updateNewmark (SolidPoint.cpp:23-38), FluidPoint (FluidPoint.cpp:23-44)
+    void updateNewmark(double dtt) {
+        double half_dt = 0.5 * dtt;
+        double half_dt_dt = half_dt * dtt;
+        Real hdt = (Real)half_dt, rdt = (Real)dtt, hdt2 = (Real)half_dt_dt;
+        parallel((int)S.size(), [&](int b, int e, int) {
+            for (int i = b; i < e; i++) {
+                SolidPt &p = S[i];
+                maskSolid(p, p.stiff);
+                for (auto &v : p.stiff) v *= p.invMass;  // Mass1D::computeAccel
+                maskSolid(p, p.stiff);
+                size_t n = p.stiff.size();
+                for (size_t j = 0; j < n; j++) p.veloc[j] += hdt * (p.accel[j] + p.stiff[j]);
+                for (size_t j = 0; j < n; j++) p.accel[j] = p.stiff[j];
+                for (size_t j = 0; j < n; j++) p.displ[j] += rdt * p.veloc[j] + hdt2 * p.accel[j];
+                for (size_t j = 0; j < n; j++) p.stiff[j] = C(0, 0);
+            }
+        });
+        parallel((int)F.size(), [&](int b, int e, int) {
+            for (int i = b; i < e; i++) {
+                FluidPt &p = F[i];
+                size_t n = p.stiff.size();
+                if (p.surface) {
+                    for (size_t j = 0; j < n; j++) p.stiff[j] = p.displ[j] = p.veloc[j] = p.accel[j] = C(0, 0);
+                    continue;
+                }
+                maskFluid(p, p.stiff);
+                for (auto &v : p.stiff) v *= p.invMass;
+                maskFluid(p, p.stiff);
+                for (size_t j = 0; j < n; j++) p.veloc[j] += hdt * (p.accel[j] + p.stiff[j]);
+                for (size_t j = 0; j < n; j++) p.accel[j] = p.stiff[j];
+                for (size_t j = 0; j < n; j++) p.displ[j] += rdt * p.veloc[j] + hdt2 * p.accel[j];
+                for (size_t j = 0; j < n; j++) p.stiff[j] = C(0, 0);
+            }
+        });
+    }
+
+    // SourceTerm::apply (core/source/SourceTerm.cpp:29-35) + SolidPoint::addToStiff
+    void applySource(int tstep) {
+        if (srcElem < 0) return;
+        Real f = stf[tstep];
+        for (int k = 0; k < 25; k++) {
+            SolidPt &p = S[E[srcElem].pts[k]];
+            for (size_t j = 0; j < srcForce[k].size(); j++) p.stiff[j] += srcForce[k][j] * f;
+        }
+    }
+
+    // SolidFluidPoint::coupleSolidFluid + SFCoupling1D (SolidFluidPoint.cpp:106-110,
+    // point/solid_fluid/SFCoupling1D.cpp:9-17)
+    void coupleSolidFluid() {
+        for (auto &q : sfs) {
+            SolidPt &sp = S[q.s];
+            FluidPt &fp = F[q.f];
+            for (int a = 0; a <= fp.nu; a++)
+                fp.stiff[a] += q.ns * sp.displ[a * 3 + 0] + q.nz * sp.displ[a * 3 + 2];
+            for (int a = 0; a <= sp.nu; a++) {
+                sp.stiff[a * 3 + 0] -= q.ns_mf * fp.stiff[a];
+                sp.stiff[a * 3 + 2] -= q.nz_mf * fp.stiff[a];
+            }
+        }
+    }
+
+    // ---------------- elements ----------------
+    // Gradient::computeGrad6 (element/grad/Gradient.cpp:206-265)
+    void grad6(const Elem &el, const std::vector<std::array<CMat, 3>> &ui, std::vector<std::array<CMat, 6>> &eij,
+               int Nu, int nyq) {
+        const RMat &Gx = el.axial ? G_GLJ : G_GLL;    // sG_xii
+        const RMat &GTx = el.axial ? GT_GLJ : GT_GLL; // sGT_xii
+        const RMat &Ge = G_GLL;
+        C ii(zero, one);
+        for (int alpha = 0; alpha <= Nu - nyq; alpha++) {
+            const auto &u = ui[alpha];
+            auto &e = eij[alpha];
+            C ia = (Real)alpha * ii;
+            CMat v0, v1, v2, GU0, GU1, GU2, UG0, UG1, UG2;
+            for (int k = 0; k < 25; k++) {
+                v0[k] = u[0][k] + ia * u[1][k];
+                v1[k] = ia * u[0][k] - u[1][k];
+                v2[k] = ia * u[2][k];
+            }
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    C a0(0, 0), a1(0, 0), a2(0, 0), b0(0, 0), b1(0, 0), b2(0, 0);
+                    for (int m = 0; m < 5; m++) {
+                        Real gt = GTx[i * 5 + m];  // (G^T u)(i,j) = sum_m GT(i,m) u(m,j)
+                        a0 += gt * u[0][m * 5 + j];
+                        a1 += gt * u[1][m * 5 + j];
+                        a2 += gt * u[2][m * 5 + j];
+                        Real g = Ge[m * 5 + j];    // (u G)(i,j) = sum_m u(i,m) G(m,j)
+                        b0 += u[0][i * 5 + m] * g;
+                        b1 += u[1][i * 5 + m] * g;
+                        b2 += u[2][i * 5 + m] * g;
+                    }
+                    int k = i * 5 + j;
+                    GU0[k] = a0;
+                    GU1[k] = a1;
+                    GU2[k] = a2;
+                    UG0[k] = b0;
+                    UG1[k] = b1;
+                    UG2[k] = b2;
+                }
+            for (int k = 0; k < 25; k++) {
+                e[0][k] = el.dzdeta[k] * GU0[k] + el.dzdxii[k] * UG0[k];
+                e[1][k] = el.inv_s[k] * v0[k];
+                e[2][k] = el.dsdeta[k] * GU2[k] + el.dsdxii[k] * UG2[k];
+                e[3][k] = el.dsdeta[k] * GU1[k] + el.dsdxii[k] * UG1[k] + el.inv_s[k] * v2[k];
+                e[4][k] = el.dsdeta[k] * GU0[k] + el.dsdxii[k] * UG0[k] + el.dzdeta[k] * GU2[k] + el.dzdxii[k] * UG2[k];
+                e[5][k] = el.dzdeta[k] * GU1[k] + el.dzdxii[k] * UG1[k] + el.inv_s[k] * v1[k];
+            }
+            if (alpha == 0) {
+                // the reference's alpha = 0 branch computes the real part only
+                for (int c = 0; c < 6; c++)
+                    for (int k = 0; k < 25; k++) e[c][k] = C(e[c][k].real(), zero);
+            }
+            if (el.axial) {
+                for (int j = 0; j < 5; j++) {
+                    C d0(0, 0), d1(0, 0), d2(0, 0);
+                    for (int m = 0; m < 5; m++) {
+                        Real gt = GTx[0 * 5 + m];
+                        d0 += gt * v0[m * 5 + j];
+                        d1 += gt * v1[m * 5 + j];
+                        d2 += gt * v2[m * 5 + j];
+                    }
+                    Real dz = el.dzdeta[j];
+                    if (alpha == 0) {
+                        e[1][j] += C((dz * d0).real(), zero);
+                        e[5][j] += C((dz * d1).real(), zero);  // = -dz * (GT u1)
+                    } else {
+                        e[1][j] += dz * d0;
+                        e[5][j] += dz * d1;
+                        e[3][j] += dz * d2;
+                    }
+                    if (alpha == 1) {
+                        C f0(0, 0), f1(0, 0);
+                        for (int m = 0; m < 5; m++) {
+                            f0 += v0[0 * 5 + m] * Ge[m * 5 + j];
+                            f1 += v1[0 * 5 + m] * Ge[m * 5 + j];
+                        }
+                        e[1][j] += el.dzdxii[j] * f0;
+                        e[5][j] += el.dzdxii[j] * f1;
+                    }
+                }
+            }
+        }
+        if (nyq)
+            for (int c = 0; c < 6; c++) eij[Nu][c].fill(C(0, 0));
+    }
+
+    // Gradient::computeQuad6 (Gradient.cpp:267-317)
+    void quad6(const Elem &el, std::vector<std::array<CMat, 3>> &fi, const std::vector<std::array<CMat, 6>> &sij,
+               int Nu, int nyq) {
+        const RMat &Gx = el.axial ? G_GLJ : G_GLL;
+        const RMat &GTe = GT_GLL;
+        C ii(zero, one);
+        for (int b = 0; b <= Nu - nyq; b++) {
+            const auto &s = sij[b];
+            auto &f = fi[b];
+            C ib = -(Real)b * ii;
+            CMat g0, g1, g2, X0, X1, X2, Y0, Y1, Y2;
+            for (int k = 0; k < 25; k++) {
+                g0[k] = s[1][k] + ib * s[5][k];
+                g1[k] = ib * s[1][k] - s[5][k];
+                g2[k] = ib * s[3][k];
+                X0[k] = el.dzdeta[k] * s[0][k] + el.dsdeta[k] * s[4][k];
+                X1[k] = el.dzdeta[k] * s[5][k] + el.dsdeta[k] * s[3][k];
+                X2[k] = el.dzdeta[k] * s[4][k] + el.dsdeta[k] * s[2][k];
+                Y0[k] = el.dzdxii[k] * s[0][k] + el.dsdxii[k] * s[4][k];
+                Y1[k] = el.dzdxii[k] * s[5][k] + el.dsdxii[k] * s[3][k];
+                Y2[k] = el.dzdxii[k] * s[4][k] + el.dsdxii[k] * s[2][k];
+            }
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    C a0(0, 0), a1(0, 0), a2(0, 0);
+                    for (int m = 0; m < 5; m++) {
+                        Real gx = Gx[i * 5 + m];   // (G X)(i,j)
+                        Real gt = GTe[m * 5 + j];  // (Y GT)(i,j)
+                        a0 += gx * X0[m * 5 + j] + Y0[i * 5 + m] * gt;
+                        a1 += gx * X1[m * 5 + j] + Y1[i * 5 + m] * gt;
+                        a2 += gx * X2[m * 5 + j] + Y2[i * 5 + m] * gt;
+                    }
+                    int k = i * 5 + j;
+                    f[0][k] = a0 + el.inv_s[k] * g0[k];
+                    f[1][k] = a1 + el.inv_s[k] * g1[k];
+                    f[2][k] = a2 + el.inv_s[k] * g2[k];
+                }
+            if (el.axial) {
+                for (int i = 0; i < 5; i++)
+                    for (int j = 0; j < 5; j++) {
+                        Real gx0 = Gx[i * 5 + 0];
+                        f[0][i * 5 + j] += gx0 * (el.dzdeta[j] * g0[j]);
+                        f[1][i * 5 + j] += gx0 * (el.dzdeta[j] * g1[j]);
+                        f[2][i * 5 + j] += gx0 * (el.dzdeta[j] * g2[j]);
+                    }
+                if (b == 1) {
+                    for (int j = 0; j < 5; j++) {
+                        C h0(0, 0), h1(0, 0);
+                        for (int m = 0; m < 5; m++) {
+                            h0 += el.dzdxii[m] * g0[m] * GTe[m * 5 + j];
+                            h1 += el.dzdxii[m] * g1[m] * GTe[m * 5 + j];
+                        }
+                        f[0][j] += h0;
+                        f[1][j] += h1;
+                    }
+                }
+            }
+            if (b == 0)
+                for (int c = 0; c < 3; c++)
+                    for (int k = 0; k < 25; k++) f[c][k] = C(f[c][k].real(), zero);
+        }
+        if (nyq)
+            for (int c = 0; c < 3; c++) fi[Nu][c].fill(C(0, 0));
+    }
+
+    // CrdTransTIsoSolid (element/crd/CrdTransTIsoSolid.cpp:14-42)
+    void spz2rtz(const Elem &el, std::vector<std::array<CMat, 6>> &u, int Nu) {
+        for (int a = 0; a <= Nu; a++) {
+            auto &ua = u[a];
+            for (int k = 0; k < 25; k++) {
+                C sum02 = ua[0][k] + ua[2][k], dif02 = ua[0][k] - ua[2][k], u3 = ua[3][k];
+                ua[0][k] = half * (sum02 + el.cos2t[k] * dif02 - el.sin2t[k] * ua[4][k]);
+                ua[2][k] = sum02 - ua[0][k];
+                ua[4][k] = el.cos2t[k] * ua[4][k] + el.sin2t[k] * dif02;
+                ua[3][k] = el.cos1t[k] * u3 + el.sin1t[k] * ua[5][k];
+                ua[5][k] = el.cos1t[k] * ua[5][k] - el.sin1t[k] * u3;
+            }
+        }
+    }
+    void rtz2spz(const Elem &el, std::vector<std::array<CMat, 6>> &u, int Nu) {
+        for (int a = 0; a <= Nu; a++) {
+            auto &ua = u[a];
+            for (int k = 0; k < 25; k++) {
+                C sum02 = ua[0][k] + ua[2][k], dif02 = (ua[0][k] - ua[2][k]) * half, u3 = ua[3][k];
+                ua[0][k] = half * sum02 + el.cos2t[k] * dif02 + el.sin2t[k] * ua[4][k];
+                ua[2][k] = sum02 - ua[0][k];
+                ua[4][k] = el.cos2t[k] * ua[4][k] - el.sin2t[k] * dif02;
+                ua[3][k] = el.cos1t[k] * u3 - el.sin1t[k] * ua[5][k];
+                ua[5][k] = el.cos1t[k] * ua[5][k] + el.sin1t[k] * u3;
+            }
+        }
+    }
+
+    // material on one point (generic over complex 1-D modes and real 3-D samples)
+    template <class T>
+    void hooke(const Elem &el, const Real *mp, int stride, int k, const T e[6], T s[6]) {
+        if (el.mat == AXG_MAT_ISOTROPIC) {
+            // Isotropic1D/3D::strainToStress (elastic/1D/Isotropic1D.cpp:9-26, 3D/Isotropic3D.cpp:10-28)
+            Real lam = mp[k], mu = mp[stride + k], mu2 = two * mu;
+            T sii = lam * (e[0] + e[1] + e[2]);
+            s[0] = sii + mu2 * e[0];
+            s[1] = sii + mu2 * e[1];
+            s[2] = sii + mu2 * e[2];
+            s[3] = mu * e[3];
+            s[4] = mu * e[4];
+            s[5] = mu * e[5];
+        } else if (el.mat == AXG_MAT_TRANSVERSELY_ISO) {
+            // TransverselyIsotropic1D/3D (elastic/1D/TransverselyIsotropic1D.cpp:9-27)
+            Real A = mp[k], Cc = mp[stride + k], Fv = mp[2 * stride + k], L = mp[3 * stride + k], N = mp[4 * stride + k];
+            Real N2 = two * N;
+            T e01 = e[0] + e[1];
+            T tmp = A * e01 + Fv * e[2];
+            s[0] = tmp - N2 * e[1];
+            s[1] = tmp - N2 * e[0];
+            s[2] = Cc * e[2] + Fv * e01;
+            s[3] = L * e[3];
+            s[4] = L * e[4];
+            s[5] = N * e[5];
+        } else {
+            // Anisotropic1D/3D (elastic/1D/Anisotropic1D.cpp:9-54)
+            static const int idx[6][6] = {{0, 1, 2, 3, 4, 5},     {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
+                                          {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
+            for (int a = 0; a < 6; a++) {
+                T acc = mp[idx[a][0] * stride + k] * e[0];
+                for (int b = 1; b < 6; b++) acc = acc + mp[idx[a][b] * stride + k] * e[b];
+                s[a] = acc;
+            }
+        }
+    }
+
+    // Attenuation1D_Full / _CG4 (attenuation/1D/*.cpp)
+    void att1D(Elem &el, std::vector<std::array<CMat, 6>> &stress, const std::vector<std::array<CMat, 6>> &strain,
+               int Nu) {
+        bool cg4 = el.att == AXG_ATT_CG4;
+        int np = cg4 ? 4 : 25;
+        static const int cgk[4] = {6, 8, 16, 18};
+        auto K = [&](int i4) { return cg4 ? cgk[i4] : i4; };
+        // applyToStress
+        for (int s = 0; s < nsls; s++)
+            for (int a = 0; a <= Nu; a++)
+                for (int c = 0; c < 6; c++)
+                    for (int i = 0; i < np; i++) stress[a][c][K(i)] -= el.memvar1[s][(a * 6 + c) * np + i];
+        // updateMemoryVariables
+        for (int s = 0; s < nsls; s++)
+            for (size_t j = 0; j < el.stressR1.size(); j++)
+                el.memvar1[s][j] = slsA[s] * el.memvar1[s][j] + slsB[s] * el.stressR1[j];
+        for (int a = 0; a <= Nu; a++) {
+            for (int i = 0; i < np; i++) {
+                int k = K(i);
+                C e[6];
+                for (int c = 0; c < 6; c++) e[c] = strain[a][c][k];
+                C eii3 = (e[0] + e[1] + e[2]) * third;
+                C *sr = &el.stressR1[(size_t)a * 6 * np];
+                if (doKappa) {
+                    C sii3 = el.dkappa3[i] * eii3;
+                    sr[0 * np + i] = sii3 + el.dmu2[i] * (e[0] - eii3);
+                    sr[1 * np + i] = sii3 + el.dmu2[i] * (e[1] - eii3);
+                    sr[2 * np + i] = sii3 + el.dmu2[i] * (e[2] - eii3);
+                } else {
+                    sr[0 * np + i] = el.dmu2[i] * (e[0] - eii3);
+                    sr[1 * np + i] = el.dmu2[i] * (e[1] - eii3);
+                    sr[2 * np + i] = -(sr[0 * np + i] + sr[1 * np + i]);
+                }
+                sr[3 * np + i] = el.dmu[i] * e[3];
+                sr[4 * np + i] = el.dmu[i] * e[4];
+                sr[5 * np + i] = el.dmu[i] * e[5];
+            }
+        }
+        for (int s = 0; s < nsls; s++)
+            for (size_t j = 0; j < el.stressR1.size(); j++) el.memvar1[s][j] += slsG[s] * el.stressR1[j];
+    }
+
+    // Attenuation3D_Full / _CG4 (attenuation/3D/*.cpp) on the flat physical buffers
+    void att3D(Elem &el, Real *stressR, const Real *strainR, int Nr) {
+        bool cg4 = el.att == AXG_ATT_CG4;
+        int np = cg4 ? 4 : 25;
+        static const int cgk[4] = {6, 8, 16, 18};
+        auto K = [&](int i4) { return cg4 ? cgk[i4] : i4; };
+        // applyToStress
+        for (int s = 0; s < nsls; s++)
+            for (int r = 0; r < Nr; r++)
+                for (int c = 0; c < 6; c++)
+                    for (int i = 0; i < np; i++) stressR[r * 150 + c * 25 + K(i)] -= el.memvar3[s][(r * 6 + c) * np + i];
+        for (int s = 0; s < nsls; s++)
+            for (size_t j = 0; j < el.stressR3.size(); j++)
+                el.memvar3[s][j] = slsA[s] * el.memvar3[s][j] + slsB[s] * el.stressR3[j];
+        for (int r = 0; r < Nr; r++) {
+            for (int i = 0; i < np; i++) {
+                int k = K(i);
+                Real e[6];
+                for (int c = 0; c < 6; c++) e[c] = strainR[r * 150 + c * 25 + k];
+                Real eii3 = third * (e[0] + e[1] + e[2]);
+                Real *sr = &el.stressR3[(size_t)r * 6 * np];
+                int ii = r * np + i;
+                if (doKappa) {
+                    Real sii = el.dkappa3[ii] * eii3;
+                    sr[0 * np + i] = sii + el.dmu2[ii] * (e[0] - eii3);
+                    sr[1 * np + i] = sii + el.dmu2[ii] * (e[1] - eii3);
+                    sr[2 * np + i] = sii + el.dmu2[ii] * (e[2] - eii3);
+                } else {
+                    sr[0 * np + i] = el.dmu2[ii] * (e[0] - eii3);
+                    sr[1 * np + i] = el.dmu2[ii] * (e[1] - eii3);
+                    sr[2 * np + i] = -(sr[0 * np + i] + sr[1 * np + i]);
+                }
+                sr[3 * np + i] = el.dmu[ii] * e[3];
+                sr[4 * np + i] = el.dmu[ii] * e[4];
+                sr[5 * np + i] = el.dmu[ii] * e[5];
+            }
+        }
+        for (int s = 0; s < nsls; s++)
+            for (size_t j = 0; j < el.stressR3.size(); j++) el.memvar3[s][j] += slsG[s] * el.stressR3[j];
+    }
+
+    void ensureWork(Work &w, int nu, int nr) {
+        if ((int)w.displ.size() < nu + 1) {
+            w.displ.resize(nu + 1);
+            w.stiff.resize(nu + 1);
+            w.strain6.resize(nu + 1);
+            w.stress6.resize(nu + 1);
+            w.fdispl.resize(nu + 1);
+            w.fstiff.resize(nu + 1);
+            w.fstrain.resize(nu + 1);
+            w.fstress.resize(nu + 1);
+        }
+        if ((int)w.flatR.size() < nr * 150) {
+            w.flatR.resize((size_t)nr * 150);
+            w.flatS.resize((size_t)nr * 150);
+        }
+        if ((int)w.colC.size() < nr) {
+            w.colC.resize(nr);
+            w.colR.resize(nr);
+        }
+    }
+
+    // SolidElement::computeStiff / displToStiff (element/SolidElement.cpp:43-65, 404-443)
+    void solidStiff(Elem &el, Work &w) {
+        int Nr = el.nr, Nu = el.nu, nyq = (Nr % 2 == 0) ? 1 : 0;
+        ensureWork(w, Nu, Nr);
+        // SolidPoint::scatterDisplToElement (SolidPoint.cpp:175-195)
+        for (int k = 0; k < 25; k++) {
+            const SolidPt &p = S[el.pts[k]];
+            int pn = (p.nr % 2 == 0) ? 1 : 0;
+            for (int a = 0; a <= el.maxNu; a++)
+                for (int c = 0; c < 3; c++)
+                    w.displ[a][c][k] = (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0);
+        }
+        grad6(el, w.displ, w.strain6, Nu, nyq);
+        if (el.tiso) spz2rtz(el, w.strain6, Nu);
+        if (!el.is3D) {
+            for (int a = 0; a <= Nu; a++)
+                for (int k = 0; k < 25; k++) {
+                    C e[6], s[6];
+                    for (int c = 0; c < 6; c++) e[c] = w.strain6[a][c][k];
+                    hooke<C>(el, el.matv.data(), 25, k, e, s);
+                    for (int c = 0; c < 6; c++) w.stress6[a][c][k] = s[c];
+                }
+            if (el.att != AXG_ATT_NONE) att1D(el, w.stress6, w.strain6, Nu);
+        } else {
+            // FieldFFT::transformF2P (element/fft/FieldFFT.cpp:19-25): columns 25*c + k
+            for (int c = 0; c < 6; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int a = 0; a <= Nu; a++) w.colC[a] = w.strain6[a][c][k];
+                    c2r<Real>(Nr, w.colC.data(), w.colR.data(), w.wa, w.wb);
+                    for (int r = 0; r < Nr; r++) w.flatR[r * 150 + c * 25 + k] = w.colR[r];
+                }
+            // Elastic3D::strainToStress on every sample
+            for (int r = 0; r < Nr; r++)
+                for (int k = 0; k < 25; k++) {
+                    Real e[6], s[6];
+                    for (int c = 0; c < 6; c++) e[c] = w.flatR[r * 150 + c * 25 + k];
+                    hooke<Real>(el, el.matv.data() + (size_t)r * 25, Nr * 25, k, e, s);
+                    for (int c = 0; c < 6; c++) w.flatS[r * 150 + c * 25 + k] = s[c];
+                }
+            if (el.att != AXG_ATT_NONE) att3D(el, w.flatS.data(), w.flatR.data(), Nr);
+            // FieldFFT::transformP2F (FieldFFT.cpp:43-49)
+            for (int c = 0; c < 6; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int r = 0; r < Nr; r++) w.colR[r] = w.flatS[r * 150 + c * 25 + k];
+                    r2c<Real>(Nr, w.colR.data(), w.colC.data(), w.wa, w.wb);
+                    for (int a = 0; a <= Nu; a++) w.stress6[a][c][k] = w.colC[a];
+                }
+        }
+        if (el.tiso) rtz2spz(el, w.stress6, Nu);
+        quad6(el, w.stiff, w.stress6, Nu, nyq);
+        // SolidPoint::gatherStiffFromElement (SolidPoint.cpp:197-209)
+        for (int k = 0; k < 25; k++) {
+            SolidPt &p = S[el.pts[k]];
+            int pn = (p.nr % 2 == 0) ? 1 : 0;
+            for (int a = 0; a <= p.nu - pn; a++)
+                for (int c = 0; c < 3; c++) p.stiff[a * 3 + c] -= w.stiff[a][c][k];
+            if (pn)
+                for (int c = 0; c < 3; c++) p.stiff[p.nu * 3 + c] = C(0, 0);
+        }
+    }
+
+    // FluidElement::computeStiff / displToStiff (element/FluidElement.cpp:43-65, 333-355)
+    void fluidStiff(Elem &el, Work &w) {
+        int Nr = el.nr, Nu = el.nu, nyq = (Nr % 2 == 0) ? 1 : 0;
+        ensureWork(w, Nu, Nr);
+        for (int k = 0; k < 25; k++) {
+            const FluidPt &p = F[el.pts[k]];
+            int pn = (p.nr % 2 == 0) ? 1 : 0;
+            for (int a = 0; a <= el.maxNu; a++) w.fdispl[a][k] = (a <= p.nu - pn) ? p.displ[a] : C(0, 0);
+        }
+        const RMat &GTx = el.axial ? GT_GLJ : GT_GLL;
+        const RMat &Gx = el.axial ? G_GLJ : G_GLL;
+        const RMat &Ge = G_GLL;
+        const RMat &GTe = GT_GLL;
+        C ii(zero, one);
+        // Gradient::computeGrad (Gradient.cpp:26-56)
+        for (int a = 0; a <= Nu - nyq; a++) {
+            const CMat &u = w.fdispl[a];
+            C ia = (Real)a * ii;
+            CMat GU, UG, v;
+            for (int k = 0; k < 25; k++) v[k] = ia * u[k];
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    C x(0, 0), y(0, 0);
+                    for (int m = 0; m < 5; m++) {
+                        x += GTx[i * 5 + m] * u[m * 5 + j];
+                        y += u[i * 5 + m] * Ge[m * 5 + j];
+                    }
+                    GU[i * 5 + j] = x;
+                    UG[i * 5 + j] = y;
+                }
+            auto &e = w.fstrain[a];
+            for (int k = 0; k < 25; k++) {
+                e[0][k] = el.dzdeta[k] * GU[k] + el.dzdxii[k] * UG[k];
+                e[1][k] = el.inv_s[k] * v[k];
+                e[2][k] = el.dsdeta[k] * GU[k] + el.dsdxii[k] * UG[k];
+            }
+            if (a == 0)
+                for (int c = 0; c < 3; c++)
+                    for (int k = 0; k < 25; k++) e[c][k] = C(e[c][k].real(), zero);
+            if (el.axial && a > 0) {
+                for (int j = 0; j < 5; j++) {
+                    C d(0, 0);
+                    for (int m = 0; m < 5; m++) d += GTx[m] * v[m * 5 + j];
+                    e[1][j] += el.dzdeta[j] * d;
+                }
+            }
+        }
+        if (nyq)
+            for (int c = 0; c < 3; c++) w.fstrain[Nu][c].fill(C(0, 0));
+        if (!el.is3D) {
+            // Acoustic1D::strainToStress (acoustic/Acoustic1D.cpp:8-16)
+            for (int a = 0; a <= Nu; a++)
+                for (int c = 0; c < 3; c++)
+                    for (int k = 0; k < 25; k++) w.fstress[a][c][k] = el.matv[k] * w.fstrain[a][c][k];
+        } else {
+            // FieldFFT (N3) + Acoustic3D::strainToStress (acoustic/Acoustic3D.cpp:9-16)
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int a = 0; a <= Nu; a++) w.colC[a] = w.fstrain[a][c][k];
+                    c2r<Real>(Nr, w.colC.data(), w.colR.data(), w.wa, w.wb);
+                    for (int r = 0; r < Nr; r++) w.colR[r] *= el.matv[(size_t)r * 25 + k];
+                    r2c<Real>(Nr, w.colR.data(), w.colC.data(), w.wa, w.wb);
+                    for (int a = 0; a <= Nu; a++) w.fstress[a][c][k] = w.colC[a];
+                }
+        }
+        // Gradient::computeQuad (Gradient.cpp:58-82)
+        for (int b = 0; b <= Nu - nyq; b++) {
+            const auto &s = w.fstress[b];
+            C ib = -(Real)b * ii;
+            CMat g, X, Y;
+            for (int k = 0; k < 25; k++) {
+                g[k] = ib * s[1][k];
+                X[k] = el.dzdeta[k] * s[0][k] + el.dsdeta[k] * s[2][k];
+                Y[k] = el.dzdxii[k] * s[0][k] + el.dsdxii[k] * s[2][k];
+            }
+            CMat &f = w.fstiff[b];
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    C acc(0, 0);
+                    for (int m = 0; m < 5; m++) acc += Gx[i * 5 + m] * X[m * 5 + j] + Y[i * 5 + m] * GTe[m * 5 + j];
+                    f[i * 5 + j] = acc + el.inv_s[i * 5 + j] * g[i * 5 + j];
+                }
+            if (el.axial && b > 0)
+                for (int i = 0; i < 5; i++)
+                    for (int j = 0; j < 5; j++) f[i * 5 + j] += Gx[i * 5 + 0] * (el.dzdeta[j] * g[j]);
+            if (b == 0)
+                for (int k = 0; k < 25; k++) f[k] = C(f[k].real(), zero);
+        }
+        if (nyq) w.fstiff[Nu].fill(C(0, 0));
+        // FluidPoint::gatherStiffFromElement (FluidPoint.cpp:183-195)
+        for (int k = 0; k < 25; k++) {
+            FluidPt &p = F[el.pts[k]];
+            int pn = (p.nr % 2 == 0) ? 1 : 0;
+            for (int a = 0; a <= p.nu - pn; a++) p.stiff[a] -= w.fstiff[a][k];
+            if (pn) p.stiff[p.nu] = C(0, 0);
+        }
+    }
+
+    template <class Fn>
+    void parallel(int n, Fn fn) {
+        if (nthread <= 1 || n < 256) {
+            fn(0, n, 0);
+            return;
+        }
+        std::vector<std::thread> th;
+        int chunk = (n + nthread - 1) / nthread;
+        for (int t = 0; t < nthread; t++) {
+            int b = t * chunk, e = std::min(n, b + chunk);
+            if (b >= e) break;
+            th.emplace_back([&, b, e, t]() {
+                set_ftz();
+                fn(b, e, t);
+            });
+        }
+        for (auto &x : th) x.join();
+    }
+
+    // Domain::computeStiff (core/domain/Domain.cpp:82-94).  One thread: element
+    // order, exactly as the reference.  Several threads: colour by colour.
+    void computeStiff() {
+        if (nthread <= 1) {
+            for (auto &el : E) el.fluid ? fluidStiff(el, works[0]) : solidStiff(el, works[0]);
+            return;
+        }
+        for (auto &col : colours) {
+            parallel((int)col.size(), [&](int b, int e, int t) {
+                for (int i = b; i < e; i++) {
+                    Elem &el = E[col[i]];
+                    el.fluid ? fluidStiff(el, works[t]) : solidStiff(el, works[t]);
+                }
+            });
+        }
+    }
+
+    // SolidElement::computeGroundMotion + PointwiseRecorder::record
+    // (SolidElement.cpp:189-216, output/pointwise/PointwiseRecorder.cpp:62-93)
+    void record(int tstep) {
+        C ii(zero, one);
+        for (size_t ir = 0; ir < recs.size(); ir++) {
+            const Rec &rc = recs[ir];
+            const Elem &el = E[rc.elem];
+            Real u[3] = {0, 0, 0};
+            for (int k = 0; k < 25; k++) {
+                if (std::abs(rc.w[k]) < 1e-10) continue;
+                const SolidPt &p = S[el.pts[k]];
+                int pn = (p.nr % 2 == 0) ? 1 : 0;
+                auto D = [&](int a, int c) { return (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0); };
+                Real up[3] = {D(0, 0).real(), D(0, 1).real(), D(0, 2).real()};
+                for (int a = 1; a <= el.maxNu - (int)(el.nr % 2 == 0); a++) {
+                    C expval = two * std::exp((Real)a * rc.phi * ii);
+                    for (int c = 0; c < 3; c++) up[c] += (expval * D(a, c)).real();
+                }
+                for (int c = 0; c < 3; c++) u[c] += rc.w[k] * up[c];
+            }
+            double *o = &seis[((size_t)tstep * recs.size() + ir) * 3];
+            if (rtz) {
+                Real cost = (Real)std::cos(rc.theta), sint = (Real)std::sin(rc.theta);
+                Real ur = u[0] * sint + u[2] * cost;
+                Real ut = u[0] * cost - u[2] * sint;
+                o[0] = ut;
+                o[1] = u[1];
+                o[2] = ur;
+            } else {
+                o[0] = u[0];
+                o[1] = u[1];
+                o[2] = u[2];
+            }
+        }
+    }
+
+    int64_t checkStability() {
+        int64_t bad = -1;
+        for (auto &p : S)
+            for (auto &v : p.displ)
+                if (!std::isfinite(v.real()) || !std::isfinite(v.imag())) {
+                    if (bad < 0 || p.gidx < bad) bad = p.gidx;
+                    break;
+                }
+        for (auto &p : F)
+            for (auto &v : p.displ)
+                if (!std::isfinite(v.real()) || !std::isfinite(v.imag())) {
+                    if (bad < 0 || p.gidx < bad) bad = p.gidx;
+                    break;
+                }
+        return bad;
+    }
+
+    void resetZero() {
+        for (auto &p : S) {
+            std::fill(p.displ.begin(), p.displ.end(), C(0, 0));
+            p.veloc = p.accel = p.stiff = p.displ;
+        }
+        for (auto &p : F) {
+            std::fill(p.displ.begin(), p.displ.end(), C(0, 0));
+            p.veloc = p.accel = p.stiff = p.displ;
+        }
+        for (auto &el : E) {
+            for (auto &v : el.stressR1) v = C(0, 0);
+            for (auto &m : el.memvar1)
+                for (auto &v : m) v = C(0, 0);
+            for (auto &v : el.stressR3) v = 0;
+            for (auto &m : el.memvar3)
+                for (auto &v : m) v = 0;
+        }
+        std::fill(seis.begin(), seis.end(), 0.);
+    }
+
+    // Newmark::solve loop body (core/newmark/Newmark.cpp:47-93)
+    void run(int first, int count, int stab) {
+        typedef std::chrono::steady_clock clk;
+        for (int i = 0; i < 5; i++) cost[i] = 0;
+        for (int tstep = first; tstep < first + count; tstep++) {
+            auto t0 = clk::now();
+            updateNewmark(dt);
+            auto t1 = clk::now();
+            applySource(tstep - 1);
+            computeStiff();
+            auto t2 = clk::now();
+            coupleSolidFluid();
+            auto t3 = clk::now();
+            record(tstep - 1);
+            if (stab > 0 && tstep % stab == 0 && checkStability() >= 0)
+                throw std::runtime_error("Domain::checkStability || Simulation blew up.");
+            auto t4 = clk::now();
+            cost[1] += std::chrono::duration<double>(t1 - t0).count() + std::chrono::duration<double>(t3 - t2).count();
+            cost[0] += std::chrono::duration<double>(t2 - t1).count();
+            cost[4] += std::chrono::duration<double>(t4 - t3).count();
+        }
+    }
+
+    // canonical field layout
+    int64_t fieldSize(int which) const {
+        int64_t n = 0;
+        if (which < 4)
+            for (auto &p : S) n += (int64_t)p.displ.size() * 2;
+        else
+            for (auto &p : F) n += (int64_t)p.displ.size() * 2;
+        return n;
+    }
+    template <class Fn>
+    void forField(int which, Fn fn) {
+        int f = which % 4;
+        if (which < 4) {
+            for (auto &p : S) fn(f == 0 ? p.displ : f == 1 ? p.veloc : f == 2 ? p.accel : p.stiff);
+        } else {
+            for (auto &p : F) fn(f == 0 ? p.displ : f == 1 ? p.veloc : f == 2 ? p.accel : p.stiff);
+        }
+    }
+};
+
+struct Base {
+    virtual ~Base() {}
+};
+template <class Real>
+struct Holder : Base {
+    Ref<Real> r;
+};
+
+}  // namespace
+
+struct refcpu {
+    int dbl = 0;
+    std::unique_ptr<Base> b;
+    template <class Real>
+    Ref<Real> &get() {
+        return static_cast<Holder<Real> *>(b.get())->r;
+    }
+};
+
+#define DISPATCH(h, ...)                         \
+    do {                                         \
+        if ((h)->dbl) {                          \
+            auto &R = (h)->get<double>();        \
+            __VA_ARGS__;                         \
+        } else {                                 \
+            auto &R = (h)->get<float>();         \
+            __VA_ARGS__;                         \
+        }                                        \
+    } while (0)
+
+#define GUARD(...)                         \
+    try {                                  \
+        set_ftz();                         \
+        __VA_ARGS__;                       \
+        return 0;                          \
+    } catch (const std::exception &e) {    \
+        g_err = e.what();                  \
+        return 1;                          \
+    }
+
+extern "C" {
+
+int refcpu_create(const axg_mesh_desc *desc, int use_double, refcpu **h) {
+    GUARD({
+        std::unique_ptr<refcpu> x(new refcpu());
+        x->dbl = use_double;
+        if (use_double) {
+            auto *hd = new Holder<double>();
+            x->b.reset(hd);
+            hd->r.init(desc);
+        } else {
+            auto *hf = new Holder<float>();
+            x->b.reset(hf);
+            hf->r.init(desc);
+        }
+        *h = x.release();
+    })
+}
+
+void refcpu_destroy(refcpu *h) { delete h; }
+
+int refcpu_set_threads(refcpu *h, int n) { GUARD(DISPATCH(h, R.setThreads(n))) }
+int refcpu_reset_zero(refcpu *h) { GUARD(DISPATCH(h, R.resetZero())) }
+int refcpu_update_newmark(refcpu *h, double dt) { GUARD(DISPATCH(h, R.updateNewmark(dt))) }
+int refcpu_apply_source(refcpu *h, int32_t tstep) { GUARD(DISPATCH(h, R.applySource(tstep))) }
+int refcpu_compute_stiff(refcpu *h) { GUARD(DISPATCH(h, R.computeStiff())) }
+int refcpu_couple_solid_fluid(refcpu *h) { GUARD(DISPATCH(h, R.coupleSolidFluid())) }
+int refcpu_record(refcpu *h, int32_t tstep, double t) {
+    (void)t;
+    GUARD(DISPATCH(h, R.record(tstep)))
+}
+int refcpu_check_stability(refcpu *h, int64_t *bad) {
+    GUARD(DISPATCH(h, *bad = R.checkStability()))
+}
+int refcpu_run(refcpu *h, int32_t first, int32_t count, int32_t stab) {
+    GUARD(DISPATCH(h, R.run(first, count, stab)))
+}
+int refcpu_get_seismograms(refcpu *h, double *out, int64_t n) {
+    GUARD(DISPATCH(h, {
+        if ((int64_t)R.seis.size() > n) throw std::runtime_error("refcpu || buffer too small");
+        std::copy(R.seis.begin(), R.seis.end(), out);
+    }))
+}
+int refcpu_get_field(refcpu *h, int32_t which, double *out, int64_t n) {
+    GUARD(DISPATCH(h, {
+        if (R.fieldSize(which) != n) throw std::runtime_error("refcpu || size mismatch");
+        int64_t i = 0;
+        R.forField(which, [&](auto &v) {
+            for (auto &z : v) {
+                out[i++] = z.real();
+                out[i++] = z.imag();
+            }
+        });
+    }))
+}
+int refcpu_set_field(refcpu *h, int32_t which, const double *in, int64_t n) {
+    GUARD(DISPATCH(h, {
+        if (R.fieldSize(which) != n) throw std::runtime_error("refcpu || size mismatch");
+        int64_t i = 0;
+        R.forField(which, [&](auto &v) {
+            for (auto &z : v) {
+                z = {(decltype(z.real()))in[i], (decltype(z.real()))in[i + 1]};
+                i += 2;
+            }
+        });
+    }))
+}
+int refcpu_last_run_cost(refcpu *h, double *c5) { GUARD(DISPATCH(h, for (int i = 0; i < 5; i++) c5[i] = R.cost[i])) }
+
+int refcpu_fft_c2r(int nr, int use_double, const double *in_c, double *out_r) {
+    GUARD({
+        int nc = nr / 2 + 1;
+        if (use_double) {
+            std::vector<Cx<double>> X(nc), wa, wb;
+            std::vector<double> x(nr);
+            for (int k = 0; k < nc; k++) X[k] = {in_c[2 * k], in_c[2 * k + 1]};
+            c2r<double>(nr, X.data(), x.data(), wa, wb);
+            for (int i = 0; i < nr; i++) out_r[i] = x[i];
+        } else {
+            std::vector<Cx<float>> X(nc), wa, wb;
+            std::vector<float> x(nr);
+            for (int k = 0; k < nc; k++) X[k] = {(float)in_c[2 * k], (float)in_c[2 * k + 1]};
+            c2r<float>(nr, X.data(), x.data(), wa, wb);
+            for (int i = 0; i < nr; i++) out_r[i] = x[i];
+        }
+    })
+}
+int refcpu_fft_r2c(int nr, int use_double, const double *in_r, double *out_c) {
+    GUARD({
+        int nc = nr / 2 + 1;
+        if (use_double) {
+            std::vector<Cx<double>> X(nc), wa, wb;
+            std::vector<double> x(in_r, in_r + nr);
+            r2c<double>(nr, x.data(), X.data(), wa, wb);
+            for (int k = 0; k < nc; k++) {
+                out_c[2 * k] = X[k].real();
+                out_c[2 * k + 1] = X[k].imag();
+            }
+        } else {
+            std::vector<Cx<float>> X(nc), wa, wb;
+            std::vector<float> x(nr);
+            for (int i = 0; i < nr; i++) x[i] = (float)in_r[i];
+            r2c<float>(nr, x.data(), X.data(), wa, wb);
+            for (int k = 0; k < nc; k++) {
+                out_c[2 * k] = X[k].real();
+                out_c[2 * k + 1] = X[k].imag();
+            }
+        }
+    })
+}
+const char *refcpu_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""ctypes wrapper of oracle/_build/librefcpu.so (test infrastructure only)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librefcpu.so")
+_lib = None
+
+f64p = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("oracle: %s is missing; run oracle/build.py" % LIB_PATH)
+        # the mesh descriptor type lives in libaxg; load it first so ctypes types match
+        import axisem3d_amd._abi as abi
+        l = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "refcpu_create": ([C.POINTER(abi.MeshDesc), C.c_int, C.POINTER(vp)], C.c_int),
+            "refcpu_destroy": ([vp], None),
+            "refcpu_set_threads": ([vp, C.c_int], C.c_int),
+            "refcpu_reset_zero": ([vp], C.c_int),
+            "refcpu_update_newmark": ([vp, C.c_double], C.c_int),
+            "refcpu_apply_source": ([vp, C.c_int32], C.c_int),
+            "refcpu_compute_stiff": ([vp], C.c_int),
+            "refcpu_couple_solid_fluid": ([vp], C.c_int),
+            "refcpu_record": ([vp, C.c_int32, C.c_double], C.c_int),
+            "refcpu_check_stability": ([vp, C.POINTER(C.c_int64)], C.c_int),
+            "refcpu_run": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+            "refcpu_get_seismograms": ([vp, f64p, C.c_int64], C.c_int),
+            "refcpu_get_field": ([vp, C.c_int32, f64p, C.c_int64], C.c_int),
+            "refcpu_set_field": ([vp, C.c_int32, f64p, C.c_int64], C.c_int),
+            "refcpu_last_run_cost": ([vp, f64p], C.c_int),
+            "refcpu_fft_c2r": ([C.c_int, C.c_int, f64p, f64p], C.c_int),
+            "refcpu_fft_r2c": ([C.c_int, C.c_int, f64p, f64p], C.c_int),
+            "refcpu_last_error": ([], C.c_char_p),
+        }
+        for name, (a, r) in sig.items():
+            fn = getattr(l, name)
+            fn.argtypes = a
+            fn.restype = r
+        _lib = l
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().refcpu_last_error().decode())
+
+
+class RefDomain:
+    """The oracle's Domain on the same mesh descriptor as the GPU path."""
+
+    def __init__(self, mesh, double=False, threads=1):
+        self.mesh = mesh
+        h = C.c_void_p()
+        _check(lib().refcpu_create(mesh.desc, int(bool(double)), C.byref(h)))
+        self._h = h
+        d = mesh.d
+        self.nstep, self.nrec, self.dt, self.t_shift = d.nstep, d.nrec, d.dt, d.t_shift
+        if threads != 1:
+            self.set_threads(threads)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().refcpu_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def set_threads(self, n):
+        _check(lib().refcpu_set_threads(self._h, n))
+
+    def resetZero(self):
+        _check(lib().refcpu_reset_zero(self._h))
+
+    def updateNewmark(self, dt):
+        _check(lib().refcpu_update_newmark(self._h, dt))
+
+    def applySource(self, tstep):
+        _check(lib().refcpu_apply_source(self._h, tstep))
+
+    def computeStiff(self):
+        _check(lib().refcpu_compute_stiff(self._h))
+
+    def coupleSolidFluid(self):
+        _check(lib().refcpu_couple_solid_fluid(self._h))
+
+    def record(self, tstep, t=0.):
+        _check(lib().refcpu_record(self._h, tstep, t))
+
+    def checkStability(self):
+        bad = C.c_int64(-1)
+        _check(lib().refcpu_check_stability(self._h, C.byref(bad)))
+        return bad.value
+
+    def run(self, first, count, stab_interval=0):
+        _check(lib().refcpu_run(self._h, first, count, stab_interval))
+
+    def cost(self):
+        c = (C.c_double * 5)()
+        _check(lib().refcpu_last_run_cost(self._h, c))
+        return dict(zip(("ELEMENT", "POINT", "ASSEMBLE", "WAIT", "OTHER"), list(c)))
+
+    def seismograms(self):
+        n = self.nstep * self.nrec * 3
+        out = np.zeros(max(n, 1))
+        _check(lib().refcpu_get_seismograms(self._h, out.ctypes.data_as(f64p), out.size))
+        return out[:n].reshape(self.nstep, self.nrec, 3)
+
+    def get_field(self, which):
+        n = self.mesh.field_size(which)
+        out = np.zeros(n)
+        _check(lib().refcpu_get_field(self._h, which, out.ctypes.data_as(f64p), n))
+        return out
+
+    def set_field(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(lib().refcpu_set_field(self._h, which, a.ctypes.data_as(f64p), a.size))
+
+
+def fft_c2r(X, nr, double=True):
+    X = np.ascontiguousarray(np.stack([np.real(X), np.imag(X)], -1).ravel(), dtype=np.float64)
+    out = np.zeros(nr)
+    _check(lib().refcpu_fft_c2r(nr, int(double), X.ctypes.data_as(f64p), out.ctypes.data_as(f64p)))
+    return out
+
+
+def fft_r2c(x, nr, double=True):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(2 * (nr // 2 + 1))
+    _check(lib().refcpu_fft_r2c(nr, int(double), x.ctypes.data_as(f64p), out.ctypes.data_as(f64p)))
+    return out[0::2] + 1j * out[1::2]

@@ -1,0 +1,44 @@
+"""Shared helpers of the test-suite."""
+import numpy as np
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def random_field(mesh, which, seed, scale=1e-6):
+    """Random complex point field in the canonical layout, masked like the
+    reference masks displacements (imaginary alpha=0, axis, Nyquist)."""
+    import axisem3d_amd._abi as abi
+    rng = np.random.default_rng(seed)
+    n = mesh.field_size(which)
+    x = rng.standard_normal(n) * scale
+    nr = mesh.array("point_nr")
+    fl = mesh.array("point_flags")
+    solid = which < 4
+    ncomp = 3 if solid else 1
+    pos = 0
+    for p in range(mesh.d.npoint):
+        if not (fl[p] & (abi.PT_SOLID if solid else abi.PT_FLUID)):
+            continue
+        nu = nr[p] // 2
+        blk = x[pos:pos + (nu + 1) * ncomp * 2].reshape(nu + 1, ncomp, 2)
+        blk[0, :, 1] = 0.
+        if fl[p] & abi.PT_AXIAL:
+            if solid:
+                blk[0, 0:2, :] = 0.
+                if nu >= 1:
+                    blk[1, 2, :] = 0.
+                    # a field that survives the axis rotation: u_phi = i u_s
+                    blk[1, 1, 0] = -blk[1, 0, 1]
+                    blk[1, 1, 1] = blk[1, 0, 0]
+                blk[2:, :, :] = 0.
+            else:
+                blk[1:, :, :] = 0.
+        if nr[p] % 2 == 0:
+            blk[nu] = 0.
+        pos += (nu + 1) * ncomp * 2
+    return x

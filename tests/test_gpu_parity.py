@@ -1,0 +1,113 @@
+"""GPU (HIP, gfx950) against the CPU oracle on the same seeded meshes.
+
+Tolerances: FP32 station seismograms within 1e-4 relative L2 of the oracle's
+FP32 run (the project's parity bar); single-operator outputs within 1e-5
+relative L2 (one step of FP32 arithmetic in a different summation order).
+"""
+import numpy as np
+import pytest
+
+import axisem3d_amd as ax
+from axisem3d_amd import _abi
+from oracle.refcpu import RefDomain
+from tests.helpers import rel_l2, random_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh_small(layers, **kw):
+    cfg = ax.shell_config(0, layers=layers, nelem_theta=kw.pop("nelem_theta", 12), **kw)
+    return ax.ShellMesh(cfg)
+
+
+@pytest.mark.parametrize("material,nu", [(_abi.MAT_ISOTROPIC, 4), (_abi.MAT_TRANSVERSELY_ISO, 6),
+                                         (_abi.MAT_ANISOTROPIC, 5)])
+def test_compute_stiff_1d_operator(small_layers, material, nu):
+    print("case compute_stiff material=%d nu=%d" % (material, nu), flush=True)
+    m = _mesh_small(small_layers, material=material, nu_const=nu, nstep=10)
+    g = ax.Domain(m)
+    r = RefDomain(m, double=True)
+    u = random_field(m, _abi.FIELD_SOLID_DISPL, seed=1)
+    g.set_field(_abi.FIELD_SOLID_DISPL, u)
+    r.set_field(_abi.FIELD_SOLID_DISPL, u)
+    g.computeStiff()
+    r.computeStiff()
+    sg = g.get_field(_abi.FIELD_SOLID_STIFF)
+    sr = r.get_field(_abi.FIELD_SOLID_STIFF)
+    assert np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-5
+
+
+def test_newmark_operator(small_layers):
+    print("case newmark", flush=True)
+    m = _mesh_small(small_layers, nstep=10)
+    g = ax.Domain(m)
+    r = RefDomain(m, double=False)
+    for i, w in enumerate((_abi.FIELD_SOLID_DISPL, _abi.FIELD_SOLID_VELOC, _abi.FIELD_SOLID_ACCEL)):
+        x = random_field(m, w, seed=10 + i)
+        g.set_field(w, x)
+        r.set_field(w, x)
+    st = np.random.default_rng(5).standard_normal(m.field_size(_abi.FIELD_SOLID_STIFF)) * 1e3
+    g.set_field(_abi.FIELD_SOLID_STIFF, st)
+    r.set_field(_abi.FIELD_SOLID_STIFF, st)
+    g.updateNewmark(m.d.dt)
+    r.updateNewmark(m.d.dt)
+    for w in (_abi.FIELD_SOLID_DISPL, _abi.FIELD_SOLID_VELOC, _abi.FIELD_SOLID_ACCEL, _abi.FIELD_SOLID_STIFF):
+        a, b = g.get_field(w), r.get_field(w)
+        assert rel_l2(a, b) < 1e-6, w
+
+
+@pytest.mark.parametrize("material", [_abi.MAT_ISOTROPIC, _abi.MAT_TRANSVERSELY_ISO])
+def test_seismograms_small(small_layers, material):
+    print("case seismograms_small material=%d" % material, flush=True)
+    m = _mesh_small(small_layers, material=material, nu_const=4, nstep=600, nrec=6)
+    g = ax.Domain(m)
+    ax.Newmark(g, 100, 100).solve()
+    sg = g.seismograms()
+    r32 = RefDomain(m, double=False)
+    r32.run(1, m.d.nstep, 100)
+    r64 = RefDomain(m, double=True)
+    r64.run(1, m.d.nstep, 100)
+    s32, s64 = r32.seismograms(), r64.seismograms()
+    assert np.isfinite(sg).all() and np.abs(sg).max() > 0
+    e32 = rel_l2(sg, s32)
+    e64 = rel_l2(sg, s64)
+    print("gpu vs oracle fp32 %.3e, vs fp64 %.3e, oracle fp32 vs fp64 %.3e" % (e32, e64, rel_l2(s32, s64)))
+    assert e32 < 1e-4
+    assert e64 < 1e-3
+
+
+def test_per_operator_driver_matches_batched(small_layers):
+    print("case per_operator", flush=True)
+    m = _mesh_small(small_layers, nstep=50, nrec=4)
+    g1 = ax.Domain(m)
+    ax.Newmark(g1, 10, 10).solve(per_operator=True)
+    g2 = ax.Domain(m)
+    ax.Newmark(g2, 10, 10).solve()
+    assert np.array_equal(g1.seismograms(), g2.seismograms())
+
+
+def test_fluid_solid_seismograms(fluid_layers):
+    print("case fluid_solid", flush=True)
+    cfg = ax.shell_config(1, layers=fluid_layers, nelem_theta=8, material=_abi.MAT_ISOTROPIC, dv_amp=0.,
+                          nu_mode=0, nu_const=3, nstep=400, nrec=5)
+    m = ax.ShellMesh(cfg)
+    assert m.stats["nsf"] > 0 and m.stats["nfluid_elem"] > 0
+    g = ax.Domain(m)
+    ax.Newmark(g, 100, 100).solve()
+    r = RefDomain(m, double=False)
+    r.run(1, m.d.nstep, 100)
+    e = rel_l2(g.seismograms(), r.seismograms())
+    print("fluid-solid gpu vs oracle fp32 %.3e" % e)
+    assert e < 1e-4
+
+
+def test_stability_reports_point(small_layers):
+    print("case stability", flush=True)
+    m = _mesh_small(small_layers, nstep=10)
+    g = ax.Domain(m)
+    x = np.zeros(m.field_size(_abi.FIELD_SOLID_DISPL), dtype=np.float32)
+    x[600] = np.nan
+    g.set_field(_abi.FIELD_SOLID_DISPL, x)
+    with pytest.raises(RuntimeError, match="Domain::checkStability"):
+        g.checkStability(m.d.dt, 1, 0.)
