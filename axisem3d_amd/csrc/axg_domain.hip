@@ -71,6 +71,7 @@ struct DevBuf {
 };
 
 struct Bucket1D {
+    int colour = 0;     // elements of one colour share no GLL point
     int phase = 0;      // 0 solid, 1 fluid
     int material = 0;
     bool tiso = false;
@@ -120,6 +121,7 @@ struct axg_ctx {
     int64_t att_mem_stride = 0;
     int32_t nsls = 0, do_kappa = 0;
     float sls_a[8] = {0}, sls_b[8] = {0}, sls_g[8] = {0};
+    int ncolour = 0;
     std::vector<std::unique_ptr<Bucket1D>> buckets1d;
     std::vector<std::unique_ptr<Bucket3D>> buckets3d;
     DevBuf<float> d_fft_twiddle;
@@ -145,10 +147,14 @@ struct axg_ctx {
     // graph
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+    int graph_chunk = 0;
     DevBuf<int32_t> d_tstep;
 };
 
 namespace {
+
+// Kernel classes timed by the profiling mode of axg_run
+enum { KC_ELEMENT = 0, KC_NEWMARK = 1, KC_OTHER = 2, KC_N = 3 };
 
 void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int phase, int mat, int tiso, int att,
                      const std::vector<int32_t> &elems) {
@@ -249,7 +255,7 @@ Elem1DArgs makeArgs1D(axg_ctx *c, const Bucket1D &b) {
     a.ep_amax = c->d_ep_amax.p;
     Phase &P = b.phase == 0 ? c->S : c->F;
     a.displ = P.displ.p;
-    a.stiff = reinterpret_cast<float *>(P.stiff.p);
+    a.stiff = P.stiff.p;
     a.att_memvar = c->d_att_memvar.p;
     a.att_stressR = c->d_att_stressR.p;
     a.att_mem_off = c->d_att_mem_off.p;
@@ -290,30 +296,35 @@ void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
     }
 }
 
-void doNewmark(axg_ctx *c, double dt) {
+void doNewmark(axg_ctx *c, double dt, int *tstep_counter = nullptr) {
     double half_dt = 0.5 * dt;
     double half_dt_dt = half_dt * dt;
     const int B = 256;
     if (c->S.nrow)
         hipLaunchKernelGGL(newmark_solid_kernel, dim3((c->S.nrow + B - 1) / B), dim3(B), 0, c->stream, c->S.view(),
                            c->S.displ.p, c->S.veloc.p, c->S.accel.p, c->S.stiff.p, (float)dt, (float)half_dt,
-                           (float)half_dt_dt);
+                           (float)half_dt_dt, tstep_counter);
     if (c->F.nrow)
         hipLaunchKernelGGL(newmark_fluid_kernel, dim3((c->F.nrow + B - 1) / B), dim3(B), 0, c->stream, c->F.view(),
                            c->F.displ.p, c->F.veloc.p, c->F.accel.p, c->F.stiff.p, (float)dt, (float)half_dt,
-                           (float)half_dt_dt);
+                           (float)half_dt_dt, c->S.nrow ? nullptr : tstep_counter);
 }
 
-void doSource(axg_ctx *c, int tstep) {
+void doSource(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
     if (c->src_elem < 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::applySource || time step out of range.");
     hipLaunchKernelGGL(source_kernel, dim3(1), dim3(256), 0, c->stream, c->d_src_force.p, c->d_src_row.p,
-                       c->d_src_nu.p, c->d_stf.p, tstep, c->S.stiff.p);
+                       c->d_src_nu.p, c->d_stf.p, tstep, tstep_ptr, c->S.stiff.p);
 }
 
 void doStiff(axg_ctx *c) {
-    for (auto &b : c->buckets1d) launchBucket1D(c, *b);
-    for (auto &b : c->buckets3d) launchBucket3D(c, *b);
+    // colour by colour: launches of one colour are point-disjoint
+    for (int col = 0; col < c->ncolour; col++) {
+        for (auto &b : c->buckets3d)
+            if (b->colour == col) launchBucket3D(c, *b);
+        for (auto &b : c->buckets1d)
+            if (b->colour == col) launchBucket1D(c, *b);
+    }
 }
 
 void doCouple(axg_ctx *c) {
@@ -324,13 +335,12 @@ void doCouple(axg_ctx *c) {
                        c->S.displ.p, c->S.stiff.p, c->F.stiff.p);
 }
 
-void doRecord(axg_ctx *c, int tstep) {
+void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
     if (c->nrec == 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::record || time step out of range.");
     hipLaunchKernelGGL(record_solid_kernel, dim3(c->nrec), dim3(256), 0, c->stream, c->nrec, c->d_rec_elem.p,
                        c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p, c->d_rec_sint.p, c->d_ep_row.p,
-                       c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz,
-                       c->d_seis.p + (int64_t)tstep * c->nrec * 3);
+                       c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz, tstep, tstep_ptr, c->d_seis.p);
 }
 
 int64_t doStability(axg_ctx *c) {
@@ -441,10 +451,29 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
             c->d_att_stressR.zero(s);
         }
     }
-    // buckets
+    // buckets: (colour, phase, material, TIso, attenuation, 1-D/3-D).  Greedy
+    // colouring makes the elements of one launch point-disjoint.
     {
-        std::vector<std::vector<int32_t>> ie(64), ia(64);
-        std::vector<std::vector<int32_t>> e3(64);
+        std::vector<int32_t> colour(NE, -1);
+        {
+            std::vector<std::vector<int32_t>> pe(d->npoint);
+            for (int e = 0; e < NE; e++)
+                for (int k = 0; k < 25; k++) pe[d->elem_points[(size_t)e * 25 + k]].push_back(e);
+            std::vector<char> used;
+            for (int e = 0; e < NE; e++) {
+                used.assign(256, 0);
+                for (int k = 0; k < 25; k++)
+                    for (int o : pe[d->elem_points[(size_t)e * 25 + k]])
+                        if (colour[o] >= 0) used[colour[o]] = 1;
+                int cc = 0;
+                while (cc < 255 && used[cc]) cc++;
+                colour[e] = cc;
+            }
+        }
+        c->ncolour = 1 + *std::max_element(colour.begin(), colour.end());
+        const int NK = 64;
+        std::vector<std::vector<int32_t>> ie((size_t)c->ncolour * NK), ia((size_t)c->ncolour * NK),
+            e3((size_t)c->ncolour * NK);
         for (int e = 0; e < NE; e++) {
             int fl = d->elem_flags[e];
             bool fluid = fl & AXG_EL_FLUID;
@@ -455,7 +484,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
             if ((mat == AXG_MAT_TRANSVERSELY_ISO || mat == AXG_MAT_ANISOTROPIC) && !tiso)
                 throw std::runtime_error("SolidElement::SolidElement || material needs the TIso rotation.");
             int att = d->elem_att[e];
-            int key = (fluid ? 1 : 0) * 32 + mat * 8 + (tiso ? 4 : 0) + att;
+            int key = colour[e] * NK + (fluid ? 1 : 0) * 32 + mat * 8 + (tiso ? 4 : 0) + att;
             if (fl & AXG_EL_3D) {
                 e3[key].push_back(e);
                 continue;
@@ -468,13 +497,15 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 ia[key].push_back(a);
             }
         }
-        for (int key = 0; key < 64; key++) {
+        for (int key = 0; key < c->ncolour * NK; key++) {
+            int kk = key % NK;
             if (!ie[key].empty()) {
                 auto b = std::make_unique<Bucket1D>();
-                b->phase = key / 32;
-                b->material = (key / 8) % 4;
-                b->tiso = (key / 4) % 2;
-                b->att = key % 4;
+                b->colour = key / NK;
+                b->phase = kk / 32;
+                b->material = (kk / 8) % 4;
+                b->tiso = (kk / 4) % 2;
+                b->att = kk % 4;
                 b->nitem = (int32_t)ie[key].size();
                 b->item_elem.upload(ie[key], s);
                 b->item_alpha.upload(ia[key], s);
@@ -482,7 +513,8 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 c->buckets1d.push_back(std::move(b));
             }
             if (!e3[key].empty()) {
-                createBuckets3D(c, d, key / 32, (key / 8) % 4, (key / 4) % 2, key % 4, e3[key]);
+                createBuckets3D(c, d, kk / 32, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
+                c->buckets3d.back()->colour = key / NK;
             }
         }
     }
@@ -597,8 +629,71 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
     HIPCHECK(hipStreamSynchronize(s));
 }
 
-// Kernel classes timed by the profiling mode of axg_run
-enum { KC_ELEMENT = 0, KC_NEWMARK = 1, KC_OTHER = 2, KC_N = 3 };
+// one step of Newmark::solve on the stream; with use_counter the source and
+// receiver kernels take the step index from the device counter (graph replay)
+void enqueueStep(axg_ctx *c, int tstep, bool use_counter) {
+    int *ctr = use_counter ? c->d_tstep.p : nullptr;
+    doNewmark(c, c->dt, ctr);
+    doSource(c, tstep - 1, ctr);
+    doStiff(c);
+    doCouple(c);
+    doRecord(c, tstep - 1, ctr);
+}
+
+// steps per captured graph: divides the stability interval so checks stay on time
+int graphChunk(int stab) {
+    const int maxChunk = 32;
+    if (stab <= 0) return maxChunk;
+    for (int k = std::min(maxChunk, stab); k >= 1; k--)
+        if (stab % k == 0) return k;
+    return 1;
+}
+
+void checkStable(axg_ctx *c, int tstep) {
+    int64_t bad = doStability(c);
+    if (bad >= 0)
+        throw std::runtime_error("Domain::checkStability || Simulation blew up at point " + std::to_string(bad) +
+                                 ", step " + std::to_string(tstep) + ".");
+}
+
+void runStepsGraph(axg_ctx *c, int first, int count, int stab) {
+    int chunk = graphChunk(stab);
+    if (c->graph_exec && c->graph_chunk != chunk) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        (void)hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+    }
+    if (!c->graph_exec && count >= chunk) {
+        HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i < chunk; i++) enqueueStep(c, first + i, true);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(c->stream, &c->graph));
+        HIPCHECK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_chunk = chunk;
+    }
+    int start = first - 2;  // Newmark advances the counter to tstep - 1 first thing
+    HIPCHECK(hipMemcpyAsync(c->d_tstep.p, &start, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    int tstep = first;
+    while (tstep + chunk <= first + count) {
+        HIPCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+        tstep += chunk;
+        if (stab > 0 && (tstep - 1) % stab == 0) checkStable(c, tstep - 1);
+    }
+    for (; tstep < first + count; tstep++) {
+        enqueueStep(c, tstep, true);
+        if (stab > 0 && tstep % stab == 0) checkStable(c, tstep);
+    }
+    // the host copy of `start` must outlive the async copy
+    HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
 
 void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
     std::vector<hipEvent_t> ev;
@@ -617,12 +712,7 @@ void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
         doCouple(c);
         doRecord(c, tstep - 1);
         if (profile) HIPCHECK(hipEventRecord(ev[k + 3], c->stream));
-        if (stab > 0 && tstep % stab == 0) {
-            int64_t bad = doStability(c);
-            if (bad >= 0)
-                throw std::runtime_error("Domain::checkStability || Simulation blew up at point " +
-                                         std::to_string(bad) + ", step " + std::to_string(tstep) + ".");
-        }
+        if (stab > 0 && tstep % stab == 0) checkStable(c, tstep);
     }
     for (int i = 0; i < KC_N; i++) c->class_ms[i] = 0.;
     if (profile) {
@@ -781,11 +871,13 @@ int axg_get_wisdom(axg_ctx *c, int32_t *nu) {
 int axg_run(axg_ctx *c, int32_t first, int32_t count, int32_t stab_interval, int32_t flags) {
     CTX_CHECK(c);
     bool profile = (flags & AXG_RUN_PROFILE) != 0;
+    bool graph = (flags & AXG_RUN_GRAPH) != 0 && !profile;
     if (first < 1 || count < 0 || first + count - 1 > c->nstep)
         return fail(AXG_EINVAL, "Newmark::solve || step range outside the source-time function.");
     try {
         HIPCHECK(hipEventRecord(c->ev0, c->stream));
-        runSteps(c, first, count, stab_interval, profile);
+        if (graph) runStepsGraph(c, first, count, stab_interval);
+        else runSteps(c, first, count, stab_interval, profile);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(c->ev1, c->stream));
         HIPCHECK(hipEventSynchronize(c->ev1));

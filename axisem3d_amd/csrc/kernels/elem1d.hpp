@@ -12,7 +12,8 @@
 //                                  SOLVER/src/core/element/material/elastic/1D/*.cpp
 //   Attenuation1D_Full / _CG4      SOLVER/src/core/element/material/attenuation/1D/*.cpp
 //   Gradient::computeQuad6         Gradient.cpp:267-317
-//   Point::gatherStiffFromElement  SolidPoint.cpp:197-209 (assembly by atomic add)
+//   Point::gatherStiffFromElement  SolidPoint.cpp:197-209 (assembly: one launch per
+//                                  mesh colour, so no two threads touch one point)
 // and the fluid equivalents (Gradient::computeGrad/computeQuad, Gradient.cpp:26-82,
 // Acoustic1D.cpp:8-16, FluidPoint.cpp:169-195).
 //
@@ -40,7 +41,7 @@ struct Elem1DArgs {
     const int32_t *ep_row;    // [nelem*25] first row of the point in its phase's field arrays
     const int32_t *ep_amax;   // [nelem*25] Nu_p - nyquist_p
     const float2 *displ;      // solid: rows x 3, fluid: rows x 1
-    float *stiff;             // same layout, as floats (atomic accumulation)
+    float2 *stiff;            // same layout; elements of one launch share no point
     // attenuation (memory variables per element, per mode)
     float2 *att_memvar;       // [nsls][...]
     float2 *att_stressR;
@@ -307,14 +308,13 @@ __global__ __launch_bounds__(256) void solid1d_kernel(Elem1DArgs a) {
         }
         f[c] = acc;
     }
-    // gatherStiffFromElement: stiff -= f on rows alpha <= Nu_p - nyquist_p
+    // gatherStiffFromElement: stiff -= f on rows alpha <= Nu_p - nyquist_p.
+    // The elements of one launch share no GLL point (mesh colouring), so a
+    // plain read-modify-write is race free and the sum order is deterministic.
     if (alpha <= amax) {
-        float *sp = a.stiff + ((int64_t)row + alpha) * 6;
+        float2 *sp = a.stiff + ((int64_t)row + alpha) * 3;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            atomicAdd(sp + 2 * c, -f[c].x);
-            if (alpha > 0) atomicAdd(sp + 2 * c + 1, -f[c].y);
-        }
+        for (int c = 0; c < 3; c++) sp[c] = csub(sp[c], f[c]);
     }
 }
 
@@ -394,9 +394,8 @@ __global__ __launch_bounds__(256) void fluid1d_kernel(Elem1DArgs a) {
         acc.y += g0 * w.y;
     }
     if (alpha <= amax) {
-        float *sp = a.stiff + ((int64_t)row + alpha) * 2;
-        atomicAdd(sp, -acc.x);
-        if (alpha > 0) atomicAdd(sp + 1, -acc.y);
+        float2 *sp = a.stiff + (int64_t)row + alpha;
+        *sp = csub(*sp, acc);
     }
 }
 

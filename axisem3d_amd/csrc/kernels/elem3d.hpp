@@ -5,7 +5,7 @@
 
 namespace axg {
 struct Bucket3D {
-    int phase = 0, material = 0, att = 0;
+    int colour = 0, phase = 0, material = 0, att = 0;
     bool tiso = false;
     int32_t nelem = 0;
 };

@@ -44,8 +44,11 @@ __device__ __forceinline__ void mask_solid_row(float2 s[3], int alpha, int nu, i
 __global__ __launch_bounds__(256) void newmark_solid_kernel(PhasePoints P, float2 *__restrict__ displ,
                                                             float2 *__restrict__ veloc, float2 *__restrict__ accel,
                                                             float2 *__restrict__ stiff, float dt, float half_dt,
-                                                            float half_dt_dt) {
+                                                            float half_dt_dt, int *__restrict__ tstep_counter) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
+    // the first kernel of a step advances the device-side step counter that the
+    // source and receiver kernels of the same step read (graph replay)
+    if (tstep_counter && r == 0) *tstep_counter += 1;
     if (r >= P.nrow) return;
     int p = P.row_point[r];
     int alpha = r - P.pt_row[p];
@@ -81,8 +84,9 @@ __global__ __launch_bounds__(256) void newmark_solid_kernel(PhasePoints P, float
 __global__ __launch_bounds__(256) void newmark_fluid_kernel(PhasePoints P, float2 *__restrict__ displ,
                                                             float2 *__restrict__ veloc, float2 *__restrict__ accel,
                                                             float2 *__restrict__ stiff, float dt, float half_dt,
-                                                            float half_dt_dt) {
+                                                            float half_dt_dt, int *__restrict__ tstep_counter) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tstep_counter && r == 0) *tstep_counter += 1;
     if (r >= P.nrow) return;
     int p = P.row_point[r];
     int alpha = r - P.pt_row[p];
@@ -115,9 +119,10 @@ __global__ __launch_bounds__(256) void newmark_fluid_kernel(PhasePoints P, float
 // SourceTerm::apply + Element::addSourceTerm (SourceTerm.cpp:29-35, Element.cpp:31-35)
 __global__ void source_kernel(const float *__restrict__ force, const int32_t *__restrict__ src_row,
                               const int32_t *__restrict__ src_nu, const float *__restrict__ stf, int tstep,
-                              float2 *__restrict__ stiff) {
+                              const int *__restrict__ tstep_ptr, float2 *__restrict__ stiff) {
     int t = threadIdx.x;  // 25 points x 3 rows x 3 components
     if (t >= 225) return;
+    if (tstep_ptr) tstep = *tstep_ptr;
     int k = t / 9, a = (t / 3) % 3, c = t % 3;
     if (a > src_nu[k]) return;
     float f = stf[tstep];
@@ -175,11 +180,14 @@ __global__ __launch_bounds__(256) void record_solid_kernel(int32_t nrec, const i
                                                            const int32_t *__restrict__ ep_row,
                                                            const int32_t *__restrict__ ep_amax,
                                                            const int32_t *__restrict__ elem_nr,
-                                                           const float2 *__restrict__ displ, int rtz,
-                                                           float *__restrict__ out) {
+                                                           const float2 *__restrict__ displ, int rtz, int tstep,
+                                                           const int *__restrict__ tstep_ptr,
+                                                           float *__restrict__ seis) {
     __shared__ float red[3][256];
     int irec = blockIdx.x;
     if (irec >= nrec) return;
+    if (tstep_ptr) tstep = *tstep_ptr;
+    float *out = seis + (int64_t)tstep * nrec * 3;
     int64_t e = rec_elem[irec];
     int nr = elem_nr[e];
     int nu = nr / 2;

@@ -143,7 +143,7 @@ class Newmark:
         dt = d.dt
         d.resetZero()
         if not per_operator:
-            d.run(1, d.nstep, self.stab)
+            d.run(1, d.nstep, self.stab, use_graph=True)
             return
         for tstep in range(1, d.nstep + 1):
             d.updateNewmark(dt)
