@@ -27,6 +27,7 @@
 
 namespace axg {
 __constant__ float cG[2][25];
+__constant__ float2 cRoot3[3], cRoot5[5], cRoot7[7], cRoot11[11], cRoot13[13];
 void set_last_error(const std::string &msg);
 }  // namespace axg
 
@@ -80,6 +81,13 @@ struct Bucket1D {
     int32_t nitem = 0;
 };
 
+struct Bucket3D {
+    int colour = 0, phase = 0, material = 0, att = 0;
+    bool tiso = false;
+    int32_t nelem = 0, maxM = 0;
+    DevBuf<int32_t> elems;
+};
+
 struct Phase {
     int32_t npt = 0, nrow = 0;
     int ncomp = 3;
@@ -124,7 +132,15 @@ struct axg_ctx {
     int ncolour = 0;
     std::vector<std::unique_ptr<Bucket1D>> buckets1d;
     std::vector<std::unique_ptr<Bucket3D>> buckets3d;
-    DevBuf<float> d_fft_twiddle;
+    // 3-D elements: material relaid out per pass row, FFT tables
+    std::vector<float> h_mat3d;
+    std::vector<int64_t> h_mat3d_off;
+    std::vector<char> nr_used;
+    DevBuf<float> d_mat3d;
+    DevBuf<int64_t> d_mat3d_off;
+    DevBuf<float2> d_twiddle;
+    DevBuf<int32_t> d_tw_off, d_factors;
+    size_t max_lds = 0;
     // solid-fluid coupling
     int32_t nsf = 0, nsf_row = 0;
     DevBuf<int32_t> d_row_sf, d_sf_srow, d_sf_frow, d_sf_row0;
@@ -156,13 +172,150 @@ namespace {
 // Kernel classes timed by the profiling mode of axg_run
 enum { KC_ELEMENT = 0, KC_NEWMARK = 1, KC_OTHER = 2, KC_N = 3 };
 
-void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int phase, int mat, int tiso, int att,
-                     const std::vector<int32_t> &elems) {
-    (void)c; (void)d; (void)phase; (void)mat; (void)tiso; (void)att; (void)elems;
-    throw std::runtime_error("FieldFFT::transformF2P || azimuth-dependent (3-D) elements are not supported by this build yet.");
+// radices of the Stockham FFT, largest power-of-two factor first
+std::vector<int> fftFactors(int n) {
+    std::vector<int> f;
+    int m = n;
+    for (int p : {4, 2, 3, 5, 7, 11, 13}) {
+        while (m % p == 0) {
+            f.push_back(p);
+            m /= p;
+        }
+    }
+    if (m != 1) f.clear();
+    return f;
 }
+
+void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att,
+                     const std::vector<int32_t> &elems) {
+    if (phase != 0)
+        throw std::runtime_error("Acoustic3D::strainToStress || azimuth-dependent fluid elements are not supported by this build yet.");
+    if (att != AXG_ATT_NONE)
+        throw std::runtime_error("Attenuation3D::applyToStress || 3-D attenuation is not supported by this build yet.");
+    const int P = (mat == AXG_MAT_ISOTROPIC) ? 2 : (mat == AXG_MAT_TRANSVERSELY_ISO) ? 5 : 21;
+    if (c->h_mat3d_off.empty()) c->h_mat3d_off.assign(d->nelem, 0);
+    if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
+    // size classes by number of modes M = Nr/2 + 1 (LDS is sized by the class)
+    const int cls[] = {9, 17, 33, 49, 65, 97, 137};
+    std::vector<std::vector<int32_t>> byClass(7);
+    for (int e : elems) {
+        int nr = c->elem_nr[e];
+        if (nr > E3D_MAX_NR || fftFactors(nr).empty() || (int)fftFactors(nr).size() > E3D_MAX_FACTORS)
+            throw std::runtime_error("FieldFFT::transformF2P || Nr = " + std::to_string(nr) +
+                                     " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
+        c->nr_used[nr] = 1;
+        int M = nr / 2 + 1;
+        int k = 0;
+        while (cls[k] < M) k++;
+        byClass[k].push_back(e);
+        // material: descriptor [param][n][i*5+j] -> [i][param][n][j]
+        int64_t off = (int64_t)c->h_mat3d.size();
+        c->h_mat3d_off[e] = off;
+        c->h_mat3d.resize(off + (size_t)P * nr * 25);
+        const float *src = d->mat_pool + d->mat_off[e];
+        if (d->mat_off[e + 1] - d->mat_off[e] != (int64_t)P * nr * 25)
+            throw std::runtime_error("Elastic3D::checkCompatibility || Incompatible size.");
+        for (int i = 0; i < 5; i++)
+            for (int p = 0; p < P; p++)
+                for (int n = 0; n < nr; n++)
+                    for (int j = 0; j < 5; j++)
+                        c->h_mat3d[off + (((size_t)i * P + p) * nr + n) * 5 + j] =
+                            src[(size_t)p * nr * 25 + (size_t)n * 25 + i * 5 + j];
+    }
+    for (int k = 0; k < 7; k++) {
+        auto &v = byClass[k];
+        if (v.empty()) continue;
+        std::sort(v.begin(), v.end(), [&](int a, int b) { return c->elem_nr[a] > c->elem_nr[b]; });
+        auto b = std::make_unique<Bucket3D>();
+        b->colour = colour;
+        b->phase = phase;
+        b->material = mat;
+        b->tiso = tiso != 0;
+        b->att = att;
+        b->nelem = (int32_t)v.size();
+        b->maxM = cls[k];
+        b->elems.upload(v, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->buckets3d.push_back(std::move(b));
+    }
+}
+
+void finishBuckets3D(axg_ctx *c) {
+    if (c->buckets3d.empty()) return;
+    hipStream_t s = c->stream;
+    c->d_mat3d.upload(c->h_mat3d, s);
+    c->d_mat3d_off.upload(c->h_mat3d_off, s);
+    std::vector<float2> tw;
+    std::vector<int32_t> twoff(E3D_MAX_NR + 1, 0), fac((size_t)(E3D_MAX_NR + 1) * (E3D_MAX_FACTORS + 1), 0);
+    for (int n = 1; n <= E3D_MAX_NR; n++) {
+        if (!c->nr_used[n]) continue;
+        twoff[n] = (int32_t)tw.size();
+        for (int t = 0; t < n; t++) {
+            double ang = -2. * 3.141592653589793238463 * t / n;
+            tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+        }
+        auto f = fftFactors(n);
+        fac[(size_t)n * (E3D_MAX_FACTORS + 1)] = (int32_t)f.size();
+        for (size_t q = 0; q < f.size(); q++) fac[(size_t)n * (E3D_MAX_FACTORS + 1) + 1 + q] = f[q];
+    }
+    c->d_twiddle.upload(tw, s);
+    c->d_tw_off.upload(twoff, s);
+    c->d_factors.upload(fac, s);
+    HIPCHECK(hipStreamSynchronize(s));
+    c->h_mat3d.clear();
+    c->h_mat3d.shrink_to_fit();
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, c->device));
+    c->max_lds = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+    for (auto &b : c->buckets3d) {
+        size_t lds = ((size_t)105 * b->maxM + 75 * E3D_STAGE_MODES) * sizeof(float2);
+        if (lds > c->max_lds)
+            throw std::runtime_error("FieldFFT || element needs " + std::to_string(lds) + " B of LDS, device has " +
+                                     std::to_string(c->max_lds) + " (not supported by this build).");
+    }
+}
+
+template <int MAT, bool TISO>
+void launchSolid3D(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+    static bool attrSet = false;
+    if (!attrSet) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&solid3d_kernel<MAT, TISO>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attrSet = true;
+    }
+    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO>), dim3(a.nelem), dim3(E3D_THREADS), lds, s, a);
+}
+
 void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
-    (void)c; (void)b;
+    if (b.nelem == 0) return;
+    Elem3DArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.elems = b.elems.p;
+    a.nelem = b.nelem;
+    a.maxM = b.maxM;
+    a.elem_flags = c->d_elem_flags.p;
+    a.elem_nr = c->d_elem_nr.p;
+    a.geom = c->d_geom.p;
+    a.tiso = c->d_tiso.p;
+    a.mat3d = c->d_mat3d.p;
+    a.mat3d_off = c->d_mat3d_off.p;
+    a.ep_row = c->d_ep_row.p;
+    a.ep_amax = c->d_ep_amax.p;
+    a.displ = c->S.displ.p;
+    a.stiff = c->S.stiff.p;
+    a.twiddle = c->d_twiddle.p;
+    a.tw_off = c->d_tw_off.p;
+    a.factors = c->d_factors.p;
+    size_t lds = ((size_t)105 * b.maxM + 75 * E3D_STAGE_MODES) * sizeof(float2);
+    hipStream_t s = c->stream;
+    if (b.material == AXG_MAT_ISOTROPIC) {
+        if (b.tiso) launchSolid3D<0, true>(a, lds, s);
+        else launchSolid3D<0, false>(a, lds, s);
+    } else if (b.material == AXG_MAT_TRANSVERSELY_ISO) {
+        launchSolid3D<1, true>(a, lds, s);
+    } else {
+        launchSolid3D<2, true>(a, lds, s);
+    }
 }
 
 int fail(int code, const std::string &msg) {
@@ -382,6 +535,19 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 G[1][i * 5 + j] = (float)gj.G[i][j];
             }
         HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cG), G, sizeof(G)));
+        auto roots = [](int R) {
+            std::vector<float2> w(R);
+            for (int m = 0; m < R; m++) {
+                double ang = -2. * 3.141592653589793238463 * m / R;
+                w[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+            }
+            return w;
+        };
+        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot3), roots(3).data(), 3 * sizeof(float2)));
+        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot5), roots(5).data(), 5 * sizeof(float2)));
+        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot7), roots(7).data(), 7 * sizeof(float2)));
+        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot11), roots(11).data(), 11 * sizeof(float2)));
+        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot13), roots(13).data(), 13 * sizeof(float2)));
     }
     buildPhase(d, c->S, true);
     buildPhase(d, c->F, false);
@@ -512,11 +678,9 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 HIPCHECK(hipStreamSynchronize(s));
                 c->buckets1d.push_back(std::move(b));
             }
-            if (!e3[key].empty()) {
-                createBuckets3D(c, d, kk / 32, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
-                c->buckets3d.back()->colour = key / NK;
-            }
+            if (!e3[key].empty()) createBuckets3D(c, d, key / NK, kk / 32, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
         }
+        finishBuckets3D(c);
     }
 
     // ---- solid-fluid coupling ----

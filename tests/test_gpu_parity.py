@@ -131,3 +131,54 @@ def test_stability_reports_point(small_layers):
     g.set_field(_abi.FIELD_SOLID_DISPL, x)
     with pytest.raises(RuntimeError, match="Domain::checkStability"):
         g.checkStability(m.d.dt, 1, 0.)
+
+
+# ---------------------------------------------------------------- 3-D path
+def _mesh3d(layers, material, nu_min, nu_max, nstep, nrec=4, nelem_theta=12, **kw):
+    cfg = ax.shell_config(1, layers=layers, nelem_theta=nelem_theta, material=material, dv_amp=0.05,
+                          nu_mode=1, nu_min=nu_min, nu_max=nu_max, nstep=nstep, nrec=nrec, **kw)
+    m = ax.ShellMesh(cfg)
+    assert m.stats["n3d_elem"] == m.stats["nsolid_elem"]
+    return m
+
+
+@pytest.mark.parametrize("material,nu_min,nu_max", [
+    (_abi.MAT_ISOTROPIC, 3, 12), (_abi.MAT_TRANSVERSELY_ISO, 3, 12), (_abi.MAT_ANISOTROPIC, 3, 12),
+    (_abi.MAT_TRANSVERSELY_ISO, 40, 128)])
+def test_compute_stiff_3d_operator(small_layers, material, nu_min, nu_max):
+    """One FFT-path stiffness evaluation on random displacement; the Nu range
+    40..128 covers lucky lengths with radices 2, 3, 4, 5, 7, 11 and 13."""
+    print("case compute_stiff_3d material=%d nu=%d..%d" % (material, nu_min, nu_max), flush=True)
+    m = _mesh3d(small_layers, material, nu_min, nu_max, nstep=10, nelem_theta=8)
+    nr = np.unique(m.array("point_nr"))
+    print("Nr present:", nr.tolist(), flush=True)
+    g = ax.Domain(m)
+    r = RefDomain(m, double=True)
+    u = random_field(m, _abi.FIELD_SOLID_DISPL, seed=3)
+    g.set_field(_abi.FIELD_SOLID_DISPL, u)
+    r.set_field(_abi.FIELD_SOLID_DISPL, u)
+    g.computeStiff()
+    r.computeStiff()
+    sg = g.get_field(_abi.FIELD_SOLID_STIFF)
+    sr = r.get_field(_abi.FIELD_SOLID_STIFF)
+    err = rel_l2(sg, sr)
+    print("3d stiff rel L2 %.3e" % err)
+    assert np.abs(sr).max() > 0
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("material", [_abi.MAT_TRANSVERSELY_ISO, _abi.MAT_ANISOTROPIC])
+def test_seismograms_3d(small_layers, material):
+    print("case seismograms_3d material=%d" % material, flush=True)
+    m = _mesh3d(small_layers, material, 3, 12, nstep=300)
+    g = ax.Domain(m)
+    ax.Newmark(g, 100, 100).solve()
+    r32 = RefDomain(m, double=False)
+    r32.run(1, m.d.nstep, 100)
+    r64 = RefDomain(m, double=True)
+    r64.run(1, m.d.nstep, 100)
+    sg, s32, s64 = g.seismograms(), r32.seismograms(), r64.seismograms()
+    e32, e64 = rel_l2(sg, s32), rel_l2(sg, s64)
+    print("3d gpu vs oracle fp32 %.3e, vs fp64 %.3e" % (e32, e64))
+    assert e32 < 1e-4
+    assert e64 < 1e-3
