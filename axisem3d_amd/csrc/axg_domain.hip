@@ -27,7 +27,6 @@
 
 namespace axg {
 __constant__ float cG[2][25];
-__constant__ float2 cRoot3[3], cRoot5[5], cRoot7[7], cRoot11[11], cRoot13[13];
 void set_last_error(const std::string &msg);
 }  // namespace axg
 
@@ -245,8 +244,15 @@ void finishBuckets3D(axg_ctx *c) {
     hipStream_t s = c->stream;
     c->d_mat3d.upload(c->h_mat3d, s);
     c->d_mat3d_off.upload(c->h_mat3d_off, s);
+    // W_R^m of the odd radices 3, 5, 7, 11, 13 first (kRootTable entries), then
+    // W_N^t for every N in use
     std::vector<float2> tw;
-    std::vector<int32_t> twoff(E3D_MAX_NR + 1, 0), fac((size_t)(E3D_MAX_NR + 1) * (E3D_MAX_FACTORS + 1), 0);
+    for (int R : {3, 5, 7, 11, 13})
+        for (int m = 0; m < R; m++) {
+            double ang = -2. * 3.141592653589793238463 * m / R;
+            tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+        }
+    std::vector<int32_t> twoff(E3D_MAX_NR + 1, 0), fac((size_t)(E3D_MAX_NR + 1) * 8, 0);
     for (int n = 1; n <= E3D_MAX_NR; n++) {
         if (!c->nr_used[n]) continue;
         twoff[n] = (int32_t)tw.size();
@@ -254,9 +260,10 @@ void finishBuckets3D(axg_ctx *c) {
             double ang = -2. * 3.141592653589793238463 * t / n;
             tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
         }
-        auto f = fftFactors(n);
-        fac[(size_t)n * (E3D_MAX_FACTORS + 1)] = (int32_t)f.size();
-        for (size_t q = 0; q < f.size(); q++) fac[(size_t)n * (E3D_MAX_FACTORS + 1) + 1 + q] = f[q];
+        const int radix[7] = {4, 2, 3, 5, 7, 11, 13};
+        for (int r : fftFactors(n))
+            for (int q = 0; q < 7; q++)
+                if (r == radix[q]) fac[(size_t)n * 8 + q]++;
     }
     c->d_twiddle.upload(tw, s);
     c->d_tw_off.upload(twoff, s);
@@ -268,7 +275,7 @@ void finishBuckets3D(axg_ctx *c) {
     HIPCHECK(hipGetDeviceProperties(&prop, c->device));
     c->max_lds = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
     for (auto &b : c->buckets3d) {
-        size_t lds = ((size_t)105 * b->maxM + 75 * E3D_STAGE_MODES) * sizeof(float2);
+        size_t lds = ((size_t)32 * b->maxM + kRootTable + 75 * std::min(E3D_STAGE_MODES, b->maxM)) * sizeof(float2);
         if (lds > c->max_lds)
             throw std::runtime_error("FieldFFT || element needs " + std::to_string(lds) + " B of LDS, device has " +
                                      std::to_string(c->max_lds) + " (not supported by this build).");
@@ -306,7 +313,7 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     a.twiddle = c->d_twiddle.p;
     a.tw_off = c->d_tw_off.p;
     a.factors = c->d_factors.p;
-    size_t lds = ((size_t)105 * b.maxM + 75 * E3D_STAGE_MODES) * sizeof(float2);
+    size_t lds = ((size_t)32 * b.maxM + kRootTable + 75 * std::min(E3D_STAGE_MODES, b.maxM)) * sizeof(float2);
     hipStream_t s = c->stream;
     if (b.material == AXG_MAT_ISOTROPIC) {
         if (b.tiso) launchSolid3D<0, true>(a, lds, s);
@@ -535,19 +542,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 G[1][i * 5 + j] = (float)gj.G[i][j];
             }
         HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cG), G, sizeof(G)));
-        auto roots = [](int R) {
-            std::vector<float2> w(R);
-            for (int m = 0; m < R; m++) {
-                double ang = -2. * 3.141592653589793238463 * m / R;
-                w[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-            }
-            return w;
-        };
-        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot3), roots(3).data(), 3 * sizeof(float2)));
-        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot5), roots(5).data(), 5 * sizeof(float2)));
-        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot7), roots(7).data(), 7 * sizeof(float2)));
-        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot11), roots(11).data(), 11 * sizeof(float2)));
-        HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cRoot13), roots(13).data(), 13 * sizeof(float2)));
+
     }
     buildPhase(d, c->S, true);
     buildPhase(d, c->F, false);

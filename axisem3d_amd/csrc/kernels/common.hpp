@@ -10,9 +10,14 @@ namespace axg {
 // (Gradient::setGMat, SOLVER/src/core/element/grad/Gradient.cpp:371-380)
 extern __constant__ float cG[2][25];
 
-struct cf {
-    float x, y;
-};
+// Returns x unchanged but opaque to the optimiser: per-thread index arithmetic
+// derived from it is recomputed inside a loop instead of being hoisted out of
+// it and kept live (hoisting dozens of addresses across a long loop body
+// exhausts the register file).
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

@@ -20,7 +20,7 @@
 namespace axg {
 
 constexpr int E3D_THREADS = 512;
-constexpr int E3D_STAGE_MODES = 8;   // modes of displacement staged at once
+constexpr int E3D_STAGE_MODES = 64;  // modes of displacement staged at once
 constexpr int E3D_MAX_NR = 272;      // 15 * Nr / radix <= threads * butterflies-per-thread
 constexpr int E3D_MAX_FACTORS = 12;
 
@@ -41,22 +41,10 @@ struct Elem3DArgs {
     float2 *stiff;
     const float2 *twiddle;       // W_N^t = exp(-2 pi i t / N), concatenated per N
     const int32_t *tw_off;       // [N] offset of the table of N
-    const int32_t *factors;      // [N][E3D_MAX_FACTORS+1]: count, radices
+    const int32_t *factors;      // [N][8]: stage counts of radices 4, 2, 3, 5, 7, 11, 13
 };
 
 // ---------------------------------------------------------------- FFT stage
-// roots of unity W_R^m = exp(-2 pi i m / R) of the odd radices, in constant
-// memory so the unrolled small DFTs read them as scalars
-extern __constant__ float2 cRoot3[3], cRoot5[5], cRoot7[7], cRoot11[11], cRoot13[13];
-
-template <int R>
-__device__ __forceinline__ float2 root(int m) {
-    if (R == 3) return cRoot3[m];
-    if (R == 5) return cRoot5[m];
-    if (R == 7) return cRoot7[m];
-    if (R == 11) return cRoot11[m];
-    return cRoot13[m];
-}
 
 template <int R>
 struct MaxB {
@@ -66,8 +54,18 @@ struct MaxB {
 // one Stockham stage on nseq sequences of length N stored [seq][N] in `buf`:
 // for butterfly j: inputs x[j + r N/R] * W_{Ns R}^{(j % Ns) r}, radix-R DFT,
 // outputs y[(j / Ns) Ns R + (j % Ns) + r Ns].  sign = -1 forward, +1 inverse.
+// LDS offsets of the W_R^m tables of the odd radices inside the twiddle area
 template <int R>
-__device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, int sign, const float2 *__restrict__ tw) {
+constexpr int rootOff() {
+    return (R == 3) ? 0 : (R == 5) ? 3 : (R == 7) ? 8 : (R == 11) ? 15 : 26;
+}
+constexpr int kRootTable = 39;
+
+// One Stockham stage.  `tw` points into LDS: W_N^t for t < N, followed by the
+// radix tables (loop-invariant constants would otherwise be hoisted into
+// registers across the whole element loop).
+template <int R>
+__device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, int sign, const float2 *tw) {
     constexpr int B = MaxB<R>::v;
     const int nb = N / R;
     const int total = nseq * nb;
@@ -75,7 +73,7 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
     int dst[B];
 #pragma unroll
     for (int q = 0; q < B; q++) {
-        int t = threadIdx.x + q * E3D_THREADS;
+        int t = opaque((int)threadIdx.x) + q * E3D_THREADS;
         dst[q] = -1;
         if (t < total) {
             int s = t / nb, j = t - s * nb;
@@ -101,10 +99,10 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
     for (int q = 0; q < B; q++) {
         if (dst[q] < 0) continue;
         float2 y[R];
-        if (R == 2) {
+        if constexpr (R == 2) {
             y[0] = cadd(v[q][0], v[q][1]);
             y[1] = csub(v[q][0], v[q][1]);
-        } else if (R == 4) {
+        } else if constexpr (R == 4) {
             float2 a0 = cadd(v[q][0], v[q][2]), a1 = csub(v[q][0], v[q][2]);
             float2 b0 = cadd(v[q][1], v[q][3]), b1 = csub(v[q][1], v[q][3]);
             // multiply b1 by -i (forward) or +i (inverse)
@@ -120,7 +118,7 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
                 float2 acc = v[q][0];
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    float2 w = root<R>((r * k) % R);
+                    float2 w = tw[N + rootOff<R>() + (r * k) % R];
                     if (sign > 0) w.y = -w.y;
                     acc = cadd(acc, cmul(v[q][r], w));
                 }
@@ -133,24 +131,41 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
     __syncthreads();
 }
 
+// the full transform: fac = stage counts [n4, n2, n3, n5, n7, n11, n13] of N,
+// applied in this order (one call site per radix keeps register use per stage)
 __device__ __forceinline__ void fft_inplace(float2 *buf, int nseq, int N, int sign, const int32_t *fac,
-                                            const float2 *__restrict__ tw) {
-    int nf = fac[0];
+                                            const float2 *tw) {
     int Ns = 1;
 #pragma unroll 1
-    for (int f = 0; f < nf; f++) {
-        int R = fac[1 + f];
-        switch (R) {
-            case 2: fft_stage<2>(buf, nseq, N, Ns, sign, tw); break;
-            case 3: fft_stage<3>(buf, nseq, N, Ns, sign, tw); break;
-            case 4: fft_stage<4>(buf, nseq, N, Ns, sign, tw); break;
-            case 5: fft_stage<5>(buf, nseq, N, Ns, sign, tw); break;
-            case 7: fft_stage<7>(buf, nseq, N, Ns, sign, tw); break;
-            case 11: fft_stage<11>(buf, nseq, N, Ns, sign, tw); break;
-            default: fft_stage<13>(buf, nseq, N, Ns, sign, tw); break;
-        }
-        Ns *= R;
+    for (int f = 0; f < fac[0]; f++) {
+        fft_stage<4>(buf, nseq, N, Ns, sign, tw);
+        Ns *= 4;
     }
+#pragma unroll 1
+    for (int f = 0; f < fac[1]; f++) {
+        fft_stage<2>(buf, nseq, N, Ns, sign, tw);
+        Ns *= 2;
+    }
+#pragma unroll 1
+    for (int f = 0; f < fac[2]; f++) {
+        fft_stage<3>(buf, nseq, N, Ns, sign, tw);
+        Ns *= 3;
+    }
+#pragma unroll 1
+    for (int f = 0; f < fac[3]; f++) {
+        fft_stage<5>(buf, nseq, N, Ns, sign, tw);
+        Ns *= 5;
+    }
+#pragma unroll 1
+    for (int f = 0; f < fac[4]; f++) {
+        fft_stage<7>(buf, nseq, N, Ns, sign, tw);
+        Ns *= 7;
+    }
+    if (fac[5]) {
+        fft_stage<11>(buf, nseq, N, Ns, sign, tw);
+        Ns *= 11;
+    }
+    if (fac[6]) fft_stage<13>(buf, nseq, N, Ns, sign, tw);
 }
 
 // ---------------------------------------------------------------- stress
@@ -194,21 +209,50 @@ __device__ __forceinline__ void stress3d(const float *__restrict__ mp, int Nr, i
     }
 }
 
+// same, with the parameters already in registers: P[p] of one (sample, point)
+template <int MAT>
+__device__ __forceinline__ void stress3d_reg(const float *P, const float e[6], float s[6]) {
+    if (MAT == 0) {
+        float lam = P[0], mu = P[1], mu2 = 2.f * mu;
+        float sii = lam * (e[0] + e[1] + e[2]);
+        s[0] = sii + mu2 * e[0];
+        s[1] = sii + mu2 * e[1];
+        s[2] = sii + mu2 * e[2];
+        s[3] = mu * e[3];
+        s[4] = mu * e[4];
+        s[5] = mu * e[5];
+    } else {
+        float A = P[0], Cc = P[1], F = P[2], L = P[3], N = P[4], N2 = 2.f * N;
+        float e01 = e[0] + e[1];
+        float tmp = A * e01 + F * e[2];
+        s[0] = tmp - N2 * e[1];
+        s[1] = tmp - N2 * e[0];
+        s[2] = Cc * e[2] + F * e01;
+        s[3] = L * e[3];
+        s[4] = L * e[4];
+        s[5] = N * e[5];
+    }
+}
+
 template <int MAT>
 struct MatParams {
     static constexpr int v = (MAT == 0) ? 2 : (MAT == 1) ? 5 : 21;
 };
 
 // ---------------------------------------------------------------- kernel
+// Work split: item (j, alpha) -> thread tid + q * T (q < QB).  The same thread
+// owns that item in every pass, so the element's stiffness accumulates in its
+// registers: acc[q][ip][c] = f_c(ip, j) of mode alpha.
 template <int MAT, bool TISO>
 __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
-    const int T = E3D_THREADS;
+    constexpr int T = E3D_THREADS;
+    constexpr int QB = 2;  // 5 * M <= QB * T
     const int tid = threadIdx.x;
     const int Mmax = a.maxM;
-    float2 *acc = lds;                  // [25][M][3]
-    float2 *buf = acc + 75 * Mmax;      // spectral [30][M] / sequences [15][Nr]
+    float2 *buf = lds;                  // spectral [30][M] / sequences [15][Nr]
     float2 *stg = buf + 30 * Mmax;      // [E3D_STAGE_MODES][3][25]
+    float2 *twl = stg + 75 * min(E3D_STAGE_MODES, Mmax);  // W_Nr^t, t < Nr
     const int64_t e = a.elems[blockIdx.x];
     const int Nr = a.elem_nr[e];
     const int Nu = Nr / 2;
@@ -217,37 +261,46 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     const bool axial = (a.elem_flags[e] & 2) != 0;
     const float *Gx = cG[axial ? 1 : 0];
     const float *Ge = cG[0];
-    const int32_t *fac = a.factors + Nr * (E3D_MAX_FACTORS + 1);
-    const float2 *tw = a.twiddle + a.tw_off[Nr];
+    const int32_t *fac = a.factors + Nr * 8;
+    for (int t = tid; t < Nr; t += E3D_THREADS) twl[t] = a.twiddle[a.tw_off[Nr] + t];
+    for (int t = tid; t < kRootTable; t += E3D_THREADS) twl[Nr + t] = a.twiddle[t];  // radix tables
+    const float2 *tw = twl;  // read by every FFT stage; visible after the first barrier
     const float invNr = 1.f / (float)Nr;
     constexpr int NPAR = MatParams<MAT>::v;
 
-    for (int t = tid; t < 75 * M; t += T) acc[t] = czero();
+    float2 acc[QB][5][3];
+#pragma unroll
+    for (int q = 0; q < QB; q++)
+#pragma unroll
+        for (int ip = 0; ip < 5; ip++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[q][ip][c] = czero();
 
 #pragma unroll 1
     for (int i = 0; i < 5; i++) {
+        const int tl = opaque(tid);
         // ---------------- strain spectrum of points (i, 0..4), modes 0..Nu
+#pragma unroll 1
         for (int a0 = 0; a0 < M; a0 += E3D_STAGE_MODES) {
-            int nch = min(E3D_STAGE_MODES, M - a0);
+            const int nch = min(E3D_STAGE_MODES, M - a0);
             __syncthreads();
-            for (int t = tid; t < nch * 25; t += T) {
+            for (int t = tl; t < nch * 25; t += T) {
                 int q = t / 25, k = t - q * 25, al = a0 + q;
                 int row = a.ep_row[e * 25 + k];
                 bool ok = al <= a.ep_amax[e * 25 + k];
+                const float2 *src = a.displ + ((int64_t)row + al) * 3;
 #pragma unroll
-                for (int c = 0; c < 3; c++)
-                    stg[(q * 3 + c) * 25 + k] = ok ? a.displ[((int64_t)row + al) * 3 + c] : czero();
+                for (int c = 0; c < 3; c++) stg[(q * 3 + c) * 25 + k] = ok ? src[c] : czero();
             }
             __syncthreads();
-            for (int t = tid; t < nch * 5; t += T) {
+            for (int t = tl; t < nch * 5; t += T) {
                 int q = t / 5, j = t - q * 5, al = a0 + q;
                 float2 e6[6];
                 if (nyq && al == Nu) {
 #pragma unroll
                     for (int c = 0; c < 6; c++) e6[c] = czero();
                 } else {
-                    const float2(*U)[25] = reinterpret_cast<const float2(*)[25]>(stg + q * 75);
-                    // strain at (i, j) from the staged 5x5x3 displacement
+                    const float2 *U = stg + q * 75;  // U[c*25 + m*5 + n]
                     PointGeom g = load_geom(a.geom, e, i * 5 + j);
                     float2 GU[3], UG[3];
 #pragma unroll
@@ -256,7 +309,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
                         for (int m = 0; m < 5; m++) {
                             float gx = Gx[m * 5 + i], ge = Ge[m * 5 + j];
-                            float2 um = U[c][m * 5 + j], un = U[c][i * 5 + m];
+                            float2 um = U[c * 25 + m * 5 + j], un = U[c * 25 + i * 5 + m];
                             x.x += gx * um.x;
                             x.y += gx * um.y;
                             y.x += un.x * ge;
@@ -266,7 +319,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                         UG[c] = y;
                     }
                     float fa = (float)al;
-                    float2 u0 = U[0][i * 5 + j], u1 = U[1][i * 5 + j], u2 = U[2][i * 5 + j];
+                    float2 u0 = U[0 * 25 + i * 5 + j], u1 = U[1 * 25 + i * 5 + j], u2 = U[2 * 25 + i * 5 + j];
                     float2 v0 = cadd(u0, cimul(fa, u1)), v1 = csub(cimul(fa, u0), u1), v2 = cimul(fa, u2);
                     e6[0] = cadd(cscale(g.dzdeta, GU[0]), cscale(g.dzdxii, UG[0]));
                     e6[1] = cscale(g.inv_s, v0);
@@ -295,27 +348,43 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         }
         __syncthreads();
 
+        // material of this pass row, issued before the C2R transform so that the
+        // loads overlap it (iso / TI: 2 or 5 parameters per sample and point)
+        constexpr int SB = 3;  // 5 * Nr <= SB * T
+        constexpr bool PREF = NPAR <= 5;
+        float pm[PREF ? SB : 1][PREF ? NPAR : 1];
+        const float *mp = a.mat3d + a.mat3d_off[e] + (int64_t)i * NPAR * Nr * 5;
+        if constexpr (PREF) {
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                int t = tl + q * T;
+                if (t < 5 * Nr) {
+                    int n = t / 5, j = t - n * 5;
+#pragma unroll
+                    for (int p = 0; p < NPAR; p++) pm[q][p] = mp[((int64_t)p * Nr + n) * 5 + j];
+                }
+            }
+        }
+
         // ---------------- pack 30 spectra into 15 Hermitian-extended sequences
         {
             constexpr int PB = 4;  // 15 * M <= PB * T
             float2 X[PB], Y[PB];
-            int bk[PB];
 #pragma unroll
             for (int q = 0; q < PB; q++) {
-                int t = tid + q * T;
-                bk[q] = -1;
+                int t = tl + q * T;
                 if (t < 15 * M) {
                     int b = t / M, k = t - b * M;
                     X[q] = buf[(2 * b) * M + k];
                     Y[q] = buf[(2 * b + 1) * M + k];
-                    bk[q] = t;
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < PB; q++) {
-                if (bk[q] < 0) continue;
-                int b = bk[q] / M, k = bk[q] - b * M;
+                int t = tl + q * T;
+                if (t >= 15 * M) continue;
+                int b = t / M, k = t - b * M;
                 float2 *z = buf + b * Nr;
                 if (k == 0 || 2 * k == Nr) {
                     // DC and Nyquist: the imaginary parts are ignored (FFTW c2r)
@@ -332,8 +401,10 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 
         // ---------------- stress at every (sample n, point j)
         {
-            const float *mp = a.mat3d + a.mat3d_off[e] + (int64_t)i * NPAR * Nr * 5;
-            for (int t = tid; t < 5 * Nr; t += T) {
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                int t = tl + q * T;
+                if (t >= 5 * Nr) continue;
                 int n = t / 5, j = t - n * 5;
                 float ev[6], sv[6];
 #pragma unroll
@@ -342,7 +413,8 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                     float2 z = buf[(col >> 1) * Nr + n];
                     ev[c] = (col & 1) ? z.y : z.x;
                 }
-                stress3d<MAT>(mp, Nr, n, j, ev, sv);
+                if constexpr (PREF) stress3d_reg<MAT>(pm[q], ev, sv);
+                else stress3d<MAT>(mp, Nr, n, j, ev, sv);
 #pragma unroll
                 for (int c = 0; c < 6; c++) {
                     int col = c * 5 + j;
@@ -359,25 +431,23 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         {
             constexpr int PB = 4;
             float2 Z0[PB], Z1[PB];
-            int bk[PB];
 #pragma unroll
             for (int q = 0; q < PB; q++) {
-                int t = tid + q * T;
-                bk[q] = -1;
+                int t = tl + q * T;
                 if (t < 15 * M) {
                     int b = t / M, k = t - b * M;
                     const float2 *z = buf + b * Nr;
                     Z0[q] = z[k];
-                    Z1[q] = z[(Nr - k) % Nr];
-                    bk[q] = t;
+                    Z1[q] = z[k == 0 ? 0 : Nr - k];
                 }
             }
             __syncthreads();
             const float h = 0.5f * invNr;
 #pragma unroll
             for (int q = 0; q < PB; q++) {
-                if (bk[q] < 0) continue;
-                int b = bk[q] / M, k = bk[q] - b * M;
+                int t = tl + q * T;
+                if (t >= 15 * M) continue;
+                int b = t / M, k = t - b * M;
                 float2 za = Z0[q], zb = Z1[q];
                 buf[(2 * b) * M + k] = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
                 buf[(2 * b + 1) * M + k] = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
@@ -385,20 +455,15 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             __syncthreads();
         }
 
-        // ---------------- rotate back, quadrature operands, accumulate
-        // thread (j, alpha): X-term to all rows of column j, Y/g-terms to (i, j)
+        // ---------------- rotate back, quadrature operands, accumulate in registers.
+        // A thread reads the stress of its own (j, alpha) items and overwrites the
+        // same LDS slots with its Y operands, so no barrier is needed in between.
         {
-            // 1) operands into registers; Y (+ axial V) stored over the spectra
-            const int nitem = 5 * M;
-            constexpr int QB = 2;  // 5 * M <= QB * T
-            float2 Xo[QB][3], Yo[QB][3], go[QB][3];
-            int it_[QB];
 #pragma unroll
             for (int q = 0; q < QB; q++) {
-                int t = tid + q * T;
-                it_[q] = -1;
-                if (t < nitem && !(nyq && (t / 5) == Nu)) {
-                    int al = t / 5, j = t - al * 5;
+                int t = tl + q * T;
+                int al = t / 5, j = t - al * 5;
+                if (t < 5 * M && !(nyq && al == Nu)) {
                     float2 s6[6];
 #pragma unroll
                     for (int c = 0; c < 6; c++) s6[c] = buf[(c * 5 + j) * M + al];
@@ -408,76 +473,84 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                     }
                     PointGeom g = load_geom(a.geom, e, i * 5 + j);
                     float nb = -(float)al;
-                    Xo[q][0] = cadd(cscale(g.dzdeta, s6[0]), cscale(g.dsdeta, s6[4]));
-                    Xo[q][1] = cadd(cscale(g.dzdeta, s6[5]), cscale(g.dsdeta, s6[3]));
-                    Xo[q][2] = cadd(cscale(g.dzdeta, s6[4]), cscale(g.dsdeta, s6[2]));
-                    Yo[q][0] = cadd(cscale(g.dzdxii, s6[0]), cscale(g.dsdxii, s6[4]));
-                    Yo[q][1] = cadd(cscale(g.dzdxii, s6[5]), cscale(g.dsdxii, s6[3]));
-                    Yo[q][2] = cadd(cscale(g.dzdxii, s6[4]), cscale(g.dsdxii, s6[2]));
-                    go[q][0] = cadd(s6[1], cimul(nb, s6[5]));
-                    go[q][1] = csub(cimul(nb, s6[1]), s6[5]);
-                    go[q][2] = cimul(nb, s6[3]);
-                    if (axial && i == 0 && al == 1) {
-                        // (dzdxii g) G^T_eta on row 0 has the shape of the Y term
-                        Yo[q][0] = cadd(Yo[q][0], cscale(g.dzdxii, go[q][0]));
-                        Yo[q][1] = cadd(Yo[q][1], cscale(g.dzdxii, go[q][1]));
+                    float2 X[3], Y[3], gg[3];
+                    X[0] = cadd(cscale(g.dzdeta, s6[0]), cscale(g.dsdeta, s6[4]));
+                    X[1] = cadd(cscale(g.dzdeta, s6[5]), cscale(g.dsdeta, s6[3]));
+                    X[2] = cadd(cscale(g.dzdeta, s6[4]), cscale(g.dsdeta, s6[2]));
+                    Y[0] = cadd(cscale(g.dzdxii, s6[0]), cscale(g.dsdxii, s6[4]));
+                    Y[1] = cadd(cscale(g.dzdxii, s6[5]), cscale(g.dsdxii, s6[3]));
+                    Y[2] = cadd(cscale(g.dzdxii, s6[4]), cscale(g.dsdxii, s6[2]));
+                    gg[0] = cadd(s6[1], cimul(nb, s6[5]));
+                    gg[1] = csub(cimul(nb, s6[1]), s6[5]);
+                    gg[2] = cimul(nb, s6[3]);
+                    // own point: inv_s g
+#pragma unroll
+                    for (int ip = 0; ip < 5; ip++)
+                        if (ip == i)
+#pragma unroll
+                            for (int c = 0; c < 3; c++) acc[q][ip][c] = cadd(acc[q][ip][c], cscale(g.inv_s, gg[c]));
+                    // axial row 0: G_xi(:,0) dzdeta g has the shape of the X term,
+                    // (dzdxii g) G^T_eta that of the Y term
+                    if (axial && i == 0) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) X[c] = cadd(X[c], cscale(g.dzdeta, gg[c]));
+                        if (al == 1) {
+                            Y[0] = cadd(Y[0], cscale(g.dzdxii, gg[0]));
+                            Y[1] = cadd(Y[1], cscale(g.dzdxii, gg[1]));
+                        }
                     }
-                    it_[q] = t;
+                    // X term: column j of every row.  For axial elements the W term
+                    // uses G_xi(ip, 0) and the X term G_xi(ip, i); on row 0 both are
+                    // G_xi(ip, 0), so they were merged above.
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+#pragma unroll
+                        for (int ip = 0; ip < 5; ip++)
+                            acc[q][ip][c] = cadd(acc[q][ip][c], cscale(Gx[ip * 5 + i], X[c]));
+#pragma unroll
+                    for (int c = 0; c < 3; c++) buf[(c * 5 + j) * M + al] = Y[c];
                 }
             }
             __syncthreads();
+            // Y term: row i, f(i, j) += sum_m Y(i, m) G_eta(j, m)
 #pragma unroll
             for (int q = 0; q < QB; q++) {
-                if (it_[q] < 0) continue;
-                int al = it_[q] / 5, j = it_[q] - al * 5;
+                int t = tl + q * T;
+                int al = t / 5, j = t - al * 5;
+                if (t < 5 * M && !(nyq && al == Nu)) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) buf[(c * 5 + j) * M + al] = Yo[q][c];
-            }
-            __syncthreads();
-            // 2) accumulate
+                    for (int c = 0; c < 3; c++) {
+                        float2 f = czero();
 #pragma unroll
-            for (int q = 0; q < QB; q++) {
-                if (it_[q] < 0) continue;
-                int al = it_[q] / 5, j = it_[q] - al * 5;
-                PointGeom g = load_geom(a.geom, e, i * 5 + j);
+                        for (int m = 0; m < 5; m++) {
+                            float ge = Ge[j * 5 + m];
+                            float2 y = buf[(c * 5 + m) * M + al];
+                            f.x += y.x * ge;
+                            f.y += y.y * ge;
+                        }
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    // own point: Y G^T_eta + inv_s g
-                    float2 f = cscale(g.inv_s, go[q][c]);
-#pragma unroll
-                    for (int m = 0; m < 5; m++) {
-                        float ge = Ge[j * 5 + m];
-                        float2 y = buf[(c * 5 + m) * M + al];
-                        f.x += y.x * ge;
-                        f.y += y.y * ge;
-                    }
-                    float2 *ap = acc + ((i * 5 + j) * M + al) * 3 + c;
-                    *ap = cadd(*ap, f);
-                    // X term (and axial G_xi(:,0) dzdeta g on row 0) to column j
-                    float2 w = (axial && i == 0) ? cscale(g.dzdeta, go[q][c]) : czero();
-#pragma unroll
-                    for (int ip = 0; ip < 5; ip++) {
-                        float gx = Gx[ip * 5 + i];
-                        float2 add = cscale(gx, Xo[q][c]);
-                        if (axial && i == 0) add = cadd(add, cscale(Gx[ip * 5 + 0], w));
-                        float2 *bp = acc + ((ip * 5 + j) * M + al) * 3 + c;
-                        *bp = cadd(*bp, add);
+                        for (int ip = 0; ip < 5; ip++)
+                            if (ip == i) acc[q][ip][c] = cadd(acc[q][ip][c], f);
                     }
                 }
             }
-            __syncthreads();
         }
     }
 
-    // ---------------- gatherStiffFromElement: stiff -= acc on valid rows
-    for (int t = tid; t < 25 * M; t += T) {
-        int k = t / M, al = t - k * M;
-        if (al > a.ep_amax[e * 25 + k]) continue;
-        if (nyq && al == Nu) continue;
-        float2 *sp = a.stiff + ((int64_t)a.ep_row[e * 25 + k] + al) * 3;
-        const float2 *ap = acc + (k * M + al) * 3;
+    // ---------------- gatherStiffFromElement: stiff -= f on valid rows
 #pragma unroll
-        for (int c = 0; c < 3; c++) sp[c] = csub(sp[c], ap[c]);
+    for (int q = 0; q < QB; q++) {
+        int t = tid + q * T;
+        int al = t / 5, j = t - al * 5;
+        if (t >= 5 * M || (nyq && al == Nu)) continue;
+#pragma unroll
+        for (int ip = 0; ip < 5; ip++) {
+            int k = ip * 5 + j;
+            if (al > a.ep_amax[e * 25 + k]) continue;
+            float2 *sp = a.stiff + ((int64_t)a.ep_row[e * 25 + k] + al) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) sp[c] = csub(sp[c], acc[q][ip][c]);
+        }
     }
 }
 
