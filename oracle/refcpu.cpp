@@ -480,6 +480,7 @@ struct Ref {
     // SourceTerm::apply (core/source/SourceTerm.cpp:29-35) + SolidPoint::addToStiff
     void applySource(int tstep) {
         if (srcElem < 0) return;
+        if (tstep < 0 || tstep >= (int)stf.size()) throw std::runtime_error("Domain::applySource || time step out of range.");
         Real f = stf[tstep];
         for (int k = 0; k < 25; k++) {
             SolidPt &p = S[E[srcElem].pts[k]];
@@ -1020,6 +1021,8 @@ struct Ref {
     // SolidElement::computeGroundMotion + PointwiseRecorder::record
     // (SolidElement.cpp:189-216, output/pointwise/PointwiseRecorder.cpp:62-93)
     void record(int tstep) {
+        if (recs.empty()) return;
+        if (tstep < 0 || tstep >= nstep) throw std::runtime_error("Domain::record || time step out of range.");
         C ii(zero, one);
         for (size_t ir = 0; ir < recs.size(); ir++) {
             const Rec &rc = recs[ir];
@@ -1093,6 +1096,8 @@ struct Ref {
     // Newmark::solve loop body (core/newmark/Newmark.cpp:47-93)
     void run(int first, int count, int stab) {
         typedef std::chrono::steady_clock clk;
+        if (first < 1 || count < 0 || first + count - 1 > (int)stf.size())
+            throw std::runtime_error("Newmark::solve || step range outside the source-time function.");
         for (int i = 0; i < 5; i++) cost[i] = 0;
         for (int tstep = first; tstep < first + count; tstep++) {
             auto t0 = clk::now();
@@ -1283,6 +1288,19 @@ int refcpu_fft_r2c(int nr, int use_double, const double *in_r, double *out_c) {
         }
     })
 }
+// nodes, weights and derivative matrix (row-major G[i][j] = l_i'(x_j)) of the
+// shared spectral header; kind 0 = GLL, 1 = GLJ(0,1)
+int refcpu_spectral(int kind, double *x, double *w, double *G) {
+    GUARD({
+        const auto &b = kind ? axg::spectral::glj() : axg::spectral::gll();
+        for (int i = 0; i < 5; i++) {
+            x[i] = b.x[i];
+            w[i] = b.w[i];
+            for (int j = 0; j < 5; j++) G[i * 5 + j] = b.G[i][j];
+        }
+    })
+}
+
 const char *refcpu_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"

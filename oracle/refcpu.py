@@ -38,6 +38,7 @@ def lib():
             "refcpu_last_run_cost": ([vp, f64p], C.c_int),
             "refcpu_fft_c2r": ([C.c_int, C.c_int, f64p, f64p], C.c_int),
             "refcpu_fft_r2c": ([C.c_int, C.c_int, f64p, f64p], C.c_int),
+            "refcpu_spectral": ([C.c_int, f64p, f64p, f64p], C.c_int),
             "refcpu_last_error": ([], C.c_char_p),
         }
         for name, (a, r) in sig.items():
@@ -137,3 +138,11 @@ def fft_r2c(x, nr, double=True):
     out = np.zeros(2 * (nr // 2 + 1))
     _check(lib().refcpu_fft_r2c(nr, int(double), x.ctypes.data_as(f64p), out.ctypes.data_as(f64p)))
     return out[0::2] + 1j * out[1::2]
+
+
+def spectral(kind):
+    """(nodes, weights, G) of the spectral header: kind 0 = GLL, 1 = GLJ(0,1)."""
+    x, w, G = np.zeros(5), np.zeros(5), np.zeros(25)
+    _check(lib().refcpu_spectral(kind, x.ctypes.data_as(f64p), w.ctypes.data_as(f64p),
+                                 G.ctypes.data_as(f64p)))
+    return x, w, G.reshape(5, 5)

@@ -1,0 +1,412 @@
+"""Pins of the CPU oracle (oracle/refcpu.cpp) and of the preloop it shares with
+the product, against what fixes them independently of their own code:
+
+* the reference's tabulated nPol=4 spectral constants
+  (SOLVER/src/preloop/spectral/SpectralConstants.cpp:43-49) -> tests/golden;
+* a brute-force DFT (numpy) with the reference's FFTW conventions: C2R
+  unnormalised, R2C times 1/Nr (SOLVER/src/core/fft/SolverFFTW_N6.cpp:45-53);
+* the lucky-number rule by prime factorisation (PreloopFFTW.cpp:59-109);
+* the mass integral of the layered model (Material.cpp:246-253, 336-363:
+  mass = sum of rho * integral factor = 2 * int rho r^2 dr over the meridian);
+* rigid-body motions lie in the null space of the stiffness operator, which
+  is self-adjoint and positive (SolidElement::test, SolidElement.cpp:96-187);
+* modes never couple for 1-D material; the 3-D (FFT) path equals the 1-D
+  (Fourier-space) path for a laterally constant 3-D material;
+* monopole / dipole / quadrupole moment tensors excite only alpha = 0 / 1 / 2
+  (Earthquake.cpp:73-83) and a moment-tensor source exerts no net force;
+* Newmark (beta = 0, gamma = 1/2) is the central-difference scheme:
+  d[n+1] - 2 d[n] + d[n-1] = dt^2 a[n] and v[n] = (d[n+1] - d[n-1]) / (2 dt);
+* the SLS coefficients (AttAxiSEM.cpp:24-56) integrate the memory-variable
+  ODE dR/dt = -w (R - y sigma) exactly for piecewise-linear sigma.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import axisem3d_amd._abi as abi
+from axisem3d_amd.mesh import ShellMesh, shell_config
+
+from tests.helpers import random_field, rel_l2
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def refcpu():
+    from oracle import refcpu as r
+    r.lib()
+    return r
+
+
+# ---------------------------------------------------------------------------
+# small meshes
+# ---------------------------------------------------------------------------
+
+def _small_layers(fluid=False):
+    layers = [dict(r_bottom=5.0e6, r_top=5.7e6, nelem_radial=2, fluid=0,
+                   vp_bottom=11000., vp_top=10300., vs_bottom=6200., vs_top=5600.,
+                   rho_bottom=5500., rho_top=4400., q_mu=0., q_kappa=57823.),
+              dict(r_bottom=5.7e6, r_top=6.371e6, nelem_radial=2, fluid=0,
+                   vp_bottom=8900., vp_top=5800., vs_bottom=4800., vs_top=3200.,
+                   rho_bottom=3500., rho_top=2600., q_mu=0., q_kappa=57823.)]
+    if fluid:
+        layers.insert(0, dict(r_bottom=3.0e6, r_top=5.0e6, nelem_radial=2, fluid=1,
+                              vp_bottom=10300., vp_top=8000., vs_bottom=0., vs_top=0.,
+                              rho_bottom=12200., rho_top=9900., q_mu=0., q_kappa=57823.))
+    return layers
+
+
+def small_mesh(material=0, nu=None, dv_amp=0.0, fluid=False, nelem_theta=8, **kw):
+    over = dict(nelem_theta=nelem_theta, material=material, dv_amp=dv_amp, nstep=60,
+                nrec=4, nthread=1)
+    if nu is None:
+        over.update(nu_mode=0, nu_const=3)
+    else:
+        over.update(nu_mode=1, nu_min=nu[0], nu_max=nu[1])
+    over.update(kw)
+    cfg = shell_config(0, layers=_small_layers(fluid), **over)
+    return ShellMesh(cfg)
+
+
+def solid_rows(mesh):
+    """(point, nu) of every solid point in field order."""
+    nr = mesh.array("point_nr")
+    fl = mesh.array("point_flags")
+    return [(p, int(nr[p]) // 2) for p in range(mesh.d.npoint) if fl[p] & abi.PT_SOLID]
+
+
+def field_from(mesh, fn):
+    """Solid field (canonical real layout) from fn(point, alpha, nu) -> 3 complex."""
+    out = []
+    for p, nu in solid_rows(mesh):
+        for a in range(nu + 1):
+            v = fn(p, a, nu)
+            for c in range(3):
+                out += [np.real(v[c]), np.imag(v[c])]
+    return np.array(out)
+
+
+def mode_weights(mesh):
+    """Per real entry of a solid field: 1 for alpha = 0, 2 for alpha > 0 (Parseval)."""
+    return field_from(mesh, lambda p, a, nu: [1. + 1j] * 3 if a == 0 else [2. + 2j] * 3)
+
+
+def stiff_of(dom, mesh, u):
+    dom.set_field(abi.FIELD_SOLID_DISPL, u)
+    dom.set_field(abi.FIELD_SOLID_STIFF, np.zeros_like(u))
+    if mesh.field_size(abi.FIELD_FLUID_DISPL):
+        z = np.zeros(mesh.field_size(abi.FIELD_FLUID_DISPL))
+        dom.set_field(abi.FIELD_FLUID_DISPL, z)
+        dom.set_field(abi.FIELD_FLUID_STIFF, z)
+    dom.computeStiff()
+    return dom.get_field(abi.FIELD_SOLID_STIFF)
+
+
+# ---------------------------------------------------------------------------
+# spectral constants
+# ---------------------------------------------------------------------------
+
+def test_spectral_constants_match_reference_table(refcpu):
+    g = json.load(open(os.path.join(GOLDEN, "spectral_npol4.json")))
+    for kind, (xn, wn, gn) in enumerate([("pgll", "wgll", "Ggll"), ("pglj", "wglj", "Gglj")]):
+        x, w, G = refcpu.spectral(kind)
+        Gref = np.array(g[gn]).reshape(5, 5)
+        assert np.abs(x - g[xn]).max() < 1e-11
+        assert np.abs(w - g[wn]).max() < 1e-11
+        assert np.abs(G - Gref).max() < 1e-11
+        # orientation: G[i][j] = l_i'(x_j), not its transpose
+        assert np.abs(G.T - Gref).max() > 1.0
+
+
+def test_spectral_quadrature_and_derivative_exactness(refcpu):
+    xg, wg, Gg = refcpu.spectral(0)
+    xj, wj, Gj = refcpu.spectral(1)
+    for k in range(8):  # GLL with 5 points is exact to degree 7
+        exact = (1. - (-1.) ** (k + 1)) / (k + 1)
+        assert abs(np.dot(wg, xg ** k) - exact) < 1e-13
+    for k in range(7):  # GLJ(0,1) weight (1+x), exact to degree 6
+        exact = (1. - (-1.) ** (k + 1)) / (k + 1) + (1. - (-1.) ** (k + 2)) / (k + 2)
+        assert abs(np.dot(wj, xj ** k) - exact) < 1e-13
+    for x, G in ((xg, Gg), (xj, Gj)):
+        for k in range(5):  # derivative of the interpolant of x^k at the nodes
+            d = G.T @ (x ** k)
+            assert np.abs(d - k * x ** max(k - 1, 0) * (k > 0)).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# FFT conventions
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nr", [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 16, 18, 26, 54, 130, 256, 260, 286])
+def test_fft_matches_dft(refcpu, nr):
+    rng = np.random.default_rng(nr)
+    nc = nr // 2 + 1
+    X = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    X[0] = X[0].real
+    if nr % 2 == 0:
+        X[-1] = X[-1].real
+    x_ref = np.fft.irfft(X, nr) * nr  # unnormalised C2R
+    for dbl, tol in ((True, 1e-12), (False, 2e-6)):
+        x = refcpu.fft_c2r(X, nr, double=dbl)
+        assert rel_l2(x, x_ref) < tol
+        xr = rng.standard_normal(nr)
+        Y = refcpu.fft_r2c(xr, nr, double=dbl)
+        assert rel_l2(Y, np.fft.rfft(xr) / nr) < tol
+
+
+# ---------------------------------------------------------------------------
+# mesh: lucky numbers and mass
+# ---------------------------------------------------------------------------
+
+def _is_lucky(n, force_odd):
+    if force_odd and n % 2 == 0:
+        return False
+    if not force_odd and n % 2 == 1 and n > 10:
+        return False
+    m, big = n, 0
+    for p in (2, 3, 5, 7, 11, 13):
+        while m % p == 0:
+            m //= p
+            big += p >= 11
+    return m == 1 and big <= 1
+
+
+def test_point_nr_are_lucky():
+    mesh = small_mesh(material=1, nu=(8, 64), dv_amp=0.05)
+    nr = mesh.array("point_nr")
+    assert nr.max() > 10
+    for n in np.unique(nr):
+        assert _is_lucky(int(n), False) or _is_lucky(int(n), True), n
+    # the smallest lucky number at or above 2 Nu + 1 for uniform Nu away from the axis
+    m2 = small_mesh(nu=None, nu_const=7)
+    sz = m2.array("point_sz").reshape(-1, 2)
+    far = sz[:, 0] > 0.3 * 6.371e6
+    assert set(np.unique(m2.array("point_nr")[far]).tolist()) == {16}
+
+
+def test_mass_is_the_layer_integral():
+    layers = _small_layers(fluid=True)
+    mesh = small_mesh(fluid=True, nelem_theta=12)
+    fl = mesh.array("point_flags")
+    ms = mesh.array("inv_mass_solid").astype(np.float64)
+    mf = mesh.array("inv_mass_fluid").astype(np.float64)
+    tot_s = (1. / ms[ms > 0]).sum()
+    tot_f = (1. / mf[mf > 0]).sum()
+    solid_int = fluid_int = 0.
+    for L in layers:
+        r = np.linspace(L["r_bottom"], L["r_top"], 20001)
+        t = (r - r[0]) / (r[-1] - r[0])
+        rho = L["rho_bottom"] + t * (L["rho_top"] - L["rho_bottom"])
+        vp = L["vp_bottom"] + t * (L["vp_top"] - L["vp_bottom"])
+        if L["fluid"]:
+            fluid_int += 2. * np.trapezoid(r ** 2 / (rho * vp ** 2), r)
+        else:
+            solid_int += 2. * np.trapezoid(rho * r ** 2, r)
+    assert (fl & abi.PT_FLUID).any()
+    assert abs(tot_s / solid_int - 1.) < 1e-5
+    assert abs(tot_f / fluid_int - 1.) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# stiffness operator
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("material,nu,dv", [(0, None, 0.), (1, None, 0.), (2, None, 0.),
+                                            (1, (6, 20), 0.05), (2, (6, 20), 0.05)])
+def test_stiffness_self_adjoint_positive_rigid_null(refcpu, material, nu, dv):
+    mesh = small_mesh(material=material, nu=nu, dv_amp=dv)
+    dom = refcpu.RefDomain(mesh, double=True)
+    w = mode_weights(mesh)
+    u = random_field(mesh, abi.FIELD_SOLID_DISPL, 1, scale=1.)
+    v = random_field(mesh, abi.FIELD_SOLID_DISPL, 2, scale=1.)
+    Ku = -stiff_of(dom, mesh, u)
+    Kv = -stiff_of(dom, mesh, v)
+    # <u, K v> = <v, K u> under the Parseval weights
+    a, b = np.dot(w * u, Kv), np.dot(w * v, Ku)
+    scale = np.linalg.norm(Ku) * np.linalg.norm(v)
+    assert abs(a - b) < 1e-10 * scale
+    assert np.dot(w * u, Ku) > 0.
+    normK = np.linalg.norm(Ku) / np.linalg.norm(u)
+    # translation along z: u_z = 1 at alpha = 0
+    tz = field_from(mesh, lambda p, a, nu: [0., 0., 1.] if a == 0 else [0., 0., 0.])
+    # translation along x: u_s = 1/2, u_phi = i/2 at alpha = 1
+    tx = field_from(mesh, lambda p, a, nu: [0.5, 0.5j, 0.] if a == 1 else [0., 0., 0.])
+    for t in (tz, tx):
+        r = np.linalg.norm(stiff_of(dom, mesh, t)) / (normK * np.linalg.norm(t))
+        assert r < 1e-11, r
+
+
+def test_modes_do_not_couple_for_1d_material(refcpu):
+    mesh = small_mesh(material=2, nu=None, nu_const=4)
+    dom = refcpu.RefDomain(mesh, double=True)
+    rng = np.random.default_rng(5)
+    for alpha in (0, 1, 2, 3):
+        def fn(p, a, nu):
+            if a != alpha:
+                return [0., 0., 0.]
+            return list(rng.standard_normal(3) + (1j * rng.standard_normal(3) if a else 0.))
+        u = field_from(mesh, fn)
+        mask = field_from(mesh, lambda p, a, nu: [1. + 1j] * 3 if a == alpha else [0.] * 3) != 0
+        s = stiff_of(dom, mesh, u)
+        assert np.abs(s[mask]).max() > 0.
+        assert np.abs(s[~mask]).max() == 0.
+
+
+class _Lifted3D:
+    """The same mesh with every solid element's 1-D material block repeated on
+    its Nr azimuthal samples and flagged 3-D (a laterally constant 3-D model)."""
+
+    def __init__(self, mesh):
+        import ctypes as C
+        self.mesh = mesh
+        ne = mesh.d.nelem
+        flags = mesh.array("elem_flags").copy()
+        off = mesh.array("mat_off")
+        pool = mesh.array("mat_pool")
+        enr = mesh.array("point_nr")[mesh.array("elem_points").reshape(ne, 25)].max(axis=1)
+        blocks, new_off = [], [0]
+        for e in range(ne):
+            blk = pool[off[e]:off[e + 1]]
+            if not flags[e] & abi.EL_FLUID:
+                npar = blk.size // 25
+                blk = np.repeat(blk.reshape(npar, 1, 25), enr[e], axis=1).ravel()
+                flags[e] |= abi.EL_3D
+            blocks.append(blk)
+            new_off.append(new_off[-1] + blk.size)
+        self._flags = flags.astype(np.int32)
+        self._off = np.array(new_off, dtype=np.int64)
+        self._pool = np.concatenate(blocks).astype(np.float32)
+        d = abi.MeshDesc.from_buffer_copy(mesh.d)
+        d.elem_flags = self._flags.ctypes.data_as(abi.i32p)
+        d.mat_off = self._off.ctypes.data_as(abi.i64p)
+        d.mat_pool = self._pool.ctypes.data_as(abi.f32p)
+        self._d = d
+        self.desc = C.pointer(d)
+
+    @property
+    def d(self):
+        return self._d
+
+    def field_size(self, which):
+        return self.mesh.field_size(which)
+
+
+@pytest.mark.parametrize("material", [0, 1, 2])
+def test_3d_path_equals_1d_path_for_constant_material(refcpu, material):
+    m1 = small_mesh(material=material, nu=(6, 20), dv_amp=0.)
+    assert not (m1.array("elem_flags") & abi.EL_3D).any()
+    m3 = _Lifted3D(m1)
+    u = random_field(m1, abi.FIELD_SOLID_DISPL, 3, scale=1.)
+    s1 = stiff_of(refcpu.RefDomain(m1, double=True), m1, u)
+    s3 = stiff_of(refcpu.RefDomain(m3, double=True), m3, u)
+    assert rel_l2(s3, s1) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# source
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("moment,alpha", [((1., 1., 1., 0., 0., 0.), 0),
+                                          ((1., -.5, -.5, 0., 0., 0.), 0),
+                                          ((0., 0., 0., 1., 0., 0.), 1),
+                                          ((0., 0., 0., 0., 1., 0.), 1),
+                                          ((0., 1., -1., 0., 0., 0.), 2),
+                                          ((0., 0., 0., 0., 0., 1.), 2)])
+def test_source_mode_separation_and_no_net_force(moment, alpha):
+    mesh = small_mesh(moment=tuple(1e20 * m for m in moment))
+    f = mesh.array("src_force").reshape(25, 3, 3, 2).astype(np.float64)
+    amp = np.abs(f).max()
+    assert amp > 0.
+    for a in range(3):
+        if a != alpha:
+            assert np.abs(f[:, a]).max() == 0.
+    # no net force: the translation modes (alpha = 0 and 1; alpha = 2 rows are
+    # dropped at near-axis points whose Nu < 2) sum to zero over the element
+    assert np.abs(f[:, :2].sum(axis=0)).max() < 1e-6 * amp
+
+
+# ---------------------------------------------------------------------------
+# Newmark
+# ---------------------------------------------------------------------------
+
+def test_newmark_is_central_difference(refcpu):
+    mesh = small_mesh()
+    dom = refcpu.RefDomain(mesh, double=True)
+    dt = 0.37
+    n = mesh.field_size(abi.FIELD_SOLID_DISPL)
+    im, off = mesh.array("inv_mass_solid"), mesh.array("mass_solid_off")
+    invm = field_from(mesh, lambda p, a, nu: [float(im[off[p]]) * (1 + 1j)] * 3)
+    for k in (abi.FIELD_SOLID_DISPL, abi.FIELD_SOLID_VELOC, abi.FIELD_SOLID_ACCEL):
+        dom.set_field(k, random_field(mesh, k, 10 + k, scale=dt ** (2 - k)))
+    ds, vs, acc = [dom.get_field(abi.FIELD_SOLID_DISPL)], [], []
+    for step in range(6):
+        F = random_field(mesh, abi.FIELD_SOLID_STIFF, 20 + step, scale=1e18)
+        dom.set_field(abi.FIELD_SOLID_STIFF, F)
+        dom.updateNewmark(dt)
+        ds.append(dom.get_field(abi.FIELD_SOLID_DISPL))
+        vs.append(dom.get_field(abi.FIELD_SOLID_VELOC))
+        acc.append(dom.get_field(abi.FIELD_SOLID_ACCEL))
+        assert rel_l2(acc[-1], invm * F) < 1e-14
+        assert np.abs(dom.get_field(abi.FIELD_SOLID_STIFF)).max() == 0.
+    assert n == ds[0].size
+    for k in range(1, 5):
+        # displ after update k is d[k+1]; accel/veloc after update k are a[k], v[k]
+        lhs = ds[k + 1] - 2. * ds[k] + ds[k - 1]
+        assert rel_l2(lhs, dt * dt * acc[k]) < 1e-9
+        assert rel_l2((ds[k + 1] - ds[k - 1]) / (2. * dt), vs[k]) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# attenuation coefficients
+# ---------------------------------------------------------------------------
+
+def test_sls_coefficients_integrate_memory_ode_exactly():
+    from scipy.integrate import solve_ivp
+    mesh = small_mesh(attenuation=abi.ATT_FULL, nsls=5)
+    al = mesh.array("sls_alpha").astype(np.float64)
+    be = mesh.array("sls_beta").astype(np.float64)
+    ga = mesh.array("sls_gamma").astype(np.float64)
+    dt = mesh.d.dt
+    assert al.size == 5 and np.all((al > 0) & (al < 1))
+    w = -np.log(al) / dt
+    y = (be + ga) / (1. - al)          # normalised weights y_s / sum(y)
+    assert abs(y.sum() - 1.) < 1e-5
+    rng = np.random.default_rng(0)
+    nstep = 12
+    sig = rng.standard_normal(nstep + 1)
+    t = np.arange(nstep + 1) * dt
+    for s in range(5):
+        R = 0.
+        for n in range(nstep):
+            R = al[s] * R + be[s] * sig[n] + ga[s] * sig[n + 1]
+        # the ODE under the piecewise-linear interpolant of the samples
+        rhs = lambda tt, r: -w[s] * (r - y[s] * np.interp(tt, t, sig))
+        sol = solve_ivp(rhs, (0., t[-1]), [0.], rtol=1e-10, atol=1e-12, max_step=dt / 50)
+        assert abs(R - sol.y[0, -1]) < 2e-5 * max(1., abs(sol.y[0, -1]))
+
+
+# ---------------------------------------------------------------------------
+# precision convergence of the whole time loop
+# ---------------------------------------------------------------------------
+
+def test_fp32_oracle_converges_to_fp64(refcpu):
+    mesh = small_mesh(material=1, nu=(6, 20), dv_amp=0.05, fluid=True, nstep=200)
+    s = []
+    for dbl in (False, True):
+        dom = refcpu.RefDomain(mesh, double=dbl)
+        dom.run(1, mesh.d.nstep, 0)
+        s.append(dom.seismograms())
+    assert np.abs(s[1]).max() > 0.
+    assert rel_l2(s[0], s[1]) < 1e-4
+
+
+def test_oracle_rejects_steps_outside_the_stf(refcpu):
+    mesh = small_mesh()
+    dom = refcpu.RefDomain(mesh)
+    for first, count in ((0, 5), (mesh.d.nstep, 2)):
+        with pytest.raises(RuntimeError, match="Newmark::solve"):
+            dom.run(first, count, 0)
+    with pytest.raises(RuntimeError, match="Domain::applySource"):
+        dom.applySource(mesh.d.nstep)
