@@ -30,7 +30,10 @@
 #include "axg.h"
 #include "spectral.hpp"
 
+#include <sys/mman.h>
+
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cfloat>
 #include <cmath>
@@ -204,10 +207,49 @@ struct GRF {
         }
         norm = std::sqrt(2. / nmode);
     }
-    double operator()(double x, double y, double z) const {
+    double exact(double x, double y, double z) const {
         double f = 0.;
         for (const auto &m : k) f += std::cos(m[0] * x + m[1] * y + m[2] * z + m[3]);
         return f * norm;
+    }
+    // The field is tabulated once on a Cartesian lattice covering [-R, R]^3 with
+    // spacing ell / 4 and read back by trilinear interpolation: the models sample
+    // it at every azimuthal slice of every GLL point (~10^8 samples for a
+    // 20000-quad mesh), which the 48-mode sum makes the dominant mesh-build cost.
+    int gn = 0;
+    double glo = 0., gh = 1.;
+    std::vector<float> grid;
+    void tabulate(double R, double ell, int nthread) {
+        gh = ell / 4.;
+        gn = (int)std::ceil(2. * R / gh) + 3;
+        glo = -R - gh;
+        grid.assign((size_t)gn * gn * gn, 0.f);
+        std::vector<std::thread> th;
+        nthread = std::max(1, nthread);
+        for (int t = 0; t < nthread; t++)
+            th.emplace_back([&, t]() {
+                for (int iz = t; iz < gn; iz += nthread)
+                    for (int iy = 0; iy < gn; iy++)
+                        for (int ix = 0; ix < gn; ix++)
+                            grid[((size_t)iz * gn + iy) * gn + ix] =
+                                (float)exact(glo + ix * gh, glo + iy * gh, glo + iz * gh);
+            });
+        for (auto &x : th) x.join();
+    }
+    double operator()(double x, double y, double z) const {
+        if (gn == 0) return exact(x, y, z);
+        double fx = (x - glo) / gh, fy = (y - glo) / gh, fz = (z - glo) / gh;
+        int ix = std::min(std::max((int)fx, 0), gn - 2);
+        int iy = std::min(std::max((int)fy, 0), gn - 2);
+        int iz = std::min(std::max((int)fz, 0), gn - 2);
+        double tx = fx - ix, ty = fy - iy, tz = fz - iz;
+        auto at = [&](int a, int b, int cc) { return (double)grid[((size_t)(iz + cc) * gn + (iy + b)) * gn + (ix + a)]; };
+        double c00 = at(0, 0, 0) * (1. - tx) + at(1, 0, 0) * tx;
+        double c10 = at(0, 1, 0) * (1. - tx) + at(1, 1, 0) * tx;
+        double c01 = at(0, 0, 1) * (1. - tx) + at(1, 0, 1) * tx;
+        double c11 = at(0, 1, 1) * (1. - tx) + at(1, 1, 1) * tx;
+        double c0 = c00 * (1. - ty) + c10 * ty, c1 = c01 * (1. - ty) + c11 * ty;
+        return c0 * (1. - tz) + c1 * tz;
     }
 };
 
@@ -240,18 +282,24 @@ struct Layer {
     double rho(double r) const { return valAt(p.r_bottom, p.r_top, p.rho_bottom, p.rho_top, r); }
 };
 
+// fn(begin, end) over [0, n) in chunks of 16 handed out dynamically (element
+// costs vary with Nr by two orders of magnitude)
 void parallelFor(int n, int nthread, const std::function<void(int, int)> &fn) {
     if (nthread <= 1 || n < 64) {
         fn(0, n);
         return;
     }
+    std::atomic<int> next(0);
+    const int chunk = 16;
     std::vector<std::thread> th;
-    int chunk = (n + nthread - 1) / nthread;
-    for (int t = 0; t < nthread; t++) {
-        int b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        th.emplace_back(fn, b, e);
-    }
+    for (int t = 0; t < nthread; t++)
+        th.emplace_back([&]() {
+            for (;;) {
+                int b = next.fetch_add(chunk);
+                if (b >= n) break;
+                fn(b, std::min(n, b + chunk));
+            }
+        });
     for (auto &x : th) x.join();
 }
 
@@ -276,6 +324,35 @@ bool isSPD6(const double C[6][6]) {
 }  // namespace
 
 // ===========================================================================
+// Allocator of the multi-GB material pools: 2 MiB-aligned and advised as huge
+// pages, so that first touch costs one fault per 2 MiB instead of per 4 KiB.
+template <class T>
+struct HugeAlloc {
+    using value_type = T;
+    static constexpr size_t kHuge = size_t(2) << 20;
+    HugeAlloc() = default;
+    template <class U>
+    HugeAlloc(const HugeAlloc<U> &) {}
+    T *allocate(size_t n) {
+        size_t bytes = n * sizeof(T);
+        if (bytes < 2 * kHuge) return static_cast<T *>(::operator new(bytes));
+        size_t sz = (bytes + kHuge - 1) / kHuge * kHuge;
+        void *p = std::aligned_alloc(kHuge, sz);
+        if (!p) throw std::bad_alloc();
+        madvise(p, sz, MADV_HUGEPAGE);
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t n) {
+        if (n * sizeof(T) < 2 * kHuge) ::operator delete(p);
+        else std::free(p);
+    }
+    template <class U>
+    bool operator==(const HugeAlloc<U> &) const { return true; }
+    template <class U>
+    bool operator!=(const HugeAlloc<U> &) const { return false; }
+};
+typedef std::vector<float, HugeAlloc<float>> PoolVec;
+
 struct axg_mesh {
     axg_mesh_desc desc;
     // backing storage
@@ -285,7 +362,8 @@ struct axg_mesh {
     std::vector<float> inv_mass_solid, inv_mass_fluid;
     std::vector<int32_t> elem_points, elem_flags, elem_material, elem_att;
     std::vector<int64_t> mat_off, att_off;
-    std::vector<float> mat_pool, att_pool, elem_geom, elem_tiso;
+    PoolVec mat_pool, att_pool;
+    std::vector<float> elem_geom, elem_tiso;
     std::vector<float> sls_alpha, sls_beta, sls_gamma;
     std::vector<int32_t> sf_point;
     std::vector<int64_t> sf_off;
@@ -709,6 +787,10 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         B.grfVs = GRF(c.seed * 31ULL + 1ULL, c.dv_corr_len);
         B.grfVp = GRF(c.seed * 31ULL + 2ULL, c.dv_corr_len);
         B.grfAniso = GRF(c.seed * 31ULL + 3ULL, c.dv_corr_len);
+        const double Rout = B.rnode.back();
+        B.grfVs.tabulate(Rout, c.dv_corr_len, B.nthread);
+        B.grfVp.tabulate(Rout, c.dv_corr_len, B.nthread);
+        if (c.material == AXG_MAT_ANISOTROPIC) B.grfAniso.tabulate(Rout, c.dv_corr_len, B.nthread);
     }
     const bool att = c.attenuation != AXG_ATT_NONE;
     std::vector<double> slsW, slsY;
@@ -732,8 +814,6 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         int nr = 1;
         bool is3D = false, tiso = false;
         int material = 0;
-        std::vector<float> mat;    // pool block
-        std::vector<float> attv;   // attenuation block
         double mass[nP * nP] = {0.};  // rho * iFact (solid) or iFact / (rho vp^2) (fluid)
         double dtElem = DBL_MAX;
         double rhoPt[nP * nP] = {0.};
@@ -822,29 +902,50 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     AttFactors af;
     if (att) af = makeAttFactors(slsW, slsY, c.f_min, c.f_max, c.f_ref, dt);
 
+    // pool offsets: [param][row][k] material blocks, [dkappa | dmu][row][k or CG4 k]
+    // attenuation blocks; pass 2 writes the blocks in place
+    m.elem_att.resize(NQ);
+    m.mat_off.assign(NQ + 1, 0);
+    m.att_off.assign(NQ + 1, 0);
+    for (int e = 0; e < NQ; e++) {
+        const QuadInfo &q = B.quads[e];
+        const ElemOut &o = out[e];
+        int64_t nrows = o.is3D ? q.nr : 1;
+        int64_t npar = q.fluid ? 1 : c.material == AXG_MAT_ISOTROPIC ? 2 : c.material == AXG_MAT_TRANSVERSELY_ISO ? 5 : 21;
+        m.elem_att[e] = (q.fluid || !att) ? AXG_ATT_NONE : c.attenuation;
+        int64_t natt = m.elem_att[e] ? 2 * nrows * (c.attenuation == AXG_ATT_CG4 ? 4 : nP * nP) : 0;
+        m.mat_off[e + 1] = m.mat_off[e] + npar * nrows * nP * nP;
+        m.att_off[e + 1] = m.att_off[e] + natt;
+    }
+    m.mat_pool.resize(m.mat_off[NQ]);
+    m.att_pool.resize(m.att_off[NQ]);
+
     parallelFor(NQ, B.nthread, [&](int b, int eEnd) {
         std::vector<std::array<double, 6>> v;
+        std::vector<double> A, C, F, L, N, kp, mu, dkp, dmu, kpNew, muNew;
+        std::vector<std::array<double, 21>> Cij;
         for (int e = b; e < eEnd; e++) {
             const QuadInfo &q = B.quads[e];
             ElemOut &o = out[e];
             int nrows = o.is3D ? q.nr : 1;
+            float *mat = m.mat_pool.data() + m.mat_off[e];
+            float *attv = m.att_pool.data() + m.att_off[e];
             o.nr = q.nr;
             sampleElem(e, v, nrows);
             const int NPE = nP * nP;
             if (q.fluid) {
                 // Material::createAcoustic (Material.cpp:257-278): K = iFact / rho
                 o.material = AXG_MAT_ACOUSTIC;
-                o.mat.resize((size_t)nrows * NPE);
                 for (int row = 0; row < nrows; row++)
                     for (int k = 0; k < NPE; k++)
-                        o.mat[(size_t)row * NPE + k] = (float)(q.iFact[k] / v[(size_t)row * NPE + k][4]);
+                        mat[(size_t)row * NPE + k] = (float)(q.iFact[k] / v[(size_t)row * NPE + k][4]);
                 continue;
             }
             o.material = c.material;
             o.tiso = c.material != AXG_MAT_ISOTROPIC;
             // A C F L N scaled by the integral factor (Material.cpp:313-330)
             size_t n = (size_t)nrows * NPE;
-            std::vector<double> A(n), C(n), F(n), L(n), N(n), kp(n), mu(n);
+            for (auto *x : {&A, &C, &F, &L, &N, &kp, &mu}) x->assign(n, 0.);
             for (int row = 0; row < nrows; row++) {
                 for (int k = 0; k < NPE; k++) {
                     size_t i = (size_t)row * NPE + k;
@@ -857,7 +958,6 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 }
             }
             // full anisotropy: Voigt tensor in the (theta, phi, r) frame
-            std::vector<std::array<double, 21>> Cij;
             if (c.material == AXG_MAT_ANISOTROPIC) {
                 Cij.resize(n);
                 for (size_t i = 0; i < n; i++) {
@@ -896,7 +996,6 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 }
             }
             // attenuation: Voigt average, modulus factors (Material.cpp:336-363, 436-469)
-            std::vector<double> dkp, dmu;
             if (att) {
                 for (size_t i = 0; i < n; i++) {
                     if (c.material == AXG_MAT_ANISOTROPIC) {
@@ -917,7 +1016,8 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 double dKpF, kpA, kpN, dMuF, muA, muN;
                 attModulusFactors(af, Qkp, c.do_kappa != 0, dKpF, kpA, kpN);
                 attModulusFactors(af, Qmu, true, dMuF, muA, muN);
-                std::vector<double> kpNew(kp), muNew(mu);
+                kpNew = kp;
+                muNew = mu;
                 if (c.attenuation == AXG_ATT_CG4) {
                     // Quad::computeWeightsCG4 on the integral factor (Quad.cpp:438-466)
                     const double *f = q.iFact;
@@ -978,29 +1078,27 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                         N[i] += dm;
                     }
                 }
-                o.attv.resize(dkp.size() + dmu.size());
-                for (size_t i = 0; i < dkp.size(); i++) o.attv[i] = (float)dkp[i];
-                for (size_t i = 0; i < dmu.size(); i++) o.attv[dkp.size() + i] = (float)dmu[i];
+                if ((int64_t)(dkp.size() + dmu.size()) != m.att_off[e + 1] - m.att_off[e])
+                    throw std::runtime_error("Material::createAttenuation || block size mismatch.");
+                for (size_t i = 0; i < dkp.size(); i++) attv[i] = (float)dkp[i];
+                for (size_t i = 0; i < dmu.size(); i++) attv[dkp.size() + i] = (float)dmu[i];
             }
             if (c.material == AXG_MAT_ISOTROPIC) {
-                o.mat.resize(2 * n);
                 for (size_t i = 0; i < n; i++) {
-                    o.mat[i] = (float)F[i];      // lambda = F
-                    o.mat[n + i] = (float)L[i];  // mu = L
+                    mat[i] = (float)F[i];      // lambda = F
+                    mat[n + i] = (float)L[i];  // mu = L
                 }
             } else if (c.material == AXG_MAT_TRANSVERSELY_ISO) {
-                o.mat.resize(5 * n);
                 for (size_t i = 0; i < n; i++) {
-                    o.mat[i] = (float)A[i];
-                    o.mat[n + i] = (float)C[i];
-                    o.mat[2 * n + i] = (float)F[i];
-                    o.mat[3 * n + i] = (float)L[i];
-                    o.mat[4 * n + i] = (float)N[i];
+                    mat[i] = (float)A[i];
+                    mat[n + i] = (float)C[i];
+                    mat[2 * n + i] = (float)F[i];
+                    mat[3 * n + i] = (float)L[i];
+                    mat[4 * n + i] = (float)N[i];
                 }
             } else {
-                o.mat.resize(21 * n);
                 for (size_t i = 0; i < n; i++)
-                    for (int t = 0; t < 21; t++) o.mat[t * n + i] = (float)Cij[i][t];
+                    for (int t = 0; t < 21; t++) mat[t * n + i] = (float)Cij[i][t];
             }
         }
     });
@@ -1041,9 +1139,6 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     m.elem_points.resize((size_t)NQ * 25);
     m.elem_flags.resize(NQ);
     m.elem_material.resize(NQ);
-    m.elem_att.resize(NQ);
-    m.mat_off.assign(NQ + 1, 0);
-    m.att_off.assign(NQ + 1, 0);
     m.elem_geom.resize((size_t)NQ * 125);
     m.elem_tiso.resize((size_t)NQ * 100);
     for (int e = 0; e < NQ; e++) {
@@ -1057,9 +1152,6 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         if (o.tiso) f |= AXG_EL_TISO;
         m.elem_flags[e] = f;
         m.elem_material[e] = o.material;
-        m.elem_att[e] = (q.fluid || !att) ? AXG_ATT_NONE : c.attenuation;
-        m.mat_off[e + 1] = m.mat_off[e] + (int64_t)o.mat.size();
-        m.att_off[e + 1] = m.att_off[e] + (int64_t)(m.elem_att[e] ? o.attv.size() : 0);
         // Quad::createGraident (Quad.cpp:509-525)
         for (int k = 0; k < 25; k++) {
             const double(*J)[2] = q.J[k];
@@ -1085,12 +1177,6 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         else m.nsolid_elem++;
         if (o.is3D) m.n3d_elem++;
         m.max_nr = std::max<int64_t>(m.max_nr, q.nr);
-    }
-    m.mat_pool.resize(m.mat_off[NQ]);
-    m.att_pool.resize(m.att_off[NQ]);
-    for (int e = 0; e < NQ; e++) {
-        std::copy(out[e].mat.begin(), out[e].mat.end(), m.mat_pool.begin() + m.mat_off[e]);
-        if (m.elem_att[e]) std::copy(out[e].attv.begin(), out[e].attv.end(), m.att_pool.begin() + m.att_off[e]);
     }
     if (att) {
         for (int i = 0; i < c.nsls; i++) {
