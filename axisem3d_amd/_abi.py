@@ -107,6 +107,8 @@ MAT_ISOTROPIC, MAT_TRANSVERSELY_ISO, MAT_ANISOTROPIC, MAT_ACOUSTIC = 0, 1, 2, 3
 ATT_NONE, ATT_FULL, ATT_CG4 = 0, 1, 2
 FIELD_SOLID_DISPL, FIELD_SOLID_VELOC, FIELD_SOLID_ACCEL, FIELD_SOLID_STIFF = 0, 1, 2, 3
 FIELD_FLUID_DISPL, FIELD_FLUID_VELOC, FIELD_FLUID_ACCEL, FIELD_FLUID_STIFF = 4, 5, 6, 7
+KF_SOLID3D, KF_SOLID1D, KF_FLUID, KF_NEWMARK, KF_OTHER, KF_N = 0, 1, 2, 3, 4, 5
+KF_NAMES = ("solid3d", "solid1d", "fluid", "newmark", "other")
 E_OK, E_INVAL, E_UNSUPPORTED, E_HIP, E_UNSTABLE, E_NOMEM = 0, 1, 2, 3, 4, 5
 
 
@@ -151,6 +153,7 @@ def declare_domain_api(lib):
         "axg_last_run_timing": ([vp, f64p, f64p, C.c_int32], C.c_int),
         "axg_step_cost": ([vp, f64p, f64p], C.c_int),
         "axg_class_bytes": ([vp, f64p, C.c_int32], C.c_int),
+        "axg_kernel_profile": ([vp, f64p, f64p, i64p, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

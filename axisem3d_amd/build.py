@@ -43,6 +43,11 @@ def build(verbose=True):
     tmp = lib + ".tmp"
     run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, lib)
+    # the C++ caller of the Domain / Newmark facade (include/axg_domain.hpp)
+    exe = os.path.join(BUILD, "axg_solve")
+    run(["g++", "-O2", "-std=c++17", "-Wall"] + INC + [os.path.join(CSRC, "host", "axg_solve.cpp"), "-o", exe + ".tmp",
+         "-L" + BUILD, "-laxg", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     return lib
 
 

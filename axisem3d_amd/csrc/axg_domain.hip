@@ -159,6 +159,17 @@ struct axg_ctx {
     double class_bytes[3] = {0., 0., 0.};
     double last_ms = 0.;
     double class_ms[4] = {0., 0., 0., 0.};
+    // kernel families of a profiled run (AXG_KF_*): an event pair per launch
+    struct ProfEv {
+        hipEvent_t a = nullptr, b = nullptr;
+        int fam = 0;
+    };
+    bool profiling = false;
+    std::vector<ProfEv> prof_ev;
+    size_t prof_used = 0;
+    double fam_ms[AXG_KF_N] = {0.};
+    double fam_bytes[AXG_KF_N] = {0.};
+    int64_t fam_launches[AXG_KF_N] = {0};
     // graph
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -170,6 +181,28 @@ namespace {
 
 // Kernel classes timed by the profiling mode of axg_run
 enum { KC_ELEMENT = 0, KC_NEWMARK = 1, KC_OTHER = 2, KC_N = 3 };
+
+// Brackets the launches of one kernel family with HIP events when the run is
+// profiled (events are pooled across runs; destroyed with the context).
+struct FamScope {
+    axg_ctx *c;
+    long idx = -1;
+    FamScope(axg_ctx *ctx, int fam) : c(ctx) {
+        if (!c->profiling) return;
+        if (c->prof_used == c->prof_ev.size()) {
+            axg_ctx::ProfEv e;
+            HIPCHECK(hipEventCreate(&e.a));
+            HIPCHECK(hipEventCreate(&e.b));
+            c->prof_ev.push_back(e);
+        }
+        idx = (long)c->prof_used++;
+        c->prof_ev[idx].fam = fam;
+        HIPCHECK(hipEventRecord(c->prof_ev[idx].a, c->stream));
+    }
+    ~FamScope() {
+        if (idx >= 0) (void)hipEventRecord(c->prof_ev[idx].b, c->stream);
+    }
+};
 
 // radices of the Stockham FFT, largest power-of-two factor first
 std::vector<int> fftFactors(int n) {
@@ -295,6 +328,7 @@ void launchSolid3D(const Elem3DArgs &a, size_t lds, hipStream_t s) {
 
 void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     if (b.nelem == 0) return;
+    FamScope fs(c, AXG_KF_SOLID3D);
     Elem3DArgs a;
     std::memset(&a, 0, sizeof(a));
     a.elems = b.elems.p;
@@ -440,6 +474,7 @@ void launchSolid1DAtt(const Elem1DArgs &a, int att, hipStream_t s) {
 
 void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
     if (b.nitem == 0) return;
+    FamScope fs(c, b.phase == 1 ? AXG_KF_FLUID : AXG_KF_SOLID1D);
     Elem1DArgs a = makeArgs1D(c, b);
     hipStream_t s = c->stream;
     if (b.phase == 1) {
@@ -457,6 +492,7 @@ void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
 }
 
 void doNewmark(axg_ctx *c, double dt, int *tstep_counter = nullptr) {
+    FamScope fs(c, AXG_KF_NEWMARK);
     double half_dt = 0.5 * dt;
     double half_dt_dt = half_dt * dt;
     const int B = 256;
@@ -473,6 +509,7 @@ void doNewmark(axg_ctx *c, double dt, int *tstep_counter = nullptr) {
 void doSource(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
     if (c->src_elem < 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::applySource || time step out of range.");
+    FamScope fs(c, AXG_KF_OTHER);
     hipLaunchKernelGGL(source_kernel, dim3(1), dim3(256), 0, c->stream, c->d_src_force.p, c->d_src_row.p,
                        c->d_src_nu.p, c->d_stf.p, tstep, tstep_ptr, c->S.stiff.p);
 }
@@ -489,6 +526,7 @@ void doStiff(axg_ctx *c) {
 
 void doCouple(axg_ctx *c) {
     if (c->nsf_row == 0) return;
+    FamScope fs(c, AXG_KF_OTHER);
     const int B = 256;
     hipLaunchKernelGGL(couple1d_kernel, dim3((c->nsf_row + B - 1) / B), dim3(B), 0, c->stream, c->nsf_row,
                        c->d_row_sf.p, c->d_sf_srow.p, c->d_sf_frow.p, c->d_sf_row0.p, c->d_sf_normal.p,
@@ -498,6 +536,7 @@ void doCouple(axg_ctx *c) {
 void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
     if (c->nrec == 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::record || time step out of range.");
+    FamScope fs(c, AXG_KF_OTHER);
     hipLaunchKernelGGL(record_solid_kernel, dim3(c->nrec), dim3(256), 0, c->stream, c->nrec, c->d_rec_elem.p,
                        c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p, c->d_rec_sint.p, c->d_ep_row.p,
                        c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz, tstep, tstep_ptr, c->d_seis.p);
@@ -784,6 +823,43 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
         c->updates_step = up;
         c->class_bytes[0] = bel + (by - bconst0);
         c->class_bytes[1] = bnm;
+
+        // the same bytes by kernel family; a point's displacement/stiffness bytes
+        // are shared equally between the elements of its phase that touch it
+        std::vector<int> incS(d->npoint, 0), incF(d->npoint, 0);
+        for (int e = 0; e < NE; e++) {
+            bool fluid = d->elem_flags[e] & AXG_EL_FLUID;
+            for (int k = 0; k < 25; k++) (fluid ? incF : incS)[d->elem_points[(size_t)e * 25 + k]]++;
+        }
+        for (int f = 0; f < AXG_KF_N; f++) c->fam_bytes[f] = 0.;
+        for (int e = 0; e < NE; e++) {
+            int fl = d->elem_flags[e];
+            bool fluid = fl & AXG_EL_FLUID;
+            int fam = fluid ? AXG_KF_FLUID : (fl & AXG_EL_3D) ? AXG_KF_SOLID3D : AXG_KF_SOLID1D;
+            double b = 500. + ((fl & AXG_EL_TISO) ? 400. : 0.) + 4. * (double)(d->mat_off[e + 1] - d->mat_off[e]);
+            int att = d->elem_att[e];
+            if (att != AXG_ATT_NONE) {
+                int np = att == AXG_ATT_CG4 ? 4 : 25;
+                double per = (2. * (c->nsls + 1) * 6 * np + 2. * np);
+                b += (fl & AXG_EL_3D) ? per * 4. * c->elem_nr[e] : per * 8. * (double)(c->elem_nr[e] / 2 + 1);
+            }
+            for (int k = 0; k < 25; k++) {
+                int p = d->elem_points[(size_t)e * 25 + k];
+                double rows = d->point_nr[p] / 2 + 1;
+                b += (fluid ? 16. : 48.) * rows / (double)(fluid ? incF[p] : incS[p]);
+            }
+            c->fam_bytes[fam] += b;
+        }
+        c->fam_bytes[AXG_KF_NEWMARK] = bnm;
+        // source (25 x 3 x 3 complex RMW), coupling (per row: two solid displacement
+        // reads, fluid and two solid stiffness RMW), receivers (25 points' rows)
+        double other = c->src_elem >= 0 ? 225. * 16. : 0.;
+        other += 64. * (double)c->nsf_row;
+        for (int r = 0; r < c->nrec; r++) {
+            int e = d->rec_elem[r];
+            for (int k = 0; k < 25; k++) other += 24. * (d->point_nr[d->elem_points[(size_t)e * 25 + k]] / 2 + 1);
+        }
+        c->fam_bytes[AXG_KF_OTHER] = other;
     }
     HIPCHECK(hipStreamSynchronize(s));
 }
@@ -856,6 +932,12 @@ void runStepsGraph(axg_ctx *c, int first, int count, int stab) {
 
 void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
     std::vector<hipEvent_t> ev;
+    c->prof_used = 0;
+    c->profiling = profile;
+    struct Off {
+        axg_ctx *c;
+        ~Off() { c->profiling = false; }
+    } off{c};
     if (profile) {
         ev.resize((size_t)count * 4);
         for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
@@ -875,6 +957,17 @@ void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
     }
     for (int i = 0; i < KC_N; i++) c->class_ms[i] = 0.;
     if (profile) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        for (int f = 0; f < AXG_KF_N; f++) {
+            c->fam_ms[f] = 0.;
+            c->fam_launches[f] = 0;
+        }
+        for (size_t i = 0; i < c->prof_used; i++) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, c->prof_ev[i].a, c->prof_ev[i].b));
+            c->fam_ms[c->prof_ev[i].fam] += ms;
+            c->fam_launches[c->prof_ev[i].fam]++;
+        }
         HIPCHECK(hipStreamSynchronize(c->stream));
         for (int i = 0; i < count; i++) {
             float a = 0.f, b = 0.f, d = 0.f;
@@ -1098,6 +1191,17 @@ int axg_last_run_timing(axg_ctx *c, double *ms_total, double *ms_per_class, int3
     CTX_CHECK(c);
     if (ms_total) *ms_total = c->last_ms;
     for (int i = 0; i < nclass; i++) ms_per_class[i] = i < KC_N ? c->class_ms[i] : 0.;
+    return AXG_OK;
+}
+
+int axg_kernel_profile(axg_ctx *c, double *ms, double *bytes_step, int64_t *launches, int32_t nfam) {
+    CTX_CHECK(c);
+    for (int f = 0; f < nfam; f++) {
+        bool ok = f < AXG_KF_N;
+        if (ms) ms[f] = ok ? c->fam_ms[f] : 0.;
+        if (bytes_step) bytes_step[f] = ok ? c->fam_bytes[f] : 0.;
+        if (launches) launches[f] = ok ? c->fam_launches[f] : 0;
+    }
     return AXG_OK;
 }
 

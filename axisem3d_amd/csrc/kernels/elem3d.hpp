@@ -69,6 +69,8 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
     constexpr int B = MaxB<R>::v;
     const int nb = N / R;
     const int total = nseq * nb;
+    const uint32_t mnb = udiv_magic(nb), mNs = udiv_magic(Ns);
+    const int wmul = nb / Ns;  // W_{Ns R}^{jm} = W_N^{jm N / (Ns R)}
     float2 v[B][R];
     int dst[B];
 #pragma unroll
@@ -76,10 +78,10 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
         int t = opaque((int)threadIdx.x) + q * E3D_THREADS;
         dst[q] = -1;
         if (t < total) {
-            int s = t / nb, j = t - s * nb;
-            int jm = j % Ns;
+            int s = udiv(t, nb, mnb), j = t - s * nb;
+            int jq = udiv(j, Ns, mNs), jm = j - jq * Ns;
             const float2 *x = buf + s * N;
-            const int wstep = jm * (N / (Ns * R));  // W_{Ns R}^{jm} = W_N^{wstep}
+            const int wstep = jm * wmul;
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 float2 xv = x[j + r * nb];
@@ -91,7 +93,7 @@ __device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, 
                     v[q][r] = cmul(xv, w);
                 }
             }
-            dst[q] = s * N + (j / Ns) * Ns * R + jm;
+            dst[q] = s * N + jq * Ns * R + jm;
         }
     }
     __syncthreads();
@@ -266,6 +268,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     for (int t = tid; t < kRootTable; t += E3D_THREADS) twl[Nr + t] = a.twiddle[t];  // radix tables
     const float2 *tw = twl;  // read by every FFT stage; visible after the first barrier
     const float invNr = 1.f / (float)Nr;
+    const uint32_t mM = udiv_magic(M);
     constexpr int NPAR = MatParams<MAT>::v;
 
     float2 acc[QB][5][3];
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             for (int q = 0; q < PB; q++) {
                 int t = tl + q * T;
                 if (t < 15 * M) {
-                    int b = t / M, k = t - b * M;
+                    int b = udiv(t, M, mM), k = t - b * M;
                     X[q] = buf[(2 * b) * M + k];
                     Y[q] = buf[(2 * b + 1) * M + k];
                 }
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             for (int q = 0; q < PB; q++) {
                 int t = tl + q * T;
                 if (t >= 15 * M) continue;
-                int b = t / M, k = t - b * M;
+                int b = udiv(t, M, mM), k = t - b * M;
                 float2 *z = buf + b * Nr;
                 if (k == 0 || 2 * k == Nr) {
                     // DC and Nyquist: the imaginary parts are ignored (FFTW c2r)
@@ -435,7 +438,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             for (int q = 0; q < PB; q++) {
                 int t = tl + q * T;
                 if (t < 15 * M) {
-                    int b = t / M, k = t - b * M;
+                    int b = udiv(t, M, mM), k = t - b * M;
                     const float2 *z = buf + b * Nr;
                     Z0[q] = z[k];
                     Z1[q] = z[k == 0 ? 0 : Nr - k];
@@ -447,7 +450,7 @@ __global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             for (int q = 0; q < PB; q++) {
                 int t = tl + q * T;
                 if (t >= 15 * M) continue;
-                int b = t / M, k = t - b * M;
+                int b = udiv(t, M, mM), k = t - b * M;
                 float2 za = Z0[q], zb = Z1[q];
                 buf[(2 * b) * M + k] = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
                 buf[(2 * b + 1) * M + k] = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));

@@ -93,6 +93,15 @@ class Domain:
         return {"steps": count, "total_ms": total.value, "element_ms": cls[0], "newmark_ms": cls[1],
                 "other_ms": cls[2]}
 
+    def kernel_profile(self):
+        """Per kernel family after run_profiled: summed ms, launches and
+        algorithmic HBM bytes per step."""
+        n = _abi.KF_N
+        ms, by, ln = (C.c_double * n)(), (C.c_double * n)(), (C.c_int64 * n)()
+        _check(self._lib.axg_kernel_profile(self._h, ms, by, ln, n))
+        return {name: {"ms": ms[i], "launches": ln[i], "bytes_step": by[i]}
+                for i, name in enumerate(_abi.KF_NAMES)}
+
     def kernel_bytes(self):
         """Algorithmic HBM bytes per step of the element-stiffness and Newmark kernel classes."""
         b = (C.c_double * 3)()

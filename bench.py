@@ -1,17 +1,36 @@
 """Benchmark of the MI355X time loop: GLL-point x mode updates per second.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload N] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload N]
+                    [--no-cpu] [--no-traffic]
 
 One step = one pass of Newmark::solve's loop body (Newmark.cpp:47-93) over the
 whole synthetic mesh resident in HBM: Newmark update, source, element stiffness
 (with on-device assembly), solid-fluid coupling and station recording.  The
-timed region is bracketed by a device synchronisation and timed with HIP events
-on the domain's stream.  Prints ONE JSON line.
+timed region is a run of graph-replayed steps bracketed by HIP events on the
+domain's stream, with the stream synchronised before and after.  Prints ONE
+JSON line.
+
+Workloads: 0 = 4000-quad isotropic shell at Nu = 4 (the reference's
+CPU-runnable case), 1 = 20000-quad shell with a 4000-quad fluid outer core,
+transversely isotropic mantle with a 3-D random velocity field and per-element
+Nu in [8, 128] (the default: the headline single-GPU case).
+
+The roofline figure is for the kernel family that takes the most time: its
+algorithmic HBM bytes per step divided by its measured time per step (HIP
+events around each of its launches, on the launch stream).  "traffic" is the
+HBM bytes the family really moved per step, from two rocprofv3 counter passes
+(FETCH_SIZE, WRITE_SIZE) run as child processes over a few eager steps and
+scaled by the same counters' reading on a plain device copy of known size.
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -25,10 +44,29 @@ WORKLOADS = {
     1: "shell-20000q-solid+fluid-ti3d-nu8..128",
 }
 
+# kernel name prefix -> family (the families of axg_kernel_profile)
+FAMILY_OF_KERNEL = (
+    ("axg::solid3d_kernel", "solid3d"),
+    ("axg::solid1d_kernel", "solid1d"),
+    ("axg::fluid", "fluid"),
+    ("axg::newmark", "newmark"),
+    ("axg::", "other"),
+)
+PROBE_STEPS = 3
+COPY_BYTES = 256 << 20
+
+
+def family_of(kernel_name):
+    name = kernel_name.replace("void ", "", 1).strip()
+    for prefix, fam in FAMILY_OF_KERNEL:
+        if name.startswith(prefix):
+            return fam
+    return None
+
 
 def cpu_baseline(mesh, seconds=20.0):
     """Time the FP32 oracle (a restatement of the reference's CPU loop) on a
-    bounded number of steps of the same workload, with all host threads."""
+    bounded number of steps of the same workload, with the host's threads."""
     from oracle.refcpu import RefDomain
     threads = min(16, os.cpu_count() or 1)
     ref = RefDomain(mesh, double=False, threads=threads)
@@ -42,8 +80,84 @@ def cpu_baseline(mesh, seconds=20.0):
     el = time.perf_counter() - t0
     upd = mesh.updates_per_step()
     return {"value": upd * nsample / el, "unit": "GLL-pt·mode updates/s", "cores": threads, "kind": "port",
-            "sample": "%d steps of the same mesh (steps 2..%d), FP32 oracle, %d threads, %.1f s"
-                      % (nsample, nsample + 1, threads, el)}
+            "sample": "%d step(s) of the same mesh (steps 2..%d) after one probe step, FP32 oracle, "
+                      "%d threads, %.1f s" % (nsample, nsample + 1, threads, el)}
+
+
+def probe(workload):
+    """Child of a rocprofv3 counter pass: a streaming kernel that reads and
+    writes COPY_BYTES (b = -a), then PROBE_STEPS eager steps after two warm-up
+    steps."""
+    import torch
+    import axisem3d_amd as ax
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)  # one streaming kernel: reads and writes COPY_BYTES
+    torch.cuda.synchronize()
+    del a, b
+    mesh = ax.ShellMesh(which=workload)
+    dom = ax.Domain(mesh)
+    dom.resetZero()
+    dom.run(1, 2, 0)
+    dom.run(3, PROBE_STEPS, 0)
+    print("probe done", flush=True)
+
+
+def counter_pass(counter, workload, outdir):
+    """One rocprofv3 pass; returns {family: KiB per step} and the copy kernel's KiB."""
+    rp = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if rp is None:
+        print("bench.py: rocprofv3 not found; traffic not measured", file=sys.stderr)
+        return None
+    d = os.path.join(outdir, counter)
+    cmd = ["timeout", "-s", "KILL", "120", rp, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
+           "--", sys.executable, os.path.abspath(__file__), "--probe", "--workload", str(workload)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, cwd=ROOT)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        print("bench.py: rocprofv3 %s pass wrote no counters (rc %d): %s"
+              % (counter, r.returncode, r.stdout[-2000:]), file=sys.stderr)
+        return None
+    fam_kib, copy_kib, names = {}, [], set()
+    for f in files:
+        for row in csv.DictReader(open(f)):
+            if row.get("Counter_Name") != counter:
+                continue
+            v = float(row["Counter_Value"])
+            name = row["Kernel_Name"]
+            names.add(name[:80])
+            fam = family_of(name)
+            if fam is not None:
+                fam_kib[fam] = fam_kib.get(fam, 0.) + v
+            elif "elementwise" in name and "neg" in name:
+                copy_kib.append(v)  # the streaming calibration kernel of probe()
+    if not copy_kib:
+        print("bench.py: no device copy kernel in the %s pass; kernels seen: %s"
+              % (counter, sorted(names)[:20]), file=sys.stderr)
+        return None
+    # the probe runs 2 + PROBE_STEPS steps; every family launches each step
+    steps = 2 + PROBE_STEPS
+    return {k: v / steps for k, v in fam_kib.items()}, max(copy_kib)
+
+
+def measure_traffic(workload, family):
+    """HBM bytes per step of `family` from FETCH_SIZE and WRITE_SIZE, each
+    scaled so that a plain copy of COPY_BYTES reads (writes) COPY_BYTES."""
+    keep = os.environ.get("AXG_PMC_DIR")
+    outdir = keep or tempfile.mkdtemp(prefix="axg_pmc_")
+    try:
+        total = 0.
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            res = counter_pass(counter, workload, outdir)
+            if res is None:
+                return None, None
+            fam_kib, copy_kib = res
+            scale = COPY_BYTES / (copy_kib * 1024.)
+            total += fam_kib.get(family, 0.) * 1024. * scale
+        return total, None
+    finally:
+        if not keep:
+            shutil.rmtree(outdir, ignore_errors=True)
 
 
 def main():
@@ -51,14 +165,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", type=int, default=0)
+    ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole mesh lives in one GPU's HBM); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        probe(args.workload)
+        return
 
     import axisem3d_amd as ax
 
@@ -79,19 +198,22 @@ def main():
     # the step; the library synchronises the stream before and after
     dom.run(args.warmup + 1, args.steps, 0, use_graph=True)
     ms = dom.last_run_ms()
-    # kernel-class timing (same steps again, profiled) for the roofline figure
+    # per-family kernel timing (the same steps again, eager, an event pair per launch)
     dom.resetZero()
     if args.warmup:
         dom.run(1, args.warmup, 0)
-    prof = dom.run_profiled(args.warmup + 1, args.steps)
-    el_ms = prof["element_ms"] / prof["steps"]
-    nm_ms = prof["newmark_ms"] / prof["steps"]
-    by_el, by_nm = dom.kernel_bytes()
-    cand = {"element": (el_ms, by_el), "newmark": (nm_ms, by_nm)}
-    dom_name = max(cand, key=lambda k: cand[k][0])
-    kms, kby = cand[dom_name]
-    achieved = kby / (kms * 1e-3) / 1e9
+    dom.run_profiled(args.warmup + 1, args.steps)
+    fams = dom.kernel_profile()
+    dom_name = max(fams, key=lambda k: fams[k]["ms"])
+    fam = fams[dom_name]
+    kms = fam["ms"] / args.steps
+    launches_step = fam["launches"] / args.steps
+    achieved = fam["bytes_step"] / (kms * 1e-3) / 1e9
     value = upd_step * args.steps / (ms * 1e-3)
+    traffic = None
+    if not args.no_traffic:
+        del dom
+        traffic, _ = measure_traffic(which, dom_name)
     out = {
         "metric": METRIC,
         "value": value,
@@ -121,9 +243,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per step of the kernel family (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes_per_step": fam["bytes_step"],
             "kernel_ms_per_step": kms,
-            "algorithmic_bytes_per_launch": kby,
+            "launches_per_step": launches_step,
+            "avg_launch_ms": kms / max(launches_step, 1e-9),
+            "families_ms_per_step": {k: v["ms"] / args.steps for k, v in fams.items()},
             "step_bytes_frac": bytes_step / (ms * 1e-3 / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
         "mesh_build_s": t_mesh,

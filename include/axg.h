@@ -263,6 +263,19 @@ int axg_set_field(axg_ctx *ctx, int32_t which, const float *in, int64_t nfloat);
 /* timing of the last axg_run: total ms and, after a profiled run, ms per
  * kernel class [element stiffness, Newmark, source/coupling/record] */
 int axg_last_run_timing(axg_ctx *ctx, double *ms_total, double *ms_per_class, int32_t nclass);
+/* kernel families timed launch by launch in a profiled run (the GPU analogue
+ * of the reference's per-bucket cost report, Domain.cpp:349-382) */
+#define AXG_KF_SOLID3D 0 /* solid elements with azimuth-dependent material */
+#define AXG_KF_SOLID1D 1 /* solid elements, Fourier-space material */
+#define AXG_KF_FLUID 2   /* fluid elements */
+#define AXG_KF_NEWMARK 3 /* point updates, solid and fluid */
+#define AXG_KF_OTHER 4   /* source, solid-fluid coupling, receivers */
+#define AXG_KF_N 5
+/* after axg_run(..., AXG_RUN_PROFILE): summed ms and number of launches per
+ * family over the run, and the family's algorithmic HBM bytes per step
+ * (element families: constants read once, each point-mode's displacement
+ * read and stiffness accumulated once, shared between the point's elements) */
+int axg_kernel_profile(axg_ctx *ctx, double *ms, double *bytes_step, int64_t *launches, int32_t nfam);
 /* algorithmic HBM bytes of one step and updates (GLL-pt x mode) of one step */
 int axg_step_cost(axg_ctx *ctx, double *bytes_step, double *updates_step);
 /* algorithmic HBM bytes per step of each kernel class (same classes as

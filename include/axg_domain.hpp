@@ -1,0 +1,125 @@
+// axg_domain.hpp -- the reference's Domain / Newmark stepping interface,
+// restated in C++ over the C ABI of axg.h so that a C++ caller written against
+//   class Domain   (SOLVER/src/core/domain/Domain.h:55-88)
+//   class Newmark  (SOLVER/src/core/newmark/Newmark.h:9-22)
+// keeps its call sites.  Every failure is rethrown as
+// std::runtime_error("Class::method || message"), which the reference's top
+// level catches and prints (SOLVER/src/axisem.cpp:193-204).
+//
+// Differences that follow from the whole mesh living on one GPU:
+//   * Domain is built from an axg_mesh_desc (the arrays Mesh::release would
+//     hand to the objects) instead of addPoint / addElement calls;
+//   * assembleStiff is a no-op (no ranks); the colour-ordered element
+//     kernels already sum every point's element forces on the device;
+//   * Newmark::solve replays the step from a captured HIP graph.
+#pragma once
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "axg.h"
+
+namespace axg {
+
+inline void throwIfFailed(int rc, const char *where) {
+    if (rc == AXG_OK) return;
+    const char *m = axg_last_error();
+    std::string msg = (m && *m) ? m : std::string(where) + " || error code " + std::to_string(rc);
+    throw std::runtime_error(msg);
+}
+
+class Domain {
+public:
+    explicit Domain(const axg_mesh_desc *desc, int device = 0) : mDesc(desc) {
+        throwIfFailed(axg_create(desc, device, &mCtx), "Domain::Domain");
+    }
+    ~Domain() { axg_destroy(mCtx); }
+    Domain(const Domain &) = delete;
+    Domain &operator=(const Domain &) = delete;
+
+    // Domain.cpp:67-74
+    void resetZero() const { throwIfFailed(axg_reset_zero(mCtx), "Domain::resetZero"); }
+    // Domain.cpp:82-94 (with the on-device assembly)
+    void computeStiff() const { throwIfFailed(axg_compute_stiff(mCtx), "Domain::computeStiff"); }
+    // Domain.cpp:96-109
+    void applySource(int tstep) const { throwIfFailed(axg_apply_source(mCtx, tstep), "Domain::applySource"); }
+    // Domain.cpp:111-163
+    void assembleStiff(int phase = 0) const { throwIfFailed(axg_assemble_stiff(mCtx, phase), "Domain::assembleStiff"); }
+    // Domain.cpp:165-177
+    void updateNewmark(double dt) const { throwIfFailed(axg_update_newmark(mCtx, dt), "Domain::updateNewmark"); }
+    // Domain.cpp:179-191
+    void coupleSolidFluid() const { throwIfFailed(axg_couple_solid_fluid(mCtx), "Domain::coupleSolidFluid"); }
+    // Domain.cpp:207-220
+    void record(int tstep, double t) const { throwIfFailed(axg_record(mCtx, tstep, t), "Domain::record"); }
+    // Domain.cpp:237-275: throws "Domain::checkStability || ..." when a
+    // displacement is not finite
+    void checkStability(double dt, int tstep, double t) const {
+        (void)dt;
+        (void)tstep;
+        (void)t;
+        int64_t bad = -1;
+        throwIfFailed(axg_check_stability(mCtx, &bad), "Domain::checkStability");
+    }
+    // Domain.cpp:384-402
+    void learnWisdom(int tstep) const {
+        (void)tstep;
+        throwIfFailed(axg_learn_wisdom(mCtx, 1e-3f), "Domain::learnWisdom");
+    }
+
+    // the STF's time axis (SourceTimeFunction::getShift / getDeltaT / getSize)
+    double getShift() const { return mDesc->t_shift; }
+    double getDeltaT() const { return mDesc->dt; }
+    int getSize() const { return mDesc->nstep; }
+
+    // recorded station traces [nstep][nrec][3]
+    std::vector<float> seismograms() const {
+        std::vector<float> out((size_t)mDesc->nstep * mDesc->nrec * 3);
+        throwIfFailed(axg_get_seismograms(mCtx, out.data(), (int64_t)out.size()), "Domain::seismograms");
+        return out;
+    }
+    axg_ctx *context() const { return mCtx; }
+
+private:
+    const axg_mesh_desc *mDesc;
+    axg_ctx *mCtx = nullptr;
+};
+
+class Newmark {
+public:
+    // Newmark.h:11; randomDispl is not supported (Domain::initDisplTinyRandom
+    // is a debugging aid of the reference)
+    Newmark(Domain *&domain, int reportInterval, int checkStabInterval, bool randomDispl = false)
+        : mDomain(domain), mReportInterval(reportInterval), mCheckStabInterval(checkStabInterval) {
+        if (randomDispl) throw std::runtime_error("Newmark::Newmark || random initial displacement is not supported.");
+    }
+
+    // Newmark.cpp:19-107: resetZero, then steps 1..maxStep of
+    // updateNewmark, applySource(tstep-1), computeStiff, coupleSolidFluid,
+    // assembleStiff, record(tstep-1, t), checkStability every
+    // checkStabInterval steps; replayed from a captured HIP graph in chunks of
+    // reportInterval steps (or fewer) so progress can be reported.
+    void solve(int verbose) const {
+        mDomain->resetZero();
+        const int maxStep = mDomain->getSize();
+        int chunk = mReportInterval > 0 ? mReportInterval : maxStep;
+        for (int first = 1; first <= maxStep; first += chunk) {
+            int count = first + chunk - 1 <= maxStep ? chunk : maxStep - first + 1;
+            int rc = axg_run(mDomain->context(), first, count, mCheckStabInterval, AXG_RUN_GRAPH);
+            throwIfFailed(rc, "Newmark::solve");
+            if (verbose) {
+                double t = -mDomain->getShift() + (first + count - 1) * mDomain->getDeltaT();
+                std::printf("  SIMULATION TIME / sec     =   %g\n  TIME STEP / TOTAL STEPS   =   %d / %d\n", t,
+                            first + count - 1, maxStep);
+            }
+        }
+    }
+
+private:
+    Domain *mDomain;
+    int mReportInterval;
+    int mCheckStabInterval;
+};
+
+}  // namespace axg

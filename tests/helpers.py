@@ -43,3 +43,43 @@ def random_field(mesh, which, seed, scale=1e-6):
             blk[nu] = 0.
         pos += (nu + 1) * ncomp * 2
     return x
+
+
+class Lifted3D:
+    """The same mesh with every solid element's 1-D material block repeated on
+    its Nr azimuthal samples and flagged 3-D (a laterally constant 3-D model)."""
+
+    def __init__(self, mesh):
+        import ctypes as C
+        import axisem3d_amd._abi as abi
+        self.mesh = mesh
+        ne = mesh.d.nelem
+        flags = mesh.array("elem_flags").copy()
+        off = mesh.array("mat_off")
+        pool = mesh.array("mat_pool")
+        enr = mesh.array("point_nr")[mesh.array("elem_points").reshape(ne, 25)].max(axis=1)
+        blocks, new_off = [], [0]
+        for e in range(ne):
+            blk = pool[off[e]:off[e + 1]]
+            if not flags[e] & abi.EL_FLUID:
+                npar = blk.size // 25
+                blk = np.repeat(blk.reshape(npar, 1, 25), enr[e], axis=1).ravel()
+                flags[e] |= abi.EL_3D
+            blocks.append(blk)
+            new_off.append(new_off[-1] + blk.size)
+        self._flags = flags.astype(np.int32)
+        self._off = np.array(new_off, dtype=np.int64)
+        self._pool = np.concatenate(blocks).astype(np.float32)
+        d = abi.MeshDesc.from_buffer_copy(mesh.d)
+        d.elem_flags = self._flags.ctypes.data_as(abi.i32p)
+        d.mat_off = self._off.ctypes.data_as(abi.i64p)
+        d.mat_pool = self._pool.ctypes.data_as(abi.f32p)
+        self._d = d
+        self.desc = C.pointer(d)
+
+    @property
+    def d(self):
+        return self._d
+
+    def field_size(self, which):
+        return self.mesh.field_size(which)

@@ -36,3 +36,18 @@ def test_no_gpu_fallback_message():
         assert "HIP" in str(e)
     else:
         raise AssertionError("axg_create succeeded without a GPU")
+
+
+def test_cpp_facade_driver_links_and_fails_loudly_without_gpu():
+    """The C++ Domain / Newmark facade (include/axg_domain.hpp) compiles into a
+    program linked against libaxg; without a device it reports the HIP error
+    and exits non-zero instead of running anything on the CPU."""
+    import subprocess
+    import torch
+    exe = os.path.join(ROOT, "axisem3d_amd", "_build", "axg_solve")
+    assert os.path.exists(exe)
+    if torch.cuda.device_count() > 0:
+        return
+    r = subprocess.run([exe, "0", "5", "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "HIP" in r.stderr

@@ -182,3 +182,94 @@ def test_seismograms_3d(small_layers, material):
     print("3d gpu vs oracle fp32 %.3e, vs fp64 %.3e" % (e32, e64))
     assert e32 < 1e-4
     assert e64 < 1e-3
+
+
+@pytest.mark.parametrize("material", [_abi.MAT_ISOTROPIC, _abi.MAT_TRANSVERSELY_ISO, _abi.MAT_ANISOTROPIC])
+def test_3d_kernel_equals_1d_kernel_for_constant_material(small_layers, material):
+    """The FFT (3-D) element kernel on a laterally constant 3-D material gives the
+    Fourier-space (1-D) kernel's stiffness."""
+    from tests.helpers import Lifted3D
+    print("case lifted3d material=%d" % material, flush=True)
+    m1 = _mesh_small(small_layers, material=material, nu_mode=1, nu_min=6, nu_max=40, nstep=10)
+    m3 = Lifted3D(m1)
+    u = random_field(m1, _abi.FIELD_SOLID_DISPL, seed=4)
+    out = []
+    for m in (m1, m3):
+        g = ax.Domain(m)
+        g.set_field(_abi.FIELD_SOLID_DISPL, u)
+        g.computeStiff()
+        out.append(g.get_field(_abi.FIELD_SOLID_STIFF))
+        g.close()
+    assert np.abs(out[0]).max() > 0
+    assert rel_l2(out[1], out[0]) < 1e-5
+
+
+@pytest.mark.parametrize("dv", [0.0, 0.05])
+def test_rigid_translations_have_no_stiffness(small_layers, dv):
+    """u_z = 1 (alpha = 0) and u_s = 1/2, u_phi = i/2 (alpha = 1) are rigid
+    translations: zero strain, so zero stiffness, on both element kernels."""
+    print("case rigid dv=%g" % dv, flush=True)
+    m = _mesh_small(small_layers, material=_abi.MAT_TRANSVERSELY_ISO, nu_mode=1, nu_min=6, nu_max=40,
+                    dv_amp=dv, nstep=10)
+    g = ax.Domain(m)
+    nr, fl = m.array("point_nr"), m.array("point_flags")
+    rows = [(p, int(nr[p]) // 2) for p in range(m.d.npoint) if fl[p] & _abi.PT_SOLID]
+
+    def field(fn):
+        out = []
+        for p, nu in rows:
+            for a in range(nu + 1):
+                for v in fn(a):
+                    out += [np.real(v), np.imag(v)]
+        return np.array(out)
+
+    u = random_field(m, _abi.FIELD_SOLID_DISPL, seed=8, scale=1.0)
+    g.set_field(_abi.FIELD_SOLID_DISPL, u)
+    g.computeStiff()
+    normK = np.linalg.norm(g.get_field(_abi.FIELD_SOLID_STIFF)) / np.linalg.norm(u)
+    for t in (field(lambda a: [0., 0., 1.] if a == 0 else [0., 0., 0.]),
+              field(lambda a: [0.5, 0.5j, 0.] if a == 1 else [0., 0., 0.])):
+        g.set_field(_abi.FIELD_SOLID_STIFF, np.zeros_like(t))
+        g.set_field(_abi.FIELD_SOLID_DISPL, t)
+        g.computeStiff()
+        r = np.linalg.norm(g.get_field(_abi.FIELD_SOLID_STIFF)) / (normK * np.linalg.norm(t))
+        assert r < 1e-5, r
+
+
+def test_kernel_profile_families(small_layers):
+    print("case kernel_profile", flush=True)
+    m = _mesh_small(small_layers, material=_abi.MAT_TRANSVERSELY_ISO, nu_mode=1, nu_min=6, nu_max=40,
+                    dv_amp=0.05, nstep=20)
+    g = ax.Domain(m)
+    g.resetZero()
+    p = g.run_profiled(1, 5)
+    fam = g.kernel_profile()
+    assert fam["solid3d"]["launches"] >= 5 and fam["newmark"]["launches"] == 5
+    assert fam["solid3d"]["ms"] > 0 and fam["solid3d"]["bytes_step"] > 0
+    total_bytes, _ = g.step_cost()
+    # the step's algorithmic bytes are the element and Newmark families' bytes
+    # (source, coupling and receivers are accounted separately)
+    core = sum(v["bytes_step"] for k, v in fam.items() if k != "other")
+    assert abs(core / total_bytes - 1.) < 1e-6
+    assert sum(v["ms"] for v in fam.values()) <= p["total_ms"] * 1.05
+
+
+def test_cpp_facade_newmark_solve_matches_python_driver(tmp_path):
+    """axg_solve (C++: Domain + Newmark::solve of include/axg_domain.hpp) and the
+    Python driver produce the same seismograms on the same mesh."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "axisem3d_amd", "_build", "axg_solve")
+    out = str(tmp_path / "seis.bin")
+    print("case cpp_facade", flush=True)
+    r = subprocess.run([exe, "0", "120", "8", out], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    cpp = np.fromfile(out, dtype=np.float32)
+    m = ax.ShellMesh(ax.shell_config(0, nelem_theta=8, nstep=120))
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, 120, 100, use_graph=True)
+    py = g.seismograms().ravel()
+    assert cpp.size == py.size and np.abs(py).max() > 0
+    assert rel_l2(cpp, py) < 1e-6

@@ -28,7 +28,7 @@ import pytest
 import axisem3d_amd._abi as abi
 from axisem3d_amd.mesh import ShellMesh, shell_config
 
-from tests.helpers import random_field, rel_l2
+from tests.helpers import Lifted3D, random_field, rel_l2
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -254,50 +254,11 @@ def test_modes_do_not_couple_for_1d_material(refcpu):
         assert np.abs(s[~mask]).max() == 0.
 
 
-class _Lifted3D:
-    """The same mesh with every solid element's 1-D material block repeated on
-    its Nr azimuthal samples and flagged 3-D (a laterally constant 3-D model)."""
-
-    def __init__(self, mesh):
-        import ctypes as C
-        self.mesh = mesh
-        ne = mesh.d.nelem
-        flags = mesh.array("elem_flags").copy()
-        off = mesh.array("mat_off")
-        pool = mesh.array("mat_pool")
-        enr = mesh.array("point_nr")[mesh.array("elem_points").reshape(ne, 25)].max(axis=1)
-        blocks, new_off = [], [0]
-        for e in range(ne):
-            blk = pool[off[e]:off[e + 1]]
-            if not flags[e] & abi.EL_FLUID:
-                npar = blk.size // 25
-                blk = np.repeat(blk.reshape(npar, 1, 25), enr[e], axis=1).ravel()
-                flags[e] |= abi.EL_3D
-            blocks.append(blk)
-            new_off.append(new_off[-1] + blk.size)
-        self._flags = flags.astype(np.int32)
-        self._off = np.array(new_off, dtype=np.int64)
-        self._pool = np.concatenate(blocks).astype(np.float32)
-        d = abi.MeshDesc.from_buffer_copy(mesh.d)
-        d.elem_flags = self._flags.ctypes.data_as(abi.i32p)
-        d.mat_off = self._off.ctypes.data_as(abi.i64p)
-        d.mat_pool = self._pool.ctypes.data_as(abi.f32p)
-        self._d = d
-        self.desc = C.pointer(d)
-
-    @property
-    def d(self):
-        return self._d
-
-    def field_size(self, which):
-        return self.mesh.field_size(which)
-
-
 @pytest.mark.parametrize("material", [0, 1, 2])
 def test_3d_path_equals_1d_path_for_constant_material(refcpu, material):
     m1 = small_mesh(material=material, nu=(6, 20), dv_amp=0.)
     assert not (m1.array("elem_flags") & abi.EL_3D).any()
-    m3 = _Lifted3D(m1)
+    m3 = Lifted3D(m1)
     u = random_field(m1, abi.FIELD_SOLID_DISPL, 3, scale=1.)
     s1 = stiff_of(refcpu.RefDomain(m1, double=True), m1, u)
     s3 = stiff_of(refcpu.RefDomain(m3, double=True), m3, u)
