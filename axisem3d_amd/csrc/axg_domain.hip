@@ -83,9 +83,15 @@ struct Bucket1D {
 struct Bucket3D {
     int colour = 0, phase = 0, material = 0, att = 0;
     bool tiso = false;
-    int32_t nelem = 0, maxM = 0;
+    int32_t nelem = 0, maxM = 0, maxNr = 0, slots = 1;
     DevBuf<int32_t> elems;
 };
+
+// dynamic LDS of a 3-D launch: forward and inverse twiddles + roots, the xi
+// operands [5][3][maxM], and each wave's three sequences [3][maxNr]
+inline size_t lds3D(const Bucket3D &b) {
+    return (2 * (size_t)b.maxNr + kRootTable + 15 * (size_t)b.maxM + 3 * W3D_WAVES * (size_t)b.maxNr) * sizeof(float2);
+}
 
 struct Phase {
     int32_t npt = 0, nrow = 0;
@@ -138,7 +144,7 @@ struct axg_ctx {
     DevBuf<float> d_mat3d;
     DevBuf<int64_t> d_mat3d_off;
     DevBuf<float2> d_twiddle;
-    DevBuf<int32_t> d_tw_off, d_factors;
+    DevBuf<int32_t> d_tw_off, d_plan, d_nstage;
     size_t max_lds = 0;
     // solid-fluid coupling
     int32_t nsf = 0, nsf_row = 0;
@@ -204,11 +210,18 @@ struct FamScope {
     }
 };
 
-// radices of the Stockham FFT, largest power-of-two factor first
+// radices of the Stockham FFT: the power of two as 16s with one 8, 4 or 2
+// for the rest, then the odd primes; empty if n has another prime factor
 std::vector<int> fftFactors(int n) {
     std::vector<int> f;
-    int m = n;
-    for (int p : {4, 2, 3, 5, 7, 11, 13}) {
+    int m = n, p2 = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        p2++;
+    }
+    for (; p2 >= 4; p2 -= 4) f.push_back(16);
+    if (p2 > 0) f.push_back(1 << p2);
+    for (int p : {3, 5, 7, 11, 13}) {
         while (m % p == 0) {
             f.push_back(p);
             m /= p;
@@ -217,6 +230,8 @@ std::vector<int> fftFactors(int n) {
     if (m != 1) f.clear();
     return f;
 }
+
+uint32_t hostMagic(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
 
 void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att,
                      const std::vector<int32_t> &elems) {
@@ -227,34 +242,31 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
     const int P = (mat == AXG_MAT_ISOTROPIC) ? 2 : (mat == AXG_MAT_TRANSVERSELY_ISO) ? 5 : 21;
     if (c->h_mat3d_off.empty()) c->h_mat3d_off.assign(d->nelem, 0);
     if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
-    // size classes by number of modes M = Nr/2 + 1 (LDS is sized by the class)
-    const int cls[] = {9, 17, 33, 49, 65, 97, 137};
-    std::vector<std::vector<int32_t>> byClass(7);
+    // classes by modes per lane S (M = Nr/2 + 1 <= 64 S): register and LDS use
+    // follow the class
+    std::vector<std::vector<int32_t>> byClass(W3D_MAX_S + 1);
     for (int e : elems) {
         int nr = c->elem_nr[e];
-        if (nr > E3D_MAX_NR || fftFactors(nr).empty() || (int)fftFactors(nr).size() > E3D_MAX_FACTORS)
+        if (nr > E3D_MAX_NR || fftFactors(nr).empty() || (int)fftFactors(nr).size() > E3D_MAX_STAGES)
             throw std::runtime_error("FieldFFT::transformF2P || Nr = " + std::to_string(nr) +
                                      " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
         c->nr_used[nr] = 1;
         int M = nr / 2 + 1;
-        int k = 0;
-        while (cls[k] < M) k++;
-        byClass[k].push_back(e);
-        // material: descriptor [param][n][i*5+j] -> [i][param][n][j]
+        byClass[(M + 63) / 64].push_back(e);
+        // material: descriptor [param][n][point] -> [point][param][n]
         int64_t off = (int64_t)c->h_mat3d.size();
         c->h_mat3d_off[e] = off;
         c->h_mat3d.resize(off + (size_t)P * nr * 25);
         const float *src = d->mat_pool + d->mat_off[e];
         if (d->mat_off[e + 1] - d->mat_off[e] != (int64_t)P * nr * 25)
             throw std::runtime_error("Elastic3D::checkCompatibility || Incompatible size.");
-        for (int i = 0; i < 5; i++)
-            for (int p = 0; p < P; p++)
-                for (int n = 0; n < nr; n++)
-                    for (int j = 0; j < 5; j++)
-                        c->h_mat3d[off + (((size_t)i * P + p) * nr + n) * 5 + j] =
-                            src[(size_t)p * nr * 25 + (size_t)n * 25 + i * 5 + j];
+        float *dst = c->h_mat3d.data() + off;
+        for (int p = 0; p < P; p++)
+            for (int n = 0; n < nr; n++)
+                for (int k = 0; k < 25; k++)
+                    dst[((size_t)k * P + p) * nr + n] = src[((size_t)p * nr + n) * 25 + k];
     }
-    for (int k = 0; k < 7; k++) {
+    for (int k = 1; k <= W3D_MAX_S; k++) {
         auto &v = byClass[k];
         if (v.empty()) continue;
         std::sort(v.begin(), v.end(), [&](int a, int b) { return c->elem_nr[a] > c->elem_nr[b]; });
@@ -265,7 +277,9 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
         b->tiso = tiso != 0;
         b->att = att;
         b->nelem = (int32_t)v.size();
-        b->maxM = cls[k];
+        b->slots = k;
+        b->maxNr = c->elem_nr[v.front()];
+        b->maxM = b->maxNr / 2 + 1;
         b->elems.upload(v, c->stream);
         HIPCHECK(hipStreamSynchronize(c->stream));
         c->buckets3d.push_back(std::move(b));
@@ -285,7 +299,8 @@ void finishBuckets3D(axg_ctx *c) {
             double ang = -2. * 3.141592653589793238463 * m / R;
             tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
         }
-    std::vector<int32_t> twoff(E3D_MAX_NR + 1, 0), fac((size_t)(E3D_MAX_NR + 1) * 8, 0);
+    std::vector<int32_t> twoff(E3D_MAX_NR + 1, 0), nstage(E3D_MAX_NR + 1, 0);
+    std::vector<int32_t> plan((size_t)(E3D_MAX_NR + 1) * E3D_MAX_STAGES * FS_N, 0);
     for (int n = 1; n <= E3D_MAX_NR; n++) {
         if (!c->nr_used[n]) continue;
         twoff[n] = (int32_t)tw.size();
@@ -293,14 +308,25 @@ void finishBuckets3D(axg_ctx *c) {
             double ang = -2. * 3.141592653589793238463 * t / n;
             tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
         }
-        const int radix[7] = {4, 2, 3, 5, 7, 11, 13};
-        for (int r : fftFactors(n))
-            for (int q = 0; q < 7; q++)
-                if (r == radix[q]) fac[(size_t)n * 8 + q]++;
+        std::vector<int> f = fftFactors(n);
+        nstage[n] = (int32_t)f.size();
+        int Ns = 1;
+        for (size_t st = 0; st < f.size(); st++) {
+            int32_t *dsc = &plan[((size_t)n * E3D_MAX_STAGES + st) * FS_N];
+            const int R = f[st], nb = n / R;
+            dsc[FS_R] = R;
+            dsc[FS_NS] = Ns;
+            dsc[FS_NB] = nb;
+            dsc[FS_MNB] = (int32_t)hostMagic((uint32_t)nb);
+            dsc[FS_MNS] = (int32_t)hostMagic((uint32_t)Ns);
+            dsc[FS_WMUL] = nb / Ns;
+            Ns *= R;
+        }
     }
     c->d_twiddle.upload(tw, s);
     c->d_tw_off.upload(twoff, s);
-    c->d_factors.upload(fac, s);
+    c->d_plan.upload(plan, s);
+    c->d_nstage.upload(nstage, s);
     HIPCHECK(hipStreamSynchronize(s));
     c->h_mat3d.clear();
     c->h_mat3d.shrink_to_fit();
@@ -308,22 +334,29 @@ void finishBuckets3D(axg_ctx *c) {
     HIPCHECK(hipGetDeviceProperties(&prop, c->device));
     c->max_lds = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
     for (auto &b : c->buckets3d) {
-        size_t lds = ((size_t)32 * b->maxM + kRootTable + 75 * std::min(E3D_STAGE_MODES, b->maxM)) * sizeof(float2);
+        size_t lds = lds3D(*b);
         if (lds > c->max_lds)
             throw std::runtime_error("FieldFFT || element needs " + std::to_string(lds) + " B of LDS, device has " +
                                      std::to_string(c->max_lds) + " (not supported by this build).");
     }
 }
 
-template <int MAT, bool TISO>
-void launchSolid3D(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+template <int MAT, bool TISO, int S>
+void launchSolid3DS(const Elem3DArgs &a, size_t lds, hipStream_t s) {
     static bool attrSet = false;
     if (!attrSet) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&solid3d_kernel<MAT, TISO>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&solid3d_kernel<MAT, TISO, S>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attrSet = true;
     }
-    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO>), dim3(a.nelem), dim3(E3D_THREADS), lds, s, a);
+    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+}
+
+template <int MAT, bool TISO>
+void launchSolid3D(const Elem3DArgs &a, int slots, size_t lds, hipStream_t s) {
+    if (slots == 1) launchSolid3DS<MAT, TISO, 1>(a, lds, s);
+    else if (slots == 2) launchSolid3DS<MAT, TISO, 2>(a, lds, s);
+    else launchSolid3DS<MAT, TISO, 3>(a, lds, s);
 }
 
 void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
@@ -334,6 +367,7 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     a.elems = b.elems.p;
     a.nelem = b.nelem;
     a.maxM = b.maxM;
+    a.maxNr = b.maxNr;
     a.elem_flags = c->d_elem_flags.p;
     a.elem_nr = c->d_elem_nr.p;
     a.geom = c->d_geom.p;
@@ -346,16 +380,17 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     a.stiff = c->S.stiff.p;
     a.twiddle = c->d_twiddle.p;
     a.tw_off = c->d_tw_off.p;
-    a.factors = c->d_factors.p;
-    size_t lds = ((size_t)32 * b.maxM + kRootTable + 75 * std::min(E3D_STAGE_MODES, b.maxM)) * sizeof(float2);
+    a.plan = c->d_plan.p;
+    a.nstage = c->d_nstage.p;
+    size_t lds = lds3D(b);
     hipStream_t s = c->stream;
     if (b.material == AXG_MAT_ISOTROPIC) {
-        if (b.tiso) launchSolid3D<0, true>(a, lds, s);
-        else launchSolid3D<0, false>(a, lds, s);
+        if (b.tiso) launchSolid3D<0, true>(a, b.slots, lds, s);
+        else launchSolid3D<0, false>(a, b.slots, lds, s);
     } else if (b.material == AXG_MAT_TRANSVERSELY_ISO) {
-        launchSolid3D<1, true>(a, lds, s);
+        launchSolid3D<1, true>(a, b.slots, lds, s);
     } else {
-        launchSolid3D<2, true>(a, lds, s);
+        launchSolid3D<2, true>(a, b.slots, lds, s);
     }
 }
 

@@ -1,39 +1,52 @@
 // elem3d.hpp -- stiffness of solid elements whose material varies in azimuth
 // (the reference's "3D" path, SolidElement::displToStiff with mElem3D,
-// SOLVER/src/core/element/SolidElement.cpp:404-443).  One workgroup owns one
-// element and keeps everything in LDS:
+// SOLVER/src/core/element/SolidElement.cpp:404-443).
 //
-//   for each row i of GLL points (5 passes of 5 points):
-//     strain spectrum of the 5 points, all modes      Gradient::computeGrad6 + CrdTransTIsoSolid
-//     C2R of 30 columns (packed as 15 complex FFTs)   FieldFFT::transformF2P, SolverFFTW_N6::computeC2R
-//     stress at every azimuthal sample               Isotropic3D / TransverselyIsotropic3D / Anisotropic3D
-//     R2C x 1/Nr                                      FieldFFT::transformP2F, SolverFFTW_N6::computeR2C
-//     rotate back, quadrature operands, accumulate    CrdTransTIsoSolid::transformRTZ_SPZ, Gradient::computeQuad6
-//   subtract the element's stiffness from its 25 points (mesh colouring: no races)
+// One workgroup of 5 waves owns one element; wave w owns GLL row ipol = w.
+// In step k (k = 0..4) wave w handles the point (w, k) on its own:
 //
-// The FFT is a mixed-radix (2,3,4,5,7,11,13) Stockham transform in LDS, done
-// in place: every stage reads all its inputs into registers, synchronises,
-// then writes.  Two real columns share one complex transform (z = x + i y).
+//   strain spectrum, all modes       Gradient::computeGrad6 + CrdTransTIsoSolid
+//                                    (displacements of row w and column k read
+//                                    straight from the point field, L2-resident)
+//   C2R of its 6 columns             FieldFFT::transformF2P, SolverFFTW_N6::computeC2R
+//   stress at every azimuthal sample Isotropic3D / TransverselyIsotropic3D / Anisotropic3D
+//   R2C x 1/Nr                       FieldFFT::transformP2F, SolverFFTW_N6::computeR2C
+//   quadrature operands              CrdTransTIsoSolid::transformRTZ_SPZ, Gradient::computeQuad6
+//
+// The FFTs run in a wave-private LDS area, so they need no workgroup barrier.
+// The quadrature couples a point to its row (the eta term, which stays in the
+// wave: the wave accumulates the forces of its whole row in registers) and to
+// its column (the xi term, which crosses rows): the xi operands of the step's
+// five points go through a double-buffered LDS exchange, one barrier per step.
+// At the end each wave subtracts its row's forces from the point field (mesh
+// colouring: no other element of the launch touches these points).
+//
+// Lane l of a wave holds the modes alpha = l + 64 s (s < S) in the spectral
+// phases and the samples n = l + 64 t in the stress phase.  The FFT is a
+// mixed-radix (16, 8, 4, 2, 3, 5, 7, 11, 13) Stockham transform done in place: each stage
+// reads its inputs into registers, then writes; the LDS operations of one wave
+// complete in issue order, so a compiler fence between the two halves
+// suffices.  Two real columns share one complex transform (z = x + i y).
 #pragma once
 #include "common.hpp"
 
 namespace axg {
 
-constexpr int E3D_THREADS = 512;
-constexpr int E3D_STAGE_MODES = 64;  // modes of displacement staged at once
-constexpr int E3D_MAX_NR = 272;      // 15 * Nr / radix <= threads * butterflies-per-thread
-constexpr int E3D_MAX_FACTORS = 12;
-
+constexpr int W3D_WAVES = 5;
+constexpr int W3D_THREADS = 64 * W3D_WAVES;
+constexpr int W3D_MAX_S = 3;                        // modes per lane: M <= 64 S
+constexpr int E3D_MAX_NR = 2 * 64 * W3D_MAX_S - 2;  // Nr with Nr/2 + 1 <= 192
 
 struct Elem3DArgs {
     const int32_t *elems;
     int32_t nelem;
-    int32_t maxM;
+    int32_t maxM;                // largest Nu+1 of the launch (LDS strides)
+    int32_t maxNr;               // largest Nr of the launch
     const int32_t *elem_flags;
     const int32_t *elem_nr;
     const float *geom;
     const float *tiso;
-    const float *mat3d;          // per element: [i][param][n][j]
+    const float *mat3d;          // per element: [point][param][sample n]
     const int64_t *mat3d_off;    // [nelem]
     const int32_t *ep_row;
     const int32_t *ep_amax;
@@ -41,142 +54,207 @@ struct Elem3DArgs {
     float2 *stiff;
     const float2 *twiddle;       // W_N^t = exp(-2 pi i t / N), concatenated per N
     const int32_t *tw_off;       // [N] offset of the table of N
-    const int32_t *factors;      // [N][8]: stage counts of radices 4, 2, 3, 5, 7, 11, 13
+    const int32_t *plan;         // [N][E3D_MAX_STAGES][8] stage descriptors (FftStage)
+    const int32_t *nstage;       // [N] number of stages
 };
 
-// ---------------------------------------------------------------- FFT stage
+// One Stockham stage of a length-N transform, precomputed on the host:
+// radix R, span Ns (product of the radices before it), nb = N / R butterflies
+// per sequence, multiply-high reciprocals of nb and Ns, and wmul = nb / Ns
+// (W_{Ns R}^{m} = W_N^{m wmul}).
+enum FftStage { FS_R = 0, FS_NS, FS_NB, FS_MNB, FS_MNS, FS_WMUL, FS_N = 8 };
+constexpr int E3D_MAX_STAGES = 12;
 
-template <int R>
-struct MaxB {
-    static constexpr int v = (R == 2) ? 4 : (R == 3) ? 3 : (R <= 7) ? 2 : 1;  // 15 Nr / R <= B * threads for Nr <= 272
-};
-
-// one Stockham stage on nseq sequences of length N stored [seq][N] in `buf`:
-// for butterfly j: inputs x[j + r N/R] * W_{Ns R}^{(j % Ns) r}, radix-R DFT,
-// outputs y[(j / Ns) Ns R + (j % Ns) + r Ns].  sign = -1 forward, +1 inverse.
-// LDS offsets of the W_R^m tables of the odd radices inside the twiddle area
+// offsets of the W_R^m tables of the odd radices in the root table
 template <int R>
 constexpr int rootOff() {
     return (R == 3) ? 0 : (R == 5) ? 3 : (R == 7) ? 8 : (R == 11) ? 15 : 26;
 }
 constexpr int kRootTable = 39;
 
-// One Stockham stage.  `tw` points into LDS: W_N^t for t < N, followed by the
-// radix tables (loop-invariant constants would otherwise be hoisted into
-// registers across the whole element loop).
+// orders the LDS accesses of one wave around this point (code motion only:
+// a wave's LDS instructions are executed in issue order)
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- FFT
+// butterflies per lane of one radix-R stage on 3 sequences of length <= NMAX
+template <int R, int NMAX>
+struct WaveB {
+    static constexpr int v = (3 * (NMAX / R) + 63) / 64;
+};
+
+// In-register DFT of a power-of-two length R <= 16, y_k = sum_r x_r W^{rk}
+// with W = exp(sign 2 pi i / R): even/odd split down to radix 4.
 template <int R>
-__device__ __forceinline__ void fft_stage(float2 *buf, int nseq, int N, int Ns, int sign, const float2 *tw) {
-    constexpr int B = MaxB<R>::v;
-    const int nb = N / R;
-    const int total = nseq * nb;
-    const uint32_t mnb = udiv_magic(nb), mNs = udiv_magic(Ns);
-    const int wmul = nb / Ns;  // W_{Ns R}^{jm} = W_N^{jm N / (Ns R)}
+__device__ __forceinline__ void dft_pow2(float2 (&x)[R], int sign) {
+    if constexpr (R == 2) {
+        float2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        float2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+        float2 b0 = cadd(x[1], x[3]), b1 = csub(x[1], x[3]);
+        // b1 times W^1 = -i (forward) or +i (inverse)
+        float2 b1r = (sign < 0) ? make_float2(b1.y, -b1.x) : make_float2(-b1.y, b1.x);
+        x[0] = cadd(a0, b0);
+        x[2] = csub(a0, b0);
+        x[1] = cadd(a1, b1r);
+        x[3] = csub(a1, b1r);
+    } else {
+        // cos and sin of 2 pi k / 16, k = 0..7 (W_8^k = W_16^{2k})
+        constexpr float c16[8] = {1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                                  0.f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
+        constexpr float s16[8] = {0.f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+                                  1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+        constexpr int H = R / 2;
+        constexpr int step = 16 / R;
+        float2 e[H], o[H];
+#pragma unroll
+        for (int i = 0; i < H; i++) {
+            e[i] = x[2 * i];
+            o[i] = x[2 * i + 1];
+        }
+        dft_pow2<H>(e, sign);
+        dft_pow2<H>(o, sign);
+        const float sg = (sign < 0) ? -1.f : 1.f;
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            float2 t;
+            if (k == 0) {
+                t = o[0];
+            } else if (2 * k == H) {
+                t = (sign < 0) ? make_float2(o[k].y, -o[k].x) : make_float2(-o[k].y, o[k].x);
+            } else {
+                t = cmul(o[k], make_float2(c16[k * step], sg * s16[k * step]));
+            }
+            x[k] = cadd(e[k], t);
+            x[k + H] = csub(e[k], t);
+        }
+    }
+}
+
+// One Stockham stage on 3 sequences [3][N] at `seq`: butterfly j of sequence
+// s reads x[j + r nb] * W_{Ns R}^{(j % Ns) r}, does a radix-R DFT and writes
+// y[(j / Ns) Ns R + (j % Ns) + r Ns].  sign = -1 forward, +1 inverse.  `tw`
+// holds W_N^{sign t} for t < N; `root` the forward odd-radix root tables.
+template <int R, int NMAX>
+__device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, uint32_t mnb, uint32_t mNs, int wmul,
+                                           int sign, const float2 *tw, const float2 *root, int lane) {
+    constexpr int B = WaveB<R, NMAX>::v;
+    const int total = 3 * nb;
     float2 v[B][R];
     int dst[B];
 #pragma unroll
     for (int q = 0; q < B; q++) {
-        int t = opaque((int)threadIdx.x) + q * E3D_THREADS;
+        int t = lane + q * 64;
         dst[q] = -1;
         if (t < total) {
             int s = udiv(t, nb, mnb), j = t - s * nb;
             int jq = udiv(j, Ns, mNs), jm = j - jq * Ns;
-            const float2 *x = buf + s * N;
+            const float2 *x = seq + s * N + j;
             const int wstep = jm * wmul;
+            const float2 *tp = tw;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                float2 xv = x[j + r * nb];
+                float2 xv = x[r * nb];
                 if (r == 0) {
                     v[q][r] = xv;
                 } else {
-                    float2 w = tw[wstep * r];  // the table holds the forward sign
-                    if (sign > 0) w.y = -w.y;
-                    v[q][r] = cmul(xv, w);
+                    tp += wstep;
+                    v[q][r] = cmul(xv, *tp);
                 }
             }
             dst[q] = s * N + jq * Ns * R + jm;
         }
     }
-    __syncthreads();
+    wave_lds_fence();
 #pragma unroll
     for (int q = 0; q < B; q++) {
         if (dst[q] < 0) continue;
-        float2 y[R];
-        if constexpr (R == 2) {
-            y[0] = cadd(v[q][0], v[q][1]);
-            y[1] = csub(v[q][0], v[q][1]);
-        } else if constexpr (R == 4) {
-            float2 a0 = cadd(v[q][0], v[q][2]), a1 = csub(v[q][0], v[q][2]);
-            float2 b0 = cadd(v[q][1], v[q][3]), b1 = csub(v[q][1], v[q][3]);
-            // multiply b1 by -i (forward) or +i (inverse)
-            float2 b1r = (sign < 0) ? make_float2(b1.y, -b1.x) : make_float2(-b1.y, b1.x);
-            y[0] = cadd(a0, b0);
-            y[2] = csub(a0, b0);
-            y[1] = cadd(a1, b1r);
-            y[3] = csub(a1, b1r);
+        float2 *o = seq + dst[q];
+        if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
+            dft_pow2<R>(v[q], sign);
+#pragma unroll
+            for (int r = 0; r < R; r++) o[r * Ns] = v[q][r];
         } else {
-            // radix-R DFT y_k = sum_r v_r W_R^{rk}
+            // odd R, outputs in pairs (k, R-k): with a_r = v_r + v_{R-r},
+            // b_r = v_r - v_{R-r} (r = 1..H) and theta = 2 pi r k / R,
+            //   y_k, y_{R-k} = v_0 + sum_r a_r cos(theta) +- i sign sum_r b_r sin(theta)
+            // (a quarter of the multiplications of the plain DFT sum)
+            constexpr int H = (R - 1) / 2;
+            float2 av[H], bv[H];
+            float2 y0 = v[q][0];
 #pragma unroll
-            for (int k = 0; k < R; k++) {
-                float2 acc = v[q][0];
+            for (int r = 1; r <= H; r++) {
+                av[r - 1] = cadd(v[q][r], v[q][R - r]);
+                bv[r - 1] = csub(v[q][r], v[q][R - r]);
+                y0 = cadd(y0, av[r - 1]);
+            }
+            o[0] = y0;
+            const float2 *rt = root + rootOff<R>();  // (cos, -sin)(2 pi m / R)
+            int m[H];
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    float2 w = tw[N + rootOff<R>() + (r * k) % R];
-                    if (sign > 0) w.y = -w.y;
-                    acc = cadd(acc, cmul(v[q][r], w));
+            for (int r = 0; r < H; r++) m[r] = 0;
+#pragma unroll 1
+            for (int k = 1; k <= H; k++) {
+                float2 A = v[q][0], Bs = czero();
+#pragma unroll
+                for (int r = 0; r < H; r++) {
+                    m[r] += r + 1;
+                    if (m[r] >= R) m[r] -= R;
+                    const float2 wr = rt[m[r]];
+                    A.x += wr.x * av[r].x;
+                    A.y += wr.x * av[r].y;
+                    Bs.x -= wr.y * bv[r].x;
+                    Bs.y -= wr.y * bv[r].y;
                 }
-                y[k] = acc;
+                // i sign Bs
+                const float2 iB = (sign > 0) ? make_float2(-Bs.y, Bs.x) : make_float2(Bs.y, -Bs.x);
+                o[k * Ns] = cadd(A, iB);
+                o[(R - k) * Ns] = csub(A, iB);
             }
         }
-#pragma unroll
-        for (int r = 0; r < R; r++) buf[dst[q] + r * Ns] = y[r];
     }
-    __syncthreads();
+    wave_lds_fence();
 }
 
-// the full transform: fac = stage counts [n4, n2, n3, n5, n7, n11, n13] of N,
-// applied in this order (one call site per radix keeps register use per stage)
-__device__ __forceinline__ void fft_inplace(float2 *buf, int nseq, int N, int sign, const int32_t *fac,
-                                            const float2 *tw) {
-    int Ns = 1;
+// the whole transform of length N <= NMAX from its stage descriptors
+template <int NMAX>
+__device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t *plan, int nst, const float2 *tw,
+                                     const float2 *root, int lane) {
 #pragma unroll 1
-    for (int f = 0; f < fac[0]; f++) {
-        fft_stage<4>(buf, nseq, N, Ns, sign, tw);
-        Ns *= 4;
+    for (int st = 0; st < nst; st++) {
+        const int32_t *d = plan + st * FS_N;
+        const int R = d[FS_R], Ns = d[FS_NS], nb = d[FS_NB], wmul = d[FS_WMUL];
+        const uint32_t mnb = (uint32_t)d[FS_MNB], mNs = (uint32_t)d[FS_MNS];
+        switch (R) {
+            case 16: wfft_stage<16, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 8: wfft_stage<8, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 4: wfft_stage<4, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 2: wfft_stage<2, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 3: wfft_stage<3, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 5: wfft_stage<5, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 7: wfft_stage<7, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 11: wfft_stage<11, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            default: wfft_stage<13, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+        }
     }
-#pragma unroll 1
-    for (int f = 0; f < fac[1]; f++) {
-        fft_stage<2>(buf, nseq, N, Ns, sign, tw);
-        Ns *= 2;
-    }
-#pragma unroll 1
-    for (int f = 0; f < fac[2]; f++) {
-        fft_stage<3>(buf, nseq, N, Ns, sign, tw);
-        Ns *= 3;
-    }
-#pragma unroll 1
-    for (int f = 0; f < fac[3]; f++) {
-        fft_stage<5>(buf, nseq, N, Ns, sign, tw);
-        Ns *= 5;
-    }
-#pragma unroll 1
-    for (int f = 0; f < fac[4]; f++) {
-        fft_stage<7>(buf, nseq, N, Ns, sign, tw);
-        Ns *= 7;
-    }
-    if (fac[5]) {
-        fft_stage<11>(buf, nseq, N, Ns, sign, tw);
-        Ns *= 11;
-    }
-    if (fac[6]) fft_stage<13>(buf, nseq, N, Ns, sign, tw);
 }
 
 // ---------------------------------------------------------------- stress
 template <int MAT>
-__device__ __forceinline__ void stress3d(const float *__restrict__ mp, int Nr, int n, int j, const float e[6],
-                                         float s[6]) {
-    // parameter p of sample n, point j (within the pass row): mp[(p * Nr + n) * 5 + j]
-    auto P = [&](int p) { return mp[((int64_t)p * Nr + n) * 5 + j]; };
-    if (MAT == 0) {
+struct MatParams {
+    static constexpr int v = (MAT == 0) ? 2 : (MAT == 1) ? 5 : 21;
+};
+
+// Hooke's law on one sample; P(p) = parameter p of this point and sample
+template <int MAT, class ParFn>
+__device__ __forceinline__ void stress3d(ParFn P, const float e[6], float s[6]) {
+    if constexpr (MAT == 0) {
         // Isotropic3D::strainToStress (Isotropic3D.cpp:10-28)
         float lam = P(0), mu = P(1), mu2 = 2.f * mu;
         float sii = lam * (e[0] + e[1] + e[2]);
@@ -186,7 +264,7 @@ __device__ __forceinline__ void stress3d(const float *__restrict__ mp, int Nr, i
         s[3] = mu * e[3];
         s[4] = mu * e[4];
         s[5] = mu * e[5];
-    } else if (MAT == 1) {
+    } else if constexpr (MAT == 1) {
         // TransverselyIsotropic3D::strainToStress (TransverselyIsotropic3D.cpp:10-28)
         float A = P(0), Cc = P(1), F = P(2), L = P(3), N = P(4), N2 = 2.f * N;
         float e01 = e[0] + e[1];
@@ -198,9 +276,9 @@ __device__ __forceinline__ void stress3d(const float *__restrict__ mp, int Nr, i
         s[4] = L * e[4];
         s[5] = N * e[5];
     } else {
-        // Anisotropic3D::strainToStress (Anisotropic3D.cpp:10-54)
-        const int idx[6][6] = {{0, 1, 2, 3, 4, 5},     {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
-                               {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
+        // Anisotropic3D::strainToStress (Anisotropic3D.cpp:10-54), upper triangle
+        constexpr int idx[6][6] = {{0, 1, 2, 3, 4, 5},     {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
+                                   {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
 #pragma unroll
         for (int a = 0; a < 6; a++) {
             float acc = 0.f;
@@ -211,348 +289,251 @@ __device__ __forceinline__ void stress3d(const float *__restrict__ mp, int Nr, i
     }
 }
 
-// same, with the parameters already in registers: P[p] of one (sample, point)
-template <int MAT>
-__device__ __forceinline__ void stress3d_reg(const float *P, const float e[6], float s[6]) {
-    if (MAT == 0) {
-        float lam = P[0], mu = P[1], mu2 = 2.f * mu;
-        float sii = lam * (e[0] + e[1] + e[2]);
-        s[0] = sii + mu2 * e[0];
-        s[1] = sii + mu2 * e[1];
-        s[2] = sii + mu2 * e[2];
-        s[3] = mu * e[3];
-        s[4] = mu * e[4];
-        s[5] = mu * e[5];
-    } else {
-        float A = P[0], Cc = P[1], F = P[2], L = P[3], N = P[4], N2 = 2.f * N;
-        float e01 = e[0] + e[1];
-        float tmp = A * e01 + F * e[2];
-        s[0] = tmp - N2 * e[1];
-        s[1] = tmp - N2 * e[0];
-        s[2] = Cc * e[2] + F * e01;
-        s[3] = L * e[3];
-        s[4] = L * e[4];
-        s[5] = N * e[5];
-    }
-}
-
-template <int MAT>
-struct MatParams {
-    static constexpr int v = (MAT == 0) ? 2 : (MAT == 1) ? 5 : 21;
-};
-
 // ---------------------------------------------------------------- kernel
-// Work split: item (j, alpha) -> thread tid + q * T (q < QB).  The same thread
-// owns that item in every pass, so the element's stiffness accumulates in its
-// registers: acc[q][ip][c] = f_c(ip, j) of mode alpha.
-template <int MAT, bool TISO>
-__global__ __launch_bounds__(E3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
+template <int MAT, bool TISO, int S>
+__global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
-    constexpr int T = E3D_THREADS;
-    constexpr int QB = 2;  // 5 * M <= QB * T
-    const int tid = threadIdx.x;
-    const int Mmax = a.maxM;
-    float2 *buf = lds;                  // spectral [30][M] / sequences [15][Nr]
-    float2 *stg = buf + 30 * Mmax;      // [E3D_STAGE_MODES][3][25]
-    float2 *twl = stg + 75 * min(E3D_STAGE_MODES, Mmax);  // W_Nr^t, t < Nr
+    constexpr int NPAR = MatParams<MAT>::v;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // row ipol of this wave
     const int64_t e = a.elems[blockIdx.x];
     const int Nr = a.elem_nr[e];
     const int Nu = Nr / 2;
     const int M = Nu + 1;
-    const int nyq = (Nr % 2 == 0) ? 1 : 0;
+    const bool nyq = (Nr % 2 == 0);
     const bool axial = (a.elem_flags[e] & 2) != 0;
     const float *Gx = cG[axial ? 1 : 0];
     const float *Ge = cG[0];
-    const int32_t *fac = a.factors + Nr * 8;
-    for (int t = tid; t < Nr; t += E3D_THREADS) twl[t] = a.twiddle[a.tw_off[Nr] + t];
-    for (int t = tid; t < kRootTable; t += E3D_THREADS) twl[Nr + t] = a.twiddle[t];  // radix tables
-    const float2 *tw = twl;  // read by every FFT stage; visible after the first barrier
-    const float invNr = 1.f / (float)Nr;
-    const uint32_t mM = udiv_magic(M);
-    constexpr int NPAR = MatParams<MAT>::v;
+    const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
+    const int nst = a.nstage[Nr];
 
-    float2 acc[QB][5][3];
-#pragma unroll
-    for (int q = 0; q < QB; q++)
-#pragma unroll
-        for (int ip = 0; ip < 5; ip++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) acc[q][ip][c] = czero();
+    float2 *twf = lds;                                   // W_N^t, t < Nr
+    float2 *twi = twf + a.maxNr;                         // W_N^{-t}
+    float2 *root = twi + a.maxNr;                        // odd-radix roots
+    float2 *xb = root + kRootTable;                      // [5][3][maxM] xi operands
+    float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [3][Nr]
+    for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
+        float2 v = a.twiddle[a.tw_off[Nr] + t];
+        twf[t] = v;
+        twi[t] = make_float2(v.x, -v.y);
+    }
+    for (int t = threadIdx.x; t < kRootTable; t += W3D_THREADS) root[t] = a.twiddle[t];
+    __syncthreads();
+
+    const int32_t *epr = a.ep_row + e * 25;
+    const int32_t *epa = a.ep_amax + e * 25;
+    const float *mbase = a.mat3d + a.mat3d_off[e];
+    const float h = 0.5f / (float)Nr;
 
 #pragma unroll 1
-    for (int i = 0; i < 5; i++) {
-        const int tl = opaque(tid);
-        // ---------------- strain spectrum of points (i, 0..4), modes 0..Nu
+    for (int k = 0; k < 5; k++) {
+        const int lane = opaque((int)(threadIdx.x & 63));
+        const int ip = w * 5 + k;  // this wave's point in this step
+        PointGeom g = load_geom(a.geom, e, ip);
+        float ts1 = 0.f, tc1 = 0.f, ts2 = 0.f, tc2 = 0.f;
+        if (TISO) {
+            const float *tp = a.tiso + e * 100 + ip;
+            ts1 = tp[0];
+            tc1 = tp[25];
+            ts2 = tp[50];
+            tc2 = tp[75];
+        }
+
+        // ---------------- strain spectrum -> 3 Hermitian-extended sequences
 #pragma unroll 1
-        for (int a0 = 0; a0 < M; a0 += E3D_STAGE_MODES) {
-            const int nch = min(E3D_STAGE_MODES, M - a0);
-            __syncthreads();
-            for (int t = tl; t < nch * 25; t += T) {
-                int q = t / 25, k = t - q * 25, al = a0 + q;
-                int row = a.ep_row[e * 25 + k];
-                bool ok = al <= a.ep_amax[e * 25 + k];
-                const float2 *src = a.displ + ((int64_t)row + al) * 3;
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            if (al >= M) continue;
+            float2 e6[6];
+            if (nyq && al == Nu) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) stg[(q * 3 + c) * 25 + k] = ok ? src[c] : czero();
-            }
-            __syncthreads();
-            for (int t = tl; t < nch * 5; t += T) {
-                int q = t / 5, j = t - q * 5, al = a0 + q;
-                float2 e6[6];
-                if (nyq && al == Nu) {
+                for (int c = 0; c < 6; c++) e6[c] = czero();
+            } else {
+                float2 GU[3], UG[3], u[3];
 #pragma unroll
-                    for (int c = 0; c < 6; c++) e6[c] = czero();
-                } else {
-                    const float2 *U = stg + q * 75;  // U[c*25 + m*5 + n]
-                    PointGeom g = load_geom(a.geom, e, i * 5 + j);
-                    float2 GU[3], UG[3];
+                for (int c = 0; c < 3; c++) GU[c] = UG[c] = u[c] = czero();
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    // column k: point (m, k); row w: point (w, m)
+                    const int pc = m * 5 + k, pr = w * 5 + m;
+                    const float gx = Gx[m * 5 + w], ge = Ge[m * 5 + k];
+                    const bool okc = al <= epa[pc], okr = al <= epa[pr];
+                    const float2 *uc = a.displ + ((int64_t)epr[pc] + al) * 3;
+                    const float2 *ur = a.displ + ((int64_t)epr[pr] + al) * 3;
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        float2 x = czero(), y = czero();
-#pragma unroll
-                        for (int m = 0; m < 5; m++) {
-                            float gx = Gx[m * 5 + i], ge = Ge[m * 5 + j];
-                            float2 um = U[c * 25 + m * 5 + j], un = U[c * 25 + i * 5 + m];
-                            x.x += gx * um.x;
-                            x.y += gx * um.y;
-                            y.x += un.x * ge;
-                            y.y += un.y * ge;
-                        }
-                        GU[c] = x;
-                        UG[c] = y;
-                    }
-                    float fa = (float)al;
-                    float2 u0 = U[0 * 25 + i * 5 + j], u1 = U[1 * 25 + i * 5 + j], u2 = U[2 * 25 + i * 5 + j];
-                    float2 v0 = cadd(u0, cimul(fa, u1)), v1 = csub(cimul(fa, u0), u1), v2 = cimul(fa, u2);
-                    e6[0] = cadd(cscale(g.dzdeta, GU[0]), cscale(g.dzdxii, UG[0]));
-                    e6[1] = cscale(g.inv_s, v0);
-                    e6[2] = cadd(cscale(g.dsdeta, GU[2]), cscale(g.dsdxii, UG[2]));
-                    e6[3] = cadd(cadd(cscale(g.dsdeta, GU[1]), cscale(g.dsdxii, UG[1])), cscale(g.inv_s, v2));
-                    e6[4] = cadd(cadd(cscale(g.dsdeta, GU[0]), cscale(g.dsdxii, UG[0])),
-                                 cadd(cscale(g.dzdeta, GU[2]), cscale(g.dzdxii, UG[2])));
-                    e6[5] = cadd(cadd(cscale(g.dzdeta, GU[1]), cscale(g.dzdxii, UG[1])), cscale(g.inv_s, v1));
-                    if (axial && i == 0) {
-                        e6[1] = cadd(e6[1], cscale(g.dzdeta, cadd(GU[0], cimul(fa, GU[1]))));
-                        e6[5] = cadd(e6[5], cscale(g.dzdeta, csub(cimul(fa, GU[0]), GU[1])));
-                        e6[3] = cadd(e6[3], cscale(g.dzdeta, cimul(fa, GU[2])));
-                        if (al == 1) {
-                            e6[1] = cadd(e6[1], cscale(g.dzdxii, cadd(UG[0], cimul(fa, UG[1]))));
-                            e6[5] = cadd(e6[5], cscale(g.dzdxii, csub(cimul(fa, UG[0]), UG[1])));
-                        }
-                    }
-                    if (TISO) {
-                        const float *tp = a.tiso + e * 100 + i * 5 + j;
-                        tiso_spz_rtz(e6, tp[0], tp[25], tp[50], tp[75]);
+                        float2 vc = okc ? uc[c] : czero();
+                        float2 vr = okr ? ur[c] : czero();
+                        GU[c].x += gx * vc.x;
+                        GU[c].y += gx * vc.y;
+                        UG[c].x += vr.x * ge;
+                        UG[c].y += vr.y * ge;
+                        if (m == k) u[c] = vr;
                     }
                 }
-#pragma unroll
-                for (int c = 0; c < 6; c++) buf[(c * 5 + j) * M + al] = e6[c];
-            }
-        }
-        __syncthreads();
-
-        // material of this pass row, issued before the C2R transform so that the
-        // loads overlap it (iso / TI: 2 or 5 parameters per sample and point)
-        constexpr int SB = 3;  // 5 * Nr <= SB * T
-        constexpr bool PREF = NPAR <= 5;
-        float pm[PREF ? SB : 1][PREF ? NPAR : 1];
-        const float *mp = a.mat3d + a.mat3d_off[e] + (int64_t)i * NPAR * Nr * 5;
-        if constexpr (PREF) {
-#pragma unroll
-            for (int q = 0; q < SB; q++) {
-                int t = tl + q * T;
-                if (t < 5 * Nr) {
-                    int n = t / 5, j = t - n * 5;
-#pragma unroll
-                    for (int p = 0; p < NPAR; p++) pm[q][p] = mp[((int64_t)p * Nr + n) * 5 + j];
+                const float fa = (float)al;
+                float2 v0 = cadd(u[0], cimul(fa, u[1])), v1 = csub(cimul(fa, u[0]), u[1]), v2 = cimul(fa, u[2]);
+                e6[0] = cadd(cscale(g.dzdeta, GU[0]), cscale(g.dzdxii, UG[0]));
+                e6[1] = cscale(g.inv_s, v0);
+                e6[2] = cadd(cscale(g.dsdeta, GU[2]), cscale(g.dsdxii, UG[2]));
+                e6[3] = cadd(cadd(cscale(g.dsdeta, GU[1]), cscale(g.dsdxii, UG[1])), cscale(g.inv_s, v2));
+                e6[4] = cadd(cadd(cscale(g.dsdeta, GU[0]), cscale(g.dsdxii, UG[0])),
+                             cadd(cscale(g.dzdeta, GU[2]), cscale(g.dzdxii, UG[2])));
+                e6[5] = cadd(cadd(cscale(g.dzdeta, GU[1]), cscale(g.dzdxii, UG[1])), cscale(g.inv_s, v1));
+                if (axial && w == 0) {
+                    // axis row: the 1/s terms become derivatives (Gradient.cpp:240-262)
+                    e6[1] = cadd(e6[1], cscale(g.dzdeta, cadd(GU[0], cimul(fa, GU[1]))));
+                    e6[5] = cadd(e6[5], cscale(g.dzdeta, csub(cimul(fa, GU[0]), GU[1])));
+                    e6[3] = cadd(e6[3], cscale(g.dzdeta, cimul(fa, GU[2])));
+                    if (al == 1) {
+                        e6[1] = cadd(e6[1], cscale(g.dzdxii, cadd(UG[0], cimul(fa, UG[1]))));
+                        e6[5] = cadd(e6[5], cscale(g.dzdxii, csub(cimul(fa, UG[0]), UG[1])));
+                    }
                 }
+                if (TISO) tiso_spz_rtz(e6, ts1, tc1, ts2, tc2);
             }
-        }
-
-        // ---------------- pack 30 spectra into 15 Hermitian-extended sequences
-        {
-            constexpr int PB = 4;  // 15 * M <= PB * T
-            float2 X[PB], Y[PB];
+            // Z_b = X_{2b} + i X_{2b+1}; Z_{N-k} = conj(X_k) + i conj(Y_k); the
+            // imaginary parts of DC and Nyquist are dropped (FFTW c2r)
 #pragma unroll
-            for (int q = 0; q < PB; q++) {
-                int t = tl + q * T;
-                if (t < 15 * M) {
-                    int b = udiv(t, M, mM), k = t - b * M;
-                    X[q] = buf[(2 * b) * M + k];
-                    Y[q] = buf[(2 * b + 1) * M + k];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < PB; q++) {
-                int t = tl + q * T;
-                if (t >= 15 * M) continue;
-                int b = udiv(t, M, mM), k = t - b * M;
-                float2 *z = buf + b * Nr;
-                if (k == 0 || 2 * k == Nr) {
-                    // DC and Nyquist: the imaginary parts are ignored (FFTW c2r)
-                    z[k] = make_float2(X[q].x, Y[q].x);
+            for (int b = 0; b < 3; b++) {
+                float2 X = e6[2 * b], Y = e6[2 * b + 1];
+                float2 *z = seq + b * Nr;
+                if (al == 0 || 2 * al == Nr) {
+                    z[al] = make_float2(X.x, Y.x);
                 } else {
-                    // Z_k = X_k + i Y_k, Z_{N-k} = conj(X_k) + i conj(Y_k)
-                    z[k] = make_float2(X[q].x - Y[q].y, X[q].y + Y[q].x);
-                    z[Nr - k] = make_float2(X[q].x + Y[q].y, -X[q].y + Y[q].x);
+                    z[al] = make_float2(X.x - Y.y, X.y + Y.x);
+                    z[Nr - al] = make_float2(X.x + Y.y, -X.y + Y.x);
                 }
             }
-            __syncthreads();
         }
-        fft_inplace(buf, 15, Nr, +1, fac, tw);
-
-        // ---------------- stress at every (sample n, point j)
-        {
-#pragma unroll
-            for (int q = 0; q < SB; q++) {
-                int t = tl + q * T;
-                if (t >= 5 * Nr) continue;
-                int n = t / 5, j = t - n * 5;
+        wave_lds_fence();
+        // C2R (pass 0), stress on the samples, R2C (pass 1): one copy of the
+        // transform code serves both directions
+#pragma unroll 1
+        for (int pass = 0; pass < 2; pass++) {
+            wfft<128 * S - 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
+            if (pass == 1) break;
+            // ---------------- stress at every sample n of the point
+            const float *mp = mbase + (int64_t)ip * NPAR * Nr;
+            auto sample = [&](int n, auto P) {
                 float ev[6], sv[6];
 #pragma unroll
-                for (int c = 0; c < 6; c++) {
-                    int col = c * 5 + j;
-                    float2 z = buf[(col >> 1) * Nr + n];
-                    ev[c] = (col & 1) ? z.y : z.x;
+                for (int b = 0; b < 3; b++) {
+                    float2 z = seq[b * Nr + n];
+                    ev[2 * b] = z.x;
+                    ev[2 * b + 1] = z.y;
                 }
-                if constexpr (PREF) stress3d_reg<MAT>(pm[q], ev, sv);
-                else stress3d<MAT>(mp, Nr, n, j, ev, sv);
+                stress3d<MAT>(P, ev, sv);
 #pragma unroll
-                for (int c = 0; c < 6; c++) {
-                    int col = c * 5 + j;
-                    float2 *z = buf + (col >> 1) * Nr + n;
-                    if (col & 1) z->y = sv[c];
-                    else z->x = sv[c];
+                for (int b = 0; b < 3; b++) seq[b * Nr + n] = make_float2(sv[2 * b], sv[2 * b + 1]);
+            };
+            if constexpr (NPAR <= 5) {
+                // all samples' parameters first (Nr <= 128 S: at most 2 S per lane)
+                float pm[2 * S][NPAR];
+#pragma unroll
+                for (int t = 0; t < 2 * S; t++) {
+                    const int n = lane + 64 * t;
+                    if (n < Nr)
+#pragma unroll
+                        for (int p = 0; p < NPAR; p++) pm[t][p] = mp[p * Nr + n];
                 }
-            }
-            __syncthreads();
-        }
-        fft_inplace(buf, 15, Nr, -1, fac, tw);
-
-        // ---------------- unpack: X_k = (Z_k + conj Z_{N-k}) / 2, Y_k = (Z_k - conj Z_{N-k}) / 2i, x 1/Nr
-        {
-            constexpr int PB = 4;
-            float2 Z0[PB], Z1[PB];
 #pragma unroll
-            for (int q = 0; q < PB; q++) {
-                int t = tl + q * T;
-                if (t < 15 * M) {
-                    int b = udiv(t, M, mM), k = t - b * M;
-                    const float2 *z = buf + b * Nr;
-                    Z0[q] = z[k];
-                    Z1[q] = z[k == 0 ? 0 : Nr - k];
+                for (int t = 0; t < 2 * S; t++) {
+                    const int n = lane + 64 * t;
+                    if (n < Nr) sample(n, [&](int p) { return pm[t][p]; });
                 }
+            } else {
+#pragma unroll 1
+                for (int n = lane; n < Nr; n += 64) sample(n, [&](int p) { return mp[p * Nr + n]; });
             }
-            __syncthreads();
-            const float h = 0.5f * invNr;
-#pragma unroll
-            for (int q = 0; q < PB; q++) {
-                int t = tl + q * T;
-                if (t >= 15 * M) continue;
-                int b = udiv(t, M, mM), k = t - b * M;
-                float2 za = Z0[q], zb = Z1[q];
-                buf[(2 * b) * M + k] = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
-                buf[(2 * b + 1) * M + k] = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
-            }
-            __syncthreads();
+            wave_lds_fence();
         }
 
-        // ---------------- rotate back, quadrature operands, accumulate in registers.
-        // A thread reads the stress of its own (j, alpha) items and overwrites the
-        // same LDS slots with its Y operands, so no barrier is needed in between.
-        {
+        // ---------------- unpack x 1/Nr, rotate back, quadrature operands:
+        // xi operand X to the exchange, eta operand Y and the own-point term
+        // inv_s g kept for the update after the exchange
+        float2 Yk[S][3], Ok[S][3];
 #pragma unroll
-            for (int q = 0; q < QB; q++) {
-                int t = tl + q * T;
-                int al = t / 5, j = t - al * 5;
-                if (t < 5 * M && !(nyq && al == Nu)) {
-                    float2 s6[6];
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
 #pragma unroll
-                    for (int c = 0; c < 6; c++) s6[c] = buf[(c * 5 + j) * M + al];
-                    if (TISO) {
-                        const float *tp = a.tiso + e * 100 + i * 5 + j;
-                        tiso_rtz_spz(s6, tp[0], tp[25], tp[50], tp[75]);
-                    }
-                    PointGeom g = load_geom(a.geom, e, i * 5 + j);
-                    float nb = -(float)al;
-                    float2 X[3], Y[3], gg[3];
-                    X[0] = cadd(cscale(g.dzdeta, s6[0]), cscale(g.dsdeta, s6[4]));
-                    X[1] = cadd(cscale(g.dzdeta, s6[5]), cscale(g.dsdeta, s6[3]));
-                    X[2] = cadd(cscale(g.dzdeta, s6[4]), cscale(g.dsdeta, s6[2]));
-                    Y[0] = cadd(cscale(g.dzdxii, s6[0]), cscale(g.dsdxii, s6[4]));
-                    Y[1] = cadd(cscale(g.dzdxii, s6[5]), cscale(g.dsdxii, s6[3]));
-                    Y[2] = cadd(cscale(g.dzdxii, s6[4]), cscale(g.dsdxii, s6[2]));
-                    gg[0] = cadd(s6[1], cimul(nb, s6[5]));
-                    gg[1] = csub(cimul(nb, s6[1]), s6[5]);
-                    gg[2] = cimul(nb, s6[3]);
-                    // own point: inv_s g
+            for (int c = 0; c < 3; c++) Yk[s][c] = Ok[s][c] = czero();
+            if (al >= M) continue;
+            float2 X[3] = {czero(), czero(), czero()};
+            if (!(nyq && al == Nu)) {
+                float2 s6[6];
 #pragma unroll
-                    for (int ip = 0; ip < 5; ip++)
-                        if (ip == i)
-#pragma unroll
-                            for (int c = 0; c < 3; c++) acc[q][ip][c] = cadd(acc[q][ip][c], cscale(g.inv_s, gg[c]));
-                    // axial row 0: G_xi(:,0) dzdeta g has the shape of the X term,
-                    // (dzdxii g) G^T_eta that of the Y term
-                    if (axial && i == 0) {
-#pragma unroll
-                        for (int c = 0; c < 3; c++) X[c] = cadd(X[c], cscale(g.dzdeta, gg[c]));
-                        if (al == 1) {
-                            Y[0] = cadd(Y[0], cscale(g.dzdxii, gg[0]));
-                            Y[1] = cadd(Y[1], cscale(g.dzdxii, gg[1]));
-                        }
-                    }
-                    // X term: column j of every row.  For axial elements the W term
-                    // uses G_xi(ip, 0) and the X term G_xi(ip, i); on row 0 both are
-                    // G_xi(ip, 0), so they were merged above.
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-#pragma unroll
-                        for (int ip = 0; ip < 5; ip++)
-                            acc[q][ip][c] = cadd(acc[q][ip][c], cscale(Gx[ip * 5 + i], X[c]));
-#pragma unroll
-                    for (int c = 0; c < 3; c++) buf[(c * 5 + j) * M + al] = Y[c];
+                for (int b = 0; b < 3; b++) {
+                    const float2 *z = seq + b * Nr;
+                    float2 za = z[al], zb = z[al == 0 ? 0 : Nr - al];
+                    s6[2 * b] = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
+                    s6[2 * b + 1] = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
                 }
-            }
-            __syncthreads();
-            // Y term: row i, f(i, j) += sum_m Y(i, m) G_eta(j, m)
+                if (TISO) tiso_rtz_spz(s6, ts1, tc1, ts2, tc2);
+                const float nb = -(float)al;
+                float2 gg[3];
+                X[0] = cadd(cscale(g.dzdeta, s6[0]), cscale(g.dsdeta, s6[4]));
+                X[1] = cadd(cscale(g.dzdeta, s6[5]), cscale(g.dsdeta, s6[3]));
+                X[2] = cadd(cscale(g.dzdeta, s6[4]), cscale(g.dsdeta, s6[2]));
+                Yk[s][0] = cadd(cscale(g.dzdxii, s6[0]), cscale(g.dsdxii, s6[4]));
+                Yk[s][1] = cadd(cscale(g.dzdxii, s6[5]), cscale(g.dsdxii, s6[3]));
+                Yk[s][2] = cadd(cscale(g.dzdxii, s6[4]), cscale(g.dsdxii, s6[2]));
+                gg[0] = cadd(s6[1], cimul(nb, s6[5]));
+                gg[1] = csub(cimul(nb, s6[1]), s6[5]);
+                gg[2] = cimul(nb, s6[3]);
+                if (axial && w == 0) {
+                    // axis row: G_xi(:,0) dzdeta g joins the xi term, (dzdxii g)
+                    // G_eta^T the eta term at alpha = 1 (Gradient.cpp:291-315)
 #pragma unroll
-            for (int q = 0; q < QB; q++) {
-                int t = tl + q * T;
-                int al = t / 5, j = t - al * 5;
-                if (t < 5 * M && !(nyq && al == Nu)) {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        float2 f = czero();
-#pragma unroll
-                        for (int m = 0; m < 5; m++) {
-                            float ge = Ge[j * 5 + m];
-                            float2 y = buf[(c * 5 + m) * M + al];
-                            f.x += y.x * ge;
-                            f.y += y.y * ge;
-                        }
-#pragma unroll
-                        for (int ip = 0; ip < 5; ip++)
-                            if (ip == i) acc[q][ip][c] = cadd(acc[q][ip][c], f);
+                    for (int c = 0; c < 3; c++) X[c] = cadd(X[c], cscale(g.dzdeta, gg[c]));
+                    if (al == 1) {
+                        Yk[s][0] = cadd(Yk[s][0], cscale(g.dzdxii, gg[0]));
+                        Yk[s][1] = cadd(Yk[s][1], cscale(g.dzdxii, gg[1]));
                     }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) Ok[s][c] = cscale(g.inv_s, gg[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) xb[(w * 3 + c) * a.maxM + al] = X[c];
+        }
+        __syncthreads();
+        // xi term of this wave's point: sum_i G_xi(w, i) X(i, k) over the column
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            if (al >= M) continue;
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const float gx = Gx[w * 5 + i];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    float2 x = xb[(i * 3 + c) * a.maxM + al];
+                    Ok[s][c].x += gx * x.x;
+                    Ok[s][c].y += gx * x.y;
                 }
             }
         }
-    }
-
-    // ---------------- gatherStiffFromElement: stiff -= f on valid rows
+        __syncthreads();  // the exchange buffer is rewritten next step
+        // gatherStiffFromElement for this step's share: the row's points get
+        // G_eta(jp, k) Y(w, k), the point (w, k) also its xi and own terms
 #pragma unroll
-    for (int q = 0; q < QB; q++) {
-        int t = tid + q * T;
-        int al = t / 5, j = t - al * 5;
-        if (t >= 5 * M || (nyq && al == Nu)) continue;
+        for (int jp = 0; jp < 5; jp++) {
+            const int pp = w * 5 + jp;
+            const int amax = epa[pp];
+            const float ge = Ge[jp * 5 + k];
+            float2 *sp0 = a.stiff + (int64_t)epr[pp] * 3;
 #pragma unroll
-        for (int ip = 0; ip < 5; ip++) {
-            int k = ip * 5 + j;
-            if (al > a.ep_amax[e * 25 + k]) continue;
-            float2 *sp = a.stiff + ((int64_t)a.ep_row[e * 25 + k] + al) * 3;
+            for (int s = 0; s < S; s++) {
+                const int al = lane + 64 * s;
+                if (al >= M || al > amax || (nyq && al == Nu)) continue;
+                float2 *sp = sp0 + (int64_t)al * 3;
 #pragma unroll
-            for (int c = 0; c < 3; c++) sp[c] = csub(sp[c], acc[q][ip][c]);
+                for (int c = 0; c < 3; c++) {
+                    float2 f = cscale(ge, Yk[s][c]);
+                    if (jp == k) f = cadd(f, Ok[s][c]);
+                    sp[c] = csub(sp[c], f);
+                }
+            }
         }
     }
 }
