@@ -137,12 +137,14 @@ __device__ __forceinline__ void dft_pow2(float2 (&x)[R], int sign) {
     }
 }
 
-// One Stockham stage on 3 sequences [3][N] at `seq`: butterfly j of sequence
-// s reads x[j + r nb] * W_{Ns R}^{(j % Ns) r}, does a radix-R DFT and writes
-// y[(j / Ns) Ns R + (j % Ns) + r Ns].  sign = -1 forward, +1 inverse.  `tw`
+// One Stockham stage on 3 interleaved sequences ([N][3] at `seq`: element n
+// of sequence s at 3 n + s): butterfly j of sequence s reads
+// x[j + r nb] * W_{Ns R}^{(j % Ns) r}, does a radix-R DFT and writes
+// y[(j / Ns) Ns R + (j % Ns) + r Ns].  Lane task t = 3 j + s, so the reads of
+// a stage are contiguous across lanes.  sign = -1 forward, +1 inverse.  `tw`
 // holds W_N^{sign t} for t < N; `root` the forward odd-radix root tables.
 template <int R, int NMAX>
-__device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, uint32_t mnb, uint32_t mNs, int wmul,
+__device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, uint32_t mNs, int wmul,
                                            int sign, const float2 *tw, const float2 *root, int lane) {
     constexpr int B = WaveB<R, NMAX>::v;
     const int total = 3 * nb;
@@ -153,14 +155,14 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
         int t = lane + q * 64;
         dst[q] = -1;
         if (t < total) {
-            int s = udiv(t, nb, mnb), j = t - s * nb;
-            int jq = udiv(j, Ns, mNs), jm = j - jq * Ns;
-            const float2 *x = seq + s * N + j;
+            const int j = t / 3, s = t - 3 * j;
+            const int jq = udiv(j, Ns, mNs), jm = j - jq * Ns;
+            const float2 *x = seq + t;
             const int wstep = jm * wmul;
             const float2 *tp = tw;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                float2 xv = x[r * nb];
+                float2 xv = x[3 * r * nb];
                 if (r == 0) {
                     v[q][r] = xv;
                 } else {
@@ -168,7 +170,7 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
                     v[q][r] = cmul(xv, *tp);
                 }
             }
-            dst[q] = s * N + jq * Ns * R + jm;
+            dst[q] = 3 * (jq * Ns * R + jm) + s;
         }
     }
     wave_lds_fence();
@@ -179,7 +181,7 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
         if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
             dft_pow2<R>(v[q], sign);
 #pragma unroll
-            for (int r = 0; r < R; r++) o[r * Ns] = v[q][r];
+            for (int r = 0; r < R; r++) o[3 * r * Ns] = v[q][r];
         } else {
             // odd R, outputs in pairs (k, R-k): with a_r = v_r + v_{R-r},
             // b_r = v_r - v_{R-r} (r = 1..H) and theta = 2 pi r k / R,
@@ -214,8 +216,8 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
                 }
                 // i sign Bs
                 const float2 iB = (sign > 0) ? make_float2(-Bs.y, Bs.x) : make_float2(Bs.y, -Bs.x);
-                o[k * Ns] = cadd(A, iB);
-                o[(R - k) * Ns] = csub(A, iB);
+                o[3 * k * Ns] = cadd(A, iB);
+                o[3 * (R - k) * Ns] = csub(A, iB);
             }
         }
     }
@@ -230,17 +232,17 @@ __device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t
     for (int st = 0; st < nst; st++) {
         const int32_t *d = plan + st * FS_N;
         const int R = d[FS_R], Ns = d[FS_NS], nb = d[FS_NB], wmul = d[FS_WMUL];
-        const uint32_t mnb = (uint32_t)d[FS_MNB], mNs = (uint32_t)d[FS_MNS];
+        const uint32_t mNs = (uint32_t)d[FS_MNS];
         switch (R) {
-            case 16: wfft_stage<16, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 8: wfft_stage<8, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 4: wfft_stage<4, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 2: wfft_stage<2, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 3: wfft_stage<3, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 5: wfft_stage<5, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 7: wfft_stage<7, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            case 11: wfft_stage<11, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
-            default: wfft_stage<13, NMAX>(seq, N, Ns, nb, mnb, mNs, wmul, sign, tw, root, lane); break;
+            case 16: wfft_stage<16, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 8: wfft_stage<8, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 4: wfft_stage<4, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 2: wfft_stage<2, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 3: wfft_stage<3, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 5: wfft_stage<5, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 7: wfft_stage<7, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 11: wfft_stage<11, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            default: wfft_stage<13, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
         }
     }
 }
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     float2 *twi = twf + a.maxNr;                         // W_N^{-t}
     float2 *root = twi + a.maxNr;                        // odd-radix roots
     float2 *xb = root + kRootTable;                      // [5][3][maxM] xi operands
-    float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [3][Nr]
+    float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [Nr][3]
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
         float2 v = a.twiddle[a.tw_off[Nr] + t];
         twf[t] = v;
@@ -357,12 +359,12 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                     const int pc = m * 5 + k, pr = w * 5 + m;
                     const float gx = Gx[m * 5 + w], ge = Ge[m * 5 + k];
                     const bool okc = al <= epa[pc], okr = al <= epa[pr];
-                    const float2 *uc = a.displ + ((int64_t)epr[pc] + al) * 3;
-                    const float2 *ur = a.displ + ((int64_t)epr[pr] + al) * 3;
+                    const float2 *uc = a.displ + (int64_t)epr[pc] * 3;  // uniform row bases
+                    const float2 *ur = a.displ + (int64_t)epr[pr] * 3;
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        float2 vc = okc ? uc[c] : czero();
-                        float2 vr = okr ? ur[c] : czero();
+                        float2 vc = okc ? uc[al * 3 + c] : czero();
+                        float2 vr = okr ? ur[al * 3 + c] : czero();
                         GU[c].x += gx * vc.x;
                         GU[c].y += gx * vc.y;
                         UG[c].x += vr.x * ge;
@@ -396,12 +398,12 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 float2 X = e6[2 * b], Y = e6[2 * b + 1];
-                float2 *z = seq + b * Nr;
+                float2 *z = seq + b;
                 if (al == 0 || 2 * al == Nr) {
-                    z[al] = make_float2(X.x, Y.x);
+                    z[3 * al] = make_float2(X.x, Y.x);
                 } else {
-                    z[al] = make_float2(X.x - Y.y, X.y + Y.x);
-                    z[Nr - al] = make_float2(X.x + Y.y, -X.y + Y.x);
+                    z[3 * al] = make_float2(X.x - Y.y, X.y + Y.x);
+                    z[3 * (Nr - al)] = make_float2(X.x + Y.y, -X.y + Y.x);
                 }
             }
         }
@@ -418,13 +420,13 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                 float ev[6], sv[6];
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
-                    float2 z = seq[b * Nr + n];
+                    float2 z = seq[3 * n + b];
                     ev[2 * b] = z.x;
                     ev[2 * b + 1] = z.y;
                 }
                 stress3d<MAT>(P, ev, sv);
 #pragma unroll
-                for (int b = 0; b < 3; b++) seq[b * Nr + n] = make_float2(sv[2 * b], sv[2 * b + 1]);
+                for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };
             if constexpr (NPAR <= 5) {
                 // all samples' parameters first (Nr <= 128 S: at most 2 S per lane)
@@ -463,8 +465,8 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                 float2 s6[6];
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
-                    const float2 *z = seq + b * Nr;
-                    float2 za = z[al], zb = z[al == 0 ? 0 : Nr - al];
+                    const float2 *z = seq + b;
+                    float2 za = z[3 * al], zb = z[3 * (al == 0 ? 0 : Nr - al)];
                     s6[2 * b] = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
                     s6[2 * b + 1] = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
                 }
@@ -526,12 +528,11 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             for (int s = 0; s < S; s++) {
                 const int al = lane + 64 * s;
                 if (al >= M || al > amax || (nyq && al == Nu)) continue;
-                float2 *sp = sp0 + (int64_t)al * 3;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     float2 f = cscale(ge, Yk[s][c]);
                     if (jp == k) f = cadd(f, Ok[s][c]);
-                    sp[c] = csub(sp[c], f);
+                    sp0[al * 3 + c] = csub(sp0[al * 3 + c], f);
                 }
             }
         }
