@@ -146,6 +146,7 @@ def declare_domain_api(lib):
         "axg_check_stability": ([vp, i64p], C.c_int),
         "axg_learn_wisdom": ([vp, C.c_float], C.c_int),
         "axg_get_wisdom": ([vp, i32p], C.c_int),
+        "axg_set_learning": ([vp, C.c_float, C.c_int32], C.c_int),
         "axg_run": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "axg_get_seismograms": ([vp, f32p, C.c_int64], C.c_int),
         "axg_get_field": ([vp, C.c_int32, f32p, C.c_int64], C.c_int),

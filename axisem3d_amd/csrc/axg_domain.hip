@@ -155,7 +155,8 @@ struct axg_ctx {
     DevBuf<float> d_src_force, d_stf;
     DevBuf<int32_t> d_src_row, d_src_nu;
     // receivers
-    DevBuf<int32_t> d_rec_elem;
+    DevBuf<int32_t> d_rec_elem, d_rec_idx_s, d_rec_idx_f;
+    int32_t nrec_solid = 0, nrec_fluid = 0;
     DevBuf<float> d_rec_w, d_rec_phi, d_rec_cost, d_rec_sint, d_seis;
     int32_t rec_rtz = 1;
     // stability
@@ -181,6 +182,11 @@ struct axg_ctx {
     hipGraph_t graph = nullptr;
     int graph_chunk = 0;
     DevBuf<int32_t> d_tstep;
+    // Nu wisdom (LearnParameters: cutoff, interval; 0 = not learning in axg_run)
+    float learn_cutoff = 1e-3f;
+    int learn_interval = 0;
+    DevBuf<double> d_wmax_s, d_wmax_f;
+    DevBuf<int32_t> d_wnu_s, d_wnu_f;
 };
 
 namespace {
@@ -399,9 +405,9 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
-#define API_GUARD(body)                                                          \
+#define API_GUARD(...)                                                           \
     try {                                                                        \
-        body;                                                                    \
+        __VA_ARGS__;                                                             \
         return AXG_OK;                                                           \
     } catch (const HipError &e) {                                                \
         return fail(AXG_EHIP, e.what());                                         \
@@ -526,6 +532,34 @@ void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
     }
 }
 
+// (re)starts learning: running maxima -1, learned orders Nu (SolidPoint.cpp:16,
+// SolidPoint.h:78-79, FluidPoint.cpp:16, FluidPoint.h:77)
+void resetWisdom(axg_ctx *c) {
+    hipStream_t s = c->stream;
+    for (int ph = 0; ph < 2; ph++) {
+        Phase &P = ph == 0 ? c->S : c->F;
+        const int nc = ph == 0 ? 3 : 1;
+        std::vector<double> m((size_t)P.npt * nc, -1.);
+        std::vector<int32_t> nu((size_t)P.npt * nc);
+        for (int q = 0; q < P.npt; q++)
+            for (int k = 0; k < nc; k++) nu[(size_t)q * nc + k] = P.pt_nu[q];
+        (ph == 0 ? c->d_wmax_s : c->d_wmax_f).upload(m, s);
+        (ph == 0 ? c->d_wnu_s : c->d_wnu_f).upload(nu, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+}
+
+void doLearn(axg_ctx *c, float cutoff, int interval, const int *tstep_ptr) {
+    FamScope fs(c, AXG_KF_OTHER);
+    const int grid = 1024;  // 4 waves per block, points strided
+    if (c->S.npt)
+        hipLaunchKernelGGL(learn_wisdom_kernel<3>, dim3(std::min(grid, (c->S.npt + 3) / 4)), dim3(256), 0, c->stream,
+                           c->S.view(), c->S.displ.p, c->d_wmax_s.p, c->d_wnu_s.p, cutoff, interval, tstep_ptr);
+    if (c->F.npt)
+        hipLaunchKernelGGL(learn_wisdom_kernel<1>, dim3(std::min(grid, (c->F.npt + 3) / 4)), dim3(256), 0, c->stream,
+                           c->F.view(), c->F.displ.p, c->d_wmax_f.p, c->d_wnu_f.p, cutoff, interval, tstep_ptr);
+}
+
 void doNewmark(axg_ctx *c, double dt, int *tstep_counter = nullptr) {
     FamScope fs(c, AXG_KF_NEWMARK);
     double half_dt = 0.5 * dt;
@@ -572,9 +606,17 @@ void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
     if (c->nrec == 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::record || time step out of range.");
     FamScope fs(c, AXG_KF_OTHER);
-    hipLaunchKernelGGL(record_solid_kernel, dim3(c->nrec), dim3(256), 0, c->stream, c->nrec, c->d_rec_elem.p,
-                       c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p, c->d_rec_sint.p, c->d_ep_row.p,
-                       c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz, tstep, tstep_ptr, c->d_seis.p);
+    if (c->nrec_solid)
+        hipLaunchKernelGGL(record_solid_kernel, dim3(c->nrec_solid), dim3(256), 0, c->stream, c->nrec_solid, c->nrec,
+                           c->d_rec_idx_s.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
+                           c->d_rec_sint.p, c->d_ep_row.p, c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz,
+                           tstep, tstep_ptr, c->d_seis.p);
+    if (c->nrec_fluid)
+        hipLaunchKernelGGL(record_fluid_kernel, dim3(c->nrec_fluid), dim3(256), 0, c->stream, c->nrec_fluid, c->nrec,
+                           c->d_rec_idx_f.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
+                           c->d_rec_sint.p, c->d_ep_row.p, c->d_ep_amax.p, c->d_elem_nr.p, c->d_elem_flags.p,
+                           c->d_geom.p, c->d_mat_pool.p, c->d_mat_off.p, c->F.displ.p, c->rec_rtz, tstep, tstep_ptr,
+                           c->d_seis.p);
 }
 
 int64_t doStability(axg_ctx *c) {
@@ -804,14 +846,24 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
     c->rec_rtz = d->rec_components == 0;
     if (c->nrec > 0) {
         std::vector<float> cost(c->nrec), sint(c->nrec);
+        std::vector<int32_t> idx_s, idx_f;
         for (int i = 0; i < c->nrec; i++) {
             int e = d->rec_elem[i];
             if (e < 0 || e >= NE) throw std::runtime_error("PointwiseRecorder::addReceiver || bad element.");
-            if (d->elem_flags[e] & AXG_EL_FLUID)
-                throw std::runtime_error("FluidElement::computeGroundMotion || fluid receivers are not supported by this build.");
+            if (d->elem_flags[e] & AXG_EL_FLUID) {
+                if (d->elem_flags[e] & AXG_EL_3D)
+                    throw std::runtime_error("FluidElement::computeGroundMotion || receivers in azimuth-dependent fluid elements are not supported by this build.");
+                idx_f.push_back(i);
+            } else {
+                idx_s.push_back(i);
+            }
             cost[i] = (float)std::cos((double)d->rec_theta[i]);
             sint[i] = (float)std::sin((double)d->rec_theta[i]);
         }
+        c->nrec_solid = (int32_t)idx_s.size();
+        c->nrec_fluid = (int32_t)idx_f.size();
+        if (!idx_s.empty()) c->d_rec_idx_s.upload(idx_s, s);
+        if (!idx_f.empty()) c->d_rec_idx_f.upload(idx_f, s);
         c->d_rec_elem.upload(d->rec_elem, c->nrec, s);
         c->d_rec_w.upload(d->rec_weights, (size_t)c->nrec * 25, s);
         c->d_rec_phi.upload(d->rec_phi, c->nrec, s);
@@ -823,6 +875,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
     }
     c->d_bad.alloc(1);
     c->d_tstep.alloc(1);
+    resetWisdom(c);
 
     // ---- algorithmic bytes / updates per step ----
     {
@@ -908,6 +961,11 @@ void enqueueStep(axg_ctx *c, int tstep, bool use_counter) {
     doStiff(c);
     doCouple(c);
     doRecord(c, tstep - 1, ctr);
+    // Domain::learnWisdom(tstep - 1) every interval steps (Domain.cpp:384-402)
+    if (c->learn_interval > 0) {
+        if (use_counter) doLearn(c, c->learn_cutoff, c->learn_interval, ctr);
+        else if ((tstep - 1) % c->learn_interval == 0) doLearn(c, c->learn_cutoff, c->learn_interval, nullptr);
+    }
 }
 
 // steps per captured graph: divides the stability interval so checks stay on time
@@ -987,6 +1045,8 @@ void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
         if (profile) HIPCHECK(hipEventRecord(ev[k + 2], c->stream));
         doCouple(c);
         doRecord(c, tstep - 1);
+        if (c->learn_interval > 0 && (tstep - 1) % c->learn_interval == 0)
+            doLearn(c, c->learn_cutoff, c->learn_interval, nullptr);
         if (profile) HIPCHECK(hipEventRecord(ev[k + 3], c->stream));
         if (stab > 0 && tstep % stab == 0) checkStable(c, tstep);
     }
@@ -1143,16 +1203,56 @@ int axg_check_stability(axg_ctx *c, int64_t *bad_point) {
     }
 }
 
+// LearnParameters clamps the cutoff to [1e-5, 0.1] (NuWisdom.cpp:136-146)
+static float clampCutoff(float cutoff) { return std::min(0.1f, std::max(1e-5f, cutoff)); }
+
 int axg_learn_wisdom(axg_ctx *c, float cutoff) {
     CTX_CHECK(c);
-    (void)cutoff;
-    return fail(AXG_EUNSUPPORTED, "Domain::learnWisdom || not supported by this build yet.");
+    API_GUARD({
+        doLearn(c, clampCutoff(cutoff), 1, nullptr);
+        HIPCHECK(hipGetLastError());
+    })
 }
 
+int axg_set_learning(axg_ctx *c, float cutoff, int32_t interval) {
+    CTX_CHECK(c);
+    if (interval < 0) return fail(AXG_EINVAL, "LearnParameters || negative interval.");
+    API_GUARD({
+        // LearnParameters: interval <= 0 in the parameter file means 5; here 0 switches learning off
+        c->learn_cutoff = clampCutoff(cutoff);
+        c->learn_interval = interval;
+        if (c->graph_exec) {
+            (void)hipGraphExecDestroy(c->graph_exec);
+            (void)hipGraphDestroy(c->graph);
+            c->graph_exec = nullptr;
+            c->graph = nullptr;
+        }
+        resetWisdom(c);
+    })
+}
+
+// Point::getNuWisdom of every point (max over the components; a solid-fluid
+// point takes the larger of its sides, SolidFluidPoint.cpp:129-131)
 int axg_get_wisdom(axg_ctx *c, int32_t *nu) {
     CTX_CHECK(c);
-    (void)nu;
-    return fail(AXG_EUNSUPPORTED, "Domain::dumpWisdom || not supported by this build yet.");
+    if (!nu) return fail(AXG_EINVAL, "Domain::dumpWisdom || null output.");
+    API_GUARD({
+        std::vector<int32_t> ns((size_t)c->S.npt * 3), nf((size_t)c->F.npt);
+        if (!ns.empty())
+            HIPCHECK(hipMemcpyAsync(ns.data(), c->d_wnu_s.p, ns.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        if (!nf.empty())
+            HIPCHECK(hipMemcpyAsync(nf.data(), c->d_wnu_f.p, nf.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        for (int p = 0; p < c->npoint; p++) nu[p] = -1;
+        for (int q = 0; q < c->S.npt; q++) {
+            int p = c->S.point_of[q];
+            nu[p] = std::max({ns[(size_t)q * 3], ns[(size_t)q * 3 + 1], ns[(size_t)q * 3 + 2]});
+        }
+        for (int q = 0; q < c->F.npt; q++) {
+            int p = c->F.point_of[q];
+            nu[p] = std::max(nu[p], nf[q]);
+        }
+    })
 }
 
 int axg_run(axg_ctx *c, int32_t first, int32_t count, int32_t stab_interval, int32_t flags) {

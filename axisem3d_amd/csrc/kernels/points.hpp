@@ -170,9 +170,37 @@ __global__ __launch_bounds__(256) void stability_kernel(int32_t nrow, const int3
     if (!ok) atomicMin(bad, compact_to_point[row_point[r]]);
 }
 
+// station output: RTZ rotation by the colatitude (PointwiseRecorder.cpp:75-93)
+__device__ __forceinline__ void store_station(float us, float up, float uz, int rtz, float cost, float sint, float *o) {
+    if (rtz) {
+        o[0] = us * cost - uz * sint;  // ut
+        o[1] = up;
+        o[2] = us * sint + uz * cost;  // ur
+    } else {
+        o[0] = us;
+        o[1] = up;
+        o[2] = uz;
+    }
+}
+
+// sum over a 256-thread block of three values per thread; thread 0 gets the sums
+__device__ __forceinline__ void block_sum3(float acc[3], float (*red)[256]) {
+    for (int c = 0; c < 3; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    for (int c = 0; c < 3; c++) acc[c] = red[c][0];
+}
+
 // SolidElement::computeGroundMotion (SolidElement.cpp:189-216) + RTZ rotation
-// (PointwiseRecorder.cpp:75-93): one block per receiver.
-__global__ __launch_bounds__(256) void record_solid_kernel(int32_t nrec, const int32_t *__restrict__ rec_elem,
+// (PointwiseRecorder.cpp:75-93): one block per solid receiver; rec_idx maps the
+// block to the receiver's slot in the station output.
+__global__ __launch_bounds__(256) void record_solid_kernel(int32_t nrec, int32_t nrec_total,
+                                                           const int32_t *__restrict__ rec_idx,
+                                                           const int32_t *__restrict__ rec_elem,
                                                            const float *__restrict__ rec_w,
                                                            const float *__restrict__ rec_phi,
                                                            const float *__restrict__ rec_cost,
@@ -184,10 +212,10 @@ __global__ __launch_bounds__(256) void record_solid_kernel(int32_t nrec, const i
                                                            const int *__restrict__ tstep_ptr,
                                                            float *__restrict__ seis) {
     __shared__ float red[3][256];
-    int irec = blockIdx.x;
-    if (irec >= nrec) return;
+    if ((int)blockIdx.x >= nrec) return;
+    const int irec = rec_idx[blockIdx.x];
     if (tstep_ptr) tstep = *tstep_ptr;
-    float *out = seis + (int64_t)tstep * nrec * 3;
+    float *out = seis + ((int64_t)tstep * nrec_total + irec) * 3;
     int64_t e = rec_elem[irec];
     int nr = elem_nr[e];
     int nu = nr / 2;
@@ -211,27 +239,157 @@ __global__ __launch_bounds__(256) void record_solid_kernel(int32_t nrec, const i
             for (int c = 0; c < 3; c++) acc[c] += wt * (2.f * (cs * u[c].x - sn * u[c].y));
         }
     }
-    for (int c = 0; c < 3; c++) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        float us = red[0][0], up = red[1][0], uz = red[2][0];
-        float *o = out + (int64_t)irec * 3;
-        if (rtz) {
-            float cost = rec_cost[irec], sint = rec_sint[irec];
-            float ur = us * sint + uz * cost;
-            float ut = us * cost - uz * sint;
-            o[0] = ut;
-            o[1] = up;
-            o[2] = ur;
+    block_sum3(acc, red);
+    if (threadIdx.x == 0) store_station(acc[0], acc[1], acc[2], rtz, rec_cost[irec], rec_sint[irec], out);
+}
+
+// FluidElement::computeGroundMotion (FluidElement.cpp:163-215) for 1-D fluid
+// elements: displacement = K grad(chi) summed with weights w / iFact (so that
+// it is grad(chi) / rho), K = the Acoustic1D modulus (Acoustic1D.cpp:8-16) and
+// grad as Gradient::computeGrad (Gradient.cpp:26-55).  One block per fluid
+// receiver; work items are (mode, GLL point).
+__global__ __launch_bounds__(256) void record_fluid_kernel(int32_t nrec, int32_t nrec_total,
+                                                           const int32_t *__restrict__ rec_idx,
+                                                           const int32_t *__restrict__ rec_elem,
+                                                           const float *__restrict__ rec_w,
+                                                           const float *__restrict__ rec_phi,
+                                                           const float *__restrict__ rec_cost,
+                                                           const float *__restrict__ rec_sint,
+                                                           const int32_t *__restrict__ ep_row,
+                                                           const int32_t *__restrict__ ep_amax,
+                                                           const int32_t *__restrict__ elem_nr,
+                                                           const int32_t *__restrict__ elem_flags,
+                                                           const float *__restrict__ geom,
+                                                           const float *__restrict__ mat_pool,
+                                                           const int64_t *__restrict__ mat_off,
+                                                           const float2 *__restrict__ chi, int rtz, int tstep,
+                                                           const int *__restrict__ tstep_ptr,
+                                                           float *__restrict__ seis) {
+    __shared__ float red[3][256];
+    if ((int)blockIdx.x >= nrec) return;
+    const int irec = rec_idx[blockIdx.x];
+    if (tstep_ptr) tstep = *tstep_ptr;
+    float *out = seis + ((int64_t)tstep * nrec_total + irec) * 3;
+    const int64_t e = rec_elem[irec];
+    const int nr = elem_nr[e];
+    const int amaxe = nr / 2 - ((nr % 2 == 0) ? 1 : 0);
+    const bool axial = (elem_flags[e] & 2) != 0;
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    const float *K = mat_pool + mat_off[e];
+    const float phi = rec_phi[irec];
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int t = threadIdx.x; t < 25 * (amaxe + 1); t += blockDim.x) {
+        const int k = t % 25, al = t / 25;
+        const float wt = rec_w[irec * 25 + k];
+        if (wt == 0.f) continue;
+        const int i = k / 5, j = k - 5 * i;
+        auto X = [&](int p) {
+            return al <= ep_amax[e * 25 + p] ? chi[(int64_t)ep_row[e * 25 + p] + al] : czero();
+        };
+        float2 GU = czero(), UG = czero();
+        for (int m = 0; m < 5; m++) {
+            float2 xc = X(m * 5 + j), xr = X(i * 5 + m);
+            GU = cadd(GU, cscale(Gx[m * 5 + i], xc));
+            UG = cadd(UG, cscale(Ge[m * 5 + j], xr));
+        }
+        PointGeom g = load_geom(geom, e, k);
+        float2 gr[3];
+        gr[0] = cadd(cscale(g.dzdeta, GU), cscale(g.dzdxii, UG));
+        gr[1] = cscale(g.inv_s, cimul((float)al, X(k)));
+        gr[2] = cadd(cscale(g.dsdeta, GU), cscale(g.dsdxii, UG));
+        if (axial && i == 0) gr[1] = cadd(gr[1], cscale(g.dzdeta, cimul((float)al, GU)));
+        const float kk = K[k] * wt;
+        if (al == 0) {
+            acc[0] += kk * gr[0].x;
+            acc[2] += kk * gr[2].x;
         } else {
-            o[0] = us;
-            o[1] = up;
-            o[2] = uz;
+            float sn, cs;
+            sincosf((float)al * phi, &sn, &cs);
+            for (int c = 0; c < 3; c++) acc[c] += kk * 2.f * (cs * gr[c].x - sn * gr[c].y);
+        }
+    }
+    block_sum3(acc, red);
+    if (threadIdx.x == 0) store_station(acc[0], acc[1], acc[2], rtz, rec_cost[irec], rec_sint[irec], out);
+}
+
+}  // namespace axg
+
+namespace axg {
+
+// SolidPoint::learnWisdom / FluidPoint::learnWisdom (SolidPoint.cpp:240-266,
+// FluidPoint.cpp:209-229): per point and displacement component, when the
+// "Hilbert" energy h2 = |u|^2 - |u_0|^2 / 2 exceeds its running maximum, the
+// learned order becomes the smallest nu' < Nu with |u|^2 - sum_{a<=nu'} |u_a|^2
+// <= h2 cutoff^2 (else Nu).  One wave per point, points strided over a fixed
+// grid; lane l holds the rows alpha = l + 64 s.  Energies are summed in FP64:
+// the tail |u|^2 - prefix is a difference of nearly equal sums whose FP32
+// rounding is of the order of the tolerance itself.
+// With `tstep_ptr`, the step index comes from the device counter and the
+// kernel returns at once unless it is a learning step.
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_incl_scan_d(double v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        double u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+template <int NCOMP>
+__global__ __launch_bounds__(256) void learn_wisdom_kernel(PhasePoints P, const float2 *__restrict__ displ,
+                                                           double *__restrict__ wmax, int32_t *__restrict__ wnu,
+                                                           float cutoff, int interval, const int *__restrict__ tstep_ptr) {
+    if (tstep_ptr && (*tstep_ptr) % interval != 0) return;
+    const int lane = threadIdx.x & 63;
+    const int nwave = gridDim.x * (blockDim.x >> 6);
+    const double cut2 = (double)cutoff * (double)cutoff;
+    for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < P.npt; q += nwave) {
+        const int64_t row0 = P.pt_row[q];
+        const int nu = P.pt_nu[q];
+        for (int c = 0; c < NCOMP; c++) {
+            double tot = 0., e0 = 0.;
+            for (int a0 = 0; a0 <= nu; a0 += 64) {
+                const int al = a0 + lane;
+                double e = 0.;
+                if (al <= nu) {
+                    float2 u = displ[(row0 + al) * NCOMP + c];
+                    e = (double)u.x * u.x + (double)u.y * u.y;
+                }
+                if (a0 == 0) e0 = __shfl(e, 0, 64);
+                tot += wave_sum_d(e);
+            }
+            const double h2 = tot - 0.5 * e0;
+            const int64_t slot = (int64_t)q * NCOMP + c;
+            if (h2 <= wmax[slot]) continue;  // uniform across the wave
+            const double tol = h2 * cut2;
+            int found = nu;
+            double carry = 0.;
+            for (int a0 = 0; a0 < nu; a0 += 64) {
+                const int al = a0 + lane;
+                double e = 0.;
+                if (al <= nu) {
+                    float2 u = displ[(row0 + al) * NCOMP + c];
+                    e = (double)u.x * u.x + (double)u.y * u.y;
+                }
+                const double pre = carry + wave_incl_scan_d(e, lane);
+                carry = __shfl(pre, 63, 64);
+                const unsigned long long hit = __ballot(al < nu && tot - pre <= tol);
+                if (hit) {
+                    found = a0 + __ffsll((long long)hit) - 1;
+                    break;
+                }
+            }
+            if (lane == 0) {
+                wmax[slot] = h2;
+                wnu[slot] = found;
+            }
         }
     }
 }

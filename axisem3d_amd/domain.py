@@ -77,7 +77,19 @@ class Domain:
         _check(rc)
 
     def learnWisdom(self, cutoff):
+        """SolidPoint/FluidPoint::learnWisdom on every point, now."""
         _check(self._lib.axg_learn_wisdom(self._h, cutoff))
+
+    def setLearning(self, cutoff, interval):
+        """LearnParameters: learn inside run() every `interval` steps (0: off);
+        restarts the learned state."""
+        _check(self._lib.axg_set_learning(self._h, cutoff, interval))
+
+    def wisdom(self):
+        """Point::getNuWisdom of every point (-1 for none)."""
+        out = np.zeros(self.mesh.d.npoint, dtype=np.int32)
+        _check(self._lib.axg_get_wisdom(self._h, out.ctypes.data_as(_abi.i32p)))
+        return out
 
     # ---- batched loop and data exchange ----
     def run(self, first, count, stab_interval=0, use_graph=False):

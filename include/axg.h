@@ -230,10 +230,18 @@ int axg_record(axg_ctx *ctx, int32_t tstep, double t);
 /* Domain::checkStability (Domain.cpp:237-275); bad_point = -1 when stable,
  * otherwise the smallest unstable point index and AXG_EUNSTABLE is returned */
 int axg_check_stability(axg_ctx *ctx, int64_t *bad_point);
-/* Domain::learnWisdom (Domain.cpp:384-402 -> SolidPoint.cpp:240-266) */
+/* Domain::learnWisdom (Domain.cpp:384-402 -> SolidPoint.cpp:240-266,
+ * FluidPoint.cpp:209-229) once, now, with the cutoff clamped to [1e-5, 0.1];
+ * energies are summed in double precision */
 int axg_learn_wisdom(axg_ctx *ctx, float cutoff);
-/* Point::getNuWisdom for every point (Domain::dumpWisdom, Domain.cpp:404-440) */
+/* Point::getNuWisdom for every point (Domain::dumpWisdom, Domain.cpp:404-440):
+ * nu_wisdom[npoint]; the (s, z) of each point are the descriptor's point_sz */
 int axg_get_wisdom(axg_ctx *ctx, int32_t *nu_wisdom);
+/* LearnParameters (NuWisdom.cpp:134-149): learn inside axg_run every
+ * `interval` steps (Domain::learnWisdom(tstep - 1) when (tstep - 1) % interval
+ * == 0) with the energy cutoff clamped to [1e-5, 0.1]; interval 0 stops
+ * learning.  Restarts the learned state (orders Nu, maxima -1). */
+int axg_set_learning(axg_ctx *ctx, float cutoff, int32_t interval);
 
 /* The loop body of Newmark::solve (Newmark.cpp:47-93) for steps
  * tstep = first .. first+count-1 (1-based), recorded every step; stability is

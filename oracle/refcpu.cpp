@@ -162,12 +162,16 @@ struct Ref {
         bool axial;
         Real invMass;
         std::vector<C> displ, veloc, accel, stiff;  // [(nu+1)*3], alpha-major
+        double wmax[3] = {-1., -1., -1.};          // SolidPoint.h:78-79
+        int wnu[3] = {0, 0, 0};
     };
     struct FluidPt {
         int nr, nu, gidx;
         bool axial, surface;
         Real invMass;
         std::vector<C> displ, veloc, accel, stiff;
+        double wmax = -1.;  // FluidPoint.h:77-78
+        int wnu = 0;
     };
     struct Elem {
         bool fluid, axial, is3D, tiso;
@@ -263,6 +267,7 @@ struct Ref {
                 s.displ.assign((s.nu + 1) * 3, C(0, 0));
                 s.veloc = s.accel = s.stiff = s.displ;
                 sIndex[p] = (int)S.size();
+                for (int k = 0; k < 3; k++) s.wnu[k] = s.nu;  // SolidPoint.cpp:16
                 S.push_back(s);
             }
             if (fl & AXG_PT_FLUID) {
@@ -278,6 +283,7 @@ struct Ref {
                 f.displ.assign(f.nu + 1, C(0, 0));
                 f.veloc = f.accel = f.stiff = f.displ;
                 fIndex[p] = (int)F.size();
+                f.wnu = f.nu;  // FluidPoint.cpp:16
                 F.push_back(f);
             }
         }
@@ -884,8 +890,9 @@ struct Ref {
         }
     }
 
-    // FluidElement::computeStiff / displToStiff (element/FluidElement.cpp:43-65, 333-355)
-    void fluidStiff(Elem &el, Work &w) {
+    // FluidElement::displToStiff up to the stress (element/FluidElement.cpp:333-355):
+    // gather, Gradient::computeGrad, Acoustic1D/3D::strainToStress -> w.fstress
+    void fluidGradStress(Elem &el, Work &w) {
         int Nr = el.nr, Nu = el.nu, nyq = (Nr % 2 == 0) ? 1 : 0;
         ensureWork(w, Nu, Nr);
         for (int k = 0; k < 25; k++) {
@@ -949,6 +956,15 @@ struct Ref {
                     for (int a = 0; a <= Nu; a++) w.fstress[a][c][k] = w.colC[a];
                 }
         }
+    }
+
+    // FluidElement::computeStiff / displToStiff (element/FluidElement.cpp:43-65, 333-355)
+    void fluidStiff(Elem &el, Work &w) {
+        int Nu = el.nu, nyq = (el.nr % 2 == 0) ? 1 : 0;
+        fluidGradStress(el, w);
+        const RMat &Gx = el.axial ? G_GLJ : G_GLL;
+        const RMat &GTe = GT_GLL;
+        C ii(zero, one);
         // Gradient::computeQuad (Gradient.cpp:58-82)
         for (int b = 0; b <= Nu - nyq; b++) {
             const auto &s = w.fstress[b];
@@ -1026,9 +1042,23 @@ struct Ref {
         C ii(zero, one);
         for (size_t ir = 0; ir < recs.size(); ir++) {
             const Rec &rc = recs[ir];
-            const Elem &el = E[rc.elem];
+            Elem &el = E[rc.elem];
             Real u[3] = {0, 0, 0};
-            for (int k = 0; k < 25; k++) {
+            if (el.fluid) {
+                // FluidElement::computeGroundMotion (FluidElement.cpp:163-215)
+                Work &w = works[0];
+                fluidGradStress(el, w);
+                for (int k = 0; k < 25; k++) {
+                    Real up[3];
+                    for (int c = 0; c < 3; c++) up[c] = w.fstress[0][c][k].real();
+                    for (int a = 1; a <= el.maxNu - (int)(el.nr % 2 == 0); a++) {
+                        C expval = two * std::exp((Real)a * rc.phi * ii);
+                        for (int c = 0; c < 3; c++) up[c] += (expval * w.fstress[a][c][k]).real();
+                    }
+                    for (int c = 0; c < 3; c++) u[c] += rc.w[k] * up[c];
+                }
+            }
+            for (int k = 0; k < 25 && !el.fluid; k++) {
                 if (std::abs(rc.w[k]) < 1e-10) continue;
                 const SolidPt &p = S[el.pts[k]];
                 int pn = (p.nr % 2 == 0) ? 1 : 0;
@@ -1054,6 +1084,41 @@ struct Ref {
                 o[2] = u[2];
             }
         }
+    }
+
+    // SolidPoint::learnWisdom / FluidPoint::learnWisdom (SolidPoint.cpp:240-266,
+    // FluidPoint.cpp:209-229), energies summed in double precision
+    template <class Fn>
+    static void learnOne(int nu, Fn energy, double cut2, double &wmax, int &wnu) {
+        double L2 = 0.;
+        for (int a = 0; a <= nu; a++) L2 += energy(a);
+        double h2 = L2 - 0.5 * energy(0);
+        if (h2 <= wmax) return;
+        wmax = h2;
+        double tol = h2 * cut2, pre = 0.;
+        for (int a = 0; a < nu; a++) {
+            pre += energy(a);
+            if (L2 - pre <= tol) {
+                wnu = a;
+                return;
+            }
+        }
+        wnu = nu;
+    }
+    void learnWisdom(float cutoff) {
+        double c = std::min(0.1, std::max(1e-5, (double)cutoff));  // NuWisdom.cpp:136-146
+        double cut2 = c * c;
+        auto en = [](const C &z) { return (double)z.real() * z.real() + (double)z.imag() * z.imag(); };
+        for (auto &p : S)
+            for (int k = 0; k < 3; k++)
+                learnOne(p.nu, [&](int a) { return en(p.displ[a * 3 + k]); }, cut2, p.wmax[k], p.wnu[k]);
+        for (auto &p : F) learnOne(p.nu, [&](int a) { return en(p.displ[a]); }, cut2, p.wmax, p.wnu);
+    }
+    // Point::getNuWisdom (SolidPoint.cpp:268-272, SolidFluidPoint.cpp:129-131)
+    void getWisdom(int32_t *out, int npoint) {
+        for (int i = 0; i < npoint; i++) out[i] = -1;
+        for (auto &p : S) out[p.gidx] = std::max({p.wnu[0], p.wnu[1], p.wnu[2]});
+        for (auto &p : F) out[p.gidx] = std::max(out[p.gidx], p.wnu);
     }
 
     int64_t checkStability() {
@@ -1245,6 +1310,8 @@ int refcpu_set_field(refcpu *h, int32_t which, const double *in, int64_t n) {
         });
     }))
 }
+int refcpu_learn_wisdom(refcpu *h, float cutoff) { GUARD(DISPATCH(h, R.learnWisdom(cutoff))) }
+int refcpu_get_wisdom(refcpu *h, int32_t *out, int npoint) { GUARD(DISPATCH(h, R.getWisdom(out, npoint))) }
 int refcpu_last_run_cost(refcpu *h, double *c5) { GUARD(DISPATCH(h, for (int i = 0; i < 5; i++) c5[i] = R.cost[i])) }
 
 int refcpu_fft_c2r(int nr, int use_double, const double *in_c, double *out_r) {

@@ -39,6 +39,8 @@ def lib():
             "refcpu_fft_c2r": ([C.c_int, C.c_int, f64p, f64p], C.c_int),
             "refcpu_fft_r2c": ([C.c_int, C.c_int, f64p, f64p], C.c_int),
             "refcpu_spectral": ([C.c_int, f64p, f64p, f64p], C.c_int),
+            "refcpu_learn_wisdom": ([vp, C.c_float], C.c_int),
+            "refcpu_get_wisdom": ([vp, C.POINTER(C.c_int32), C.c_int], C.c_int),
             "refcpu_last_error": ([], C.c_char_p),
         }
         for name, (a, r) in sig.items():
@@ -108,6 +110,14 @@ class RefDomain:
         c = (C.c_double * 5)()
         _check(lib().refcpu_last_run_cost(self._h, c))
         return dict(zip(("ELEMENT", "POINT", "ASSEMBLE", "WAIT", "OTHER"), list(c)))
+
+    def learnWisdom(self, cutoff):
+        _check(lib().refcpu_learn_wisdom(self._h, cutoff))
+
+    def wisdom(self):
+        out = np.zeros(self.mesh.d.npoint, dtype=np.int32)
+        _check(lib().refcpu_get_wisdom(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size))
+        return out
 
     def seismograms(self):
         n = self.nstep * self.nrec * 3

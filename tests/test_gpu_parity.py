@@ -273,3 +273,79 @@ def test_cpp_facade_newmark_solve_matches_python_driver(tmp_path):
     py = g.seismograms().ravel()
     assert cpp.size == py.size and np.abs(py).max() > 0
     assert rel_l2(cpp, py) < 1e-6
+
+
+def test_fluid_receivers_seismograms(fluid_layers):
+    """Stations inside the fluid outer core (FluidElement::computeGroundMotion)."""
+    print("case fluid receivers", flush=True)
+    cfg = ax.shell_config(0, layers=fluid_layers, nelem_theta=12, nstep=300, nrec=6, rec_depth=6.371e6 - 2.5e6,
+                          nu_mode=0, nu_const=4)
+    m = ax.ShellMesh(cfg)
+    assert (m.array("elem_flags")[m.array("rec_elem")] & _abi.EL_FLUID).all()
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, m.d.nstep, 100, use_graph=True)
+    r = RefDomain(m, double=False)
+    r.run(1, m.d.nstep, 100)
+    sg, sr = g.seismograms(), r.seismograms()
+    assert np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4
+
+
+def _decaying_field(m, which, seed):
+    """Random point field whose modes decay like 0.6^alpha (a spectrum the
+    wisdom cutoff cuts somewhere inside), masked like a displacement."""
+    x = random_field(m, which, seed=seed, scale=1.0)
+    fl, nr = m.array("point_flags"), m.array("point_nr")
+    solid = which < 4
+    nc = 3 if solid else 1
+    pos = 0
+    for p in range(m.d.npoint):
+        if not fl[p] & (_abi.PT_SOLID if solid else _abi.PT_FLUID):
+            continue
+        n = (nr[p] // 2 + 1) * nc * 2
+        blk = x[pos:pos + n].reshape(-1, nc * 2)
+        blk *= (0.6 ** np.arange(blk.shape[0]))[:, None]
+        pos += n
+    return x
+
+
+def test_wisdom_matches_oracle(fluid_layers):
+    """Domain::learnWisdom on identical fields: the learned Nu of every point
+    equals the oracle's, through a sequence of updates of the running maxima."""
+    print("case wisdom", flush=True)
+    m = _mesh_small(fluid_layers, material=_abi.MAT_TRANSVERSELY_ISO, nu_mode=1, nu_min=6, nu_max=80,
+                    dv_amp=0.05, nstep=10)
+    g = ax.Domain(m)
+    r = RefDomain(m, double=False)
+    assert (g.wisdom() == r.wisdom()).all()
+    for it, (scale, cutoff) in enumerate([(1.0, 1e-3), (0.5, 1e-3), (3.0, 1e-2), (10.0, 1e-4)]):
+        us = _decaying_field(m, _abi.FIELD_SOLID_DISPL, 40 + it) * scale
+        uf = _decaying_field(m, _abi.FIELD_FLUID_DISPL, 50 + it) * scale
+        for d in (g, r):
+            d.set_field(_abi.FIELD_SOLID_DISPL, us)
+            d.set_field(_abi.FIELD_FLUID_DISPL, uf)
+            d.learnWisdom(cutoff)
+        wg, wr = g.wisdom(), r.wisdom()
+        assert (wg >= 0).all()
+        assert (wg == wr).all(), (it, np.flatnonzero(wg != wr)[:10])
+
+
+def test_wisdom_learned_in_graph_run_matches_eager(fluid_layers):
+    """axg_set_learning inside graph-replayed steps == learnWisdom called by hand
+    every interval steps of an eager run (Newmark.cpp:88: learnWisdom(tstep-1))."""
+    print("case wisdom graph", flush=True)
+    m = _mesh_small(fluid_layers, nstep=80)
+    g1 = ax.Domain(m)
+    g1.resetZero()
+    g1.setLearning(1e-3, 5)
+    g1.run(1, 80, 0, use_graph=True)
+    g2 = ax.Domain(m)
+    g2.resetZero()
+    for t in range(1, 81):
+        g2.run(t, 1, 0)
+        if (t - 1) % 5 == 0:
+            g2.learnWisdom(1e-3)
+    w1, w2 = g1.wisdom(), g2.wisdom()
+    assert (w1 == w2).all()
+    assert (w1 < m.array("point_nr") // 2).any()

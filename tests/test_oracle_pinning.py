@@ -371,3 +371,92 @@ def test_oracle_rejects_steps_outside_the_stf(refcpu):
             dom.run(first, count, 0)
     with pytest.raises(RuntimeError, match="Domain::applySource"):
         dom.applySource(mesh.d.nstep)
+
+
+# ---------------------------------------------------------------------------
+# receivers in the fluid
+# ---------------------------------------------------------------------------
+
+def test_fluid_receiver_reads_grad_chi_over_rho(refcpu):
+    """chi = z at alpha = 0 has grad chi = e_z, so a station in the fluid
+    records (s, phi, z) = (0, 0, 1/rho) (FluidElement.cpp:163-215: weights /
+    iFact times K = iFact / rho)."""
+    layers = _small_layers(fluid=True)
+    mesh = ShellMesh(shell_config(0, layers=layers, nelem_theta=8, nstep=4, nrec=6, rec_depth=6.371e6 - 4.0e6,
+                                  rec_components=1, nu_mode=0, nu_const=3, nthread=1))
+    fl = mesh.array("point_flags")
+    nr = mesh.array("point_nr")
+    sz = mesh.array("point_sz").reshape(-1, 2)
+    chi = []
+    for p in range(mesh.d.npoint):
+        if fl[p] & abi.PT_FLUID:
+            rows = np.zeros((nr[p] // 2 + 1, 2))
+            rows[0, 0] = sz[p, 1]
+            chi.append(rows.ravel())
+    chi = np.concatenate(chi)
+    dom = refcpu.RefDomain(mesh, double=True)
+    dom.set_field(abi.FIELD_FLUID_DISPL, chi)
+    dom.record(0)
+    u = dom.seismograms()[0]
+    L = layers[0]
+    r = 4.0e6
+    rho = L["rho_bottom"] + (r - L["r_bottom"]) / (L["r_top"] - L["r_bottom"]) * (L["rho_top"] - L["rho_bottom"])
+    assert np.abs(u[:, 0]).max() < 1e-5 / rho and np.abs(u[:, 1]).max() < 1e-5 / rho
+    assert np.abs(u[:, 2] * rho - 1.).max() < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# Nu wisdom (SolidPoint::learnWisdom, SolidPoint.cpp:240-266)
+# ---------------------------------------------------------------------------
+
+def _wisdom_mesh():
+    return small_mesh(material=1, nu=(6, 24), dv_amp=0.05, fluid=True)
+
+
+def _spectral_field(mesh, which, fn):
+    """Field with row alpha of every point set to fn(point, alpha, nu) (all components)."""
+    fl, nr = mesh.array("point_flags"), mesh.array("point_nr")
+    solid = which < 4
+    nc = 3 if solid else 1
+    out = []
+    for p in range(mesh.d.npoint):
+        if not fl[p] & (abi.PT_SOLID if solid else abi.PT_FLUID):
+            continue
+        nu = nr[p] // 2
+        for a in range(nu + 1):
+            v = fn(p, a, nu)
+            for c in range(nc):
+                out += [v, 0.]
+    return np.array(out)
+
+
+def test_wisdom_semantics(refcpu):
+    mesh = _wisdom_mesh()
+    nr = mesh.array("point_nr")
+    nu = nr // 2
+    dom = refcpu.RefDomain(mesh, double=True)
+    # before any learning every point reports its own Nu
+    assert (dom.wisdom() == nu).all()
+
+    def learn(fn, cutoff=1e-3):
+        dom.set_field(abi.FIELD_SOLID_DISPL, _spectral_field(mesh, 0, fn))
+        dom.set_field(abi.FIELD_FLUID_DISPL, _spectral_field(mesh, 4, fn))
+        dom.learnWisdom(cutoff)
+        return dom.wisdom()
+
+    # energy only in the top mode: nothing smaller represents it
+    w = learn(lambda p, a, n: 1. if a == n else 0.)
+    assert (w == nu).all()
+    # the same shape at a smaller energy does not replace the maximum
+    w = learn(lambda p, a, n: 1. if a == 0 else 0.)
+    assert (w == nu).all()
+    # a larger energy in modes 0..3 (geometric decay) gives Nu = 3 where Nu > 3
+    w = learn(lambda p, a, n: 10. * 0.5 ** a if a <= 3 else 0.)
+    assert (w == np.minimum(nu, 3)).all()
+    # cutoff is clamped to 0.1: a tail of 1e-3 of the energy is dropped
+    dom2 = refcpu.RefDomain(mesh, double=True)
+    dom2.set_field(abi.FIELD_SOLID_DISPL, _spectral_field(mesh, 0, lambda p, a, n: 1. if a == 0 else (0.03 if a == 1 else 0.)))
+    dom2.learnWisdom(5.0)
+    ws = dom2.wisdom()
+    solid_only = (mesh.array("point_flags") & abi.PT_FLUID) == 0
+    assert (ws[solid_only] == 0).all()
