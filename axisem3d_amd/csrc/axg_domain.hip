@@ -88,9 +88,12 @@ struct Bucket3D {
 };
 
 // dynamic LDS of a 3-D launch: forward and inverse twiddles + roots, the xi
-// operands [5][3][maxM], and each wave's three sequences [3][maxNr]
+// operands [5][ncomp][maxM], and each wave's sequences [maxNr][nseq]
+// (solid: 3 components, 3 sequences; fluid: 1 component, 2 sequences)
 inline size_t lds3D(const Bucket3D &b) {
-    return (2 * (size_t)b.maxNr + kRootTable + 15 * (size_t)b.maxM + 3 * W3D_WAVES * (size_t)b.maxNr) * sizeof(float2);
+    const size_t nc = b.phase ? 1 : 3, ns = b.phase ? 2 : 3;
+    return (2 * (size_t)b.maxNr + kRootTable + 5 * nc * (size_t)b.maxM + ns * W3D_WAVES * (size_t)b.maxNr) *
+           sizeof(float2);
 }
 
 struct Phase {
@@ -241,11 +244,11 @@ uint32_t hostMagic(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
 
 void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att,
                      const std::vector<int32_t> &elems) {
-    if (phase != 0)
-        throw std::runtime_error("Acoustic3D::strainToStress || azimuth-dependent fluid elements are not supported by this build yet.");
+    if (phase != 0 && tiso)
+        throw std::runtime_error("CrdTransTIsoFluid || transversely isotropic rotation of fluid elements is not supported by this build.");
     if (att != AXG_ATT_NONE)
         throw std::runtime_error("Attenuation3D::applyToStress || 3-D attenuation is not supported by this build yet.");
-    const int P = (mat == AXG_MAT_ISOTROPIC) ? 2 : (mat == AXG_MAT_TRANSVERSELY_ISO) ? 5 : 21;
+    const int P = (phase != 0) ? 1 : (mat == AXG_MAT_ISOTROPIC) ? 2 : (mat == AXG_MAT_TRANSVERSELY_ISO) ? 5 : 21;
     if (c->h_mat3d_off.empty()) c->h_mat3d_off.assign(d->nelem, 0);
     if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
     // classes by modes per lane S (M = Nr/2 + 1 <= 64 S): register and LDS use
@@ -358,6 +361,17 @@ void launchSolid3DS(const Elem3DArgs &a, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
 }
 
+template <int S>
+void launchFluid3DS(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+    static bool attrSet = false;
+    if (!attrSet) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fluid3d_kernel<S>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attrSet = true;
+    }
+    hipLaunchKernelGGL((fluid3d_kernel<S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+}
+
 template <int MAT, bool TISO>
 void launchSolid3D(const Elem3DArgs &a, int slots, size_t lds, hipStream_t s) {
     if (slots == 1) launchSolid3DS<MAT, TISO, 1>(a, lds, s);
@@ -367,7 +381,7 @@ void launchSolid3D(const Elem3DArgs &a, int slots, size_t lds, hipStream_t s) {
 
 void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     if (b.nelem == 0) return;
-    FamScope fs(c, AXG_KF_SOLID3D);
+    FamScope fs(c, b.phase ? AXG_KF_FLUID : AXG_KF_SOLID3D);
     Elem3DArgs a;
     std::memset(&a, 0, sizeof(a));
     a.elems = b.elems.p;
@@ -390,6 +404,14 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     a.nstage = c->d_nstage.p;
     size_t lds = lds3D(b);
     hipStream_t s = c->stream;
+    if (b.phase) {
+        a.displ = c->F.displ.p;
+        a.stiff = c->F.stiff.p;
+        if (b.slots == 1) launchFluid3DS<1>(a, lds, s);
+        else if (b.slots == 2) launchFluid3DS<2>(a, lds, s);
+        else launchFluid3DS<3>(a, lds, s);
+        return;
+    }
     if (b.material == AXG_MAT_ISOTROPIC) {
         if (b.tiso) launchSolid3D<0, true>(a, b.slots, lds, s);
         else launchSolid3D<0, false>(a, b.slots, lds, s);

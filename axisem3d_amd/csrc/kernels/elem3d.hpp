@@ -81,10 +81,10 @@ __device__ __forceinline__ void wave_lds_fence() {
 }
 
 // ---------------------------------------------------------------- FFT
-// butterflies per lane of one radix-R stage on 3 sequences of length <= NMAX
-template <int R, int NMAX>
+// butterflies per lane of one radix-R stage on NSEQ sequences of length <= NMAX
+template <int R, int NMAX, int NSEQ>
 struct WaveB {
-    static constexpr int v = (3 * (NMAX / R) + 63) / 64;
+    static constexpr int v = (NSEQ * (NMAX / R) + 63) / 64;
 };
 
 // In-register DFT of a power-of-two length R <= 16, y_k = sum_r x_r W^{rk}
@@ -137,17 +137,17 @@ __device__ __forceinline__ void dft_pow2(float2 (&x)[R], int sign) {
     }
 }
 
-// One Stockham stage on 3 interleaved sequences ([N][3] at `seq`: element n
-// of sequence s at 3 n + s): butterfly j of sequence s reads
+// One Stockham stage on NSEQ interleaved sequences ([N][NSEQ] at `seq`:
+// element n of sequence s at NSEQ n + s): butterfly j of sequence s reads
 // x[j + r nb] * W_{Ns R}^{(j % Ns) r}, does a radix-R DFT and writes
-// y[(j / Ns) Ns R + (j % Ns) + r Ns].  Lane task t = 3 j + s, so the reads of
-// a stage are contiguous across lanes.  sign = -1 forward, +1 inverse.  `tw`
-// holds W_N^{sign t} for t < N; `root` the forward odd-radix root tables.
-template <int R, int NMAX>
+// y[(j / Ns) Ns R + (j % Ns) + r Ns].  Lane task t = NSEQ j + s, so the reads
+// of a stage are contiguous across lanes.  sign = -1 forward, +1 inverse.
+// `tw` holds W_N^{sign t} for t < N; `root` the forward odd-radix root tables.
+template <int R, int NMAX, int NSEQ>
 __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, uint32_t mNs, int wmul,
                                            int sign, const float2 *tw, const float2 *root, int lane) {
-    constexpr int B = WaveB<R, NMAX>::v;
-    const int total = 3 * nb;
+    constexpr int B = WaveB<R, NMAX, NSEQ>::v;
+    const int total = NSEQ * nb;
     float2 v[B][R];
     int dst[B];
 #pragma unroll
@@ -155,14 +155,14 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
         int t = lane + q * 64;
         dst[q] = -1;
         if (t < total) {
-            const int j = t / 3, s = t - 3 * j;
+            const int j = t / NSEQ, s = t - NSEQ * j;
             const int jq = udiv(j, Ns, mNs), jm = j - jq * Ns;
             const float2 *x = seq + t;
             const int wstep = jm * wmul;
             const float2 *tp = tw;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                float2 xv = x[3 * r * nb];
+                float2 xv = x[NSEQ * r * nb];
                 if (r == 0) {
                     v[q][r] = xv;
                 } else {
@@ -170,7 +170,7 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
                     v[q][r] = cmul(xv, *tp);
                 }
             }
-            dst[q] = 3 * (jq * Ns * R + jm) + s;
+            dst[q] = NSEQ * (jq * Ns * R + jm) + s;
         }
     }
     wave_lds_fence();
@@ -181,7 +181,7 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
         if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
             dft_pow2<R>(v[q], sign);
 #pragma unroll
-            for (int r = 0; r < R; r++) o[3 * r * Ns] = v[q][r];
+            for (int r = 0; r < R; r++) o[NSEQ * r * Ns] = v[q][r];
         } else {
             // odd R, outputs in pairs (k, R-k): with a_r = v_r + v_{R-r},
             // b_r = v_r - v_{R-r} (r = 1..H) and theta = 2 pi r k / R,
@@ -216,8 +216,8 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
                 }
                 // i sign Bs
                 const float2 iB = (sign > 0) ? make_float2(-Bs.y, Bs.x) : make_float2(Bs.y, -Bs.x);
-                o[3 * k * Ns] = cadd(A, iB);
-                o[3 * (R - k) * Ns] = csub(A, iB);
+                o[NSEQ * k * Ns] = cadd(A, iB);
+                o[NSEQ * (R - k) * Ns] = csub(A, iB);
             }
         }
     }
@@ -225,7 +225,7 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
 }
 
 // the whole transform of length N <= NMAX from its stage descriptors
-template <int NMAX>
+template <int NMAX, int NSEQ = 3>
 __device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t *plan, int nst, const float2 *tw,
                                      const float2 *root, int lane) {
 #pragma unroll 1
@@ -234,15 +234,15 @@ __device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t
         const int R = d[FS_R], Ns = d[FS_NS], nb = d[FS_NB], wmul = d[FS_WMUL];
         const uint32_t mNs = (uint32_t)d[FS_MNS];
         switch (R) {
-            case 16: wfft_stage<16, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 8: wfft_stage<8, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 4: wfft_stage<4, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 2: wfft_stage<2, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 3: wfft_stage<3, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 5: wfft_stage<5, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 7: wfft_stage<7, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 11: wfft_stage<11, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            default: wfft_stage<13, NMAX>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 16: wfft_stage<16, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 8: wfft_stage<8, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 4: wfft_stage<4, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 2: wfft_stage<2, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 3: wfft_stage<3, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 5: wfft_stage<5, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 7: wfft_stage<7, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 11: wfft_stage<11, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            default: wfft_stage<13, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
         }
     }
 }
@@ -534,6 +534,171 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                     if (jp == k) f = cadd(f, Ok[s][c]);
                     sp0[al * 3 + c] = csub(sp0[al * 3 + c], f);
                 }
+            }
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------- fluid
+// FluidElement::displToStiff with mElem3D (FluidElement.cpp:333-355): the same
+// organisation as the solid kernel with the scalar potential chi: gradient
+// (Gradient::computeGrad, Gradient.cpp:26-56), C2R of its 3 columns packed in
+// 2 sequences, K(phi) grad chi (Acoustic3D::strainToStress, Acoustic3D.cpp:9-16),
+// R2C x 1/Nr, Gradient::computeQuad (Gradient.cpp:58-82).
+template <int S>
+__global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
+    extern __shared__ float2 lds[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t e = a.elems[blockIdx.x];
+    const int Nr = a.elem_nr[e];
+    const int Nu = Nr / 2;
+    const int M = Nu + 1;
+    const bool nyq = (Nr % 2 == 0);
+    const bool axial = (a.elem_flags[e] & 2) != 0;
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
+    const int nst = a.nstage[Nr];
+
+    float2 *twf = lds;
+    float2 *twi = twf + a.maxNr;
+    float2 *root = twi + a.maxNr;
+    float2 *xb = root + kRootTable;                      // [5][maxM] xi operands
+    float2 *seq = xb + 5 * a.maxM + w * 2 * a.maxNr;     // this wave's [Nr][2]
+    for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
+        float2 v = a.twiddle[a.tw_off[Nr] + t];
+        twf[t] = v;
+        twi[t] = make_float2(v.x, -v.y);
+    }
+    for (int t = threadIdx.x; t < kRootTable; t += W3D_THREADS) root[t] = a.twiddle[t];
+    __syncthreads();
+
+    const int32_t *epr = a.ep_row + e * 25;
+    const int32_t *epa = a.ep_amax + e * 25;
+    const float *mbase = a.mat3d + a.mat3d_off[e];
+    const float h = 0.5f / (float)Nr;
+
+#pragma unroll 1
+    for (int k = 0; k < 5; k++) {
+        const int lane = opaque((int)(threadIdx.x & 63));
+        const int ip = w * 5 + k;
+        PointGeom g = load_geom(a.geom, e, ip);
+
+        // ---------------- gradient spectrum -> 2 Hermitian-extended sequences
+#pragma unroll 1
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            if (al >= M) continue;
+            float2 g3[3] = {czero(), czero(), czero()};
+            if (!(nyq && al == Nu)) {
+                float2 GU = czero(), UG = czero(), u = czero();
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    const int pc = m * 5 + k, pr = w * 5 + m;
+                    const float gx = Gx[m * 5 + w], ge = Ge[m * 5 + k];
+                    const float2 *uc = a.displ + (int64_t)epr[pc];
+                    const float2 *ur = a.displ + (int64_t)epr[pr];
+                    float2 vc = al <= epa[pc] ? uc[al] : czero();
+                    float2 vr = al <= epa[pr] ? ur[al] : czero();
+                    GU.x += gx * vc.x;
+                    GU.y += gx * vc.y;
+                    UG.x += vr.x * ge;
+                    UG.y += vr.y * ge;
+                    if (m == k) u = vr;
+                }
+                const float fa = (float)al;
+                g3[0] = cadd(cscale(g.dzdeta, GU), cscale(g.dzdxii, UG));
+                g3[1] = cscale(g.inv_s, cimul(fa, u));
+                g3[2] = cadd(cscale(g.dsdeta, GU), cscale(g.dsdxii, UG));
+                if (axial && w == 0) g3[1] = cadd(g3[1], cscale(g.dzdeta, cimul(fa, GU)));
+            }
+            // Z_0 = X_0 + i X_1, Z_1 = X_2 (+ i 0)
+            float2 *z0 = seq, *z1 = seq + 1;
+            if (al == 0 || 2 * al == Nr) {
+                z0[2 * al] = make_float2(g3[0].x, g3[1].x);
+                z1[2 * al] = make_float2(g3[2].x, 0.f);
+            } else {
+                z0[2 * al] = make_float2(g3[0].x - g3[1].y, g3[0].y + g3[1].x);
+                z0[2 * (Nr - al)] = make_float2(g3[0].x + g3[1].y, -g3[0].y + g3[1].x);
+                z1[2 * al] = g3[2];
+                z1[2 * (Nr - al)] = make_float2(g3[2].x, -g3[2].y);
+            }
+        }
+        wave_lds_fence();
+#pragma unroll 1
+        for (int pass = 0; pass < 2; pass++) {
+            wfft<128 * S - 2, 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
+            if (pass == 1) break;
+            // Acoustic3D: stress = K(phi) grad chi on every sample
+            const float *K = mbase + (int64_t)ip * Nr;
+            float kv[2 * S];
+#pragma unroll
+            for (int t = 0; t < 2 * S; t++) {
+                const int n = lane + 64 * t;
+                kv[t] = n < Nr ? K[n] : 0.f;
+            }
+#pragma unroll
+            for (int t = 0; t < 2 * S; t++) {
+                const int n = lane + 64 * t;
+                if (n < Nr) {
+                    float2 a0 = seq[2 * n], a1 = seq[2 * n + 1];
+                    seq[2 * n] = make_float2(kv[t] * a0.x, kv[t] * a0.y);
+                    seq[2 * n + 1] = make_float2(kv[t] * a1.x, 0.f);
+                }
+            }
+            wave_lds_fence();
+        }
+
+        // ---------------- unpack x 1/Nr, quadrature operands
+        float2 Yk[S], Ok[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            Yk[s] = Ok[s] = czero();
+            if (al >= M) continue;
+            float2 X = czero();
+            if (!(nyq && al == Nu)) {
+                float2 za = seq[2 * al], zb = seq[2 * (al == 0 ? 0 : Nr - al)];
+                float2 ya = seq[2 * al + 1], yb = seq[2 * (al == 0 ? 0 : Nr - al) + 1];
+                float2 s0 = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
+                float2 s1 = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
+                float2 s2 = make_float2(h * (ya.x + yb.x), h * (ya.y - yb.y));
+                float2 gg = cimul(-(float)al, s1);
+                X = cadd(cscale(g.dzdeta, s0), cscale(g.dsdeta, s2));
+                Yk[s] = cadd(cscale(g.dzdxii, s0), cscale(g.dsdxii, s2));
+                if (axial && w == 0) X = cadd(X, cscale(g.dzdeta, gg));
+                Ok[s] = cscale(g.inv_s, gg);
+            }
+            xb[w * a.maxM + al] = X;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            if (al >= M) continue;
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const float gx = Gx[w * 5 + i];
+                float2 x = xb[i * a.maxM + al];
+                Ok[s].x += gx * x.x;
+                Ok[s].y += gx * x.y;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jp = 0; jp < 5; jp++) {
+            const int pp = w * 5 + jp;
+            const int amax = epa[pp];
+            const float ge = Ge[jp * 5 + k];
+            float2 *sp0 = a.stiff + (int64_t)epr[pp];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const int al = lane + 64 * s;
+                if (al >= M || al > amax || (nyq && al == Nu)) continue;
+                float2 f = cscale(ge, Yk[s]);
+                if (jp == k) f = cadd(f, Ok[s]);
+                sp0[al] = csub(sp0[al], f);
             }
         }
     }

@@ -46,10 +46,11 @@ def random_field(mesh, which, seed, scale=1e-6):
 
 
 class Lifted3D:
-    """The same mesh with every solid element's 1-D material block repeated on
-    its Nr azimuthal samples and flagged 3-D (a laterally constant 3-D model)."""
+    """The same mesh with every solid (and with fluid=True every fluid)
+    element's 1-D material block repeated on its Nr azimuthal samples and
+    flagged 3-D (a laterally constant 3-D model)."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, fluid=False):
         import ctypes as C
         import axisem3d_amd._abi as abi
         self.mesh = mesh
@@ -61,7 +62,7 @@ class Lifted3D:
         blocks, new_off = [], [0]
         for e in range(ne):
             blk = pool[off[e]:off[e + 1]]
-            if not flags[e] & abi.EL_FLUID:
+            if fluid or not flags[e] & abi.EL_FLUID:
                 npar = blk.size // 25
                 blk = np.repeat(blk.reshape(npar, 1, 25), enr[e], axis=1).ravel()
                 flags[e] |= abi.EL_3D

@@ -349,3 +349,47 @@ def test_wisdom_learned_in_graph_run_matches_eager(fluid_layers):
     w1, w2 = g1.wisdom(), g2.wisdom()
     assert (w1 == w2).all()
     assert (w1 < m.array("point_nr") // 2).any()
+
+
+def test_compute_stiff_3d_fluid_operator(fluid_layers):
+    """Acoustic3D elements (fluid with a 3-D vp): fluid stiffness against the
+    FP64 oracle, and the 3-D fluid kernel on a laterally constant model against
+    the 1-D fluid kernel."""
+    from tests.helpers import Lifted3D
+    print("case fluid3d", flush=True)
+    m = _mesh_small(fluid_layers, nu_mode=1, nu_min=6, nu_max=60, dv_amp=0.05, dv_fluid=1, nstep=10)
+    fl = m.array("elem_flags")
+    assert ((fl & _abi.EL_FLUID) & ((fl & _abi.EL_3D) >> 2)).any()
+    g = ax.Domain(m)
+    r = RefDomain(m, double=True)
+    chi = random_field(m, _abi.FIELD_FLUID_DISPL, seed=7)
+    for d in (g, r):
+        d.set_field(_abi.FIELD_FLUID_DISPL, chi)
+        d.computeStiff()
+    fg, fr = g.get_field(_abi.FIELD_FLUID_STIFF), r.get_field(_abi.FIELD_FLUID_STIFF)
+    assert np.abs(fr).max() > 0
+    assert rel_l2(fg, fr) < 1e-5
+    m1 = _mesh_small(fluid_layers, nu_mode=1, nu_min=6, nu_max=60, nstep=10)
+    m3 = Lifted3D(m1, fluid=True)
+    chi = random_field(m1, _abi.FIELD_FLUID_DISPL, seed=8)
+    out = []
+    for mm in (m1, m3):
+        d = ax.Domain(mm)
+        d.set_field(_abi.FIELD_FLUID_DISPL, chi)
+        d.computeStiff()
+        out.append(d.get_field(_abi.FIELD_FLUID_STIFF))
+        d.close()
+    assert rel_l2(out[1], out[0]) < 1e-5
+
+
+def test_seismograms_3d_fluid(fluid_layers):
+    print("case fluid3d seismograms", flush=True)
+    m = _mesh_small(fluid_layers, nu_mode=1, nu_min=6, nu_max=40, dv_amp=0.05, dv_fluid=1, nstep=300)
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, m.d.nstep, 100, use_graph=True)
+    r = RefDomain(m, double=False)
+    r.run(1, m.d.nstep, 100)
+    sg, sr = g.seismograms(), r.seismograms()
+    assert np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4
