@@ -246,8 +246,8 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
                      const std::vector<int32_t> &elems) {
     if (phase != 0 && tiso)
         throw std::runtime_error("CrdTransTIsoFluid || transversely isotropic rotation of fluid elements is not supported by this build.");
-    if (att != AXG_ATT_NONE)
-        throw std::runtime_error("Attenuation3D::applyToStress || 3-D attenuation is not supported by this build yet.");
+    if (phase != 0 && att != AXG_ATT_NONE)
+        throw std::runtime_error("Attenuation3D::applyToStress || attenuation of fluid elements is not defined.");
     const int P = (phase != 0) ? 1 : (mat == AXG_MAT_ISOTROPIC) ? 2 : (mat == AXG_MAT_TRANSVERSELY_ISO) ? 5 : 21;
     if (c->h_mat3d_off.empty()) c->h_mat3d_off.assign(d->nelem, 0);
     if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
@@ -350,15 +350,22 @@ void finishBuckets3D(axg_ctx *c) {
     }
 }
 
-template <int MAT, bool TISO, int S>
-void launchSolid3DS(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+template <int MAT, bool TISO, int S, int ATT>
+void launchSolid3DSA(const Elem3DArgs &a, size_t lds, hipStream_t s) {
     static bool attrSet = false;
     if (!attrSet) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&solid3d_kernel<MAT, TISO, S>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&solid3d_kernel<MAT, TISO, S, ATT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attrSet = true;
     }
-    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S, ATT>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+}
+
+template <int MAT, bool TISO, int S>
+void launchSolid3DS(const Elem3DArgs &a, int att, size_t lds, hipStream_t s) {
+    if (att == AXG_ATT_NONE) launchSolid3DSA<MAT, TISO, S, 0>(a, lds, s);
+    else if (att == AXG_ATT_FULL) launchSolid3DSA<MAT, TISO, S, 1>(a, lds, s);
+    else launchSolid3DSA<MAT, TISO, S, 2>(a, lds, s);
 }
 
 template <int S>
@@ -373,10 +380,10 @@ void launchFluid3DS(const Elem3DArgs &a, size_t lds, hipStream_t s) {
 }
 
 template <int MAT, bool TISO>
-void launchSolid3D(const Elem3DArgs &a, int slots, size_t lds, hipStream_t s) {
-    if (slots == 1) launchSolid3DS<MAT, TISO, 1>(a, lds, s);
-    else if (slots == 2) launchSolid3DS<MAT, TISO, 2>(a, lds, s);
-    else launchSolid3DS<MAT, TISO, 3>(a, lds, s);
+void launchSolid3D(const Elem3DArgs &a, int slots, int att, size_t lds, hipStream_t s) {
+    if (slots == 1) launchSolid3DS<MAT, TISO, 1>(a, att, lds, s);
+    else if (slots == 2) launchSolid3DS<MAT, TISO, 2>(a, att, lds, s);
+    else launchSolid3DS<MAT, TISO, 3>(a, att, lds, s);
 }
 
 void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
@@ -412,13 +419,26 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
         else launchFluid3DS<3>(a, lds, s);
         return;
     }
+    a.att_pool = c->d_att_pool.p;
+    a.att_off = c->d_att_off.p;
+    a.att_memvar = c->d_att_memvar.p;
+    a.att_stressR = c->d_att_stressR.p;
+    a.att_mem_off = c->d_att_mem_off.p;
+    a.mem_stride = c->att_mem_stride;
+    a.nsls = c->nsls;
+    for (int i = 0; i < 8; i++) {
+        a.sls_alpha[i] = c->sls_a[i];
+        a.sls_beta[i] = c->sls_b[i];
+        a.sls_gamma[i] = c->sls_g[i];
+    }
+    a.do_kappa = c->do_kappa;
     if (b.material == AXG_MAT_ISOTROPIC) {
-        if (b.tiso) launchSolid3D<0, true>(a, b.slots, lds, s);
-        else launchSolid3D<0, false>(a, b.slots, lds, s);
+        if (b.tiso) launchSolid3D<0, true>(a, b.slots, b.att, lds, s);
+        else launchSolid3D<0, false>(a, b.slots, b.att, lds, s);
     } else if (b.material == AXG_MAT_TRANSVERSELY_ISO) {
-        launchSolid3D<1, true>(a, b.slots, lds, s);
+        launchSolid3D<1, true>(a, b.slots, b.att, lds, s);
     } else {
-        launchSolid3D<2, true>(a, b.slots, lds, s);
+        launchSolid3D<2, true>(a, b.slots, b.att, lds, s);
     }
 }
 

@@ -56,7 +56,62 @@ struct Elem3DArgs {
     const int32_t *tw_off;       // [N] offset of the table of N
     const int32_t *plan;         // [N][E3D_MAX_STAGES][8] stage descriptors (FftStage)
     const int32_t *nstage;       // [N] number of stages
+    // attenuation (Attenuation3D_Full / _CG4): per element, dkappa [n][np] then
+    // dmu [n][np] in att_pool at att_off[e]; memory variables of SLS s at
+    // att_memvar + s * mem_stride + att_mem_off[e] and the previous relaxed
+    // stress at att_stressR + att_mem_off[e], both as floats [i][6][n]
+    const float *att_pool;
+    const int64_t *att_off;
+    float2 *att_memvar;
+    float2 *att_stressR;
+    const int64_t *att_mem_off;
+    int64_t mem_stride;
+    int32_t nsls;
+    float sls_alpha[8], sls_beta[8], sls_gamma[8];
+    int32_t do_kappa;
 };
+
+// Attenuation3D_{Full,CG4}::applyToStress + updateMemoryVariables on one
+// sample n of one attenuating GLL point i (Attenuation3D_Full.cpp,
+// Attenuation3D_CG4.cpp; AttAxiSEM coefficients): the stress loses the sum of
+// the memory variables, which then advance with the previous and the new
+// relaxed stress sigma_R = dkappa tr(e) + 2 dmu dev(e) (deviatoric only
+// without kappa attenuation).
+__device__ __forceinline__ void attenuate3d(const Elem3DArgs &a, int64_t e, int np, int i, int Nr, int n,
+                                            const float ev[6], float sv[6]) {
+    const float *par = a.att_pool + a.att_off[e];
+    const float dkappa = par[(int64_t)n * np + i];
+    const float dmu = par[(int64_t)(Nr + n) * np + i];
+    float *sr = reinterpret_cast<float *>(a.att_stressR + a.att_mem_off[e]) + (int64_t)i * 6 * Nr + n;
+    float old[6], neu[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) old[c] = sr[c * Nr];
+    const float eii3 = (ev[0] + ev[1] + ev[2]) * (1.f / 3.f);
+    if (a.do_kappa) {
+        const float sii = 3.f * dkappa * eii3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) neu[c] = sii + 2.f * dmu * (ev[c] - eii3);
+    } else {
+        neu[0] = 2.f * dmu * (ev[0] - eii3);
+        neu[1] = 2.f * dmu * (ev[1] - eii3);
+        neu[2] = -(neu[0] + neu[1]);
+    }
+#pragma unroll
+    for (int c = 3; c < 6; c++) neu[c] = dmu * ev[c];
+    for (int sl = 0; sl < a.nsls; sl++) {
+        float *mv = reinterpret_cast<float *>(a.att_memvar + sl * a.mem_stride + a.att_mem_off[e]) +
+                    (int64_t)i * 6 * Nr + n;
+        const float al = a.sls_alpha[sl], be = a.sls_beta[sl], ga = a.sls_gamma[sl];
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            float m = mv[c * Nr];
+            sv[c] -= m;
+            mv[c * Nr] = al * m + be * old[c] + ga * neu[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) sr[c * Nr] = neu[c];
+}
 
 // One Stockham stage of a length-N transform, precomputed on the host:
 // radix R, span Ns (product of the radices before it), nb = N / R butterflies
@@ -292,7 +347,7 @@ __device__ __forceinline__ void stress3d(ParFn P, const float e[6], float s[6]) 
 }
 
 // ---------------------------------------------------------------- kernel
-template <int MAT, bool TISO, int S>
+template <int MAT, bool TISO, int S, int ATT>
 __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     constexpr int NPAR = MatParams<MAT>::v;
@@ -416,6 +471,9 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
             if (pass == 1) break;
             // ---------------- stress at every sample n of the point
             const float *mp = mbase + (int64_t)ip * NPAR * Nr;
+            // attenuating point: all 25 (Full) or (1,1),(1,3),(3,1),(3,3) (CG4)
+            const int np = (ATT == 2) ? 4 : 25;
+            const int iatt = (ATT == 1) ? ip : (ATT == 2) ? (ip == 6 ? 0 : ip == 8 ? 1 : ip == 16 ? 2 : ip == 18 ? 3 : -1) : -1;
             auto sample = [&](int n, auto P) {
                 float ev[6], sv[6];
 #pragma unroll
@@ -425,6 +483,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                     ev[2 * b + 1] = z.y;
                 }
                 stress3d<MAT>(P, ev, sv);
+                if (ATT != 0 && iatt >= 0) attenuate3d(a, e, np, iatt, Nr, n, ev, sv);
 #pragma unroll
                 for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };

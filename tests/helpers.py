@@ -71,10 +71,25 @@ class Lifted3D:
         self._flags = flags.astype(np.int32)
         self._off = np.array(new_off, dtype=np.int64)
         self._pool = np.concatenate(blocks).astype(np.float32)
+        # attenuation blocks: dkappa [1][np] then dmu [1][np] -> [nr][np] each
+        aoff = mesh.array("att_off")
+        apool = mesh.array("att_pool")
+        ablocks, anew = [], [0]
+        for e in range(ne):
+            blk = apool[aoff[e]:aoff[e + 1]]
+            if blk.size and (flags[e] & abi.EL_3D):
+                h = blk.size // 2
+                blk = np.concatenate([np.tile(blk[:h], enr[e]), np.tile(blk[h:], enr[e])])
+            ablocks.append(blk)
+            anew.append(anew[-1] + blk.size)
+        self._aoff = np.array(anew, dtype=np.int64)
+        self._apool = (np.concatenate(ablocks) if anew[-1] else np.zeros(1)).astype(np.float32)
         d = abi.MeshDesc.from_buffer_copy(mesh.d)
         d.elem_flags = self._flags.ctypes.data_as(abi.i32p)
         d.mat_off = self._off.ctypes.data_as(abi.i64p)
         d.mat_pool = self._pool.ctypes.data_as(abi.f32p)
+        d.att_off = self._aoff.ctypes.data_as(abi.i64p)
+        d.att_pool = self._apool.ctypes.data_as(abi.f32p)
         self._d = d
         self.desc = C.pointer(d)
 

@@ -460,3 +460,21 @@ def test_wisdom_semantics(refcpu):
     ws = dom2.wisdom()
     solid_only = (mesh.array("point_flags") & abi.PT_FLUID) == 0
     assert (ws[solid_only] == 0).all()
+
+
+@pytest.mark.parametrize("att", [abi.ATT_FULL, abi.ATT_CG4])
+def test_3d_attenuation_equals_1d_for_constant_material(refcpu, att):
+    """Attenuation3D (memory variables on the azimuthal samples) equals
+    Attenuation1D (memory variables per Fourier mode) for a laterally constant
+    model: both are the same linear recursion, before or after the FFT."""
+    m1 = small_mesh(material=1, nu=(6, 20), dv_amp=0., attenuation=att, nsls=3)
+    m3 = Lifted3D(m1)
+    u = [random_field(m1, abi.FIELD_SOLID_DISPL, 60 + i, scale=1.) for i in range(4)]
+    outs = []
+    for m in (m1, m3):
+        dom = refcpu.RefDomain(m, double=True)
+        seq = []
+        for x in u:   # memory variables evolve over the calls
+            seq.append(stiff_of(dom, m, x))
+        outs.append(np.concatenate(seq))
+    assert rel_l2(outs[1], outs[0]) < 1e-12
