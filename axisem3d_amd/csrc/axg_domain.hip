@@ -23,6 +23,7 @@
 #include "kernels/elem1d.hpp"
 #include "kernels/elem3d.hpp"
 #include "kernels/points.hpp"
+#include "kernels/pointfft.hpp"
 #include "preloop/spectral.hpp"
 
 namespace axg {
@@ -106,6 +107,13 @@ struct Phase {
     DevBuf<int32_t> d_row_point, d_pt_row, d_pt_nu, d_pt_flags, d_point_of;
     DevBuf<float> d_invmass;
     DevBuf<float2> displ, veloc, accel, stiff;
+    // Mass3D points: compact index, Nr, first slice of 1/mass in the pool
+    std::vector<int32_t> m3_q, m3_nr;
+    std::vector<int64_t> m3_off;
+    std::vector<float> m3_inv;
+    DevBuf<int32_t> d_m3_q, d_m3_nr;
+    DevBuf<int64_t> d_m3_off;
+    DevBuf<float> d_m3_inv;
     PhasePoints view() const {
         PhasePoints P;
         P.row_point = d_row_point.p;
@@ -151,6 +159,11 @@ struct axg_ctx {
     size_t max_lds = 0;
     // solid-fluid coupling
     int32_t nsf = 0, nsf_row = 0;
+    // SFCoupling3D points: solid row, fluid row, Nr, first normal row
+    int32_t nsf3 = 0;
+    DevBuf<int32_t> d_sf3_srow, d_sf3_frow, d_sf3_nr;
+    DevBuf<int64_t> d_sf3_off;
+    DevBuf<float> d_sf3_normal;
     DevBuf<int32_t> d_row_sf, d_sf_srow, d_sf_frow, d_sf_row0;
     DevBuf<float> d_sf_normal;
     // source
@@ -296,7 +309,9 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
 }
 
 void finishBuckets3D(axg_ctx *c) {
-    if (c->buckets3d.empty()) return;
+    bool any = false;
+    for (char u : c->nr_used) any = any || u;
+    if (!any) return;
     hipStream_t s = c->stream;
     c->d_mat3d.upload(c->h_mat3d, s);
     c->d_mat3d_off.upload(c->h_mat3d_off, s);
@@ -466,6 +481,10 @@ void buildPhase(const axg_mesh_desc *d, Phase &P, bool solid) {
     P.pt_nu.clear();
     P.pt_flags.clear();
     P.pt_invmass.clear();
+    P.m3_q.clear();
+    P.m3_nr.clear();
+    P.m3_off.clear();
+    P.m3_inv.clear();
     int32_t row = 0;
     const int64_t *moff = solid ? d->mass_solid_off : d->mass_fluid_off;
     const float *minv = solid ? d->inv_mass_solid : d->inv_mass_fluid;
@@ -473,8 +492,11 @@ void buildPhase(const axg_mesh_desc *d, Phase &P, bool solid) {
         if (!(d->point_flags[p] & flagBit)) continue;
         int nr = d->point_nr[p];
         int nu = nr / 2;
-        if (moff[p + 1] - moff[p] != 1)
-            throw std::runtime_error("Mass3D::computeAccel || azimuth-dependent mass is not supported by this build.");
+        const int64_t nm = moff[p + 1] - moff[p];
+        if (nm != 1 && nm != nr) throw std::runtime_error("Mass3D::checkCompatibility || Incompatible size.");
+        if (nm == nr && nr > E3D_MAX_NR)
+            throw std::runtime_error("Mass3D::computeAccel || Nr = " + std::to_string(nr) +
+                                     " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
         P.compact[p] = (int32_t)P.point_of.size();
         P.point_of.push_back(p);
         P.pt_row.push_back(row);
@@ -484,7 +506,16 @@ void buildPhase(const axg_mesh_desc *d, Phase &P, bool solid) {
         if (nr % 2 == 0) f |= 2;
         if (!solid && (d->point_flags[p] & AXG_PT_FLUID_SURFACE)) f |= 4;
         P.pt_flags.push_back(f);
-        P.pt_invmass.push_back(minv[moff[p]]);
+        if (nm == 1) {
+            P.pt_invmass.push_back(minv[moff[p]]);
+        } else {
+            // Mass3D: the accel is computed before the Newmark kernel, which then scales by 1
+            P.pt_invmass.push_back(1.f);
+            P.m3_q.push_back(P.compact[p]);
+            P.m3_nr.push_back(nr);
+            P.m3_off.push_back((int64_t)P.m3_inv.size());
+            P.m3_inv.insert(P.m3_inv.end(), minv + moff[p], minv + moff[p + 1]);
+        }
         if ((int64_t)row + nu + 1 > INT32_MAX / 3) throw std::runtime_error("axg_create || mesh too large.");
         row += nu + 1;
     }
@@ -502,6 +533,12 @@ void uploadPhase(Phase &P, hipStream_t s) {
     P.d_pt_flags.upload(P.pt_flags, s);
     P.d_invmass.upload(P.pt_invmass, s);
     P.d_point_of.upload(P.point_of, s);
+    if (!P.m3_q.empty()) {
+        P.d_m3_q.upload(P.m3_q, s);
+        P.d_m3_nr.upload(P.m3_nr, s);
+        P.d_m3_off.upload(P.m3_off, s);
+        P.d_m3_inv.upload(P.m3_inv, s);
+    }
     size_t n = (size_t)P.nrow * P.ncomp;
     P.displ.alloc(n);
     P.veloc.alloc(n);
@@ -602,8 +639,43 @@ void doLearn(axg_ctx *c, float cutoff, int interval, const int *tstep_ptr) {
                            c->F.view(), c->F.displ.p, c->d_wmax_f.p, c->d_wnu_f.p, cutoff, interval, tstep_ptr);
 }
 
+PointFFTArgs pointFFTArgs(axg_ctx *c) {
+    PointFFTArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.twiddle = c->d_twiddle.p;
+    a.tw_off = c->d_tw_off.p;
+    a.plan = c->d_plan.p;
+    a.nstage = c->d_nstage.p;
+    return a;
+}
+
+constexpr size_t kPointFFTLds = (size_t)PF_WAVES * PF_WAVE_LDS * sizeof(float2);
+
+// Mass3D::computeAccel of the points whose mass varies in azimuth, before the
+// Newmark kernels (which then scale those points by 1)
+void doMass3D(axg_ctx *c) {
+    for (int ph = 0; ph < 2; ph++) {
+        Phase &P = ph ? c->F : c->S;
+        if (P.m3_q.empty()) continue;
+        PointFFTArgs a = pointFFTArgs(c);
+        a.n = (int32_t)P.m3_q.size();
+        a.items = P.d_m3_q.p;
+        a.pt_row = P.d_pt_row.p;
+        a.pt_nu = P.d_pt_nu.p;
+        a.pt_flags = P.d_pt_flags.p;
+        a.pt_nr = P.d_m3_nr.p;
+        a.coef_off = P.d_m3_off.p;
+        a.coef = P.d_m3_inv.p;
+        a.field = P.stiff.p;
+        dim3 grid(std::min(1024, (a.n + PF_WAVES - 1) / PF_WAVES));
+        if (ph) hipLaunchKernelGGL(mass3d_kernel<1>, grid, dim3(256), kPointFFTLds, c->stream, a);
+        else hipLaunchKernelGGL(mass3d_kernel<3>, grid, dim3(256), kPointFFTLds, c->stream, a);
+    }
+}
+
 void doNewmark(axg_ctx *c, double dt, int *tstep_counter = nullptr) {
     FamScope fs(c, AXG_KF_NEWMARK);
+    doMass3D(c);
     double half_dt = 0.5 * dt;
     double half_dt_dt = half_dt * dt;
     const int B = 256;
@@ -636,8 +708,23 @@ void doStiff(axg_ctx *c) {
 }
 
 void doCouple(axg_ctx *c) {
-    if (c->nsf_row == 0) return;
+    if (c->nsf_row == 0 && c->nsf3 == 0) return;
     FamScope fs(c, AXG_KF_OTHER);
+    if (c->nsf3) {
+        PointFFTArgs a = pointFFTArgs(c);
+        a.n = c->nsf3;
+        a.pt_row = c->d_sf3_srow.p;
+        a.frow = c->d_sf3_frow.p;
+        a.pt_nr = c->d_sf3_nr.p;
+        a.coef_off = c->d_sf3_off.p;
+        a.coef = c->d_sf3_normal.p;
+        a.field = c->S.stiff.p;
+        a.sdispl = c->S.displ.p;
+        a.fstiff = c->F.stiff.p;
+        dim3 grid(std::min(1024, (a.n + PF_WAVES - 1) / PF_WAVES));
+        hipLaunchKernelGGL(couple3d_kernel, grid, dim3(256), kPointFFTLds, c->stream, a);
+    }
+    if (c->nsf_row == 0) return;
     const int B = 256;
     hipLaunchKernelGGL(couple1d_kernel, dim3((c->nsf_row + B - 1) / B), dim3(B), 0, c->stream, c->nsf_row,
                        c->d_row_sf.p, c->d_sf_srow.p, c->d_sf_frow.p, c->d_sf_row0.p, c->d_sf_normal.p,
@@ -833,6 +920,25 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
             }
             if (!e3[key].empty()) createBuckets3D(c, d, key / NK, kk / 32, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
         }
+        // transforms of the points with Mass3D / SFCoupling3D use the same tables
+        if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
+        for (Phase *P : {&c->S, &c->F})
+            for (int nr : P->m3_nr) c->nr_used[nr] = 1;
+        for (int q = 0; q < d->nsf; q++) {
+            int64_t rows = d->sf_off[q + 1] - d->sf_off[q];
+            int nr = d->point_nr[d->sf_point[q]];
+            if (rows == 1) continue;
+            if (rows != nr) throw std::runtime_error("SFCoupling3D::checkCompatibility || Incompatible size.");
+            if (nr > E3D_MAX_NR || fftFactors(nr).empty())
+                throw std::runtime_error("SFCoupling3D::coupleSolidFluid || Nr = " + std::to_string(nr) +
+                                         " is not supported by this build.");
+            c->nr_used[nr] = 1;
+        }
+        for (Phase *P : {&c->S, &c->F})
+            for (int nr : P->m3_nr)
+                if (fftFactors(nr).empty())
+                    throw std::runtime_error("Mass3D::computeAccel || Nr = " + std::to_string(nr) +
+                                             " has no radix plan (not a lucky number).");
         finishBuckets3D(c);
     }
 
@@ -842,18 +948,29 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
         std::vector<int32_t> row_sf, srow, frow, row0;
         std::vector<float> nrm;
         int32_t r = 0;
+        std::vector<int32_t> s3row, f3row, s3nr;
+        std::vector<int64_t> s3off;
+        std::vector<float> s3nrm;
+        int q1 = 0;
         for (int q = 0; q < d->nsf; q++) {
             int p = d->sf_point[q];
-            if (d->sf_off[q + 1] - d->sf_off[q] != 1)
-                throw std::runtime_error("SFCoupling3D::coupleSolidFluid || azimuth-dependent coupling is not supported by this build.");
             int qs = c->S.compact[p], qf = c->F.compact[p];
             if (qs < 0 || qf < 0) throw std::runtime_error("SolidFluidPoint::SolidFluidPoint || Incompatible point.");
+            if (d->sf_off[q + 1] - d->sf_off[q] != 1) {
+                s3row.push_back(c->S.pt_row[qs]);
+                f3row.push_back(c->F.pt_row[qf]);
+                s3nr.push_back(d->point_nr[p]);
+                s3off.push_back((int64_t)s3nrm.size() / 6);
+                s3nrm.insert(s3nrm.end(), d->sf_normal + d->sf_off[q] * 6, d->sf_normal + d->sf_off[q + 1] * 6);
+                continue;
+            }
             srow.push_back(c->S.pt_row[qs]);
             frow.push_back(c->F.pt_row[qf]);
             row0.push_back(r);
             int nu = d->point_nr[p] / 2;
-            for (int a = 0; a <= nu; a++) row_sf.push_back(q);
+            for (int a = 0; a <= nu; a++) row_sf.push_back(q1);
             r += nu + 1;
+            q1++;
             for (int t = 0; t < 6; t++) nrm.push_back(d->sf_normal[d->sf_off[q] * 6 + t]);
         }
         c->nsf_row = r;
@@ -862,6 +979,14 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
         c->d_sf_frow.upload(frow, s);
         c->d_sf_row0.upload(row0, s);
         c->d_sf_normal.upload(nrm, s);
+        c->nsf3 = (int32_t)s3row.size();
+        if (c->nsf3) {
+            c->d_sf3_srow.upload(s3row, s);
+            c->d_sf3_frow.upload(f3row, s);
+            c->d_sf3_nr.upload(s3nr, s);
+            c->d_sf3_off.upload(s3off, s);
+            c->d_sf3_normal.upload(s3nrm, s);
+        }
         HIPCHECK(hipStreamSynchronize(s));
     }
 

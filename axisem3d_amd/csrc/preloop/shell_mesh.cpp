@@ -817,6 +817,9 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         double mass[nP * nP] = {0.};  // rho * iFact (solid) or iFact / (rho vp^2) (fluid)
         double dtElem = DBL_MAX;
         double rhoPt[nP * nP] = {0.};
+        // fluid mass on each point's own Nr samples when vp varies in azimuth
+        // (Material::computeElementalMass evaluates the point's slices)
+        std::vector<double> massv[nP * nP];
     };
     std::vector<ElemOut> out(NQ);
 
@@ -892,6 +895,20 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 double rho = v[k][4];
                 o.rhoPt[k] = rho;
                 o.mass[k] = q.fluid ? q.iFact[k] / (rho * v[k][0] * v[k][0]) : rho * q.iFact[k];
+            }
+            if (q.fluid && threeD && c.dv_fluid) {
+                const Layer &L = B.layers[q.layer];
+                for (int k = 0; k < nP * nP; k++) {
+                    const int nrp = pointNr[q.gll[k]];
+                    const double sk = q.crd[k][0], zk = q.crd[k][1], r = std::hypot(sk, zk);
+                    o.massv[k].resize(nrp);
+                    for (int n = 0; n < nrp; n++) {
+                        const double phi = 2. * pi * n / nrp;
+                        const double dvp = c.dv_amp * std::tanh(B.grfVp(sk * std::cos(phi), sk * std::sin(phi), zk) / 2.);
+                        const double vp = L.vp(r) * (1. + dvp), rho = L.rho(r);
+                        o.massv[k][n] = q.iFact[k] / (rho * vp * vp);
+                    }
+                }
             }
         }
     });
@@ -1106,6 +1123,7 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     // ---------------- points ----------------
     const int NPT = B.npoint;
     std::vector<double> massS(NPT, 0.), massF(NPT, 0.);
+    std::vector<std::vector<double>> massFv(NPT);  // azimuth-dependent fluid mass
     std::vector<int> pflags(NPT, 0);
     for (int e = 0; e < NQ; e++) {
         const QuadInfo &q = B.quads[e];
@@ -1113,6 +1131,11 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             int p = q.gll[k];
             if (q.fluid) {
                 massF[p] += out[e].mass[k];
+                const auto &mv = out[e].massv[k];
+                if (!mv.empty()) {
+                    if (massFv[p].empty()) massFv[p].assign(mv.size(), 0.);
+                    for (size_t n = 0; n < mv.size(); n++) massFv[p][n] += mv[n];
+                }
                 pflags[p] |= AXG_PT_FLUID;
             } else {
                 massS[p] += out[e].mass[k];
@@ -1131,8 +1154,25 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         // GLLPoint::release: Mass1D = 1 / mass (GLLPoint.cpp:74-79)
         m.mass_solid_off[p + 1] = m.mass_solid_off[p] + ((pflags[p] & AXG_PT_SOLID) ? 1 : 0);
         if (pflags[p] & AXG_PT_SOLID) m.inv_mass_solid.push_back((float)(1. / massS[p]));
-        m.mass_fluid_off[p + 1] = m.mass_fluid_off[p] + ((pflags[p] & AXG_PT_FLUID) ? 1 : 0);
-        if (pflags[p] & AXG_PT_FLUID) m.inv_mass_fluid.push_back((float)(1. / massF[p]));
+        // Mass1D when the assembled mass is the same on every slice
+        // (XMath::equalRows), otherwise Mass3D with the point's Nr values
+        // (GLLPoint.cpp:86-95)
+        bool fluid3 = false;
+        if ((pflags[p] & AXG_PT_FLUID) && !massFv[p].empty()) {
+            const auto &mv = massFv[p];
+            double mx = 0., dif = 0.;
+            for (double x : mv) mx = std::max(mx, std::abs(x));
+            for (double x : mv) dif = std::max(dif, std::abs(x - mv[0]));
+            fluid3 = dif > 1e-12 * mx;
+        }
+        if (fluid3) {
+            m.mass_fluid_off[p + 1] = m.mass_fluid_off[p] + (int64_t)massFv[p].size();
+            for (double x : massFv[p]) m.inv_mass_fluid.push_back((float)(1. / x));
+        } else {
+            m.mass_fluid_off[p + 1] = m.mass_fluid_off[p] + ((pflags[p] & AXG_PT_FLUID) ? 1 : 0);
+            if (pflags[p] & AXG_PT_FLUID) m.inv_mass_fluid.push_back((float)(1. / massF[p]));
+            massFv[p].clear();
+        }
     }
 
     // ---------------- elements ----------------
@@ -1239,15 +1279,21 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         for (int p = 0; p < NPT; p++) {
             if (!onSF[p] || (pflags[p] & 3) != 3) continue;
             m.sf_point.push_back(p);
-            double mf = massF[p];
-            // SFCoupling1D: unassembled normal, then normal / fluid mass (GLLPoint.cpp:100-111)
-            m.sf_normal.push_back((float)sfn[p][0]);
-            m.sf_normal.push_back((float)sfn[p][1]);
-            m.sf_normal.push_back((float)sfn[p][2]);
-            m.sf_normal.push_back((float)(sfn[p][0] / mf));
-            m.sf_normal.push_back((float)(sfn[p][1] / mf));
-            m.sf_normal.push_back((float)(sfn[p][2] / mf));
-            m.sf_off.push_back(m.sf_off.back() + 1);
+            // SFCoupling1D: unassembled normal, then normal / fluid mass; with an
+            // azimuth-dependent fluid mass SFCoupling3D, one row per slice
+            // (GLLPoint.cpp:100-119)
+            const bool sf3 = !massFv[p].empty();
+            const int rows = sf3 ? (int)massFv[p].size() : 1;
+            for (int n = 0; n < rows; n++) {
+                double mf = sf3 ? massFv[p][n] : massF[p];
+                m.sf_normal.push_back((float)sfn[p][0]);
+                m.sf_normal.push_back((float)sfn[p][1]);
+                m.sf_normal.push_back((float)sfn[p][2]);
+                m.sf_normal.push_back((float)(sfn[p][0] / mf));
+                m.sf_normal.push_back((float)(sfn[p][1] / mf));
+                m.sf_normal.push_back((float)(sfn[p][2] / mf));
+            }
+            m.sf_off.push_back(m.sf_off.back() + rows);
         }
     }
 

@@ -162,6 +162,7 @@ struct Ref {
         bool axial;
         Real invMass;
         std::vector<C> displ, veloc, accel, stiff;  // [(nu+1)*3], alpha-major
+        std::vector<Real> invMass3;                 // Mass3D: 1/mass on the Nr slices
         double wmax[3] = {-1., -1., -1.};          // SolidPoint.h:78-79
         int wnu[3] = {0, 0, 0};
     };
@@ -170,6 +171,7 @@ struct Ref {
         bool axial, surface;
         Real invMass;
         std::vector<C> displ, veloc, accel, stiff;
+        std::vector<Real> invMass3;  // Mass3D
         double wmax = -1.;  // FluidPoint.h:77-78
         int wnu = 0;
     };
@@ -201,6 +203,11 @@ struct Ref {
         Real ns, nz, ns_mf, nz_mf;
     };
     std::vector<SF> sfs;
+    struct SF3 {
+        int s, f;
+        std::vector<Real> n;  // [Nr][6]: unassembled normal, normal / fluid mass
+    };
+    std::vector<SF3> sfs3;
 
     // source
     int srcElem = -1;
@@ -256,9 +263,11 @@ struct Ref {
             int nr = d->point_nr[p];
             int fl = d->point_flags[p];
             if (fl & AXG_PT_SOLID) {
-                if (d->mass_solid_off[p + 1] - d->mass_solid_off[p] != 1)
-                    throw std::runtime_error("refcpu || Mass3D not restated.");
+                const int64_t nm = d->mass_solid_off[p + 1] - d->mass_solid_off[p];
+                if (nm != 1 && nm != nr) throw std::runtime_error("Mass3D::checkCompatibility || Incompatible size.");
                 SolidPt s;
+                if (nm == nr)
+                    for (int i = 0; i < nr; i++) s.invMass3.push_back((Real)d->inv_mass_solid[d->mass_solid_off[p] + i]);
                 s.nr = nr;
                 s.nu = nr / 2;
                 s.gidx = p;
@@ -271,9 +280,11 @@ struct Ref {
                 S.push_back(s);
             }
             if (fl & AXG_PT_FLUID) {
-                if (d->mass_fluid_off[p + 1] - d->mass_fluid_off[p] != 1)
-                    throw std::runtime_error("refcpu || Mass3D not restated.");
+                const int64_t nm = d->mass_fluid_off[p + 1] - d->mass_fluid_off[p];
+                if (nm != 1 && nm != nr) throw std::runtime_error("Mass3D::checkCompatibility || Incompatible size.");
                 FluidPt f;
+                if (nm == nr)
+                    for (int i = 0; i < nr; i++) f.invMass3.push_back((Real)d->inv_mass_fluid[d->mass_fluid_off[p] + i]);
                 f.nr = nr;
                 f.nu = nr / 2;
                 f.gidx = p;
@@ -375,9 +386,14 @@ struct Ref {
         }
         // SF points
         for (int q = 0; q < d->nsf; q++) {
-            if (d->sf_off[q + 1] - d->sf_off[q] != 1) throw std::runtime_error("refcpu || SFCoupling3D not restated.");
+            const int64_t rows = d->sf_off[q + 1] - d->sf_off[q];
             const float *n = d->sf_normal + d->sf_off[q] * 6;
             int p = d->sf_point[q];
+            if (rows != 1) {
+                if (rows != d->point_nr[p]) throw std::runtime_error("SFCoupling3D::checkCompatibility || Incompatible size.");
+                sfs3.push_back({sIndex[p], fIndex[p], std::vector<Real>(n, n + rows * 6)});
+                continue;
+            }
             sfs.push_back({sIndex[p], fIndex[p], (Real)n[0], (Real)n[2], (Real)n[3], (Real)n[5]});
         }
         // source
@@ -447,6 +463,20 @@ struct Ref {
     }
 
     // SolidPoint::updateNewmark (SolidPoint.cpp:23-38), FluidPoint (FluidPoint.cpp:23-44)
+    // Mass3D::computeAccel (point/mass/Mass3D.cpp:13-57): each column of the
+    // (Nu+1) x nc spectrum to the Nr slices, times 1/mass, back (x 1/Nr)
+    static void mass3D(int nr, int nu, int nc, std::vector<C> &f, const std::vector<Real> &invm) {
+        std::vector<C> X(nr / 2 + 1), wa, wb;
+        std::vector<Real> x(nr);
+        for (int c = 0; c < nc; c++) {
+            for (int a = 0; a <= nu; a++) X[a] = f[a * nc + c];
+            c2r<Real>(nr, X.data(), x.data(), wa, wb);
+            for (int i = 0; i < nr; i++) x[i] *= invm[i];
+            r2c<Real>(nr, x.data(), X.data(), wa, wb);
+            for (int a = 0; a <= nu; a++) f[a * nc + c] = X[a];
+        }
+    }
+
     void updateNewmark(double dtt) {
         double half_dt = 0.5 * dtt;
         double half_dt_dt = half_dt * dtt;
@@ -455,7 +485,10 @@ struct Ref {
             for (int i = b; i < e; i++) {
                 SolidPt &p = S[i];
                 maskSolid(p, p.stiff);
-                for (auto &v : p.stiff) v *= p.invMass;  // Mass1D::computeAccel
+                if (p.invMass3.empty())
+                    for (auto &v : p.stiff) v *= p.invMass;  // Mass1D::computeAccel
+                else
+                    mass3D(p.nr, p.nu, 3, p.stiff, p.invMass3);
                 maskSolid(p, p.stiff);
                 size_t n = p.stiff.size();
                 for (size_t j = 0; j < n; j++) p.veloc[j] += hdt * (p.accel[j] + p.stiff[j]);
@@ -473,7 +506,10 @@ struct Ref {
                     continue;
                 }
                 maskFluid(p, p.stiff);
-                for (auto &v : p.stiff) v *= p.invMass;
+                if (p.invMass3.empty())
+                    for (auto &v : p.stiff) v *= p.invMass;
+                else
+                    mass3D(p.nr, p.nu, 1, p.stiff, p.invMass3);
                 maskFluid(p, p.stiff);
                 for (size_t j = 0; j < n; j++) p.veloc[j] += hdt * (p.accel[j] + p.stiff[j]);
                 for (size_t j = 0; j < n; j++) p.accel[j] = p.stiff[j];
@@ -497,6 +533,28 @@ struct Ref {
     // SolidFluidPoint::coupleSolidFluid + SFCoupling1D (SolidFluidPoint.cpp:106-110,
     // point/solid_fluid/SFCoupling1D.cpp:9-17)
     void coupleSolidFluid() {
+        // SFCoupling3D (point/solid_fluid/SFCoupling3D.cpp:9-55)
+        for (auto &q : sfs3) {
+            SolidPt &sp = S[q.s];
+            FluidPt &fp = F[q.f];
+            const int nr = sp.nr, nu = sp.nu;
+            std::vector<C> X(nr / 2 + 1), wa, wb;
+            std::vector<Real> x(nr), acc(nr, 0);
+            for (int c = 0; c < 3; c++) {  // solid displacement -> n . u on the slices
+                for (int a = 0; a <= nu; a++) X[a] = sp.displ[a * 3 + c];
+                c2r<Real>(nr, X.data(), x.data(), wa, wb);
+                for (int i = 0; i < nr; i++) acc[i] += q.n[i * 6 + c] * x[i];
+            }
+            r2c<Real>(nr, acc.data(), X.data(), wa, wb);
+            for (int a = 0; a <= fp.nu; a++) fp.stiff[a] += X[a];
+            for (int a = 0; a <= fp.nu; a++) X[a] = fp.stiff[a];
+            c2r<Real>(nr, X.data(), x.data(), wa, wb);
+            for (int c = 0; c < 3; c++) {  // fluid stiffness -> n / m_f on the slices
+                for (int i = 0; i < nr; i++) acc[i] = q.n[i * 6 + 3 + c] * x[i];
+                r2c<Real>(nr, acc.data(), X.data(), wa, wb);
+                for (int a = 0; a <= nu; a++) sp.stiff[a * 3 + c] -= X[a];
+            }
+        }
         for (auto &q : sfs) {
             SolidPt &sp = S[q.s];
             FluidPt &fp = F[q.f];

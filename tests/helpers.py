@@ -99,3 +99,54 @@ class Lifted3D:
 
     def field_size(self, which):
         return self.mesh.field_size(which)
+
+
+class LiftedMass3D:
+    """The same mesh with every point's Mass1D written as Mass3D (the inverse
+    mass repeated on the point's Nr slices) and every SFCoupling1D as
+    SFCoupling3D (its row repeated on the slices): the same operator."""
+
+    def __init__(self, mesh):
+        import ctypes as C
+        import axisem3d_amd._abi as abi
+        self.mesh = mesh
+        nr = mesh.array("point_nr")
+        self._keep = []
+        d = abi.MeshDesc.from_buffer_copy(mesh.d)
+        for side in ("solid", "fluid"):
+            off = mesh.array("mass_%s_off" % side)
+            inv = mesh.array("inv_mass_%s" % side)
+            blocks, new = [], [0]
+            for p in range(mesh.d.npoint):
+                blk = inv[off[p]:off[p + 1]]
+                if blk.size == 1:
+                    blk = np.repeat(blk, nr[p])
+                blocks.append(blk)
+                new.append(new[-1] + blk.size)
+            o = np.array(new, dtype=np.int64)
+            v = np.concatenate(blocks).astype(np.float32)
+            self._keep += [o, v]
+            setattr(d, "mass_%s_off" % side, o.ctypes.data_as(abi.i64p))
+            setattr(d, "inv_mass_%s" % side, v.ctypes.data_as(abi.f32p))
+        soff, sn, sp = mesh.array("sf_off"), mesh.array("sf_normal").reshape(-1, 6), mesh.array("sf_point")
+        rows, new = [], [0]
+        for q in range(mesh.d.nsf):
+            blk = sn[soff[q]:soff[q + 1]]
+            if blk.shape[0] == 1:
+                blk = np.repeat(blk, nr[sp[q]], axis=0)
+            rows.append(blk)
+            new.append(new[-1] + blk.shape[0])
+        o = np.array(new, dtype=np.int64)
+        v = np.concatenate(rows).astype(np.float32).ravel() if rows else np.zeros(6, np.float32)
+        self._keep += [o, v]
+        d.sf_off = o.ctypes.data_as(abi.i64p)
+        d.sf_normal = v.ctypes.data_as(abi.f32p)
+        self._d = d
+        self.desc = C.pointer(d)
+
+    @property
+    def d(self):
+        return self._d
+
+    def field_size(self, which):
+        return self.mesh.field_size(which)

@@ -478,3 +478,34 @@ def test_3d_attenuation_equals_1d_for_constant_material(refcpu, att):
             seq.append(stiff_of(dom, m, x))
         outs.append(np.concatenate(seq))
     assert rel_l2(outs[1], outs[0]) < 1e-12
+
+
+def test_mass3d_and_sfcoupling3d_equal_1d_for_constant_fields(refcpu):
+    """Mass3D (Mass3D.cpp:13-57) and SFCoupling3D (SFCoupling3D.cpp:9-55) with
+    azimuth-independent values reproduce Mass1D and SFCoupling1D: FFT, pointwise
+    product and inverse FFT of a constant is the plain product."""
+    from tests.helpers import LiftedMass3D
+    m1 = small_mesh(fluid=True, nu=(4, 16), nstep=40)
+    assert m1.d.nsf > 0
+    m3 = LiftedMass3D(m1)
+    s = []
+    for m in (m1, m3):
+        dom = refcpu.RefDomain(m, double=True)
+        dom.run(1, 40, 0)
+        s.append(dom.seismograms())
+    assert np.abs(s[0]).max() > 0
+    assert rel_l2(s[1], s[0]) < 1e-12
+
+
+def test_fluid_with_3d_vp_gets_mass3d_and_sfcoupling3d():
+    m = small_mesh(fluid=True, nu=(4, 16), dv_amp=0.05, dv_fluid=1, nstep=10)
+    nr = m.array("point_nr")
+    fl = m.array("point_flags")
+    off = m.array("mass_fluid_off")
+    fluid = np.flatnonzero(fl & abi.PT_FLUID)
+    cnt = off[fluid + 1] - off[fluid]
+    assert ((cnt == 1) | (cnt == nr[fluid])).all() and (cnt > 1).any()
+    # SFCoupling3D where the fluid mass varies (not on the axis, where all slices coincide)
+    rows = np.diff(m.array("sf_off"))
+    nsf = nr[m.array("sf_point")]
+    assert ((rows == 1) | (rows == nsf)).all() and (rows > 1).any()
