@@ -264,8 +264,9 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
     const int P = (phase != 0) ? 1 : (mat == AXG_MAT_ISOTROPIC) ? 2 : (mat == AXG_MAT_TRANSVERSELY_ISO) ? 5 : 21;
     if (c->h_mat3d_off.empty()) c->h_mat3d_off.assign(d->nelem, 0);
     if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
-    // classes by modes per lane S (M = Nr/2 + 1 <= 64 S): register and LDS use
-    // follow the class
+    // classes by modes per lane S: the (Nr + 1) / 2 modes that carry data fit in
+    // 64 S lanes (an even Nr's Nyquist mode is zero and gets no lane); register
+    // and LDS use follow the class
     std::vector<std::vector<int32_t>> byClass(W3D_MAX_S + 1);
     for (int e : elems) {
         int nr = c->elem_nr[e];
@@ -273,8 +274,7 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
             throw std::runtime_error("FieldFFT::transformF2P || Nr = " + std::to_string(nr) +
                                      " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
         c->nr_used[nr] = 1;
-        int M = nr / 2 + 1;
-        byClass[(M + 63) / 64].push_back(e);
+        byClass[((nr + 1) / 2 + 63) / 64].push_back(e);
         // material: descriptor [param][n][point] -> [point][param][n]
         int64_t off = (int64_t)c->h_mat3d.size();
         c->h_mat3d_off[e] = off;
@@ -301,7 +301,7 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
         b->nelem = (int32_t)v.size();
         b->slots = k;
         b->maxNr = c->elem_nr[v.front()];
-        b->maxM = b->maxNr / 2 + 1;
+        b->maxM = std::min(b->maxNr / 2 + 1, 64 * k);  // modes with a lane
         b->elems.upload(v, c->stream);
         HIPCHECK(hipStreamSynchronize(c->stream));
         c->buckets3d.push_back(std::move(b));

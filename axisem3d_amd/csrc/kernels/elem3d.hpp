@@ -14,12 +14,13 @@
 //   quadrature operands              CrdTransTIsoSolid::transformRTZ_SPZ, Gradient::computeQuad6
 //
 // The FFTs run in a wave-private LDS area, so they need no workgroup barrier.
-// The quadrature couples a point to its row (the eta term, which stays in the
-// wave: the wave accumulates the forces of its whole row in registers) and to
-// its column (the xi term, which crosses rows): the xi operands of the step's
-// five points go through a double-buffered LDS exchange, one barrier per step.
-// At the end each wave subtracts its row's forces from the point field (mesh
-// colouring: no other element of the launch touches these points).
+// The quadrature couples a point to its column (the xi term, which crosses
+// rows: the xi operands of the step's five points go through an LDS exchange
+// between two barriers, and each wave sums its own point's) and to its row (the
+// eta term, which stays in the wave): in every step each wave subtracts its
+// point's eta contributions from the five points of its row, and its point's
+// xi and own terms from that point, in the point field (mesh colouring: no
+// other element of the launch touches these points).
 //
 // Lane l of a wave holds the modes alpha = l + 64 s (s < S) in the spectral
 // phases and the samples n = l + 64 t in the stress phase.  The FFT is a
@@ -35,7 +36,9 @@ namespace axg {
 constexpr int W3D_WAVES = 5;
 constexpr int W3D_THREADS = 64 * W3D_WAVES;
 constexpr int W3D_MAX_S = 3;                        // modes per lane: M <= 64 S
-constexpr int E3D_MAX_NR = 2 * 64 * W3D_MAX_S - 2;  // Nr with Nr/2 + 1 <= 192
+// Nr with at most 64 S modes carrying data: (Nr + 1) / 2 <= 192 (the Nyquist
+// mode of an even Nr is zero throughout and has no lane)
+constexpr int E3D_MAX_NR = 2 * 64 * W3D_MAX_S;
 
 struct Elem3DArgs {
     const int32_t *elems;
@@ -462,12 +465,16 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                 }
             }
         }
+        if (nyq && lane == 0) {
+#pragma unroll
+            for (int b = 0; b < 3; b++) seq[3 * Nu + b] = czero();  // Nyquist: no lane when Nu = 64 S
+        }
         wave_lds_fence();
         // C2R (pass 0), stress on the samples, R2C (pass 1): one copy of the
         // transform code serves both directions
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
-            wfft<128 * S - 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
+            wfft<128 * S>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
             if (pass == 1) break;
             // ---------------- stress at every sample n of the point
             const float *mp = mbase + (int64_t)ip * NPAR * Nr;
@@ -684,10 +691,11 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
                 z1[2 * (Nr - al)] = make_float2(g3[2].x, -g3[2].y);
             }
         }
+        if (nyq && lane == 0) seq[2 * Nu] = seq[2 * Nu + 1] = czero();  // Nyquist: no lane when Nu = 64 S
         wave_lds_fence();
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
-            wfft<128 * S - 2, 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
+            wfft<128 * S, 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
             if (pass == 1) break;
             // Acoustic3D: stress = K(phi) grad chi on every sample
             const float *K = mbase + (int64_t)ip * Nr;
