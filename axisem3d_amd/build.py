@@ -3,6 +3,7 @@
 ``python -m axisem3d_amd.build`` compiles every source every time.  Outputs go
 to axisem3d_amd/_build/ (git-ignored; travels with the tree to the GPU box).
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -20,7 +21,10 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 HOST_SOURCES = ["preloop/shell_mesh.cpp", "host/errors.cpp"]
-HIP_SOURCES = ["axg_domain.hip"]
+# (source, object name, extra flags): the 3-D element launchers are compiled once
+# per size class (modes per lane) so that the classes build in parallel
+HIP_SOURCES = [("axg_domain.hip", "axg_domain", [])] + [
+    ("elem3d_launch.hip", "elem3d_s%d" % s, ["-DAXG_E3D_S=%d" % s]) for s in (1, 2, 3, 5, 7)]
 
 
 def run(cmd):
@@ -28,17 +32,20 @@ def run(cmd):
     subprocess.check_call(cmd)
 
 
-def build(verbose=True):
+def build(verbose=True, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        run(["g++"] + CXX_FLAGS + INC + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append(["g++"] + CXX_FLAGS + INC + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in HIP_SOURCES:
-        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        run([HIPCC] + HIP_FLAGS + INC + ["-c", os.path.join(CSRC, src), "-o", obj])
+    for src, name, extra in HIP_SOURCES:
+        obj = os.path.join(BUILD, name + ".hip.o")
+        cmds.append([HIPCC] + HIP_FLAGS + extra + INC + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(run, cmds))
     lib = os.path.join(BUILD, "libaxg.so")
     tmp = lib + ".tmp"
     run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])

@@ -131,6 +131,8 @@ struct Phase {
 
 struct axg_ctx {
     int device = 0;
+    float G[2][25] = {};  // GLL / GLJ derivative matrices (kernel arguments of the 3-D kernels)
+    int pf_max_nr = 0;    // largest Nr of the Mass3D / SFCoupling3D points
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int32_t npoint = 0, nelem = 0, nstep = 0, nrec = 0;
@@ -274,7 +276,7 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
             throw std::runtime_error("FieldFFT::transformF2P || Nr = " + std::to_string(nr) +
                                      " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
         c->nr_used[nr] = 1;
-        byClass[((nr + 1) / 2 + 63) / 64].push_back(e);
+        byClass[w3dClass(((nr + 1) / 2 + 63) / 64)].push_back(e);
         // material: descriptor [param][n][point] -> [point][param][n]
         int64_t off = (int64_t)c->h_mat3d.size();
         c->h_mat3d_off[e] = off;
@@ -365,42 +367,6 @@ void finishBuckets3D(axg_ctx *c) {
     }
 }
 
-template <int MAT, bool TISO, int S, int ATT>
-void launchSolid3DSA(const Elem3DArgs &a, size_t lds, hipStream_t s) {
-    static bool attrSet = false;
-    if (!attrSet) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&solid3d_kernel<MAT, TISO, S, ATT>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attrSet = true;
-    }
-    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S, ATT>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
-}
-
-template <int MAT, bool TISO, int S>
-void launchSolid3DS(const Elem3DArgs &a, int att, size_t lds, hipStream_t s) {
-    if (att == AXG_ATT_NONE) launchSolid3DSA<MAT, TISO, S, 0>(a, lds, s);
-    else if (att == AXG_ATT_FULL) launchSolid3DSA<MAT, TISO, S, 1>(a, lds, s);
-    else launchSolid3DSA<MAT, TISO, S, 2>(a, lds, s);
-}
-
-template <int S>
-void launchFluid3DS(const Elem3DArgs &a, size_t lds, hipStream_t s) {
-    static bool attrSet = false;
-    if (!attrSet) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fluid3d_kernel<S>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attrSet = true;
-    }
-    hipLaunchKernelGGL((fluid3d_kernel<S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
-}
-
-template <int MAT, bool TISO>
-void launchSolid3D(const Elem3DArgs &a, int slots, int att, size_t lds, hipStream_t s) {
-    if (slots == 1) launchSolid3DS<MAT, TISO, 1>(a, att, lds, s);
-    else if (slots == 2) launchSolid3DS<MAT, TISO, 2>(a, att, lds, s);
-    else launchSolid3DS<MAT, TISO, 3>(a, att, lds, s);
-}
-
 void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     if (b.nelem == 0) return;
     FamScope fs(c, b.phase ? AXG_KF_FLUID : AXG_KF_SOLID3D);
@@ -424,15 +390,10 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     a.tw_off = c->d_tw_off.p;
     a.plan = c->d_plan.p;
     a.nstage = c->d_nstage.p;
-    size_t lds = lds3D(b);
-    hipStream_t s = c->stream;
+    std::memcpy(a.G, c->G, sizeof(a.G));
     if (b.phase) {
         a.displ = c->F.displ.p;
         a.stiff = c->F.stiff.p;
-        if (b.slots == 1) launchFluid3DS<1>(a, lds, s);
-        else if (b.slots == 2) launchFluid3DS<2>(a, lds, s);
-        else launchFluid3DS<3>(a, lds, s);
-        return;
     }
     a.att_pool = c->d_att_pool.p;
     a.att_off = c->d_att_off.p;
@@ -447,13 +408,13 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
         a.sls_gamma[i] = c->sls_g[i];
     }
     a.do_kappa = c->do_kappa;
-    if (b.material == AXG_MAT_ISOTROPIC) {
-        if (b.tiso) launchSolid3D<0, true>(a, b.slots, b.att, lds, s);
-        else launchSolid3D<0, false>(a, b.slots, b.att, lds, s);
-    } else if (b.material == AXG_MAT_TRANSVERSELY_ISO) {
-        launchSolid3D<1, true>(a, b.slots, b.att, lds, s);
-    } else {
-        launchSolid3D<2, true>(a, b.slots, b.att, lds, s);
+    const size_t lds = lds3D(b);
+    switch (b.slots) {
+        case 1: launchElem3D<1>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
+        case 2: launchElem3D<2>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
+        case 3: launchElem3D<3>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
+        case 5: launchElem3D<5>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
+        default: launchElem3D<7>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
     }
 }
 
@@ -642,6 +603,7 @@ void doLearn(axg_ctx *c, float cutoff, int interval, const int *tstep_ptr) {
 PointFFTArgs pointFFTArgs(axg_ctx *c) {
     PointFFTArgs a;
     std::memset(&a, 0, sizeof(a));
+    a.wave_lds = pfWaveLds(c->pf_max_nr);
     a.twiddle = c->d_twiddle.p;
     a.tw_off = c->d_tw_off.p;
     a.plan = c->d_plan.p;
@@ -649,7 +611,7 @@ PointFFTArgs pointFFTArgs(axg_ctx *c) {
     return a;
 }
 
-constexpr size_t kPointFFTLds = (size_t)PF_WAVES * PF_WAVE_LDS * sizeof(float2);
+size_t pointFFTLds(const PointFFTArgs &a) { return (size_t)PF_WAVES * a.wave_lds * sizeof(float2); }
 
 // Mass3D::computeAccel of the points whose mass varies in azimuth, before the
 // Newmark kernels (which then scale those points by 1)
@@ -668,8 +630,8 @@ void doMass3D(axg_ctx *c) {
         a.coef = P.d_m3_inv.p;
         a.field = P.stiff.p;
         dim3 grid(std::min(1024, (a.n + PF_WAVES - 1) / PF_WAVES));
-        if (ph) hipLaunchKernelGGL(mass3d_kernel<1>, grid, dim3(256), kPointFFTLds, c->stream, a);
-        else hipLaunchKernelGGL(mass3d_kernel<3>, grid, dim3(256), kPointFFTLds, c->stream, a);
+        if (ph) hipLaunchKernelGGL(mass3d_kernel<1>, grid, dim3(256), pointFFTLds(a), c->stream, a);
+        else hipLaunchKernelGGL(mass3d_kernel<3>, grid, dim3(256), pointFFTLds(a), c->stream, a);
     }
 }
 
@@ -722,7 +684,7 @@ void doCouple(axg_ctx *c) {
         a.sdispl = c->S.displ.p;
         a.fstiff = c->F.stiff.p;
         dim3 grid(std::min(1024, (a.n + PF_WAVES - 1) / PF_WAVES));
-        hipLaunchKernelGGL(couple3d_kernel, grid, dim3(256), kPointFFTLds, c->stream, a);
+        hipLaunchKernelGGL(couple3d_kernel, grid, dim3(256), pointFFTLds(a), c->stream, a);
     }
     if (c->nsf_row == 0) return;
     const int B = 256;
@@ -787,10 +749,13 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 G[1][i * 5 + j] = (float)gj.G[i][j];
             }
         HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cG), G, sizeof(G)));
+        std::memcpy(c->G, G, sizeof(G));
 
     }
     buildPhase(d, c->S, true);
     buildPhase(d, c->F, false);
+    for (Phase *P : {&c->S, &c->F})
+        for (int nr : P->m3_nr) c->pf_max_nr = std::max(c->pf_max_nr, nr);
     uploadPhase(c->S, s);
     uploadPhase(c->F, s);
 
@@ -960,6 +925,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
                 s3row.push_back(c->S.pt_row[qs]);
                 f3row.push_back(c->F.pt_row[qf]);
                 s3nr.push_back(d->point_nr[p]);
+                c->pf_max_nr = std::max(c->pf_max_nr, (int)d->point_nr[p]);
                 s3off.push_back((int64_t)s3nrm.size() / 6);
                 s3nrm.insert(s3nrm.end(), d->sf_normal + d->sf_off[q] * 6, d->sf_normal + d->sf_off[q + 1] * 6);
                 continue;

@@ -1,0 +1,60 @@
+// elem3d_launch.hip -- host launchers of the 3-D element kernels for ONE size
+// class S (modes per lane), selected by -DAXG_E3D_S=<S>; the build compiles
+// this file once per class so that the classes compile in parallel.
+#include <hip/hip_runtime.h>
+
+#include "axg.h"
+#include "kernels/elem3d.hpp"
+
+#ifndef AXG_E3D_S
+#error "AXG_E3D_S (the size class) must be defined"
+#endif
+
+namespace axg {
+namespace {
+
+constexpr int S = AXG_E3D_S;
+static_assert(S == w3dClass(S), "not a built size class");
+
+// the kernels may take up to the whole 160 KB of a CU's LDS
+template <class K>
+void allowLds(K kernel) {
+    static bool done = false;
+    if (!done) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        done = true;
+    }
+}
+
+template <int MAT, bool TISO, int ATT>
+void solid(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+    allowLds(&solid3d_kernel<MAT, TISO, S, ATT>);
+    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S, ATT>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+}
+
+template <int MAT, bool TISO>
+void solidAtt(const Elem3DArgs &a, int att, size_t lds, hipStream_t s) {
+    if (att == AXG_ATT_NONE) solid<MAT, TISO, 0>(a, lds, s);
+    else if (att == AXG_ATT_FULL) solid<MAT, TISO, 1>(a, lds, s);
+    else solid<MAT, TISO, 2>(a, lds, s);
+}
+
+}  // namespace
+
+template <>
+void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s) {
+    if (phase) {
+        allowLds(&fluid3d_kernel<S>);
+        hipLaunchKernelGGL((fluid3d_kernel<S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+    } else if (mat == AXG_MAT_ISOTROPIC) {
+        if (tiso) solidAtt<0, true>(a, att, lds, s);
+        else solidAtt<0, false>(a, att, lds, s);
+    } else if (mat == AXG_MAT_TRANSVERSELY_ISO) {
+        solidAtt<1, true>(a, att, lds, s);
+    } else {
+        solidAtt<2, true>(a, att, lds, s);
+    }
+}
+
+}  // namespace axg

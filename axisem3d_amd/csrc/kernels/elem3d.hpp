@@ -35,12 +35,16 @@ namespace axg {
 
 constexpr int W3D_WAVES = 5;
 constexpr int W3D_THREADS = 64 * W3D_WAVES;
-constexpr int W3D_MAX_S = 3;                        // modes per lane: M <= 64 S
+// modes per lane S: the classes built are 1, 2, 3, 5 and 7 (a class of 4 or 6
+// runs as 5 or 7); classes above 3 hold one workgroup per CU (LDS)
+constexpr int W3D_MAX_S = 7;
+constexpr int w3dClass(int s) { return s <= 3 ? s : s <= 5 ? 5 : 7; }
 // Nr with at most 64 S modes carrying data: (Nr + 1) / 2 <= 192 (the Nyquist
 // mode of an even Nr is zero throughout and has no lane)
 constexpr int E3D_MAX_NR = 2 * 64 * W3D_MAX_S;
 
 struct Elem3DArgs {
+    float G[2][25];              // GLL (0) and GLJ (1) derivative matrices G[i][j] = l_i'(xi_j)
     const int32_t *elems;
     int32_t nelem;
     int32_t maxM;                // largest Nu+1 of the launch (LDS strides)
@@ -361,8 +365,8 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     const int M = Nu + 1;
     const bool nyq = (Nr % 2 == 0);
     const bool axial = (a.elem_flags[e] & 2) != 0;
-    const float *Gx = cG[axial ? 1 : 0];
-    const float *Ge = cG[0];
+    const float *Gx = a.G[axial ? 1 : 0];
+    const float *Ge = a.G[0];
     const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
     const int nst = a.nstage[Nr];
 
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
                 for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };
-            if constexpr (NPAR <= 5) {
+            if constexpr (NPAR <= 5 && S <= 3) {
                 // all samples' parameters first (Nr <= 128 S: at most 2 S per lane)
                 float pm[2 * S][NPAR];
 #pragma unroll
@@ -622,8 +626,8 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     const int M = Nu + 1;
     const bool nyq = (Nr % 2 == 0);
     const bool axial = (a.elem_flags[e] & 2) != 0;
-    const float *Gx = cG[axial ? 1 : 0];
-    const float *Ge = cG[0];
+    const float *Gx = a.G[axial ? 1 : 0];
+    const float *Ge = a.G[0];
     const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
     const int nst = a.nstage[Nr];
 
@@ -770,5 +774,10 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
         }
     }
 }
+
+// Launch of one 3-D bucket of class S (defined, one class per object file, in
+// elem3d_launch.hip): phase 0 solid (material, TI, attenuation kind), 1 fluid.
+template <int S>
+void launchElem3D(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s);
 
 }  // namespace axg

@@ -14,11 +14,13 @@
 namespace axg {
 
 constexpr int PF_WAVES = 4;
-// float2 slots of one wave's LDS area: twiddles 2 Nr + roots, sequences 2 Nr
-constexpr int PF_WAVE_LDS = 2 * E3D_MAX_NR + kRootTable + 2 * E3D_MAX_NR;
+// float2 slots of one wave's LDS area for points of Nr <= maxNr: twiddles
+// 2 Nr + roots, sequences 2 Nr
+constexpr int pfWaveLds(int maxNr) { return 4 * maxNr + kRootTable; }
 
 struct PointFFTArgs {
     int32_t n;                   // points in the list
+    int32_t wave_lds;            // float2 slots of each wave's LDS area, pfWaveLds(largest Nr)
     const int32_t *items;        // Mass3D: compact points; coupling: SF3D entries
     const int32_t *pt_row;       // phase rows (Mass3D) / solid rows (coupling)
     const int32_t *pt_nu;
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
     extern __shared__ float2 lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    float2 *twf = lds + w * PF_WAVE_LDS;
+    float2 *twf = lds + w * a.wave_lds;
     for (int it = blockIdx.x * PF_WAVES + w; it < a.n; it += gridDim.x * PF_WAVES) {
         const int q = a.items[it];
         const int Nr = a.pt_nr[it], nu = a.pt_nu[q], fl = a.pt_flags[q];
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(256) void couple3d_kernel(PointFFTArgs a) {
     extern __shared__ float2 lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    float2 *twf = lds + w * PF_WAVE_LDS;
+    float2 *twf = lds + w * a.wave_lds;
     for (int it = blockIdx.x * PF_WAVES + w; it < a.n; it += gridDim.x * PF_WAVES) {
         const int Nr = a.pt_nr[it], nu = Nr / 2;
         float2 *twi = twf + Nr, *root = twi + Nr, *seq = root + kRootTable;

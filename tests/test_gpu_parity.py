@@ -460,3 +460,74 @@ def test_mass3d_and_sfcoupling3d(fluid_layers):
     sg, sr = g.seismograms(), r.seismograms()
     assert np.abs(sr).max() > 0
     assert rel_l2(sg, sr) < 1e-4
+
+
+def _large_nr_layers(fluid):
+    # radial element size close to the azimuthal one, so that the Nr cap
+    # 2 pi s / spacing stays above 810
+    solid = dict(r_bottom=3780e3 if fluid else 6071e3, r_top=4080e3 if fluid else 6371e3, nelem_radial=2, fluid=0,
+                 vp_bottom=8.9e3, vp_top=5.8e3, vs_bottom=4.8e3, vs_top=3.2e3, rho_bottom=3.5e3, rho_top=2.6e3,
+                 q_mu=300., q_kappa=57823.)
+    if not fluid:
+        return [solid]
+    return [dict(r_bottom=3480e3, r_top=3780e3, nelem_radial=2, fluid=1, vp_bottom=10.0e3, vp_top=8.5e3,
+                 vs_bottom=0., vs_top=0., rho_bottom=12.0e3, rho_top=10.0e3, q_mu=0., q_kappa=57823.), solid]
+
+
+def test_large_nr_size_classes():
+    """Elements and points of Nr up to 810 (Nu up to 405): every 3-D size class
+    (1, 2, 3, 5, 7 modes per lane) on an anisotropic CG4-attenuating solid over
+    two successive stiffness calls, then a 3-D fluid with Mass3D and
+    SFCoupling3D points of large Nr through stiffness, coupling and Newmark;
+    all fields against the FP64 oracle."""
+    print("case large_nr solid", flush=True)
+    cfg = ax.shell_config(1, layers=_large_nr_layers(False), nelem_theta=104, material=_abi.MAT_ANISOTROPIC,
+                          dv_amp=0.05, nu_mode=1, nu_min=150, nu_max=405, attenuation=_abi.ATT_CG4, nsls=5,
+                          nstep=10, nrec=4)
+    m = ax.ShellMesh(cfg)
+    nr = m.array("point_nr")
+    ep = m.array("elem_points").reshape(m.d.nelem, 25)
+    meff = (nr[ep].max(1) + 1) // 2
+    assert nr.max() == 810 and {1, 2, 3, 4, 5, 6, 7} <= set(((meff + 63) // 64).tolist())
+    g = ax.Domain(m)
+    r = RefDomain(m, double=True)
+    for seed in (11, 12):
+        u = random_field(m, _abi.FIELD_SOLID_DISPL, seed=seed)
+        for d in (g, r):
+            d.set_field(_abi.FIELD_SOLID_DISPL, u)
+            d.set_field(_abi.FIELD_SOLID_STIFF, np.zeros_like(u))
+            d.computeStiff()
+        sg, sr = g.get_field(_abi.FIELD_SOLID_STIFF), r.get_field(_abi.FIELD_SOLID_STIFF)
+        err = rel_l2(sg, sr)
+        print("large-Nr solid stiff rel L2 %.3e" % err, flush=True)
+        assert np.abs(sr).max() > 0 and err < 1e-5
+    g.close()
+    del r
+
+    print("case large_nr fluid", flush=True)
+    cfg = ax.shell_config(1, layers=_large_nr_layers(True), nelem_theta=104, material=_abi.MAT_ISOTROPIC,
+                          dv_amp=0.05, dv_fluid=1, nu_mode=1, nu_min=150, nu_max=405, nstep=10, nrec=4)
+    m = ax.ShellMesh(cfg)
+    nr = m.array("point_nr")
+    fl = m.array("elem_flags")
+    fnr = nr[m.array("elem_points").reshape(m.d.nelem, 25)].max(1)
+    assert ((fl & _abi.EL_FLUID) & ((fl & _abi.EL_3D) >> 2)).astype(bool)[fnr > 384].any()
+    moff, soff = m.array("mass_fluid_off"), m.array("sf_off")
+    assert (np.diff(moff) > 384).any() and (np.diff(soff) > 384).any()
+    g = ax.Domain(m)
+    r = RefDomain(m, double=True)
+    fields = {w: random_field(m, w, seed=20 + w) for w in range(8)}
+    for d in (g, r):
+        for w, x in fields.items():
+            d.set_field(w, x)
+        d.computeStiff()
+        d.coupleSolidFluid()
+        d.updateNewmark(m.d.dt)
+    for w in range(8):
+        a, b = g.get_field(w), r.get_field(w)
+        if w in (_abi.FIELD_SOLID_STIFF, _abi.FIELD_FLUID_STIFF):
+            assert not np.abs(b).any() and not np.abs(a).any()  # Newmark clears the stiffness
+            continue
+        err = rel_l2(a, b)
+        print("large-Nr field %d rel L2 %.3e" % (w, err), flush=True)
+        assert err < 1e-5
