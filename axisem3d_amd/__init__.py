@@ -27,6 +27,7 @@ def lib():
         l = _C.CDLL(LIB_PATH, mode=_C.RTLD_GLOBAL)
         _abi.declare_mesh_api(l)
         _abi.declare_domain_api(l)
+        _abi.declare_mlp_api(l)
         _lib = l
     return _lib
 
@@ -37,5 +38,6 @@ def last_error():
 
 from .mesh import ShellMesh, shell_config  # noqa: E402
 from .domain import Domain, Newmark  # noqa: E402
+from .nu_mlp import NuMLP  # noqa: E402
 
-__all__ = ["lib", "last_error", "ShellMesh", "shell_config", "Domain", "Newmark"]
+__all__ = ["lib", "last_error", "ShellMesh", "shell_config", "Domain", "Newmark", "NuMLP"]

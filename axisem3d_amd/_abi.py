@@ -97,6 +97,7 @@ class ShellConfig(C.Structure):
         ("rec_components", C.c_int32),
         ("seed", C.c_uint64),
         ("nthread", C.c_int32),
+        ("elem_nu", C.POINTER(C.c_int32)),
     ]
 
 
@@ -110,6 +111,7 @@ FIELD_FLUID_DISPL, FIELD_FLUID_VELOC, FIELD_FLUID_ACCEL, FIELD_FLUID_STIFF = 4, 
 KF_SOLID3D, KF_SOLID1D, KF_FLUID, KF_NEWMARK, KF_OTHER, KF_N = 0, 1, 2, 3, 4, 5
 KF_NAMES = ("solid3d", "solid1d", "fluid", "newmark", "other")
 E_OK, E_INVAL, E_UNSUPPORTED, E_HIP, E_UNSTABLE, E_NOMEM = 0, 1, 2, 3, 4, 5
+NFEAT = 19
 
 
 def declare_mesh_api(lib):
@@ -124,6 +126,8 @@ def declare_mesh_api(lib):
     lib.axg_mesh_stats.restype = C.c_int
     lib.axg_mesh_free.argtypes = [C.c_void_p]
     lib.axg_mesh_free.restype = None
+    lib.axg_mesh_point_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    lib.axg_mesh_point_features.restype = C.c_int
     lib.axg_last_error.argtypes = []
     lib.axg_last_error.restype = C.c_char_p
     lib.axg_field_size.argtypes = [C.POINTER(MeshDesc), C.c_int32]
@@ -160,3 +164,23 @@ def declare_domain_api(lib):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+
+
+def declare_mlp_api(lib):
+    """Prototypes of the learned Nu predictor."""
+    vp, fp = C.c_void_p, C.POINTER(C.c_float)
+    sig = {
+        "axg_mlp_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_int, C.POINTER(vp)], C.c_int),
+        "axg_mlp_destroy": ([vp], None),
+        "axg_mlp_nparams": ([vp], C.c_int64),
+        "axg_mlp_get_params": ([vp, fp], C.c_int),
+        "axg_mlp_set_params": ([vp, fp], C.c_int),
+        "axg_mlp_set_data": ([vp, fp, fp, C.c_int64], C.c_int),
+        "axg_mlp_train": ([vp, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64], C.c_int),
+        "axg_mlp_predict": ([vp, fp, C.c_int64, fp], C.c_int),
+        "axg_mlp_gradient": ([vp, fp, fp, C.c_int32, C.c_float, fp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res

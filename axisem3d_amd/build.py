@@ -23,7 +23,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 HOST_SOURCES = ["preloop/shell_mesh.cpp", "host/errors.cpp"]
 # (source, object name, extra flags): the 3-D element launchers are compiled once
 # per size class (modes per lane) so that the classes build in parallel
-HIP_SOURCES = [("axg_domain.hip", "axg_domain", [])] + [
+HIP_SOURCES = [("axg_domain.hip", "axg_domain", []), ("nu_mlp.hip", "nu_mlp", [])] + [
     ("elem3d_launch.hip", "elem3d_s%d" % s, ["-DAXG_E3D_S=%d" % s]) for s in (1, 2, 3, 5, 7)]
 
 

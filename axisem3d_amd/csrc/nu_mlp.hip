@@ -1,0 +1,304 @@
+// nu_mlp.hip -- C ABI of the learned Nu predictor (include/axg.h, "learned Nu"
+// section): a ReLU perceptron nin -> width -> ... -> width -> 1 trained with
+// Adam on (feature, Nu) pairs, every product on the bf16 matrix cores
+// (kernels/mlp.hpp).  Training data live in HBM; one epoch is a host-drawn
+// permutation and, per batch, gather -> forward -> loss gradient -> backward ->
+// Adam, all on the predictor's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "axg.h"
+#include "host/hip_util.hpp"
+#include "kernels/mlp.hpp"
+
+namespace axg {
+void set_last_error(const std::string &msg);
+}  // namespace axg
+
+using namespace axg;
+
+struct axg_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int> dim;              // layer widths d0 .. dL (dL = 1)
+    std::vector<int64_t> woff, boff;   // offsets of W_l [d_{l+1}][d_l] and b_l in the parameter vector
+    int64_t nparam = 0;
+    int64_t step = 0;                  // Adam steps taken
+    DevBuf<float> p, g, m, v;          // parameters, gradient, Adam moments
+    int bmax = 0;                      // rows of the work buffers
+    std::vector<DevBuf<float>> act;    // act[l] [bmax][d_l]: input of layer l
+    DevBuf<float> z, yb, dz, dA, dB;   // output, batch targets, output gradient, ping-pong layer gradients
+    DevBuf<float> xs, ys;              // training set
+    DevBuf<int32_t> idx;
+    int64_t nset = 0;
+    int nlayer() const { return (int)dim.size() - 1; }
+};
+
+namespace {
+
+int fail(int code, const std::string &msg) {
+    set_last_error(msg);
+    return code;
+}
+
+#define MLP_GUARD(...)                           \
+    try {                                        \
+        __VA_ARGS__;                             \
+        return AXG_OK;                           \
+    } catch (const HipError &e) {                \
+        return fail(AXG_EHIP, e.what());         \
+    } catch (const std::exception &e) {          \
+        return fail(AXG_EINVAL, e.what());       \
+    }
+
+void gemm(axg_mlp *n, const GemmArgs &g) {
+    dim3 grid((g.N + 31) / 32, (g.M + 31) / 32);
+    hipLaunchKernelGGL(mlp_gemm_kernel, grid, dim3(256), 0, n->stream, g);
+}
+
+void ensureRows(axg_mlp *n, int rows) {
+    if (rows <= n->bmax) return;
+    n->bmax = rows;
+    n->act.resize(n->nlayer());
+    for (int l = 0; l < n->nlayer(); l++) n->act[l].alloc((size_t)rows * n->dim[l]);
+    const int wmax = *std::max_element(n->dim.begin(), n->dim.end());
+    n->z.alloc(rows);
+    n->yb.alloc(rows);
+    n->dz.alloc(rows);
+    n->dA.alloc((size_t)rows * wmax);
+    n->dB.alloc((size_t)rows * wmax);
+}
+
+// forward pass of B rows already in act[0]; output in z
+void forward(axg_mlp *n, int B) {
+    const int L = n->nlayer();
+    for (int l = 0; l < L; l++) {
+        GemmArgs g{};
+        g.M = B;
+        g.N = n->dim[l + 1];
+        g.K = n->dim[l];
+        g.A = n->act[l].p;
+        g.a_rs = n->dim[l];
+        g.a_cs = 1;
+        g.B = n->p.p + n->woff[l];  // (k, j) = W_l[j][k]
+        g.b_rs = 1;
+        g.b_cs = n->dim[l];
+        g.C = (l + 1 < L) ? n->act[l + 1].p : n->z.p;
+        g.c_rs = n->dim[l + 1];
+        g.bias = n->p.p + n->boff[l];
+        g.relu = (l + 1 < L) ? 1 : 0;
+        gemm(n, g);
+    }
+}
+
+// gradient of the loss over B rows (act[0] and yb filled) into g
+void backward(axg_mlp *n, int B, float under_weight) {
+    const int L = n->nlayer();
+    hipLaunchKernelGGL(mlp_loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, n->stream, B, n->z.p, n->yb.p,
+                       under_weight, n->dz.p);
+    const float *dZ = n->dz.p;  // [B][d_{l+1}]
+    for (int l = L - 1; l >= 0; l--) {
+        const int din = n->dim[l], dout = n->dim[l + 1];
+        // dW_l = dZ^T act[l]
+        GemmArgs w{};
+        w.M = dout;
+        w.N = din;
+        w.K = B;
+        w.A = dZ;  // (j, i) = dZ[i][j]
+        w.a_rs = 1;
+        w.a_cs = dout;
+        w.B = n->act[l].p;  // (i, k) = act[i][k]
+        w.b_rs = din;
+        w.b_cs = 1;
+        w.C = n->g.p + n->woff[l];
+        w.c_rs = din;
+        gemm(n, w);
+        hipLaunchKernelGGL(mlp_bias_grad_kernel, dim3((dout + 255) / 256), dim3(256), 0, n->stream, B, dout, dZ,
+                           n->g.p + n->boff[l]);
+        if (l == 0) break;
+        // dZ_{l-1} = (dZ W_l) * (act[l] > 0)
+        float *out = (dZ == n->dA.p) ? n->dB.p : n->dA.p;
+        GemmArgs d{};
+        d.M = B;
+        d.N = din;
+        d.K = dout;
+        d.A = dZ;
+        d.a_rs = dout;
+        d.a_cs = 1;
+        d.B = n->p.p + n->woff[l];  // (j, k) = W_l[j][k]
+        d.b_rs = din;
+        d.b_cs = 1;
+        d.C = out;
+        d.c_rs = din;
+        d.mask = n->act[l].p;
+        d.mask_rs = din;
+        gemm(n, d);
+        dZ = out;
+    }
+}
+
+void loadRows(axg_mlp *n, const float *x, const float *y, int rows) {
+    ensureRows(n, rows);
+    HIPCHECK(hipMemcpyAsync(n->act[0].p, x, sizeof(float) * rows * n->dim[0], hipMemcpyHostToDevice, n->stream));
+    if (y) HIPCHECK(hipMemcpyAsync(n->yb.p, y, sizeof(float) * rows, hipMemcpyHostToDevice, n->stream));
+}
+
+}  // namespace
+
+extern "C" void axg_mlp_destroy(axg_mlp *n);
+
+extern "C" int axg_mlp_create(int32_t nin, int32_t width, int32_t nhidden, uint64_t seed, int device,
+                              axg_mlp **out) {
+    if (!out || nin < 1 || width < 1 || nhidden < 1) return fail(AXG_EINVAL, "axg_mlp_create || invalid argument.");
+    *out = nullptr;
+    std::unique_ptr<axg_mlp, void (*)(axg_mlp *)> hold(new axg_mlp(), axg_mlp_destroy);
+    axg_mlp *n = hold.get();
+    MLP_GUARD({
+        n->device = device;
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+        n->dim.push_back(nin);
+        for (int h = 0; h < nhidden; h++) n->dim.push_back(width);
+        n->dim.push_back(1);
+        std::vector<float> h0;
+        std::mt19937_64 rng(seed);
+        for (int l = 0; l < n->nlayer(); l++) {
+            const int din = n->dim[l], dout = n->dim[l + 1];
+            n->woff.push_back((int64_t)h0.size());
+            // He initialisation for ReLU inputs
+            std::normal_distribution<double> nd(0., std::sqrt(2. / din));
+            for (int64_t i = 0; i < (int64_t)din * dout; i++) h0.push_back((float)nd(rng));
+            n->boff.push_back((int64_t)h0.size());
+            h0.insert(h0.end(), dout, 0.f);
+        }
+        n->nparam = (int64_t)h0.size();
+        n->p.upload(h0, n->stream);
+        n->g.alloc(n->nparam);
+        n->m.alloc(n->nparam);
+        n->v.alloc(n->nparam);
+        n->m.zero(n->stream);
+        n->v.zero(n->stream);
+        HIPCHECK(hipStreamSynchronize(n->stream));
+        *out = hold.release();
+    });
+}
+
+extern "C" void axg_mlp_destroy(axg_mlp *n) {
+    if (!n) return;
+    (void)hipSetDevice(n->device);
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    n->act.clear();
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+}
+
+extern "C" int64_t axg_mlp_nparams(const axg_mlp *n) { return n ? n->nparam : 0; }
+
+extern "C" int axg_mlp_get_params(axg_mlp *n, float *out) {
+    if (!n || !out) return fail(AXG_EINVAL, "axg_mlp_get_params || null argument.");
+    MLP_GUARD({
+        HIPCHECK(hipSetDevice(n->device));
+        HIPCHECK(hipMemcpyAsync(out, n->p.p, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, n->stream));
+        HIPCHECK(hipStreamSynchronize(n->stream));
+    });
+}
+
+extern "C" int axg_mlp_set_params(axg_mlp *n, const float *in) {
+    if (!n || !in) return fail(AXG_EINVAL, "axg_mlp_set_params || null argument.");
+    MLP_GUARD({
+        HIPCHECK(hipSetDevice(n->device));
+        HIPCHECK(hipMemcpyAsync(n->p.p, in, sizeof(float) * n->nparam, hipMemcpyHostToDevice, n->stream));
+        n->m.zero(n->stream);
+        n->v.zero(n->stream);
+        n->step = 0;
+        HIPCHECK(hipStreamSynchronize(n->stream));
+    });
+}
+
+extern "C" int axg_mlp_set_data(axg_mlp *n, const float *x, const float *y, int64_t rows) {
+    if (!n || !x || !y || rows < 1 || rows > INT32_MAX) return fail(AXG_EINVAL, "axg_mlp_set_data || invalid argument.");
+    MLP_GUARD({
+        HIPCHECK(hipSetDevice(n->device));
+        n->xs.upload(x, (size_t)rows * n->dim[0], n->stream);
+        n->ys.upload(y, (size_t)rows, n->stream);
+        n->idx.alloc((size_t)rows);
+        n->nset = rows;
+        HIPCHECK(hipStreamSynchronize(n->stream));
+    });
+}
+
+extern "C" int axg_mlp_train(axg_mlp *n, int32_t epochs, int32_t batch, float lr, float under_weight,
+                             uint64_t seed) {
+    if (!n || epochs < 0 || batch < 1 || !(lr > 0.f) || !(under_weight > 0.f))
+        return fail(AXG_EINVAL, "axg_mlp_train || invalid argument.");
+    if (n->nset == 0) return fail(AXG_EINVAL, "axg_mlp_train || no training data (axg_mlp_set_data).");
+    MLP_GUARD({
+        HIPCHECK(hipSetDevice(n->device));
+        ensureRows(n, batch);
+        std::mt19937_64 rng(seed);
+        std::vector<int32_t> perm(n->nset);
+        std::iota(perm.begin(), perm.end(), 0);
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const int nin = n->dim[0];
+        for (int ep = 0; ep < epochs; ep++) {
+            std::shuffle(perm.begin(), perm.end(), rng);
+            HIPCHECK(hipMemcpyAsync(n->idx.p, perm.data(), sizeof(int32_t) * n->nset, hipMemcpyHostToDevice,
+                                    n->stream));
+            for (int64_t b0 = 0; b0 < n->nset; b0 += batch) {
+                const int B = (int)std::min<int64_t>(batch, n->nset - b0);
+                const int64_t tot = (int64_t)B * nin;
+                hipLaunchKernelGGL(mlp_gather_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n->stream, B,
+                                   nin, n->idx.p + b0, n->xs.p, n->ys.p, n->act[0].p, n->yb.p);
+                forward(n, B);
+                backward(n, B, under_weight);
+                n->step++;
+                const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
+                hipLaunchKernelGGL(mlp_adam_kernel, dim3((unsigned)((n->nparam + 255) / 256)), dim3(256), 0,
+                                   n->stream, n->nparam, n->p.p, n->g.p, n->m.p, n->v.p, lr, b1, b2, eps, c1, c2);
+            }
+            // the permutation buffer is rewritten next epoch
+            HIPCHECK(hipStreamSynchronize(n->stream));
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(n->stream));
+    });
+}
+
+extern "C" int axg_mlp_predict(axg_mlp *n, const float *x, int64_t rows, float *y) {
+    if (!n || !x || !y || rows < 0) return fail(AXG_EINVAL, "axg_mlp_predict || invalid argument.");
+    MLP_GUARD({
+        HIPCHECK(hipSetDevice(n->device));
+        const int chunk = 65536;
+        for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+            const int B = (int)std::min<int64_t>(chunk, rows - r0);
+            loadRows(n, x + r0 * n->dim[0], nullptr, B);
+            forward(n, B);
+            HIPCHECK(hipMemcpyAsync(y + r0, n->z.p, sizeof(float) * B, hipMemcpyDeviceToHost, n->stream));
+            HIPCHECK(hipStreamSynchronize(n->stream));
+        }
+        HIPCHECK(hipGetLastError());
+    });
+}
+
+extern "C" int axg_mlp_gradient(axg_mlp *n, const float *x, const float *y, int32_t rows, float under_weight,
+                                float *grad) {
+    if (!n || !x || !y || !grad || rows < 1 || !(under_weight > 0.f))
+        return fail(AXG_EINVAL, "axg_mlp_gradient || invalid argument.");
+    MLP_GUARD({
+        HIPCHECK(hipSetDevice(n->device));
+        loadRows(n, x, y, rows);
+        forward(n, rows);
+        backward(n, rows, under_weight);
+        HIPCHECK(hipMemcpyAsync(grad, n->g.p, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, n->stream));
+        HIPCHECK(hipStreamSynchronize(n->stream));
+        HIPCHECK(hipGetLastError());
+    });
+}

@@ -375,6 +375,7 @@ struct axg_mesh {
     int64_t nsolid_elem = 0, nfluid_elem = 0, n3d_elem = 0, max_nr = 0, sum_nu1 = 0;
     double dt = 0.;
     double t_shift = 0.;
+    std::vector<float> point_feat;  // [npoint][AXG_NFEAT]
 
     void bind() {
         axg_mesh_desc &d = desc;
@@ -681,6 +682,12 @@ struct Builder {
             for (auto &v : elemNu) {
                 int nu = (int)std::lround(ln(rng));
                 v = std::max(c.nu_min, std::min(c.nu_max, nu));
+            }
+        } else if (c.nu_mode == 3) {
+            if (!c.elem_nu) throw std::runtime_error("NrField || nu_mode 3 needs elem_nu.");
+            for (size_t e = 0; e < elemNu.size(); e++) {
+                if (c.elem_nu[e] < 0) throw std::runtime_error("NrField || negative Nu in elem_nu.");
+                elemNu[e] = c.elem_nu[e];
             }
         }
         // Nu of a point = max over the elements sharing it; Nr = 2 Nu + 1 (ConstNrField.cpp:13-16)
@@ -1445,6 +1452,51 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     m.desc.src_elem = srcElem;
     m.desc.att_do_kappa = c.do_kappa;
     m.desc.rec_components = c.rec_components;
+
+    // ---------------- features of the learned Nu predictor ----------------
+    // radius, colatitude, period, and the azimuthal spectrum of the model
+    // perturbation that the 3-D path samples at the point (same dvs / dvp
+    // formulas as the material sampling above)
+    constexpr int kNPhi = 64, kNK = AXG_NFEAT - 3;
+    m.point_feat.assign((size_t)B.npoint * AXG_NFEAT, 0.f);
+    const double Rout = B.rnode.back();
+    std::vector<double> ck((size_t)kNK * kNPhi), sk((size_t)kNK * kNPhi);
+    for (int k = 0; k < kNK; k++)
+        for (int n = 0; n < kNPhi; n++) {
+            ck[k * kNPhi + n] = std::cos(2. * pi * k * n / kNPhi);
+            sk[k * kNPhi + n] = std::sin(2. * pi * k * n / kNPhi);
+        }
+    parallelFor(B.npoint, B.nthread, [&](int b, int e) {
+        std::vector<double> dvs(kNPhi), dvp(kNPhi);
+        for (int p = b; p < e; p++) {
+            const double sp = psz[2 * p], zp = psz[2 * p + 1];
+            float *f = &m.point_feat[(size_t)p * AXG_NFEAT];
+            f[0] = (float)(std::hypot(sp, zp) / Rout);
+            f[1] = (float)(std::atan2(sp, zp) / pi);
+            f[2] = (float)(c.period / 100.);
+            if (!threeD) continue;
+            const bool solid = (m.point_flags[p] & AXG_PT_SOLID) != 0;
+            const bool pfluid = (m.point_flags[p] & AXG_PT_FLUID) != 0;
+            const bool withVp = solid || (pfluid && c.dv_fluid);
+            for (int n = 0; n < kNPhi; n++) {
+                const double phi = 2. * pi * n / kNPhi, x = sp * std::cos(phi), y = sp * std::sin(phi);
+                dvs[n] = solid ? c.dv_amp * std::tanh(B.grfVs(x, y, zp) / 2.) : 0.;
+                dvp[n] = withVp ? c.dv_amp * std::tanh(B.grfVp(x, y, zp) / 2.) : 0.;
+            }
+            for (int k = 0; k < kNK; k++) {
+                double a2 = 0.;
+                for (const auto *v : {&dvs, &dvp}) {
+                    double re = 0., im = 0.;
+                    for (int n = 0; n < kNPhi; n++) {
+                        re += (*v)[n] * ck[k * kNPhi + n];
+                        im -= (*v)[n] * sk[k * kNPhi + n];
+                    }
+                    a2 += (re * re + im * im) / ((double)kNPhi * kNPhi);
+                }
+                f[3 + k] = (float)(std::sqrt(a2) / c.dv_amp);
+            }
+        }
+    });
 }
 
 // ===========================================================================
@@ -1594,3 +1646,12 @@ extern "C" int axg_mesh_stats(const axg_mesh *mesh, int64_t *stats, int nstats) 
 }
 
 extern "C" void axg_mesh_free(axg_mesh *mesh) { delete mesh; }
+
+extern "C" int axg_mesh_point_features(const axg_mesh *mesh, float *out) {
+    if (!mesh || !out) {
+        axg::set_last_error("axg_mesh_point_features || null argument.");
+        return AXG_EINVAL;
+    }
+    std::memcpy(out, mesh->point_feat.data(), mesh->point_feat.size() * sizeof(float));
+    return AXG_OK;
+}

@@ -29,13 +29,20 @@ def shell_config(which=0, layers=None, **overrides):
         cfg.nlayer = len(layers)
         cfg.layers = C.cast(arr, C.POINTER(_abi.ShellLayer))
         keep = arr
+    keep_nu = None
     for k, v in overrides.items():
         if k == "moment":
             for i in range(6):
                 cfg.moment[i] = v[i]
+        elif k == "elem_nu":
+            # nu_mode 3: Nu of every element in build order
+            keep_nu = np.ascontiguousarray(v, dtype=np.int32)
+            cfg.elem_nu = keep_nu.ctypes.data_as(C.POINTER(C.c_int32))
+            cfg.nu_mode = 3
         else:
             setattr(cfg, k, v)
     cfg._keep = keep  # keep the layer array alive with the config
+    cfg._keep_nu = keep_nu
     return cfg
 
 
@@ -78,6 +85,16 @@ class ShellMesh:
                 self._h = None
         except Exception:
             pass
+
+    def point_features(self):
+        """[npoint][NFEAT] float32 features of the learned Nu predictor
+        (axg_mesh_point_features)."""
+        from . import lib
+        out = np.zeros((self.d.npoint, _abi.NFEAT), dtype=np.float32)
+        rc = lib().axg_mesh_point_features(self._h, out.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise RuntimeError(lib().axg_last_error().decode())
+        return out
 
     # ---- numpy views of the descriptor ----
     def _view(self, name, n, dtype):

@@ -1,0 +1,91 @@
+"""Learned Nu predictor over the C ABI (axg_mlp_*): a ReLU perceptron whose
+products run on the bf16 matrix cores, trained with Adam on per-point
+(feature, Nu) pairs.  The features are those of ``ShellMesh.point_features``;
+the targets are the Nu that ``Domain.learnWisdom`` (the reference's
+Domain::learnWisdom, Domain.cpp:384-440) reports per point."""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class NuMLP:
+    def __init__(self, nin=_abi.NFEAT, width=256, nhidden=3, seed=1, device=0):
+        from . import lib
+        self._lib = lib()
+        h = C.c_void_p()
+        rc = self._lib.axg_mlp_create(nin, width, nhidden, seed, device, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(self._lib.axg_last_error().decode())
+        self._h = h
+        self.nin, self.width, self.nhidden = nin, width, nhidden
+        self.dims = [nin] + [width] * nhidden + [1]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.axg_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._lib.axg_last_error().decode())
+
+    @property
+    def nparams(self):
+        return int(self._lib.axg_mlp_nparams(self._h))
+
+    def params(self):
+        out = np.zeros(self.nparams, dtype=np.float32)
+        self._check(self._lib.axg_mlp_get_params(self._h, _fp(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self._check(self._lib.axg_mlp_set_params(self._h, _fp(p)))
+
+    def layers(self, p=None):
+        """[(W [dout][din], b [dout])] views of a parameter vector."""
+        p = self.params() if p is None else p
+        out, o = [], 0
+        for din, dout in zip(self.dims[:-1], self.dims[1:]):
+            W = p[o:o + din * dout].reshape(dout, din)
+            o += din * dout
+            b = p[o:o + dout]
+            o += dout
+            out.append((W, b))
+        return out
+
+    def set_data(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+        assert x.shape == (y.size, self.nin)
+        self._check(self._lib.axg_mlp_set_data(self._h, _fp(x), _fp(y), y.size))
+        self._data = (x, y)
+
+    def train(self, epochs, batch=1024, lr=1e-3, under_weight=1.0, seed=1):
+        self._check(self._lib.axg_mlp_train(self._h, epochs, batch, lr, under_weight, seed))
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        self._check(self._lib.axg_mlp_predict(self._h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def gradient(self, x, y, under_weight=1.0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+        g = np.zeros(self.nparams, dtype=np.float32)
+        self._check(self._lib.axg_mlp_gradient(self._h, _fp(x), _fp(y), y.size, under_weight, _fp(g)))
+        return g

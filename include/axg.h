@@ -170,7 +170,8 @@ typedef struct axg_shell_config {
     double dv_corr_len;               /* correlation length, metres               */
     int32_t dv_fluid;                 /* perturb vp in fluid too                   */
     /* Fourier order */
-    int32_t nu_mode;                  /* 0 const, 1 uniform per element, 2 log-normal per element */
+    int32_t nu_mode;                  /* 0 const, 1 uniform per element, 2 log-normal per element,
+                                         3 given per element (elem_nu)             */
     int32_t nu_const, nu_min, nu_max;
     double nu_median, nu_sigma;
     int32_t lucky;                    /* FFTW_LUCKY_NUMBER                         */
@@ -192,6 +193,8 @@ typedef struct axg_shell_config {
     int32_t rec_components;
     uint64_t seed;
     int32_t nthread;                  /* host threads for set-up (0 = all)         */
+    const int32_t *elem_nu;           /* nu_mode 3: Nu of every element in build order
+                                         (the order of a build with the same geometry) */
 } axg_shell_config;
 
 typedef struct axg_mesh axg_mesh;
@@ -201,6 +204,36 @@ int axg_mesh_build_shell(const axg_shell_config *cfg, axg_mesh **mesh);
 const axg_mesh_desc *axg_mesh_get_desc(const axg_mesh *mesh);
 int axg_mesh_stats(const axg_mesh *mesh, int64_t *stats, int nstats);
 void axg_mesh_free(axg_mesh *mesh);
+
+/* Features of every GLL point for the learned Nu predictor (no counterpart in
+ * the reference; they describe what the reference's NuWisdom learns from):
+ * out[npoint][AXG_NFEAT] = radius / outer radius, colatitude / pi, source period
+ * / 100 s, then for k = 0..15 the amplitude of azimuthal order k of the local
+ * model perturbation, sqrt(|dvs_k|^2 + |dvp_k|^2) / dv_amp, sampled on 64
+ * azimuths (0 for a laterally homogeneous model). */
+#define AXG_NFEAT 19
+int axg_mesh_point_features(const axg_mesh *mesh, float *out);
+
+/* ---- learned Nu predictor ------------------------------------------------
+ * No counterpart in the reference: it predicts, from axg_mesh_point_features,
+ * the per-point Nu that the reference's wisdom (Domain::learnWisdom,
+ * Domain.cpp:384-440; NuWisdom.cpp:13-45) produces, for callers that build a
+ * reduced-Nu mesh instead of running the full-Nu solve first.
+ * A ReLU perceptron nin -> width (x nhidden) -> 1 on the bf16 matrix cores with
+ * FP32 accumulation (weights kept in FP32 for Adam, rounded to bf16 at use).
+ * Loss: mean of w (pred - target)^2 with w = under_weight below the target.
+ * All arrays are host arrays; rows are [nin] floats. */
+typedef struct axg_mlp axg_mlp;
+int axg_mlp_create(int32_t nin, int32_t width, int32_t nhidden, uint64_t seed, int device, axg_mlp **mlp);
+void axg_mlp_destroy(axg_mlp *mlp);
+int64_t axg_mlp_nparams(const axg_mlp *mlp);            /* per layer l: W_l [d_{l+1}][d_l], then b_l */
+int axg_mlp_get_params(axg_mlp *mlp, float *params);
+int axg_mlp_set_params(axg_mlp *mlp, const float *params); /* also resets the Adam state */
+int axg_mlp_set_data(axg_mlp *mlp, const float *x, const float *y, int64_t rows); /* training set -> HBM */
+int axg_mlp_train(axg_mlp *mlp, int32_t epochs, int32_t batch, float lr, float under_weight, uint64_t seed);
+int axg_mlp_predict(axg_mlp *mlp, const float *x, int64_t rows, float *y);
+int axg_mlp_gradient(axg_mlp *mlp, const float *x, const float *y, int32_t rows, float under_weight,
+                     float *grad); /* loss gradient over the rows, parameter layout */
 
 /* ===================================================================== *
  *  Domain on one GPU.
