@@ -1,7 +1,7 @@
 """Benchmark of the MI355X time loop: GLL-point x mode updates per second.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload N]
-                    [--no-cpu] [--no-traffic]
+                    [--no-cpu] [--no-traffic] [--no-learned]
 
 One step = one pass of Newmark::solve's loop body (Newmark.cpp:47-93) over the
 whole synthetic mesh resident in HBM: Newmark update, source, element stiffness
@@ -21,6 +21,11 @@ events around each of its launches, on the launch stream).  "traffic" is the
 HBM bytes the family really moved per step, from two rocprofv3 counter passes
 (FETCH_SIZE, WRITE_SIZE) run as child processes over a few eager steps and
 scaled by the same counters' reading on a plain device copy of known size.
+
+"learned_nu" is the metric's second half: a perceptron trained on the Nu
+wisdom of full-Nu runs predicts the Nu of held-out models, which are then
+solved at that Nu and judged against their own full-Nu solve (seismogram
+misfit, wall-clock ratio).
 """
 import argparse
 import csv
@@ -160,6 +165,40 @@ def measure_traffic(workload, family):
             shutil.rmtree(outdir, ignore_errors=True)
 
 
+def learned_nu(n_train, nstep, n_test=2, nu_full=128, cutoff=1e-2):
+    """The second half of the metric: wall-clock to solution at the Nu a
+    trained predictor gives, against the full-Nu solve that judges its
+    seismogram misfit, on held-out seeded models (2000-quad solid + fluid TI
+    shells with a +-5 % random 3-D model).  Training data: the wisdom of
+    full-Nu runs of `n_train` other seeded models."""
+    from axisem3d_amd import nu_learning as nl
+    t0 = time.perf_counter()
+    L = nl.LearnedNu(nu_full=nu_full, nstep=nstep, cutoff=cutoff)
+    for i in range(n_train):
+        L.add_training_model(1000 + i)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mse = L.train(epochs=40)
+    t_train = time.perf_counter() - t0
+    runs = [L.evaluate(5000 + j) for j in range(n_test)]
+    L.mlp.close()
+    return {
+        "models": {"train": n_train, "test": n_test, "quads": 2000, "steps": nstep, "nu_full": nu_full},
+        "wisdom_cutoff": cutoff,
+        "misfit_max": max(r["misfit"] for r in runs),
+        "wall_speedup": sum(r["wall_full_s"] for r in runs) / sum(r["wall_pred_s"] for r in runs),
+        "solve_speedup": sum(r["solve_full_s"] for r in runs) / sum(r["solve_pred_s"] for r in runs),
+        "updates_ratio": sum(r["updates_full"] for r in runs) / sum(r["updates_pred"] for r in runs),
+        "mean_nu_full": runs[0]["mean_nu_full"],
+        "mean_nu_pred": sum(r["mean_nu_pred"] for r in runs) / n_test,
+        "flagged_points": sum(r["flagged_points"] for r in runs),
+        "train_mse": mse,
+        "data_s": t_data,
+        "train_s": t_train,
+        "per_model": runs,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +208,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learn-models", type=int, default=8)
+    ap.add_argument("--learn-steps", type=int, default=1500)
     ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -256,6 +298,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.cpu_seconds)
+    if not args.no_learned:
+        out["learned_nu"] = learned_nu(args.learn_models, args.learn_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,69 @@
+"""Learned-Nu pipeline: element Nu from point predictions (host logic), the
+full-Nu fallback reproducing the judge exactly, and one small train ->
+predict -> solve -> judge round on the GPU."""
+import numpy as np
+import pytest
+
+import axisem3d_amd as ax
+from axisem3d_amd import nu_learning as nl
+
+
+class _Mesh:
+    def __init__(self, ep, npoint):
+        self._ep = np.asarray(ep, dtype=np.int32).ravel()
+        self.d = type("D", (), {"nelem": len(ep), "npoint": npoint})()
+
+    def array(self, name):
+        assert name == "elem_points"
+        return self._ep
+
+
+def test_element_nu_is_max_over_points_with_fallback():
+    ep = [list(range(25)), list(range(20, 45))]
+    m = _Mesh(ep, 45)
+    nu = np.arange(45, dtype=np.int32)
+    flagged = np.zeros(45, dtype=bool)
+    assert nl.element_nu(m, nu, flagged, 100).tolist() == [24, 44]
+    assert nl.element_nu(m, nu, flagged, 30).tolist() == [24, 30]  # clipped to the full Nu
+    flagged[3] = True
+    assert nl.element_nu(m, nu, flagged, 100).tolist() == [100, 44]  # fallback on the flagged element
+    assert nl.element_nu(m, np.zeros(45, np.int32), np.zeros(45, bool), 100).tolist() == [1, 1]
+
+
+def test_nu_mode_3_builds_the_given_element_nu():
+    cfg = nl.model_config(7, 10, nelem_theta=12, nu=16)
+    m = ax.ShellMesh(cfg)
+    enu = np.full(m.d.nelem, 16, dtype=np.int32)
+    m3 = ax.ShellMesh(nl.model_config(7, 10, nelem_theta=12, elem_nu=enu))
+    assert np.array_equal(m3.array("point_nr"), m.array("point_nr"))
+    f1 = ax.ShellMesh(nl.model_config(7, 10, nelem_theta=12, nu=1)).point_features()
+    assert np.array_equal(f1, m.point_features())  # features do not depend on Nu
+    assert f1[:, 3:].max() > 0  # a 3-D model has azimuthal content
+
+
+@pytest.mark.gpu
+def test_full_nu_fallback_reproduces_the_judge():
+    print("case learned-nu fallback", flush=True)
+    m = ax.ShellMesh(nl.model_config(11, 120, nelem_theta=12, nu=24))
+    s_full, _, _ = nl.solve(m)
+    enu = nl.element_nu(m, np.zeros(m.d.npoint, np.int32), np.ones(m.d.npoint, bool), 24)
+    mp = ax.ShellMesh(nl.model_config(11, 120, nelem_theta=12, elem_nu=enu))
+    s_pred, _, _ = nl.solve(mp)
+    assert np.abs(s_full).max() > 0
+    assert np.array_equal(s_pred, s_full)
+
+
+@pytest.mark.gpu
+def test_learned_nu_round_trip():
+    print("case learned-nu round trip", flush=True)
+    L = nl.LearnedNu(nu_full=24, nstep=200, nelem_theta=12, cutoff=1e-2)
+    for i in range(3):
+        L.add_training_model(100 + i)
+    y = np.concatenate(L.y)
+    assert 0 < y.mean() <= 1.01
+    mse = L.train(epochs=20, batch=512)
+    assert mse < y.var()
+    r = L.evaluate(900)
+    print(r, flush=True)
+    assert np.isfinite(r["misfit"]) and r["misfit"] < 0.05
+    assert r["mean_nu_pred"] <= r["mean_nu_full"] + 1e-9
