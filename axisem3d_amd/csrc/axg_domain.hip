@@ -71,6 +71,8 @@ struct Phase {
     std::vector<float> pt_invmass;
     DevBuf<int32_t> d_row_point, d_pt_row, d_pt_nu, d_pt_flags, d_point_of;
     DevBuf<float> d_invmass;
+    DevBuf<int32_t> d_pt_sim;       // batch: simulation of each compact point
+    const float *sim_dt = nullptr;  // batch: [nsim][3] dt factors
     DevBuf<float2> displ, veloc, accel, stiff;
     // Mass3D points: compact index, Nr, first slice of 1/mass in the pool
     std::vector<int32_t> m3_q, m3_nr;
@@ -86,6 +88,8 @@ struct Phase {
         P.pt_nu = d_pt_nu.p;
         P.pt_flags = d_pt_flags.p;
         P.pt_invmass = d_invmass.p;
+        P.pt_sim = d_pt_sim.p;
+        P.sim_dt = sim_dt;
         P.nrow = nrow;
         P.npt = npt;
         return P;
@@ -135,8 +139,13 @@ struct axg_ctx {
     DevBuf<float> d_sf_normal;
     // source
     int32_t src_elem = -1;
+    int32_t nsrc = 0;  // sources (one per simulation of a batch that has one)
     DevBuf<float> d_src_force, d_stf;
     DevBuf<int32_t> d_src_row, d_src_nu;
+    // batch of independent simulations (nsim > 1): each advances with its own dt
+    int32_t nsim = 1;
+    DevBuf<float> d_sim_dt;
+    std::vector<int32_t> sim_point_off, sim_rec_off;
     // receivers
     DevBuf<int32_t> d_rec_elem, d_rec_idx_s, d_rec_idx_f;
     int32_t nrec_solid = 0, nrec_fluid = 0;
@@ -617,11 +626,11 @@ void doNewmark(axg_ctx *c, double dt, int *tstep_counter = nullptr) {
 }
 
 void doSource(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
-    if (c->src_elem < 0) return;
+    if (c->nsrc == 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::applySource || time step out of range.");
     FamScope fs(c, AXG_KF_OTHER);
-    hipLaunchKernelGGL(source_kernel, dim3(1), dim3(256), 0, c->stream, c->d_src_force.p, c->d_src_row.p,
-                       c->d_src_nu.p, c->d_stf.p, tstep, tstep_ptr, c->S.stiff.p);
+    hipLaunchKernelGGL(source_kernel, dim3(c->nsrc), dim3(256), 0, c->stream, c->d_src_force.p, c->d_src_row.p,
+                       c->d_src_nu.p, c->d_stf.p, c->nstep, tstep, tstep_ptr, c->S.stiff.p);
 }
 
 void doStiff(axg_ctx *c) {
@@ -691,7 +700,18 @@ int64_t doStability(axg_ctx *c) {
     return bad == INT32_MAX ? -1 : bad;
 }
 
-void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
+// A batch of independent simulations merged into one descriptor: the
+// simulation of every point, each simulation's dt, source and STF.
+struct BatchInfo {
+    int32_t nsim = 0;
+    std::vector<int32_t> point_sim, point_off, rec_off;
+    std::vector<double> dt;
+    std::vector<int32_t> src_elem;  // merged element index, -1: none
+    std::vector<float> src_force;   // [nsim][450]
+    std::vector<float> stf;         // [nsim][nstep]
+};
+
+void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *bi = nullptr) {
     if (!d || d->npoint <= 0 || d->nelem <= 0) throw std::runtime_error("axg_create || empty mesh descriptor.");
     c->device = device;
     HIPCHECK(hipSetDevice(device));
@@ -723,6 +743,26 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
         for (int nr : P->m3_nr) c->pf_max_nr = std::max(c->pf_max_nr, nr);
     uploadPhase(c->S, s);
     uploadPhase(c->F, s);
+    if (bi) {
+        c->nsim = bi->nsim;
+        c->sim_point_off = bi->point_off;
+        c->sim_rec_off = bi->rec_off;
+        std::vector<float> f;
+        for (double dt : bi->dt) {
+            // the same float factors as the one-simulation path (doNewmark)
+            const double h = 0.5 * dt;
+            f.push_back((float)dt);
+            f.push_back((float)h);
+            f.push_back((float)(h * dt));
+        }
+        c->d_sim_dt.upload(f, s);
+        for (Phase *P : {&c->S, &c->F}) {
+            std::vector<int32_t> ps(P->npt);
+            for (int q = 0; q < P->npt; q++) ps[q] = bi->point_sim[P->point_of[q]];
+            P->d_pt_sim.upload(ps, s);
+            P->sim_dt = c->d_sim_dt.p;
+        }
+    }
 
     // ---- elements ----
     const int NE = d->nelem;
@@ -922,22 +962,47 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
     }
 
     // ---- source ----
-    c->src_elem = d->src_elem;
-    if (c->src_elem >= 0) {
-        if (c->src_elem >= NE || (d->elem_flags[c->src_elem] & AXG_EL_FLUID))
-            throw std::runtime_error("Source::release || source element must be a solid element.");
-        std::vector<int32_t> srow(25), snu(25);
-        for (int k = 0; k < 25; k++) {
-            int p = d->elem_points[(size_t)c->src_elem * 25 + k];
-            srow[k] = c->S.pt_row[c->S.compact[p]];
-            snu[k] = std::min(2, d->point_nr[p] / 2);
+    {
+        // one source per simulation (SourceTerm of its element), STFs side by side
+        std::vector<int32_t> selem;
+        std::vector<const float *> sforce, sstf;
+        if (bi) {
+            for (int k = 0; k < bi->nsim; k++)
+                if (bi->src_elem[k] >= 0) {
+                    selem.push_back(bi->src_elem[k]);
+                    sforce.push_back(bi->src_force.data() + (size_t)450 * k);
+                    sstf.push_back(bi->stf.data() + (size_t)d->nstep * k);
+                }
+        } else if (d->src_elem >= 0) {
+            selem.push_back(d->src_elem);
+            sforce.push_back(d->src_force);
+            sstf.push_back(d->stf);
         }
-        c->d_src_row.upload(srow, s);
-        c->d_src_nu.upload(snu, s);
-        c->d_src_force.upload(d->src_force, 450, s);
+        c->src_elem = selem.empty() ? -1 : selem[0];
+        c->nsrc = (int32_t)selem.size();
+        std::vector<int32_t> srow, snu;
+        std::vector<float> force, stf;
+        for (size_t k = 0; k < selem.size(); k++) {
+            const int e = selem[k];
+            if (e >= NE || (d->elem_flags[e] & AXG_EL_FLUID))
+                throw std::runtime_error("Source::release || source element must be a solid element.");
+            for (int j = 0; j < 25; j++) {
+                int p = d->elem_points[(size_t)e * 25 + j];
+                srow.push_back(c->S.pt_row[c->S.compact[p]]);
+                snu.push_back(std::min(2, d->point_nr[p] / 2));
+            }
+            force.insert(force.end(), sforce[k], sforce[k] + 450);
+            stf.insert(stf.end(), sstf[k], sstf[k] + d->nstep);
+        }
+        if (stf.empty()) stf.assign((size_t)std::max(1, d->nstep), 0.f);
+        if (c->nsrc) {
+            c->d_src_row.upload(srow, s);
+            c->d_src_nu.upload(snu, s);
+            c->d_src_force.upload(force, s);
+        }
+        c->d_stf.upload(stf, s);
         HIPCHECK(hipStreamSynchronize(s));
     }
-    c->d_stf.upload(d->stf, (size_t)std::max(1, d->nstep), s);
 
     // ---- receivers ----
     c->nrec = d->nrec;
@@ -1039,7 +1104,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c) {
         c->fam_bytes[AXG_KF_NEWMARK] = bnm;
         // source (25 x 3 x 3 complex RMW), coupling (per row: two solid displacement
         // reads, fluid and two solid stiffness RMW), receivers (25 points' rows)
-        double other = c->src_elem >= 0 ? 225. * 16. : 0.;
+        double other = 225. * 16. * c->nsrc;
         other += 64. * (double)c->nsf_row;
         for (int r = 0; r < c->nrec; r++) {
             int e = d->rec_elem[r];
@@ -1178,6 +1243,141 @@ void runSteps(axg_ctx *c, int first, int count, int stab, bool profile) {
 }  // namespace
 
 // ===================================================================== C ABI
+namespace {
+
+// storage of a merged descriptor
+struct MergedDesc {
+    axg_mesh_desc d{};
+    std::vector<int32_t> point_nr, point_flags, elem_points, elem_flags, elem_material, elem_att, sf_point, rec_elem;
+    std::vector<double> point_sz;
+    std::vector<int64_t> mass_solid_off, mass_fluid_off, mat_off, att_off, sf_off;
+    std::vector<float> inv_mass_solid, inv_mass_fluid, mat_pool, elem_geom, elem_tiso, att_pool, sf_normal,
+        rec_weights, rec_phi, rec_theta;
+};
+
+template <class T>
+void append(std::vector<T> &v, const T *p, size_t n) {
+    if (n) v.insert(v.end(), p, p + n);
+}
+
+// append offsets[0..n] of one simulation, shifted so that they continue `off`
+void appendOffsets(std::vector<int64_t> &off, const int64_t *o, size_t n) {
+    const int64_t base = off.empty() ? 0 : off.back();
+    if (!off.empty()) off.pop_back();
+    for (size_t i = 0; i <= n; i++) off.push_back(base + o[i] - o[0]);
+}
+
+// The simulations of a batch as one mesh of disjoint parts (indices shifted),
+// with what stays per simulation in `bi`: dt, source, STF, point ranges.
+void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo &bi) {
+    const axg_mesh_desc &d0 = *ds[0];
+    bi.nsim = n;
+    bi.point_off.push_back(0);
+    bi.rec_off.push_back(0);
+    int32_t npoint = 0, nelem = 0, nrec = 0, nsf = 0;
+    const axg_mesh_desc *att0 = nullptr;
+    for (int k = 0; k < n; k++) {
+        const axg_mesh_desc &d = *ds[k];
+        if (d.npoint <= 0 || d.nelem <= 0) throw std::runtime_error("axg_create_batch || empty mesh descriptor.");
+        if (d.nstep != d0.nstep)
+            throw std::runtime_error("axg_create_batch || the simulations of a batch share the number of steps.");
+        if (d.nrec > 0 && d0.nrec > 0 && d.rec_components != d0.rec_components)
+            throw std::runtime_error("axg_create_batch || the simulations of a batch share the station components.");
+        bool att = false;
+        for (int e = 0; e < d.nelem; e++) att = att || d.elem_att[e] != AXG_ATT_NONE;
+        if (att) {
+            // the SLS coefficients depend on dt and are shared by the context
+            if (att0 && (d.nsls != att0->nsls || d.att_do_kappa != att0->att_do_kappa ||
+                         std::memcmp(d.sls_alpha, att0->sls_alpha, sizeof(float) * d.nsls) ||
+                         std::memcmp(d.sls_beta, att0->sls_beta, sizeof(float) * d.nsls) ||
+                         std::memcmp(d.sls_gamma, att0->sls_gamma, sizeof(float) * d.nsls)))
+                throw std::runtime_error("axg_create_batch || attenuating simulations of a batch need the same "
+                                         "standard linear solids (not supported by this build).");
+            att0 = &d;
+        }
+        const size_t np = d.npoint, ne = d.nelem;
+        append(m.point_nr, d.point_nr, np);
+        append(m.point_flags, d.point_flags, np);
+        append(m.point_sz, d.point_sz, 2 * np);
+        appendOffsets(m.mass_solid_off, d.mass_solid_off, np);
+        append(m.inv_mass_solid, d.inv_mass_solid + d.mass_solid_off[0], (size_t)(d.mass_solid_off[np] - d.mass_solid_off[0]));
+        appendOffsets(m.mass_fluid_off, d.mass_fluid_off, np);
+        append(m.inv_mass_fluid, d.inv_mass_fluid + d.mass_fluid_off[0], (size_t)(d.mass_fluid_off[np] - d.mass_fluid_off[0]));
+        for (size_t i = 0; i < 25 * ne; i++) m.elem_points.push_back(d.elem_points[i] + npoint);
+        append(m.elem_flags, d.elem_flags, ne);
+        append(m.elem_material, d.elem_material, ne);
+        appendOffsets(m.mat_off, d.mat_off, ne);
+        append(m.mat_pool, d.mat_pool + d.mat_off[0], (size_t)(d.mat_off[ne] - d.mat_off[0]));
+        append(m.elem_geom, d.elem_geom, 125 * ne);
+        append(m.elem_tiso, d.elem_tiso, 100 * ne);
+        append(m.elem_att, d.elem_att, ne);
+        appendOffsets(m.att_off, d.att_off, ne);
+        append(m.att_pool, d.att_pool + d.att_off[0], (size_t)(d.att_off[ne] - d.att_off[0]));
+        for (int q = 0; q < d.nsf; q++) m.sf_point.push_back(d.sf_point[q] + npoint);
+        if (d.nsf) {
+            appendOffsets(m.sf_off, d.sf_off, d.nsf);
+            append(m.sf_normal, d.sf_normal + 6 * d.sf_off[0], (size_t)(6 * (d.sf_off[d.nsf] - d.sf_off[0])));
+        }
+        for (int r = 0; r < d.nrec; r++) m.rec_elem.push_back(d.rec_elem[r] + nelem);
+        append(m.rec_weights, d.rec_weights, 25 * (size_t)d.nrec);
+        append(m.rec_phi, d.rec_phi, d.nrec);
+        append(m.rec_theta, d.rec_theta, d.nrec);
+        bi.point_sim.insert(bi.point_sim.end(), np, k);
+        bi.dt.push_back(d.dt);
+        bi.src_elem.push_back(d.src_elem >= 0 ? d.src_elem + nelem : -1);
+        if (d.src_elem >= 0) append(bi.src_force, d.src_force, 450);
+        else bi.src_force.insert(bi.src_force.end(), 450, 0.f);
+        if (d.nstep > 0) append(bi.stf, d.stf, d.nstep);
+        npoint += d.npoint;
+        nelem += d.nelem;
+        nrec += d.nrec;
+        nsf += d.nsf;
+        bi.point_off.push_back(npoint);
+        bi.rec_off.push_back(nrec);
+    }
+    if (m.sf_off.empty()) m.sf_off.push_back(0);
+    axg_mesh_desc &d = m.d;
+    d = d0;
+    d.npoint = npoint;
+    d.point_nr = m.point_nr.data();
+    d.point_flags = m.point_flags.data();
+    d.point_sz = m.point_sz.data();
+    d.mass_solid_off = m.mass_solid_off.data();
+    d.inv_mass_solid = m.inv_mass_solid.data();
+    d.mass_fluid_off = m.mass_fluid_off.data();
+    d.inv_mass_fluid = m.inv_mass_fluid.data();
+    d.nelem = nelem;
+    d.elem_points = m.elem_points.data();
+    d.elem_flags = m.elem_flags.data();
+    d.elem_material = m.elem_material.data();
+    d.mat_off = m.mat_off.data();
+    d.mat_pool = m.mat_pool.data();
+    d.elem_geom = m.elem_geom.data();
+    d.elem_tiso = m.elem_tiso.data();
+    d.elem_att = m.elem_att.data();
+    d.att_off = m.att_off.data();
+    d.att_pool = m.att_pool.data();
+    if (att0) {
+        d.nsls = att0->nsls;
+        d.sls_alpha = att0->sls_alpha;
+        d.sls_beta = att0->sls_beta;
+        d.sls_gamma = att0->sls_gamma;
+        d.att_do_kappa = att0->att_do_kappa;
+    }
+    d.nsf = nsf;
+    d.sf_point = m.sf_point.data();
+    d.sf_off = m.sf_off.data();
+    d.sf_normal = m.sf_normal.data();
+    d.src_elem = -1;  // per simulation in bi
+    d.nrec = nrec;
+    d.rec_elem = m.rec_elem.data();
+    d.rec_weights = m.rec_weights.data();
+    d.rec_phi = m.rec_phi.data();
+    d.rec_theta = m.rec_theta.data();
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t axg_field_size(const axg_mesh_desc *d, int32_t which) {
@@ -1205,6 +1405,39 @@ int axg_create(const axg_mesh_desc *desc, int device, axg_ctx **ctx) {
         return fail(uns ? AXG_EUNSUPPORTED : AXG_EINVAL, m);
     }
     *ctx = c.release();
+    return AXG_OK;
+}
+
+int axg_create_batch(const axg_mesh_desc *const *descs, int32_t nsim, int device, axg_ctx **ctx) {
+    if (!ctx) return fail(AXG_EINVAL, "axg_create_batch || null context pointer.");
+    *ctx = nullptr;
+    if (!descs || nsim < 1) return fail(AXG_EINVAL, "axg_create_batch || no simulations.");
+    for (int k = 0; k < nsim; k++)
+        if (!descs[k]) return fail(AXG_EINVAL, "axg_create_batch || null mesh descriptor.");
+    std::unique_ptr<axg_ctx> c(new axg_ctx());
+    try {
+        MergedDesc m;
+        BatchInfo bi;
+        mergeDescs(descs, nsim, m, bi);
+        createCtx(&m.d, device, c.get(), &bi);
+    } catch (const HipError &e) {
+        return fail(AXG_EHIP, e.what());
+    } catch (const std::exception &e) {
+        std::string msg = e.what();
+        bool uns = msg.find("not supported") != std::string::npos;
+        return fail(uns ? AXG_EUNSUPPORTED : AXG_EINVAL, msg);
+    }
+    *ctx = c.release();
+    return AXG_OK;
+}
+
+int axg_batch_layout(axg_ctx *c, int32_t *point_off, int32_t *rec_off, int32_t n) {
+    if (!c) return fail(AXG_EINVAL, "axg_batch_layout || null context.");
+    if (n < c->nsim + 1) return fail(AXG_EINVAL, "axg_batch_layout || arrays need nsim + 1 entries.");
+    for (int k = 0; k <= c->nsim; k++) {
+        point_off[k] = c->sim_point_off.empty() ? (k ? c->npoint : 0) : c->sim_point_off[k];
+        rec_off[k] = c->sim_rec_off.empty() ? (k ? c->nrec : 0) : c->sim_rec_off[k];
+    }
     return AXG_OK;
 }
 
@@ -1244,6 +1477,8 @@ int axg_reset_zero(axg_ctx *c) {
 
 int axg_update_newmark(axg_ctx *c, double dt) {
     CTX_CHECK(c);
+    if (c->nsim > 1 && dt != 0.)
+        return fail(AXG_EINVAL, "Newmark::solve || every simulation of a batch advances with its own dt; pass dt = 0.");
     API_GUARD({
         doNewmark(c, dt);
         HIPCHECK(hipGetLastError());

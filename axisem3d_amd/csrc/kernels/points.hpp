@@ -12,9 +12,21 @@ struct PhasePoints {
     const int32_t *pt_nu;       // [npt]
     const int32_t *pt_flags;    // [npt] bit0 axial, bit1 nyquist (Nr even), bit2 fluid surface
     const float *pt_invmass;    // [npt] Mass1D
+    const int32_t *pt_sim;      // [npt] simulation of the point in a batch (null: one simulation)
+    const float *sim_dt;        // [nsim][3] dt, dt/2, dt^2/2 of each simulation of a batch
     int32_t nrow;
     int32_t npt;
 };
+
+// time-step factors of point p: the simulation's own in a batch
+__device__ __forceinline__ void point_dt(const PhasePoints &P, int p, float &dt, float &half_dt, float &half_dt_dt) {
+    if (P.pt_sim) {
+        const float *f = P.sim_dt + 3 * P.pt_sim[p];
+        dt = f[0];
+        half_dt = f[1];
+        half_dt_dt = f[2];
+    }
+}
 
 // SolidPoint::maskField (SOLVER/src/core/point/SolidPoint.cpp:216-238) on row alpha
 __device__ __forceinline__ void mask_solid_row(float2 s[3], int alpha, int nu, int flags) {
@@ -55,6 +67,7 @@ __global__ __launch_bounds__(256) void newmark_solid_kernel(PhasePoints P, float
     int nu = P.pt_nu[p];
     int flags = P.pt_flags[p];
     float im = P.pt_invmass[p];
+    point_dt(P, p, dt, half_dt, half_dt_dt);
     int64_t b = (int64_t)r * 3;
     float2 s[3];
 #pragma unroll
@@ -97,6 +110,7 @@ __global__ __launch_bounds__(256) void newmark_fluid_kernel(PhasePoints P, float
         displ[r] = veloc[r] = accel[r] = stiff[r] = czero();
         return;
     }
+    point_dt(P, p, dt, half_dt, half_dt_dt);
     float2 s = stiff[r];
     bool zero = ((flags & 1) && alpha >= 1) || ((flags & 2) && alpha == nu);
     if (alpha == 0) s.y = 0.f;
@@ -117,12 +131,18 @@ __global__ __launch_bounds__(256) void newmark_fluid_kernel(PhasePoints P, float
 }
 
 // SourceTerm::apply + Element::addSourceTerm (SourceTerm.cpp:29-35, Element.cpp:31-35)
+// one block per source (a batch of simulations has one each): source b reads
+// force + 450 b, src_row / src_nu + 25 b and stf + nstep b
 __global__ void source_kernel(const float *__restrict__ force, const int32_t *__restrict__ src_row,
-                              const int32_t *__restrict__ src_nu, const float *__restrict__ stf, int tstep,
-                              const int *__restrict__ tstep_ptr, float2 *__restrict__ stiff) {
+                              const int32_t *__restrict__ src_nu, const float *__restrict__ stf, int32_t nstep,
+                              int tstep, const int *__restrict__ tstep_ptr, float2 *__restrict__ stiff) {
     int t = threadIdx.x;  // 25 points x 3 rows x 3 components
     if (t >= 225) return;
     if (tstep_ptr) tstep = *tstep_ptr;
+    force += 450 * blockIdx.x;
+    src_row += 25 * blockIdx.x;
+    src_nu += 25 * blockIdx.x;
+    stf += (int64_t)nstep * blockIdx.x;
     int k = t / 9, a = (t / 3) % 3, c = t % 3;
     if (a > src_nu[k]) return;
     float f = stf[tstep];

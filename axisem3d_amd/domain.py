@@ -147,6 +147,46 @@ class Domain:
         _check(self._lib.axg_set_field(self._h, which, a.ctypes.data_as(_abi.f32p), a.size))
 
 
+class DomainBatch(Domain):
+    """Many independent simulations in one context (axg_create_batch): one
+    launch per kernel and step covers all of them.  ``run`` steps every
+    simulation with its own dt; ``seismograms(k)`` / ``wisdom(k)`` are those
+    of simulation k."""
+
+    def __init__(self, meshes, device=0):
+        from . import lib
+        self._lib = lib()
+        self.meshes = list(meshes)  # keep the descriptors alive
+        arr = (C.POINTER(_abi.MeshDesc) * len(self.meshes))(*[m.desc for m in self.meshes])
+        h = C.c_void_p()
+        _check(self._lib.axg_create_batch(arr, len(self.meshes), device, C.byref(h)))
+        self._h = h
+        n = len(self.meshes)
+        po, ro = (C.c_int32 * (n + 1))(), (C.c_int32 * (n + 1))()
+        _check(self._lib.axg_batch_layout(h, po, ro, n + 1))
+        self.point_off, self.rec_off = list(po), list(ro)
+        self.nsim = n
+        self.nstep = self.meshes[0].d.nstep
+        self.nrec = self.rec_off[-1]
+        self.dt, self.t_shift = 0.0, self.meshes[0].d.t_shift
+        self.mesh = None
+
+    def updateNewmark(self, dt=0.0):
+        """Every simulation with its own dt (dt must be 0)."""
+        super().updateNewmark(dt)
+
+    def seismograms(self, k=None):
+        s = super().seismograms()
+        if k is None:
+            return s
+        return s[:, self.rec_off[k]:self.rec_off[k + 1], :]
+
+    def wisdom(self, k=None):
+        out = np.zeros(self.point_off[-1], dtype=np.int32)
+        _check(self._lib.axg_get_wisdom(self._h, out.ctypes.data_as(_abi.i32p)))
+        return out if k is None else out[self.point_off[k]:self.point_off[k + 1]]
+
+
 class Newmark:
     """Newmark(domain, reportInterval, checkStabInterval).solve(verbose)
     (SOLVER/src/core/newmark/Newmark.cpp:12-107)."""

@@ -242,6 +242,17 @@ typedef struct axg_ctx axg_ctx;
 
 /* create: copy the descriptor into HBM (Mesh::release -> Domain) */
 int axg_create(const axg_mesh_desc *desc, int device, axg_ctx **ctx);
+
+/* Many independent simulations in ONE context (no counterpart in the
+ * reference, which runs one simulation per mpirun): the meshes are laid side
+ * by side in HBM and every launch of a step covers all of them, so that small
+ * simulations fill the GPU together.  Each simulation keeps its own dt, source
+ * and STF; they share the number of steps (and, if attenuating, the standard
+ * linear solids).  Per-call axg_update_newmark takes dt = 0 (each simulation's
+ * own); axg_run steps all of them.  Seismograms are [nstep][nrec_total][3] and
+ * points / stations of simulation k start at point_off[k] / rec_off[k]. */
+int axg_create_batch(const axg_mesh_desc *const *descs, int32_t nsim, int device, axg_ctx **ctx);
+int axg_batch_layout(axg_ctx *ctx, int32_t *point_off, int32_t *rec_off, int32_t n); /* n >= nsim + 1 */
 int axg_destroy(axg_ctx *ctx);
 
 /* Domain::resetZero  (Domain.cpp:67-74) */
