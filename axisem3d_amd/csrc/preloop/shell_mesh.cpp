@@ -1604,7 +1604,7 @@ extern "C" int axg_shell_config_default(int which, axg_shell_config *cfg) {
             c.attenuation = AXG_ATT_FULL;
             c.nstep = 3000;
             break;
-        case 3:  // one member of the batched set: 2000 quads
+        case 3:  // one member of the batched set: 5000 quads (2000 at nelem_theta = 20)
             c.nelem_theta = 50;
             c.material = AXG_MAT_TRANSVERSELY_ISO;
             c.dv_amp = 0.05;

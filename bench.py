@@ -1,7 +1,7 @@
 """Benchmark of the MI355X time loop: GLL-point x mode updates per second.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload N]
-                    [--no-cpu] [--no-traffic] [--no-learned]
+                    [--no-cpu] [--no-traffic] [--no-batch] [--no-learned]
 
 One step = one pass of Newmark::solve's loop body (Newmark.cpp:47-93) over the
 whole synthetic mesh resident in HBM: Newmark update, source, element stiffness
@@ -22,7 +22,9 @@ HBM bytes the family really moved per step, from two rocprofv3 counter passes
 (FETCH_SIZE, WRITE_SIZE) run as child processes over a few eager steps and
 scaled by the same counters' reading on a plain device copy of known size.
 
-"learned_nu" is the metric's second half: a perceptron trained on the Nu
+"batched" runs many small independent simulations in one context against the
+same simulations one after the other (aggregate updates/s, per-solve
+completion times).  "learned_nu" is the metric's second half: a perceptron trained on the Nu
 wisdom of full-Nu runs predicts the Nu of held-out models, which are then
 solved at that Nu and judged against their own full-Nu solve (seismogram
 misfit, wall-clock ratio).
@@ -165,6 +167,60 @@ def measure_traffic(workload, family):
             shutil.rmtree(outdir, ignore_errors=True)
 
 
+def batched(nsim, steps, warmup=10):
+    """Many small independent simulations (config-3 family: 2000 to 5000
+    quads, Nu per element in [4, 200], random moment tensor and source depth
+    in 10..600 km, isotropic or transversely isotropic 3-D models) run in ONE
+    context against the same simulations run one after the other.  Per-solve
+    completion times are for the full 2000-step solves, from the measured
+    per-step times."""
+    import numpy as np
+    import axisem3d_amd as ax
+    from axisem3d_amd import _abi
+    rng = np.random.default_rng(2024)
+    meshes = []
+    t0 = time.perf_counter()
+    for k in range(nsim):
+        cfg = ax.shell_config(3, nelem_theta=int(rng.choice([20, 30, 40, 50])),
+                              material=_abi.MAT_ISOTROPIC if k % 2 == 0 else _abi.MAT_TRANSVERSELY_ISO,
+                              src_depth=float(rng.uniform(10e3, 600e3)), seed=7000 + k,
+                              nstep=warmup + steps)
+        meshes.append(ax.ShellMesh(cfg))
+    t_mesh = time.perf_counter() - t0
+    upd = [float(m.stats["sum_nu1"]) for m in meshes]
+    single_ms = []
+    for m in meshes:
+        d = ax.Domain(m)
+        d.resetZero()
+        d.run(1, warmup, 0, use_graph=True)
+        d.run(warmup + 1, steps, 0, use_graph=True)
+        single_ms.append(d.last_run_ms() / steps)
+        d.close()
+    b = ax.DomainBatch(meshes)
+    b.resetZero()
+    b.run(1, warmup, 0, use_graph=True)
+    b.run(warmup + 1, steps, 0, use_graph=True)
+    batch_ms = b.last_run_ms() / steps
+    b.close()
+    full = 2000  # steps of each solve
+    serial_done = np.cumsum(single_ms) * full * 1e-3
+    batch_done = np.full(nsim, batch_ms * full * 1e-3)
+    pct = lambda a: {"p50": float(np.percentile(a, 50)), "p90": float(np.percentile(a, 90)),
+                     "p99": float(np.percentile(a, 99))}
+    return {
+        "simulations": nsim,
+        "quads": [int(m.d.nelem) for m in meshes],
+        "updates_per_step": sum(upd),
+        "batched_updates_per_s": sum(upd) / (batch_ms * 1e-3),
+        "serial_updates_per_s": sum(upd) / (sum(single_ms) * 1e-3),
+        "batched_ms_per_step": batch_ms,
+        "serial_ms_per_step": sum(single_ms),
+        "completion_s_batched": pct(batch_done),
+        "completion_s_serial": pct(serial_done),
+        "mesh_build_s": t_mesh,
+    }
+
+
 def learned_nu(n_train, nstep, n_test=2, nu_full=128, cutoff=1e-2):
     """The second half of the metric: wall-clock to solution at the Nu a
     trained predictor gives, against the full-Nu solve that judges its
@@ -209,6 +265,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-sims", type=int, default=16)
+    ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--learn-models", type=int, default=8)
     ap.add_argument("--learn-steps", type=int, default=1500)
     ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
@@ -298,6 +357,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.cpu_seconds)
+    if not args.no_batch:
+        out["batched"] = batched(args.batch_sims, args.batch_steps)
     if not args.no_learned:
         out["learned_nu"] = learned_nu(args.learn_models, args.learn_steps)
     print(json.dumps(out), flush=True)
