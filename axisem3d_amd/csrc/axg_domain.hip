@@ -54,13 +54,19 @@ struct Bucket3D {
 };
 
 // dynamic LDS of a 3-D launch: forward and inverse twiddles + roots, the xi
-// operands [5][ncomp][maxM], and each wave's sequences [maxNr][nseq]
-// (solid: 3 components, 3 sequences; fluid: 1 component, 2 sequences)
+// operands [5][ncomp][maxM], each wave's sequences [maxNr][nseq] (solid: 3
+// components, 3 sequences; fluid: 1 component, 2 sequences), the element table
 inline size_t lds3D(const Bucket3D &b) {
     const size_t nc = b.phase ? 1 : 3, ns = b.phase ? 2 : 3;
-    return (2 * (size_t)b.maxNr + kRootTable + 5 * nc * (size_t)b.maxM + ns * W3D_WAVES * (size_t)b.maxNr) *
+    return (2 * (size_t)b.maxNr + kRootTable + 5 * nc * (size_t)b.maxM + ns * W3D_WAVES * (size_t)b.maxNr +
+            kTabSlots) *
            sizeof(float2);
 }
+
+// rows of zeros after a displacement field: the 3-D kernels read every mode
+// slot of their lanes (weighting rows above a point's own by zero), up to
+// 64 W3D_MAX_S rows past a point's first row
+constexpr size_t kFieldPadRows = 64 * W3D_MAX_S;
 
 struct Phase {
     int32_t npt = 0, nrow = 0;
@@ -475,7 +481,7 @@ void uploadPhase(Phase &P, hipStream_t s) {
         P.d_m3_inv.upload(P.m3_inv, s);
     }
     size_t n = (size_t)P.nrow * P.ncomp;
-    P.displ.alloc(n);
+    P.displ.alloc(n + kFieldPadRows * P.ncomp);
     P.veloc.alloc(n);
     P.accel.alloc(n);
     P.stiff.alloc(n);
@@ -1633,13 +1639,20 @@ static DevBuf<float2> *fieldBuf(axg_ctx *c, int which) {
     }
 }
 
+// float2 entries of a field (the displacement buffers carry zero rows after them)
+static size_t fieldCount(axg_ctx *c, int which) {
+    const Phase &P = which < 4 ? c->S : c->F;
+    return (size_t)P.nrow * P.ncomp;
+}
+
 int axg_get_field(axg_ctx *c, int32_t which, float *out, int64_t nfloat) {
     CTX_CHECK(c);
     if (which < 0 || which > 7) return fail(AXG_EINVAL, "axg_get_field || bad field.");
     DevBuf<float2> *b = fieldBuf(c, which);
-    if (!out || nfloat != (int64_t)b->n * 2) return fail(AXG_EINVAL, "axg_get_field || size mismatch.");
+    const size_t n = fieldCount(c, which);
+    if (!out || nfloat != (int64_t)n * 2) return fail(AXG_EINVAL, "axg_get_field || size mismatch.");
     API_GUARD({
-        if (b->n) HIPCHECK(hipMemcpyAsync(out, b->p, b->n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+        if (n) HIPCHECK(hipMemcpyAsync(out, b->p, n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
     })
 }
@@ -1648,9 +1661,10 @@ int axg_set_field(axg_ctx *c, int32_t which, const float *in, int64_t nfloat) {
     CTX_CHECK(c);
     if (which < 0 || which > 7) return fail(AXG_EINVAL, "axg_set_field || bad field.");
     DevBuf<float2> *b = fieldBuf(c, which);
-    if (!in || nfloat != (int64_t)b->n * 2) return fail(AXG_EINVAL, "axg_set_field || size mismatch.");
+    const size_t n = fieldCount(c, which);
+    if (!in || nfloat != (int64_t)n * 2) return fail(AXG_EINVAL, "axg_set_field || size mismatch.");
     API_GUARD({
-        if (b->n) HIPCHECK(hipMemcpyAsync(b->p, in, b->n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+        if (n) HIPCHECK(hipMemcpyAsync(b->p, in, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
     })
 }

@@ -84,12 +84,12 @@ struct Elem3DArgs {
 // the memory variables, which then advance with the previous and the new
 // relaxed stress sigma_R = dkappa tr(e) + 2 dmu dev(e) (deviatoric only
 // without kappa attenuation).
-__device__ __forceinline__ void attenuate3d(const Elem3DArgs &a, int64_t e, int np, int i, int Nr, int n,
-                                            const float ev[6], float sv[6]) {
-    const float *par = a.att_pool + a.att_off[e];
+// (par = att_pool + att_off[e]; mv, sr = att_memvar, att_stressR + att_mem_off[e])
+__device__ __forceinline__ void attenuate3d(const Elem3DArgs &a, const float *par, float2 *mv0, float2 *sr0, int np,
+                                            int i, int Nr, int n, const float ev[6], float sv[6]) {
     const float dkappa = par[(int64_t)n * np + i];
     const float dmu = par[(int64_t)(Nr + n) * np + i];
-    float *sr = reinterpret_cast<float *>(a.att_stressR + a.att_mem_off[e]) + (int64_t)i * 6 * Nr + n;
+    float *sr = reinterpret_cast<float *>(sr0) + (int64_t)i * 6 * Nr + n;
     float old[6], neu[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) old[c] = sr[c * Nr];
@@ -106,8 +106,7 @@ __device__ __forceinline__ void attenuate3d(const Elem3DArgs &a, int64_t e, int 
 #pragma unroll
     for (int c = 3; c < 6; c++) neu[c] = dmu * ev[c];
     for (int sl = 0; sl < a.nsls; sl++) {
-        float *mv = reinterpret_cast<float *>(a.att_memvar + sl * a.mem_stride + a.att_mem_off[e]) +
-                    (int64_t)i * 6 * Nr + n;
+        float *mv = reinterpret_cast<float *>(mv0 + sl * a.mem_stride) + (int64_t)i * 6 * Nr + n;
         const float al = a.sls_alpha[sl], be = a.sls_beta[sl], ga = a.sls_gamma[sl];
 #pragma unroll
         for (int c = 0; c < 6; c++) {
@@ -140,6 +139,53 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-element tables in LDS, read back with a uniform address into scalar
+// registers.  The kernels write the point field, so the compiler cannot read
+// the element's index and geometry arrays through the scalar cache; staging
+// them once per element keeps the loops free of dependent global loads.
+constexpr int kTabPlan = 0;                               // FFT stages [stage][FS_N]
+constexpr int kTabRow = kTabPlan + 12 * 8;                 // first row of each point [25]
+constexpr int kTabAmax = kTabRow + 25;                     // largest row the element may touch [25]
+constexpr int kTabGeom = kTabAmax + 25;                    // geometry [5][25]
+constexpr int kTabTiso = kTabGeom + 125;                   // sin t, cos t, sin 2t, cos 2t [4][25]
+constexpr int kTabWords = kTabTiso + 100;                  // 371
+constexpr int kTabSlots = (kTabWords + 1) / 2;             // in float2 slots
+
+__device__ __forceinline__ int ldsu(const int32_t *p) { return __builtin_amdgcn_readfirstlane(*p); }
+// a 64-bit value at a uniform address, kept in scalar registers
+__device__ __forceinline__ int64_t uload64(const int64_t *p) {
+    const int64_t v = *p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float ldsuf(const int32_t *p) { return __int_as_float(__builtin_amdgcn_readfirstlane(*p)); }
+
+// fill the table of element e (all threads of the workgroup; a barrier follows)
+__device__ __forceinline__ void load_elem_table(int32_t *tab, const int32_t *plan, int nst, const int32_t *ep_row,
+                                                const int32_t *ep_amax, const float *geom, const float *tiso,
+                                                int64_t e, int t, int nt) {
+    for (int i = t; i < kTabWords; i += nt) {
+        int32_t v = 0;
+        if (i < kTabRow) v = (i < nst * FS_N) ? plan[i] : 0;
+        else if (i < kTabAmax) v = ep_row[e * 25 + (i - kTabRow)];
+        else if (i < kTabGeom) v = ep_amax[e * 25 + (i - kTabAmax)];
+        else if (i < kTabTiso) v = __float_as_int(geom[e * 125 + (i - kTabGeom)]);
+        else v = tiso ? __float_as_int(tiso[e * 100 + (i - kTabTiso)]) : 0;
+        tab[i] = v;
+    }
+}
+
+__device__ __forceinline__ PointGeom tab_geom(const int32_t *tab, int k) {
+    PointGeom g;
+    g.dsdxii = ldsuf(tab + kTabGeom + 0 * 25 + k);
+    g.dsdeta = ldsuf(tab + kTabGeom + 1 * 25 + k);
+    g.dzdxii = ldsuf(tab + kTabGeom + 2 * 25 + k);
+    g.dzdeta = ldsuf(tab + kTabGeom + 3 * 25 + k);
+    g.inv_s = ldsuf(tab + kTabGeom + 4 * 25 + k);
+    return g;
 }
 
 // ---------------------------------------------------------------- FFT
@@ -293,8 +339,8 @@ __device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t
 #pragma unroll 1
     for (int st = 0; st < nst; st++) {
         const int32_t *d = plan + st * FS_N;
-        const int R = d[FS_R], Ns = d[FS_NS], nb = d[FS_NB], wmul = d[FS_WMUL];
-        const uint32_t mNs = (uint32_t)d[FS_MNS];
+        const int R = ldsu(d + FS_R), Ns = ldsu(d + FS_NS), nb = ldsu(d + FS_NB), wmul = ldsu(d + FS_WMUL);
+        const uint32_t mNs = (uint32_t)ldsu(d + FS_MNS);
         switch (R) {
             case 16: wfft_stage<16, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
             case 8: wfft_stage<8, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
@@ -367,7 +413,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     const bool axial = (a.elem_flags[e] & 2) != 0;
     const float *Gx = a.G[axial ? 1 : 0];
     const float *Ge = a.G[0];
-    const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
+    const int32_t *plan = nullptr;  // the element table's copy (below)
     const int nst = a.nstage[Nr];
 
     float2 *twf = lds;                                   // W_N^t, t < Nr
@@ -375,31 +421,40 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     float2 *root = twi + a.maxNr;                        // odd-radix roots
     float2 *xb = root + kRootTable;                      // [5][3][maxM] xi operands
     float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [Nr][3]
+    int32_t *tab = reinterpret_cast<int32_t *>(xb + 15 * a.maxM + 15 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
         float2 v = a.twiddle[a.tw_off[Nr] + t];
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
     for (int t = threadIdx.x; t < kRootTable; t += W3D_THREADS) root[t] = a.twiddle[t];
+    load_elem_table(tab, a.plan + Nr * E3D_MAX_STAGES * FS_N, nst, a.ep_row, a.ep_amax, a.geom,
+                    TISO ? a.tiso : nullptr, e, threadIdx.x, W3D_THREADS);
     __syncthreads();
+    plan = tab + kTabPlan;
 
-    const int32_t *epr = a.ep_row + e * 25;
-    const int32_t *epa = a.ep_amax + e * 25;
-    const float *mbase = a.mat3d + a.mat3d_off[e];
+    const float *mbase = a.mat3d + uload64(a.mat3d_off + e);
+    const float *att_par = nullptr;
+    float2 *att_mv = nullptr, *att_sr = nullptr;
+    if (ATT != 0) {
+        att_par = a.att_pool + uload64(a.att_off + e);
+        const int64_t mo = uload64(a.att_mem_off + e);
+        att_mv = a.att_memvar + mo;
+        att_sr = a.att_stressR + mo;
+    }
     const float h = 0.5f / (float)Nr;
 
 #pragma unroll 1
     for (int k = 0; k < 5; k++) {
         const int lane = opaque((int)(threadIdx.x & 63));
         const int ip = w * 5 + k;  // this wave's point in this step
-        PointGeom g = load_geom(a.geom, e, ip);
+        PointGeom g = tab_geom(tab, ip);
         float ts1 = 0.f, tc1 = 0.f, ts2 = 0.f, tc2 = 0.f;
         if (TISO) {
-            const float *tp = a.tiso + e * 100 + ip;
-            ts1 = tp[0];
-            tc1 = tp[25];
-            ts2 = tp[50];
-            tc2 = tp[75];
+            ts1 = ldsuf(tab + kTabTiso + ip);
+            tc1 = ldsuf(tab + kTabTiso + 25 + ip);
+            ts2 = ldsuf(tab + kTabTiso + 50 + ip);
+            tc2 = ldsuf(tab + kTabTiso + 75 + ip);
         }
 
         // ---------------- strain spectrum -> 3 Hermitian-extended sequences
@@ -419,19 +474,21 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                 for (int m = 0; m < 5; m++) {
                     // column k: point (m, k); row w: point (w, m)
                     const int pc = m * 5 + k, pr = w * 5 + m;
-                    const float gx = Gx[m * 5 + w], ge = Ge[m * 5 + k];
-                    const bool okc = al <= epa[pc], okr = al <= epa[pr];
-                    const float2 *uc = a.displ + (int64_t)epr[pc] * 3;  // uniform row bases
-                    const float2 *ur = a.displ + (int64_t)epr[pr] * 3;
+                    // rows above a point's largest alpha are read (the field
+                    // is padded) and weighted by zero
+                    const bool okc = al <= ldsu(tab + kTabAmax + pc), okr = al <= ldsu(tab + kTabAmax + pr);
+                    const float gx = okc ? Gx[m * 5 + w] : 0.f, ge = okr ? Ge[m * 5 + k] : 0.f;
+                    const float2 *uc = a.displ + (int64_t)ldsu(tab + kTabRow + pc) * 3;  // uniform row bases
+                    const float2 *ur = a.displ + (int64_t)ldsu(tab + kTabRow + pr) * 3;
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        float2 vc = okc ? uc[al * 3 + c] : czero();
-                        float2 vr = okr ? ur[al * 3 + c] : czero();
+                        float2 vc = uc[al * 3 + c];
+                        float2 vr = ur[al * 3 + c];
                         GU[c].x += gx * vc.x;
                         GU[c].y += gx * vc.y;
                         UG[c].x += vr.x * ge;
                         UG[c].y += vr.y * ge;
-                        if (m == k) u[c] = vr;
+                        if (m == k) u[c] = okr ? vr : czero();
                     }
                 }
                 const float fa = (float)al;
@@ -494,7 +551,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                     ev[2 * b + 1] = z.y;
                 }
                 stress3d<MAT>(P, ev, sv);
-                if (ATT != 0 && iatt >= 0) attenuate3d(a, e, np, iatt, Nr, n, ev, sv);
+                if (ATT != 0 && iatt >= 0) attenuate3d(a, att_par, att_mv, att_sr, np, iatt, Nr, n, ev, sv);
 #pragma unroll
                 for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };
@@ -591,9 +648,9 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
         for (int jp = 0; jp < 5; jp++) {
             const int pp = w * 5 + jp;
-            const int amax = epa[pp];
+            const int amax = ldsu(tab + kTabAmax + pp);
             const float ge = Ge[jp * 5 + k];
-            float2 *sp0 = a.stiff + (int64_t)epr[pp] * 3;
+            float2 *sp0 = a.stiff + (int64_t)ldsu(tab + kTabRow + pp) * 3;
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 const int al = lane + 64 * s;
@@ -628,7 +685,7 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     const bool axial = (a.elem_flags[e] & 2) != 0;
     const float *Gx = a.G[axial ? 1 : 0];
     const float *Ge = a.G[0];
-    const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
+    const int32_t *plan = nullptr;  // the element table's copy (below)
     const int nst = a.nstage[Nr];
 
     float2 *twf = lds;
@@ -636,24 +693,26 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     float2 *root = twi + a.maxNr;
     float2 *xb = root + kRootTable;                      // [5][maxM] xi operands
     float2 *seq = xb + 5 * a.maxM + w * 2 * a.maxNr;     // this wave's [Nr][2]
+    int32_t *tab = reinterpret_cast<int32_t *>(xb + 5 * a.maxM + 10 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
         float2 v = a.twiddle[a.tw_off[Nr] + t];
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
     for (int t = threadIdx.x; t < kRootTable; t += W3D_THREADS) root[t] = a.twiddle[t];
+    load_elem_table(tab, a.plan + Nr * E3D_MAX_STAGES * FS_N, nst, a.ep_row, a.ep_amax, a.geom, nullptr, e,
+                    threadIdx.x, W3D_THREADS);
     __syncthreads();
+    plan = tab + kTabPlan;
 
-    const int32_t *epr = a.ep_row + e * 25;
-    const int32_t *epa = a.ep_amax + e * 25;
-    const float *mbase = a.mat3d + a.mat3d_off[e];
+    const float *mbase = a.mat3d + uload64(a.mat3d_off + e);
     const float h = 0.5f / (float)Nr;
 
 #pragma unroll 1
     for (int k = 0; k < 5; k++) {
         const int lane = opaque((int)(threadIdx.x & 63));
         const int ip = w * 5 + k;
-        PointGeom g = load_geom(a.geom, e, ip);
+        PointGeom g = tab_geom(tab, ip);
 
         // ---------------- gradient spectrum -> 2 Hermitian-extended sequences
 #pragma unroll 1
@@ -666,16 +725,17 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 #pragma unroll
                 for (int m = 0; m < 5; m++) {
                     const int pc = m * 5 + k, pr = w * 5 + m;
-                    const float gx = Gx[m * 5 + w], ge = Ge[m * 5 + k];
-                    const float2 *uc = a.displ + (int64_t)epr[pc];
-                    const float2 *ur = a.displ + (int64_t)epr[pr];
-                    float2 vc = al <= epa[pc] ? uc[al] : czero();
-                    float2 vr = al <= epa[pr] ? ur[al] : czero();
+                    const bool okc = al <= ldsu(tab + kTabAmax + pc), okr = al <= ldsu(tab + kTabAmax + pr);
+                    const float gx = okc ? Gx[m * 5 + w] : 0.f, ge = okr ? Ge[m * 5 + k] : 0.f;
+                    const float2 *uc = a.displ + (int64_t)ldsu(tab + kTabRow + pc);
+                    const float2 *ur = a.displ + (int64_t)ldsu(tab + kTabRow + pr);
+                    float2 vc = uc[al];  // padded field, zero weight above the point's rows
+                    float2 vr = ur[al];
                     GU.x += gx * vc.x;
                     GU.y += gx * vc.y;
                     UG.x += vr.x * ge;
                     UG.y += vr.y * ge;
-                    if (m == k) u = vr;
+                    if (m == k) u = okr ? vr : czero();
                 }
                 const float fa = (float)al;
                 g3[0] = cadd(cscale(g.dzdeta, GU), cscale(g.dzdxii, UG));
@@ -760,9 +820,9 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 #pragma unroll
         for (int jp = 0; jp < 5; jp++) {
             const int pp = w * 5 + jp;
-            const int amax = epa[pp];
+            const int amax = ldsu(tab + kTabAmax + pp);
             const float ge = Ge[jp * 5 + k];
-            float2 *sp0 = a.stiff + (int64_t)epr[pp];
+            float2 *sp0 = a.stiff + (int64_t)ldsu(tab + kTabRow + pp);
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 const int al = lane + 64 * s;

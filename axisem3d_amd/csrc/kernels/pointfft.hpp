@@ -16,7 +16,8 @@ namespace axg {
 constexpr int PF_WAVES = 4;
 // float2 slots of one wave's LDS area for points of Nr <= maxNr: twiddles
 // 2 Nr + roots, sequences 2 Nr
-constexpr int pfWaveLds(int maxNr) { return 4 * maxNr + kRootTable; }
+// (+ the FFT stage table of the point's Nr, 48 slots)
+constexpr int pfWaveLds(int maxNr) { return 4 * maxNr + kRootTable + E3D_MAX_STAGES * FS_N / 2; }
 
 struct PointFFTArgs {
     int32_t n;                   // points in the list
@@ -38,14 +39,18 @@ struct PointFFTArgs {
     const int32_t *nstage;
 };
 
+// twiddles and roots, and the stage table into pl (LDS) so that the stages
+// read it with uniform LDS loads
 __device__ __forceinline__ void pf_load_twiddles(const PointFFTArgs &a, int Nr, float2 *twf, float2 *twi,
-                                                 float2 *root, int lane) {
+                                                 float2 *root, int32_t *pl, int lane) {
     for (int t = lane; t < Nr; t += 64) {
         float2 v = a.twiddle[a.tw_off[Nr] + t];
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
     for (int t = lane; t < kRootTable; t += 64) root[t] = a.twiddle[t];
+    const int32_t *gp = a.plan + Nr * E3D_MAX_STAGES * FS_N;
+    for (int t = lane; t < E3D_MAX_STAGES * FS_N; t += 64) pl[t] = gp[t];
     wave_lds_fence();
 }
 
@@ -98,7 +103,8 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
         const int q = a.items[it];
         const int Nr = a.pt_nr[it], nu = a.pt_nu[q], fl = a.pt_flags[q];
         float2 *twi = twf + Nr, *root = twi + Nr, *seq = root + kRootTable;
-        pf_load_twiddles(a, Nr, twf, twi, root, lane);
+        int32_t *plan = reinterpret_cast<int32_t *>(seq + 2 * Nr);
+        pf_load_twiddles(a, Nr, twf, twi, root, plan, lane);
         pf_clear(seq, Nr, lane);
         float2 *st = a.field + (int64_t)a.pt_row[q] * NCOMP;
         for (int al = lane; al <= nu; al += 64) {
@@ -113,7 +119,6 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
             for (int c = 0; c < NCOMP; c++) pf_pack(seq, Nr, al, v[c], c);
         }
         wave_lds_fence();
-        const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
         wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, root, lane);
         const float *im = a.coef + a.coef_off[it];
         for (int n = lane; n < Nr; n += 64) {
@@ -141,12 +146,12 @@ __global__ __launch_bounds__(256) void couple3d_kernel(PointFFTArgs a) {
     for (int it = blockIdx.x * PF_WAVES + w; it < a.n; it += gridDim.x * PF_WAVES) {
         const int Nr = a.pt_nr[it], nu = Nr / 2;
         float2 *twi = twf + Nr, *root = twi + Nr, *seq = root + kRootTable;
-        const int32_t *plan = a.plan + Nr * E3D_MAX_STAGES * FS_N;
+        int32_t *plan = reinterpret_cast<int32_t *>(seq + 2 * Nr);
         const float *nrm = a.coef + a.coef_off[it] * 6;
         const float2 *ud = a.sdispl + (int64_t)a.pt_row[it] * 3;
         float2 *ss = a.field + (int64_t)a.pt_row[it] * 3;
         float2 *fs = a.fstiff + a.frow[it];
-        pf_load_twiddles(a, Nr, twf, twi, root, lane);
+        pf_load_twiddles(a, Nr, twf, twi, root, plan, lane);
         // solid -> fluid
         pf_clear(seq, Nr, lane);
         for (int al = lane; al <= nu; al += 64)
