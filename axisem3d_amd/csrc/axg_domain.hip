@@ -50,7 +50,8 @@ struct Bucket3D {
     int colour = 0, phase = 0, material = 0, att = 0;
     bool tiso = false;
     int32_t nelem = 0, maxM = 0, maxNr = 0, slots = 1;
-    DevBuf<int32_t> elems;
+    std::vector<int32_t> h_elems;
+    DevBuf<int4> einfo;  // per element: id, Nr, flags | stages << 8, twiddle offset
 };
 
 // dynamic LDS of a 3-D launch: forward and inverse twiddles + roots, the xi
@@ -284,8 +285,7 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
         b->slots = k;
         b->maxNr = c->elem_nr[v.front()];
         b->maxM = std::min(b->maxNr / 2 + 1, 64 * k);  // modes with a lane
-        b->elems.upload(v, c->stream);
-        HIPCHECK(hipStreamSynchronize(c->stream));
+        b->h_elems = v;  // einfo is built with the FFT tables (finishBuckets3D)
         c->buckets3d.push_back(std::move(b));
     }
 }
@@ -333,6 +333,15 @@ void finishBuckets3D(axg_ctx *c) {
     c->d_tw_off.upload(twoff, s);
     c->d_plan.upload(plan, s);
     c->d_nstage.upload(nstage, s);
+    for (auto &b : c->buckets3d) {
+        std::vector<int4> info(b->h_elems.size());
+        for (size_t i = 0; i < info.size(); i++) {
+            const int e = b->h_elems[i], nr = c->elem_nr[e];
+            info[i] = make_int4(e, nr, c->elem_flags[e] | (nstage[nr] << 8), twoff[nr]);
+        }
+        b->einfo.upload(info, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
     HIPCHECK(hipStreamSynchronize(s));
     c->h_mat3d.clear();
     c->h_mat3d.shrink_to_fit();
@@ -352,7 +361,7 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
     FamScope fs(c, b.phase ? AXG_KF_FLUID : AXG_KF_SOLID3D);
     Elem3DArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.elems = b.elems.p;
+    a.einfo = b.einfo.p;
     a.nelem = b.nelem;
     a.maxM = b.maxM;
     a.maxNr = b.maxNr;

@@ -45,7 +45,7 @@ constexpr int E3D_MAX_NR = 2 * 64 * W3D_MAX_S;
 
 struct Elem3DArgs {
     float G[2][25];              // GLL (0) and GLJ (1) derivative matrices G[i][j] = l_i'(xi_j)
-    const int32_t *elems;
+    const int4 *einfo;           // per workgroup: element, Nr, flags | FFT stages << 8, twiddle offset
     int32_t nelem;
     int32_t maxM;                // largest Nu+1 of the launch (LDS strides)
     int32_t maxNr;               // largest Nr of the launch
@@ -405,16 +405,19 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     constexpr int NPAR = MatParams<MAT>::v;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // row ipol of this wave
-    const int64_t e = a.elems[blockIdx.x];
-    const int Nr = a.elem_nr[e];
+    const int4 info = a.einfo[blockIdx.x];  // one load: no dependent chain at the start
+    const int64_t e = __builtin_amdgcn_readfirstlane(info.x);
+    const int Nr = __builtin_amdgcn_readfirstlane(info.y);
+    const int eflags = __builtin_amdgcn_readfirstlane(info.z);
+    const int twoff = __builtin_amdgcn_readfirstlane(info.w);
     const int Nu = Nr / 2;
     const int M = Nu + 1;
     const bool nyq = (Nr % 2 == 0);
-    const bool axial = (a.elem_flags[e] & 2) != 0;
+    const bool axial = (eflags & 2) != 0;
     const float *Gx = a.G[axial ? 1 : 0];
     const float *Ge = a.G[0];
     const int32_t *plan = nullptr;  // the element table's copy (below)
-    const int nst = a.nstage[Nr];
+    const int nst = eflags >> 8;
 
     float2 *twf = lds;                                   // W_N^t, t < Nr
     float2 *twi = twf + a.maxNr;                         // W_N^{-t}
@@ -423,7 +426,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
     float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [Nr][3]
     int32_t *tab = reinterpret_cast<int32_t *>(xb + 15 * a.maxM + 15 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
-        float2 v = a.twiddle[a.tw_off[Nr] + t];
+        float2 v = a.twiddle[twoff + t];
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
@@ -533,12 +536,25 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         wave_lds_fence();
         // C2R (pass 0), stress on the samples, R2C (pass 1): one copy of the
         // transform code serves both directions
+        // the point's moduli on this lane's samples, requested before the C2R
+        // so that their latency hides behind it (up to 2 S samples per lane)
+        const float *mp = mbase + (int64_t)ip * NPAR * Nr;
+        constexpr bool kPre = (NPAR <= 5 && S <= 3);
+        float pm[kPre ? 2 * S : 1][NPAR];
+        if constexpr (kPre) {
+#pragma unroll
+            for (int t = 0; t < 2 * S; t++) {
+                const int n = lane + 64 * t;
+                if (n < Nr)
+#pragma unroll
+                    for (int p = 0; p < NPAR; p++) pm[t][p] = mp[p * Nr + n];
+            }
+        }
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
             wfft<128 * S>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
             if (pass == 1) break;
             // ---------------- stress at every sample n of the point
-            const float *mp = mbase + (int64_t)ip * NPAR * Nr;
             // attenuating point: all 25 (Full) or (1,1),(1,3),(3,1),(3,3) (CG4)
             const int np = (ATT == 2) ? 4 : 25;
             const int iatt = (ATT == 1) ? ip : (ATT == 2) ? (ip == 6 ? 0 : ip == 8 ? 1 : ip == 16 ? 2 : ip == 18 ? 3 : -1) : -1;
@@ -555,16 +571,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
                 for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };
-            if constexpr (NPAR <= 5 && S <= 3) {
-                // all samples' parameters first (Nr <= 128 S: at most 2 S per lane)
-                float pm[2 * S][NPAR];
-#pragma unroll
-                for (int t = 0; t < 2 * S; t++) {
-                    const int n = lane + 64 * t;
-                    if (n < Nr)
-#pragma unroll
-                        for (int p = 0; p < NPAR; p++) pm[t][p] = mp[p * Nr + n];
-                }
+            if constexpr (kPre) {
 #pragma unroll
                 for (int t = 0; t < 2 * S; t++) {
                     const int n = lane + 64 * t;
@@ -677,16 +684,19 @@ template <int S>
 __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t e = a.elems[blockIdx.x];
-    const int Nr = a.elem_nr[e];
+    const int4 info = a.einfo[blockIdx.x];  // one load: no dependent chain at the start
+    const int64_t e = __builtin_amdgcn_readfirstlane(info.x);
+    const int Nr = __builtin_amdgcn_readfirstlane(info.y);
+    const int eflags = __builtin_amdgcn_readfirstlane(info.z);
+    const int twoff = __builtin_amdgcn_readfirstlane(info.w);
     const int Nu = Nr / 2;
     const int M = Nu + 1;
     const bool nyq = (Nr % 2 == 0);
-    const bool axial = (a.elem_flags[e] & 2) != 0;
+    const bool axial = (eflags & 2) != 0;
     const float *Gx = a.G[axial ? 1 : 0];
     const float *Ge = a.G[0];
     const int32_t *plan = nullptr;  // the element table's copy (below)
-    const int nst = a.nstage[Nr];
+    const int nst = eflags >> 8;
 
     float2 *twf = lds;
     float2 *twi = twf + a.maxNr;
@@ -695,7 +705,7 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     float2 *seq = xb + 5 * a.maxM + w * 2 * a.maxNr;     // this wave's [Nr][2]
     int32_t *tab = reinterpret_cast<int32_t *>(xb + 5 * a.maxM + 10 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
-        float2 v = a.twiddle[a.tw_off[Nr] + t];
+        float2 v = a.twiddle[twoff + t];
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
