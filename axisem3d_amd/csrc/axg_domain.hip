@@ -54,12 +54,12 @@ struct Bucket3D {
     DevBuf<int4> einfo;  // per element: id, Nr, flags | stages << 8, twiddle offset
 };
 
-// dynamic LDS of a 3-D launch: forward and inverse twiddles + roots, the xi
+// dynamic LDS of a 3-D launch: forward and inverse twiddles, the xi
 // operands [5][ncomp][maxM], each wave's sequences [maxNr][nseq] (solid: 3
 // components, 3 sequences; fluid: 1 component, 2 sequences), the element table
 inline size_t lds3D(const Bucket3D &b) {
     const size_t nc = b.phase ? 1 : 3, ns = b.phase ? 2 : 3;
-    return (2 * (size_t)b.maxNr + kRootTable + 5 * nc * (size_t)b.maxM + ns * W3D_WAVES * (size_t)b.maxNr +
+    return (2 * (size_t)b.maxNr + 5 * nc * (size_t)b.maxM + ns * W3D_WAVES * (size_t)b.maxNr +
             kTabSlots) *
            sizeof(float2);
 }
@@ -297,14 +297,8 @@ void finishBuckets3D(axg_ctx *c) {
     hipStream_t s = c->stream;
     c->d_mat3d.upload(c->h_mat3d, s);
     c->d_mat3d_off.upload(c->h_mat3d_off, s);
-    // W_R^m of the odd radices 3, 5, 7, 11, 13 first (kRootTable entries), then
     // W_N^t for every N in use
     std::vector<float2> tw;
-    for (int R : {3, 5, 7, 11, 13})
-        for (int m = 0; m < R; m++) {
-            double ang = -2. * 3.141592653589793238463 * m / R;
-            tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
-        }
     std::vector<int32_t> twoff(E3D_MAX_NR + 1, 0), nstage(E3D_MAX_NR + 1, 0);
     std::vector<int32_t> plan((size_t)(E3D_MAX_NR + 1) * E3D_MAX_STAGES * FS_N, 0);
     for (int n = 1; n <= E3D_MAX_NR; n++) {

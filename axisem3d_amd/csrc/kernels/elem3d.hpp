@@ -126,12 +126,43 @@ __device__ __forceinline__ void attenuate3d(const Elem3DArgs &a, const float *pa
 enum FftStage { FS_R = 0, FS_NS, FS_NB, FS_MNB, FS_MNS, FS_WMUL, FS_N = 8 };
 constexpr int E3D_MAX_STAGES = 12;
 
-// offsets of the W_R^m tables of the odd radices in the root table
+
+// cos and sin of 2 pi m / R for the odd radices, m < R (compile-time roots
+// of the odd-radix butterflies)
 template <int R>
-constexpr int rootOff() {
-    return (R == 3) ? 0 : (R == 5) ? 3 : (R == 7) ? 8 : (R == 11) ? 15 : 26;
+__device__ __forceinline__ constexpr float odd_cos(int m) {
+    constexpr float c3[3] = {1.f, -0.5f, -0.5f};
+    constexpr float c5[5] = {1.f, 0.30901699437494745f, -0.80901699437494734f, -0.80901699437494734f,
+                             0.30901699437494745f};
+    constexpr float c7[7] = {1.f, 0.62348980185873359f, -0.22252093395631434f, -0.90096886790241903f,
+                             -0.90096886790241903f, -0.22252093395631434f, 0.62348980185873359f};
+    constexpr float c11[11] = {1.f, 0.84125353283118121f, 0.41541501300188644f, -0.142314838273285f,
+                               -0.65486073394528499f, -0.95949297361449737f, -0.95949297361449737f,
+                               -0.65486073394528499f, -0.142314838273285f, 0.41541501300188644f,
+                               0.84125353283118121f};
+    constexpr float c13[13] = {1.f, 0.88545602565320991f, 0.56806474673115592f, 0.12053668025532301f,
+                               -0.35460488704253545f, -0.74851074817110119f, -0.97094181742605201f,
+                               -0.97094181742605201f, -0.74851074817110119f, -0.35460488704253545f,
+                               0.12053668025532301f, 0.56806474673115592f, 0.88545602565320991f};
+    return R == 3 ? c3[m] : R == 5 ? c5[m] : R == 7 ? c7[m] : R == 11 ? c11[m] : c13[m];
 }
-constexpr int kRootTable = 39;
+template <int R>
+__device__ __forceinline__ constexpr float odd_sin(int m) {
+    constexpr float s3[3] = {0.f, 0.86602540378443865f, -0.86602540378443865f};
+    constexpr float s5[5] = {0.f, 0.95105651629515353f, 0.58778525229247325f, -0.58778525229247325f,
+                             -0.95105651629515353f};
+    constexpr float s7[7] = {0.f, 0.7818314824680298f, 0.97492791218182362f, 0.43388373911755823f,
+                             -0.43388373911755823f, -0.97492791218182362f, -0.7818314824680298f};
+    constexpr float s11[11] = {0.f, 0.54064081745559756f, 0.90963199535451833f, 0.9898214418809328f,
+                               0.75574957435425827f, 0.28173255684142967f, -0.28173255684142967f,
+                               -0.75574957435425827f, -0.9898214418809328f, -0.90963199535451833f,
+                               -0.54064081745559756f};
+    constexpr float s13[13] = {0.f, 0.46472317204376851f, 0.82298386589365635f, 0.99270887409805397f,
+                               0.93501624268541483f, 0.66312265824079519f, 0.23931566428755768f,
+                               -0.23931566428755768f, -0.66312265824079519f, -0.93501624268541483f,
+                               -0.99270887409805397f, -0.82298386589365635f, -0.46472317204376851f};
+    return R == 3 ? s3[m] : R == 5 ? s5[m] : R == 7 ? s7[m] : R == 11 ? s11[m] : s13[m];
+}
 
 // orders the LDS accesses of one wave around this point (code motion only:
 // a wave's LDS instructions are executed in issue order)
@@ -250,10 +281,10 @@ __device__ __forceinline__ void dft_pow2(float2 (&x)[R], int sign) {
 // x[j + r nb] * W_{Ns R}^{(j % Ns) r}, does a radix-R DFT and writes
 // y[(j / Ns) Ns R + (j % Ns) + r Ns].  Lane task t = NSEQ j + s, so the reads
 // of a stage are contiguous across lanes.  sign = -1 forward, +1 inverse.
-// `tw` holds W_N^{sign t} for t < N; `root` the forward odd-radix root tables.
+// `tw` holds W_N^{sign t} for t < N; the odd radices use compile-time roots.
 template <int R, int NMAX, int NSEQ>
 __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, uint32_t mNs, int wmul,
-                                           int sign, const float2 *tw, const float2 *root, int lane) {
+                                           int sign, const float2 *tw, int lane) {
     constexpr int B = WaveB<R, NMAX, NSEQ>::v;
     const int total = NSEQ * nb;
     float2 v[B][R];
@@ -305,22 +336,17 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
                 y0 = cadd(y0, av[r - 1]);
             }
             o[0] = y0;
-            const float2 *rt = root + rootOff<R>();  // (cos, -sin)(2 pi m / R)
-            int m[H];
 #pragma unroll
-            for (int r = 0; r < H; r++) m[r] = 0;
-#pragma unroll 1
             for (int k = 1; k <= H; k++) {
                 float2 A = v[q][0], Bs = czero();
 #pragma unroll
-                for (int r = 0; r < H; r++) {
-                    m[r] += r + 1;
-                    if (m[r] >= R) m[r] -= R;
-                    const float2 wr = rt[m[r]];
-                    A.x += wr.x * av[r].x;
-                    A.y += wr.x * av[r].y;
-                    Bs.x -= wr.y * bv[r].x;
-                    Bs.y -= wr.y * bv[r].y;
+                for (int r = 1; r <= H; r++) {
+                    // compile-time cos and sin of 2 pi r k / R
+                    const float cw = odd_cos<R>((r * k) % R), sw = odd_sin<R>((r * k) % R);
+                    A.x += cw * av[r - 1].x;
+                    A.y += cw * av[r - 1].y;
+                    Bs.x += sw * bv[r - 1].x;
+                    Bs.y += sw * bv[r - 1].y;
                 }
                 // i sign Bs
                 const float2 iB = (sign > 0) ? make_float2(-Bs.y, Bs.x) : make_float2(Bs.y, -Bs.x);
@@ -335,22 +361,22 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
 // the whole transform of length N <= NMAX from its stage descriptors
 template <int NMAX, int NSEQ = 3>
 __device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t *plan, int nst, const float2 *tw,
-                                     const float2 *root, int lane) {
+                                     int lane) {
 #pragma unroll 1
     for (int st = 0; st < nst; st++) {
         const int32_t *d = plan + st * FS_N;
         const int R = ldsu(d + FS_R), Ns = ldsu(d + FS_NS), nb = ldsu(d + FS_NB), wmul = ldsu(d + FS_WMUL);
         const uint32_t mNs = (uint32_t)ldsu(d + FS_MNS);
         switch (R) {
-            case 16: wfft_stage<16, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 8: wfft_stage<8, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 4: wfft_stage<4, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 2: wfft_stage<2, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 3: wfft_stage<3, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 5: wfft_stage<5, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 7: wfft_stage<7, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            case 11: wfft_stage<11, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
-            default: wfft_stage<13, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, root, lane); break;
+            case 16: wfft_stage<16, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 8: wfft_stage<8, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 4: wfft_stage<4, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 2: wfft_stage<2, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 3: wfft_stage<3, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 5: wfft_stage<5, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 7: wfft_stage<7, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            case 11: wfft_stage<11, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            default: wfft_stage<13, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
         }
     }
 }
@@ -421,8 +447,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 
     float2 *twf = lds;                                   // W_N^t, t < Nr
     float2 *twi = twf + a.maxNr;                         // W_N^{-t}
-    float2 *root = twi + a.maxNr;                        // odd-radix roots
-    float2 *xb = root + kRootTable;                      // [5][3][maxM] xi operands
+    float2 *xb = twi + a.maxNr;                          // [5][3][maxM] xi operands
     float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [Nr][3]
     int32_t *tab = reinterpret_cast<int32_t *>(xb + 15 * a.maxM + 15 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
@@ -430,7 +455,6 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
-    for (int t = threadIdx.x; t < kRootTable; t += W3D_THREADS) root[t] = a.twiddle[t];
     load_elem_table(tab, a.plan + Nr * E3D_MAX_STAGES * FS_N, nst, a.ep_row, a.ep_amax, a.geom,
                     TISO ? a.tiso : nullptr, e, threadIdx.x, W3D_THREADS);
     __syncthreads();
@@ -552,7 +576,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         }
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
-            wfft<128 * S>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
+            wfft<128 * S>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, lane);
             if (pass == 1) break;
             // ---------------- stress at every sample n of the point
             // attenuating point: all 25 (Full) or (1,1),(1,3),(3,1),(3,3) (CG4)
@@ -700,8 +724,7 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 
     float2 *twf = lds;
     float2 *twi = twf + a.maxNr;
-    float2 *root = twi + a.maxNr;
-    float2 *xb = root + kRootTable;                      // [5][maxM] xi operands
+    float2 *xb = twi + a.maxNr;                          // [5][maxM] xi operands
     float2 *seq = xb + 5 * a.maxM + w * 2 * a.maxNr;     // this wave's [Nr][2]
     int32_t *tab = reinterpret_cast<int32_t *>(xb + 5 * a.maxM + 10 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
@@ -709,7 +732,6 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
-    for (int t = threadIdx.x; t < kRootTable; t += W3D_THREADS) root[t] = a.twiddle[t];
     load_elem_table(tab, a.plan + Nr * E3D_MAX_STAGES * FS_N, nst, a.ep_row, a.ep_amax, a.geom, nullptr, e,
                     threadIdx.x, W3D_THREADS);
     __syncthreads();
@@ -769,7 +791,7 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
         wave_lds_fence();
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
-            wfft<128 * S, 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, root, lane);
+            wfft<128 * S, 2>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, lane);
             if (pass == 1) break;
             // Acoustic3D: stress = K(phi) grad chi on every sample
             const float *K = mbase + (int64_t)ip * Nr;

@@ -15,9 +15,9 @@ namespace axg {
 
 constexpr int PF_WAVES = 4;
 // float2 slots of one wave's LDS area for points of Nr <= maxNr: twiddles
-// 2 Nr + roots, sequences 2 Nr
+// 2 Nr, sequences 2 Nr
 // (+ the FFT stage table of the point's Nr, 48 slots)
-constexpr int pfWaveLds(int maxNr) { return 4 * maxNr + kRootTable + E3D_MAX_STAGES * FS_N / 2; }
+constexpr int pfWaveLds(int maxNr) { return 4 * maxNr + E3D_MAX_STAGES * FS_N / 2; }
 
 struct PointFFTArgs {
     int32_t n;                   // points in the list
@@ -39,16 +39,15 @@ struct PointFFTArgs {
     const int32_t *nstage;
 };
 
-// twiddles and roots, and the stage table into pl (LDS) so that the stages
+// twiddles, and the stage table into pl (LDS) so that the stages
 // read it with uniform LDS loads
 __device__ __forceinline__ void pf_load_twiddles(const PointFFTArgs &a, int Nr, float2 *twf, float2 *twi,
-                                                 float2 *root, int32_t *pl, int lane) {
+                                                 int32_t *pl, int lane) {
     for (int t = lane; t < Nr; t += 64) {
         float2 v = a.twiddle[a.tw_off[Nr] + t];
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
-    for (int t = lane; t < kRootTable; t += 64) root[t] = a.twiddle[t];
     const int32_t *gp = a.plan + Nr * E3D_MAX_STAGES * FS_N;
     for (int t = lane; t < E3D_MAX_STAGES * FS_N; t += 64) pl[t] = gp[t];
     wave_lds_fence();
@@ -102,9 +101,9 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
     for (int it = blockIdx.x * PF_WAVES + w; it < a.n; it += gridDim.x * PF_WAVES) {
         const int q = a.items[it];
         const int Nr = a.pt_nr[it], nu = a.pt_nu[q], fl = a.pt_flags[q];
-        float2 *twi = twf + Nr, *root = twi + Nr, *seq = root + kRootTable;
+        float2 *twi = twf + Nr, *seq = twi + Nr;
         int32_t *plan = reinterpret_cast<int32_t *>(seq + 2 * Nr);
-        pf_load_twiddles(a, Nr, twf, twi, root, plan, lane);
+        pf_load_twiddles(a, Nr, twf, twi, plan, lane);
         pf_clear(seq, Nr, lane);
         float2 *st = a.field + (int64_t)a.pt_row[q] * NCOMP;
         for (int al = lane; al <= nu; al += 64) {
@@ -119,7 +118,7 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
             for (int c = 0; c < NCOMP; c++) pf_pack(seq, Nr, al, v[c], c);
         }
         wave_lds_fence();
-        wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, root, lane);
+        wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, lane);
         const float *im = a.coef + a.coef_off[it];
         for (int n = lane; n < Nr; n += 64) {
             const float m = im[n];
@@ -128,7 +127,7 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
             seq[2 * n + 1] = make_float2(m * z1.x, m * z1.y);
         }
         wave_lds_fence();
-        wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, root, lane);
+        wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, lane);
         for (int al = lane; al <= nu; al += 64)
             for (int c = 0; c < NCOMP; c++) st[al * NCOMP + c] = pf_unpack(seq, Nr, al, c);
         wave_lds_fence();
@@ -145,19 +144,19 @@ __global__ __launch_bounds__(256) void couple3d_kernel(PointFFTArgs a) {
     float2 *twf = lds + w * a.wave_lds;
     for (int it = blockIdx.x * PF_WAVES + w; it < a.n; it += gridDim.x * PF_WAVES) {
         const int Nr = a.pt_nr[it], nu = Nr / 2;
-        float2 *twi = twf + Nr, *root = twi + Nr, *seq = root + kRootTable;
+        float2 *twi = twf + Nr, *seq = twi + Nr;
         int32_t *plan = reinterpret_cast<int32_t *>(seq + 2 * Nr);
         const float *nrm = a.coef + a.coef_off[it] * 6;
         const float2 *ud = a.sdispl + (int64_t)a.pt_row[it] * 3;
         float2 *ss = a.field + (int64_t)a.pt_row[it] * 3;
         float2 *fs = a.fstiff + a.frow[it];
-        pf_load_twiddles(a, Nr, twf, twi, root, plan, lane);
+        pf_load_twiddles(a, Nr, twf, twi, plan, lane);
         // solid -> fluid
         pf_clear(seq, Nr, lane);
         for (int al = lane; al <= nu; al += 64)
             for (int c = 0; c < 3; c++) pf_pack(seq, Nr, al, ud[al * 3 + c], c);
         wave_lds_fence();
-        wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, root, lane);
+        wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, lane);
         for (int n = lane; n < Nr; n += 64) {
             float2 z0 = seq[2 * n], z1 = seq[2 * n + 1];
             const float *nn = nrm + n * 6;
@@ -165,14 +164,14 @@ __global__ __launch_bounds__(256) void couple3d_kernel(PointFFTArgs a) {
             seq[2 * n + 1] = czero();
         }
         wave_lds_fence();
-        wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, root, lane);
+        wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, lane);
         for (int al = lane; al <= nu; al += 64) fs[al] = cadd(fs[al], pf_unpack(seq, Nr, al, 0));
         wave_lds_fence();
         // fluid -> solid (with the updated fluid stiffness)
         pf_clear(seq, Nr, lane);
         for (int al = lane; al <= nu; al += 64) pf_pack(seq, Nr, al, fs[al], 0);
         wave_lds_fence();
-        wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, root, lane);
+        wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, lane);
         for (int n = lane; n < Nr; n += 64) {
             const float y = seq[2 * n].x;
             const float *nn = nrm + n * 6;
@@ -180,7 +179,7 @@ __global__ __launch_bounds__(256) void couple3d_kernel(PointFFTArgs a) {
             seq[2 * n + 1] = make_float2(nn[5] * y, 0.f);
         }
         wave_lds_fence();
-        wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, root, lane);
+        wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, lane);
         for (int al = lane; al <= nu; al += 64)
             for (int c = 0; c < 3; c++) ss[al * 3 + c] = csub(ss[al * 3 + c], pf_unpack(seq, Nr, al, c));
         wave_lds_fence();
