@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -238,6 +239,80 @@ std::vector<int> fftFactors(int n) {
 
 uint32_t hostMagic(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
 
+// Stage plan of a length-n transform on three interleaved sequences: the
+// power of two split into radices 2, 4, 8, 16 and each stage's lane mapping
+// (one task per butterfly and sequence, or one butterfly of all sequences per
+// lane) chosen by a VALU-instruction model of the two stage kernels; odd
+// primes as they are.  {radix, lane} per stage; empty if n has another prime.
+std::vector<std::pair<int, int>> planStages(int n) {
+    std::vector<int> f = fftFactors(n);
+    if (f.empty() && n != 1) return {};
+    int p2 = 0, m = n;
+    while (m % 2 == 0) {
+        m /= 2;
+        p2++;
+    }
+    std::vector<int> odd;
+    for (int r : f)
+        if (r % 2) odd.push_back(r);
+    auto dft = [](int R) {
+        switch (R) {
+            case 2: return 4.;
+            case 3: return 14.;
+            case 4: return 16.;
+            case 5: return 36.;
+            case 7: return 64.;
+            case 8: return 52.;
+            case 11: return 130.;
+            case 13: return 170.;
+            default: return 144.;  // 16
+        }
+    };
+    auto stage = [&](int R, int &lane) {
+        const int nb = n / R;
+        const double task = std::ceil(3. * nb / 64.) * (15. + dft(R) + 6. * (R - 1) + 2. * R);
+        const bool canLane = (R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8);
+        const double ln = canLane ? std::ceil(nb / 64.) * (15. + 3. * dft(R) + 14. * (R - 1) + 6. * R) : 1e30;
+        lane = ln < task ? 1 : 0;
+        return std::min(task, ln);
+    };
+    double best = 1e300;
+    std::vector<int> bestPow;
+    std::vector<int> cur;
+    // compositions of p2 into parts 1..4 (radix 2..16), non-increasing
+    std::function<void(int, int)> rec = [&](int left, int maxPart) {
+        if (left == 0) {
+            double c = 0.;
+            int l;
+            for (int e : cur) c += stage(1 << e, l);
+            for (int r : odd) c += stage(r, l);
+            if (c < best) {
+                best = c;
+                bestPow = cur;
+            }
+            return;
+        }
+        for (int e = std::min(left, maxPart); e >= 1; e--) {
+            cur.push_back(e);
+            rec(left - e, e);
+            cur.pop_back();
+        }
+    };
+    rec(p2, 4);
+    std::vector<std::pair<int, int>> out;
+    for (int e : bestPow) {
+        int l;
+        stage(1 << e, l);
+        out.push_back({1 << e, l});
+    }
+    for (int r : odd) {
+        int l;
+        stage(r, l);
+        out.push_back({r, l});
+    }
+    return out;
+}
+
 void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att,
                      const std::vector<int32_t> &elems) {
     if (phase != 0 && tiso)
@@ -308,12 +383,14 @@ void finishBuckets3D(axg_ctx *c) {
             double ang = -2. * 3.141592653589793238463 * t / n;
             tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
         }
-        std::vector<int> f = fftFactors(n);
+        const std::vector<std::pair<int, int>> f = planStages(n);
+        if ((int)f.size() > E3D_MAX_STAGES) throw std::runtime_error("FieldFFT || too many FFT stages.");
         nstage[n] = (int32_t)f.size();
         int Ns = 1;
         for (size_t st = 0; st < f.size(); st++) {
             int32_t *dsc = &plan[((size_t)n * E3D_MAX_STAGES + st) * FS_N];
-            const int R = f[st], nb = n / R;
+            const int R = f[st].first, nb = n / R;
+            dsc[FS_LANE] = f[st].second;
             dsc[FS_R] = R;
             dsc[FS_NS] = Ns;
             dsc[FS_NB] = nb;

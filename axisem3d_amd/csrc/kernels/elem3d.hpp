@@ -123,7 +123,8 @@ __device__ __forceinline__ void attenuate3d(const Elem3DArgs &a, const float *pa
 // radix R, span Ns (product of the radices before it), nb = N / R butterflies
 // per sequence, multiply-high reciprocals of nb and Ns, and wmul = nb / Ns
 // (W_{Ns R}^{m} = W_N^{m wmul}).
-enum FftStage { FS_R = 0, FS_NS, FS_NB, FS_MNB, FS_MNS, FS_WMUL, FS_N = 8 };
+// FS_LANE: 1 = one butterfly of every sequence per lane (radix <= 8)
+enum FftStage { FS_R = 0, FS_NS, FS_NB, FS_MNB, FS_MNS, FS_WMUL, FS_LANE, FS_N = 8 };
 constexpr int E3D_MAX_STAGES = 12;
 
 
@@ -276,12 +277,56 @@ __device__ __forceinline__ void dft_pow2(float2 (&x)[R], int sign) {
     }
 }
 
+// In-register DFT of length R, y_k = sum_r x_r W^{rk} with W = exp(sign 2 pi i / R):
+// powers of two by the even/odd split, odd R by output pairs (k, R-k): with
+// a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r} (r = 1..H) and theta = 2 pi r k / R,
+//   y_k, y_{R-k} = x_0 + sum_r a_r cos(theta) +- i sign sum_r b_r sin(theta)
+// (a quarter of the multiplications of the plain DFT sum; compile-time roots).
+template <int R>
+__device__ __forceinline__ void dft_radix(float2 (&x)[R], int sign) {
+    if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
+        dft_pow2<R>(x, sign);
+    } else {
+        constexpr int H = (R - 1) / 2;
+        float2 av[H], bv[H];
+        float2 y[R];
+        y[0] = x[0];
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            av[r - 1] = cadd(x[r], x[R - r]);
+            bv[r - 1] = csub(x[r], x[R - r]);
+            y[0] = cadd(y[0], av[r - 1]);
+        }
+#pragma unroll
+        for (int k = 1; k <= H; k++) {
+            float2 A = x[0], Bs = czero();
+#pragma unroll
+            for (int r = 1; r <= H; r++) {
+                const float cw = odd_cos<R>((r * k) % R), sw = odd_sin<R>((r * k) % R);
+                A.x += cw * av[r - 1].x;
+                A.y += cw * av[r - 1].y;
+                Bs.x += sw * bv[r - 1].x;
+                Bs.y += sw * bv[r - 1].y;
+            }
+            // i sign Bs
+            const float2 iB = (sign > 0) ? make_float2(-Bs.y, Bs.x) : make_float2(Bs.y, -Bs.x);
+            y[k] = cadd(A, iB);
+            y[R - k] = csub(A, iB);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = y[r];
+    }
+}
+
 // One Stockham stage on NSEQ interleaved sequences ([N][NSEQ] at `seq`:
 // element n of sequence s at NSEQ n + s): butterfly j of sequence s reads
 // x[j + r nb] * W_{Ns R}^{(j % Ns) r}, does a radix-R DFT and writes
-// y[(j / Ns) Ns R + (j % Ns) + r Ns].  Lane task t = NSEQ j + s, so the reads
-// of a stage are contiguous across lanes.  sign = -1 forward, +1 inverse.
-// `tw` holds W_N^{sign t} for t < N; the odd radices use compile-time roots.
+// y[(j / Ns) Ns R + (j % Ns) + r Ns].  Two lane mappings: one task per
+// (butterfly, sequence), t = NSEQ j + s, so the reads of a stage are
+// contiguous across lanes (suits few butterflies of a large radix), or one
+// butterfly of all NSEQ sequences per lane, sharing its index arithmetic and
+// twiddles (suits many butterflies of a small radix); the host planner picks
+// per stage.  sign = -1 forward, +1 inverse; `tw` holds W_N^{sign t}, t < N.
 template <int R, int NMAX, int NSEQ>
 __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, uint32_t mNs, int wmul,
                                            int sign, const float2 *tw, int lane) {
@@ -317,42 +362,55 @@ __device__ __forceinline__ void wfft_stage(float2 *seq, int N, int Ns, int nb, u
     for (int q = 0; q < B; q++) {
         if (dst[q] < 0) continue;
         float2 *o = seq + dst[q];
-        if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
-            dft_pow2<R>(v[q], sign);
+        dft_radix<R>(v[q], sign);
 #pragma unroll
-            for (int r = 0; r < R; r++) o[NSEQ * r * Ns] = v[q][r];
-        } else {
-            // odd R, outputs in pairs (k, R-k): with a_r = v_r + v_{R-r},
-            // b_r = v_r - v_{R-r} (r = 1..H) and theta = 2 pi r k / R,
-            //   y_k, y_{R-k} = v_0 + sum_r a_r cos(theta) +- i sign sum_r b_r sin(theta)
-            // (a quarter of the multiplications of the plain DFT sum)
-            constexpr int H = (R - 1) / 2;
-            float2 av[H], bv[H];
-            float2 y0 = v[q][0];
+        for (int r = 0; r < R; r++) o[NSEQ * r * Ns] = v[q][r];
+    }
+    wave_lds_fence();
+}
+
+// butterflies per lane of the one-butterfly-per-lane mapping
+template <int R, int NMAX>
+struct LaneB {
+    static constexpr int v = (NMAX / R + 63) / 64;
+};
+
+template <int R, int NMAX, int NSEQ>
+__device__ __forceinline__ void wfft_stage_lane(float2 *seq, int N, int Ns, int nb, uint32_t mNs, int wmul,
+                                                int sign, const float2 *tw, int lane) {
+    constexpr int B = LaneB<R, NMAX>::v;
+    float2 v[B][NSEQ][R];
+    int dst[B];
 #pragma unroll
-            for (int r = 1; r <= H; r++) {
-                av[r - 1] = cadd(v[q][r], v[q][R - r]);
-                bv[r - 1] = csub(v[q][r], v[q][R - r]);
-                y0 = cadd(y0, av[r - 1]);
-            }
-            o[0] = y0;
+    for (int q = 0; q < B; q++) {
+        const int j = lane + q * 64;
+        dst[q] = -1;
+        if (j < nb) {
+            const int jq = udiv(j, Ns, mNs), jm = j - jq * Ns;
+            const int wstep = jm * wmul;
+            const float2 *x = seq + NSEQ * j;
 #pragma unroll
-            for (int k = 1; k <= H; k++) {
-                float2 A = v[q][0], Bs = czero();
+            for (int r = 0; r < R; r++) {
+                const float2 w = tw[r * wstep];
 #pragma unroll
-                for (int r = 1; r <= H; r++) {
-                    // compile-time cos and sin of 2 pi r k / R
-                    const float cw = odd_cos<R>((r * k) % R), sw = odd_sin<R>((r * k) % R);
-                    A.x += cw * av[r - 1].x;
-                    A.y += cw * av[r - 1].y;
-                    Bs.x += sw * bv[r - 1].x;
-                    Bs.y += sw * bv[r - 1].y;
+                for (int s = 0; s < NSEQ; s++) {
+                    const float2 xv = x[NSEQ * r * nb + s];
+                    v[q][s][r] = (r == 0) ? xv : cmul(xv, w);
                 }
-                // i sign Bs
-                const float2 iB = (sign > 0) ? make_float2(-Bs.y, Bs.x) : make_float2(Bs.y, -Bs.x);
-                o[NSEQ * k * Ns] = cadd(A, iB);
-                o[NSEQ * (R - k) * Ns] = csub(A, iB);
             }
+            dst[q] = NSEQ * (jq * Ns * R + jm);
+        }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < B; q++) {
+        if (dst[q] < 0) continue;
+        float2 *o = seq + dst[q];
+#pragma unroll
+        for (int s = 0; s < NSEQ; s++) {
+            dft_radix<R>(v[q][s], sign);
+#pragma unroll
+            for (int r = 0; r < R; r++) o[NSEQ * r * Ns + s] = v[q][s][r];
         }
     }
     wave_lds_fence();
@@ -367,6 +425,17 @@ __device__ __forceinline__ void wfft(float2 *seq, int N, int sign, const int32_t
         const int32_t *d = plan + st * FS_N;
         const int R = ldsu(d + FS_R), Ns = ldsu(d + FS_NS), nb = ldsu(d + FS_NB), wmul = ldsu(d + FS_WMUL);
         const uint32_t mNs = (uint32_t)ldsu(d + FS_MNS);
+        if (ldsu(d + FS_LANE)) {
+            switch (R) {
+                case 8: wfft_stage_lane<8, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+                case 4: wfft_stage_lane<4, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+                case 2: wfft_stage_lane<2, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+                case 3: wfft_stage_lane<3, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+                case 5: wfft_stage_lane<5, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+                default: wfft_stage_lane<7, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
+            }
+            continue;
+        }
         switch (R) {
             case 16: wfft_stage<16, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
             case 8: wfft_stage<8, NMAX, NSEQ>(seq, N, Ns, nb, mNs, wmul, sign, tw, lane); break;
