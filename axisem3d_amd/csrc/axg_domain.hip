@@ -55,14 +55,12 @@ struct Bucket3D {
     DevBuf<int4> einfo;  // per element: id, Nr, flags | stages << 8, twiddle offset
 };
 
-// dynamic LDS of a 3-D launch: forward and inverse twiddles, the xi
-// operands [5][ncomp][maxM], each wave's sequences [maxNr][nseq] (solid: 3
-// components, 3 sequences; fluid: 1 component, 2 sequences), the element table
+// dynamic LDS of a 3-D launch: forward and inverse twiddles, each wave's
+// sequences [maxNr][nseq] (solid 3, fluid 2; they also carry the xi operands
+// [ncomp][maxM] of the exchange), the element table
 inline size_t lds3D(const Bucket3D &b) {
-    const size_t nc = b.phase ? 1 : 3, ns = b.phase ? 2 : 3;
-    return (2 * (size_t)b.maxNr + 5 * nc * (size_t)b.maxM + ns * W3D_WAVES * (size_t)b.maxNr +
-            kTabSlots) *
-           sizeof(float2);
+    const size_t ns = b.phase ? 2 : 3;
+    return (2 * (size_t)b.maxNr + ns * W3D_WAVES * (size_t)b.maxNr + kTabSlots) * sizeof(float2);
 }
 
 // rows of zeros after a displacement field: the 3-D kernels read every mode

@@ -496,7 +496,9 @@ __device__ __forceinline__ void stress3d(ParFn P, const float e[6], float s[6]) 
 
 // ---------------------------------------------------------------- kernel
 template <int MAT, bool TISO, int S, int ATT>
-__global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
+// classes 1 and 2 at 96 VGPRs: five waves per SIMD, four workgroups per CU
+__global__ __launch_bounds__(W3D_THREADS) __attribute__((amdgpu_waves_per_eu(S <= 2 ? 5 : 1)))
+void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     constexpr int NPAR = MatParams<MAT>::v;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // row ipol of this wave
@@ -516,9 +518,10 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 
     float2 *twf = lds;                                   // W_N^t, t < Nr
     float2 *twi = twf + a.maxNr;                         // W_N^{-t}
-    float2 *xb = twi + a.maxNr;                          // [5][3][maxM] xi operands
-    float2 *seq = xb + 15 * a.maxM + w * 3 * a.maxNr;    // this wave's [Nr][3]
-    int32_t *tab = reinterpret_cast<int32_t *>(xb + 15 * a.maxM + 15 * a.maxNr);  // element table
+    float2 *seq0 = twi + a.maxNr;                        // the waves' sequence areas
+    float2 *seq = seq0 + w * 3 * a.maxNr;                // this wave's [Nr][3]; after the
+                                                         // R2C its xi operands [3][maxM]
+    int32_t *tab = reinterpret_cast<int32_t *>(seq0 + 15 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
         float2 v = a.twiddle[twoff + t];
         twf[t] = v;
@@ -632,7 +635,7 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         // the point's moduli on this lane's samples, requested before the C2R
         // so that their latency hides behind it (up to 2 S samples per lane)
         const float *mp = mbase + (int64_t)ip * NPAR * Nr;
-        constexpr bool kPre = (NPAR <= 5 && S <= 3);
+        constexpr bool kPre = (NPAR <= 5 && S == 3);
         float pm[kPre ? 2 * S : 1][NPAR];
         if constexpr (kPre) {
 #pragma unroll
@@ -678,16 +681,17 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
         }
 
         // ---------------- unpack x 1/Nr, rotate back, quadrature operands:
-        // xi operand X to the exchange, eta operand Y and the own-point term
-        // inv_s g kept for the update after the exchange
-        float2 Yk[S][3], Ok[S][3];
+        // xi operand X to the exchange (the wave's own sequence area, once all
+        // its lanes have read the spectrum), eta operand Y and the own-point
+        // term inv_s g kept for the update after the exchange
+        float2 Yk[S][3], Ok[S][3], Xk[S][3];
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const int al = lane + 64 * s;
 #pragma unroll
-            for (int c = 0; c < 3; c++) Yk[s][c] = Ok[s][c] = czero();
+            for (int c = 0; c < 3; c++) Yk[s][c] = Ok[s][c] = Xk[s][c] = czero();
             if (al >= M) continue;
-            float2 X[3] = {czero(), czero(), czero()};
+            float2 *X = Xk[s];
             if (!(nyq && al == Nu)) {
                 float2 s6[6];
 #pragma unroll
@@ -722,8 +726,14 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) Ok[s][c] = cscale(g.inv_s, gg[c]);
             }
+        }
+        wave_lds_fence();
 #pragma unroll
-            for (int c = 0; c < 3; c++) xb[(w * 3 + c) * a.maxM + al] = X[c];
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            if (al >= M) continue;
+#pragma unroll
+            for (int c = 0; c < 3; c++) seq[c * a.maxM + al] = Xk[s][c];
         }
         __syncthreads();
         // xi term of this wave's point: sum_i G_xi(w, i) X(i, k) over the column
@@ -736,13 +746,13 @@ __global__ __launch_bounds__(W3D_THREADS) void solid3d_kernel(Elem3DArgs a) {
                 const float gx = Gx[w * 5 + i];
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    float2 x = xb[(i * 3 + c) * a.maxM + al];
+                    float2 x = seq0[i * 3 * a.maxNr + c * a.maxM + al];
                     Ok[s][c].x += gx * x.x;
                     Ok[s][c].y += gx * x.y;
                 }
             }
         }
-        __syncthreads();  // the exchange buffer is rewritten next step
+        __syncthreads();  // the sequence areas are rewritten next step
         // gatherStiffFromElement for this step's share: the row's points get
         // G_eta(jp, k) Y(w, k), the point (w, k) also its xi and own terms
 #pragma unroll
@@ -793,9 +803,10 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 
     float2 *twf = lds;
     float2 *twi = twf + a.maxNr;
-    float2 *xb = twi + a.maxNr;                          // [5][maxM] xi operands
-    float2 *seq = xb + 5 * a.maxM + w * 2 * a.maxNr;     // this wave's [Nr][2]
-    int32_t *tab = reinterpret_cast<int32_t *>(xb + 5 * a.maxM + 10 * a.maxNr);  // element table
+    float2 *seq0 = twi + a.maxNr;                        // the waves' sequence areas
+    float2 *seq = seq0 + w * 2 * a.maxNr;                // this wave's [Nr][2]; after the
+                                                         // R2C its xi operands [maxM]
+    int32_t *tab = reinterpret_cast<int32_t *>(seq0 + 10 * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
         float2 v = a.twiddle[twoff + t];
         twf[t] = v;
@@ -883,13 +894,13 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
         }
 
         // ---------------- unpack x 1/Nr, quadrature operands
-        float2 Yk[S], Ok[S];
+        float2 Yk[S], Ok[S], Xk[S];
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const int al = lane + 64 * s;
-            Yk[s] = Ok[s] = czero();
+            Yk[s] = Ok[s] = Xk[s] = czero();
             if (al >= M) continue;
-            float2 X = czero();
+            float2 &X = Xk[s];
             if (!(nyq && al == Nu)) {
                 float2 za = seq[2 * al], zb = seq[2 * (al == 0 ? 0 : Nr - al)];
                 float2 ya = seq[2 * al + 1], yb = seq[2 * (al == 0 ? 0 : Nr - al) + 1];
@@ -902,7 +913,12 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
                 if (axial && w == 0) X = cadd(X, cscale(g.dzdeta, gg));
                 Ok[s] = cscale(g.inv_s, gg);
             }
-            xb[w * a.maxM + al] = X;
+        }
+        wave_lds_fence();  // the spectrum is read: the xi operands go to the same area
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int al = lane + 64 * s;
+            if (al < M) seq[al] = Xk[s];
         }
         __syncthreads();
 #pragma unroll
@@ -912,7 +928,7 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 #pragma unroll
             for (int i = 0; i < 5; i++) {
                 const float gx = Gx[w * 5 + i];
-                float2 x = xb[i * a.maxM + al];
+                float2 x = seq0[i * 2 * a.maxNr + al];
                 Ok[s].x += gx * x.x;
                 Ok[s].y += gx * x.y;
             }
