@@ -496,8 +496,9 @@ __device__ __forceinline__ void stress3d(ParFn P, const float e[6], float s[6]) 
 
 // ---------------------------------------------------------------- kernel
 template <int MAT, bool TISO, int S, int ATT>
-// classes 1 and 2 at 96 VGPRs: five waves per SIMD, four workgroups per CU
-__global__ __launch_bounds__(W3D_THREADS) __attribute__((amdgpu_waves_per_eu(S <= 2 ? 5 : 1)))
+// classes 1 and 2 at 96 VGPRs (five waves per SIMD: four workgroups per CU),
+// class 3 at 128 (three workgroups)
+__global__ __launch_bounds__(W3D_THREADS) __attribute__((amdgpu_waves_per_eu(S <= 2 ? 5 : S == 3 ? 4 : 1)))
 void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     constexpr int NPAR = MatParams<MAT>::v;
@@ -632,20 +633,7 @@ void solid3d_kernel(Elem3DArgs a) {
         wave_lds_fence();
         // C2R (pass 0), stress on the samples, R2C (pass 1): one copy of the
         // transform code serves both directions
-        // the point's moduli on this lane's samples, requested before the C2R
-        // so that their latency hides behind it (up to 2 S samples per lane)
         const float *mp = mbase + (int64_t)ip * NPAR * Nr;
-        constexpr bool kPre = (NPAR <= 5 && S == 3);
-        float pm[kPre ? 2 * S : 1][NPAR];
-        if constexpr (kPre) {
-#pragma unroll
-            for (int t = 0; t < 2 * S; t++) {
-                const int n = lane + 64 * t;
-                if (n < Nr)
-#pragma unroll
-                    for (int p = 0; p < NPAR; p++) pm[t][p] = mp[p * Nr + n];
-            }
-        }
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
             wfft<128 * S>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, lane);
@@ -667,16 +655,8 @@ void solid3d_kernel(Elem3DArgs a) {
 #pragma unroll
                 for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };
-            if constexpr (kPre) {
-#pragma unroll
-                for (int t = 0; t < 2 * S; t++) {
-                    const int n = lane + 64 * t;
-                    if (n < Nr) sample(n, [&](int p) { return pm[t][p]; });
-                }
-            } else {
 #pragma unroll 1
-                for (int n = lane; n < Nr; n += 64) sample(n, [&](int p) { return mp[p * Nr + n]; });
-            }
+            for (int n = lane; n < Nr; n += 64) sample(n, [&](int p) { return mp[p * Nr + n]; });
             wave_lds_fence();
         }
 
