@@ -16,6 +16,8 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -129,6 +131,7 @@ struct axg_ctx {
     std::vector<float> h_mat3d;
     std::vector<int64_t> h_mat3d_off;
     std::vector<char> nr_used;
+    std::vector<int> nr_class;  // size class of the 3-D elements of each Nr (0: points only)
     DevBuf<float> d_mat3d;
     DevBuf<int64_t> d_mat3d_off;
     DevBuf<float2> d_twiddle;
@@ -237,24 +240,19 @@ std::vector<int> fftFactors(int n) {
 
 uint32_t hostMagic(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
 
-// Stage plan of a length-n transform on three interleaved sequences: the
-// power of two split into radices 2, 4, 8, 16 and each stage's lane mapping
-// (one task per butterfly and sequence, or one butterfly of all sequences per
-// lane) chosen by a VALU-instruction model of the two stage kernels; odd
-// primes as they are.  {radix, lane} per stage; empty if n has another prime.
-std::vector<std::pair<int, int>> planStages(int n) {
-    std::vector<int> f = fftFactors(n);
-    if (f.empty() && n != 1) return {};
-    int p2 = 0, m = n;
-    while (m % 2 == 0) {
-        m /= 2;
-        p2++;
-    }
-    std::vector<int> odd;
-    for (int r : f)
-        if (r % 2) odd.push_back(r);
-    auto dft = [](int R) {
-        switch (R) {
+// Stage plans of a length-n transform on three interleaved sequences: the
+// power of two split into radices 2, 4, 8, 16 (every composition), odd primes
+// as they are, and each stage's lane mapping (one task per butterfly and
+// sequence, or one butterfly of all sequences per lane).  The model plan picks
+// the composition and mappings by a VALU-instruction model of the two stage
+// kernels (deterministic, the default, like FFTW_ESTIMATE); the measured
+// planner (axg_set_fft_planner) times the candidates on the device, like
+// FFTW_MEASURE.  A plan is {radix, lane} per stage; empty if n has another prime.
+using FftPlan = std::vector<std::pair<int, int>>;
+
+double stageModel(int n, int R, int lane) {
+    auto dft = [](int r) {
+        switch (r) {
             case 2: return 4.;
             case 3: return 14.;
             case 4: return 16.;
@@ -266,28 +264,31 @@ std::vector<std::pair<int, int>> planStages(int n) {
             default: return 144.;  // 16
         }
     };
-    auto stage = [&](int R, int &lane) {
-        const int nb = n / R;
-        const double task = std::ceil(3. * nb / 64.) * (15. + dft(R) + 6. * (R - 1) + 2. * R);
-        const bool canLane = (R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8);
-        const double ln = canLane ? std::ceil(nb / 64.) * (15. + 3. * dft(R) + 14. * (R - 1) + 6. * R) : 1e30;
-        lane = ln < task ? 1 : 0;
-        return std::min(task, ln);
-    };
-    double best = 1e300;
-    std::vector<int> bestPow;
+    const int nb = n / R;
+    if (lane) return std::ceil(nb / 64.) * (15. + 3. * dft(R) + 14. * (R - 1) + 6. * R);
+    return std::ceil(3. * nb / 64.) * (15. + dft(R) + 6. * (R - 1) + 2. * R);
+}
+
+bool laneCapable(int R) { return R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8; }
+
+// every candidate plan and, in *model, the index of the model's choice
+std::vector<FftPlan> planCandidates(int n, int *model) {
+    std::vector<FftPlan> out;
+    std::vector<int> f = fftFactors(n);
+    if (f.empty() && n != 1) return out;
+    int p2 = 0, m = n;
+    while (m % 2 == 0) {
+        m /= 2;
+        p2++;
+    }
+    std::vector<int> odd;
+    for (int r : f)
+        if (r % 2) odd.push_back(r);
+    std::vector<std::vector<int>> comps;
     std::vector<int> cur;
-    // compositions of p2 into parts 1..4 (radix 2..16), non-increasing
     std::function<void(int, int)> rec = [&](int left, int maxPart) {
         if (left == 0) {
-            double c = 0.;
-            int l;
-            for (int e : cur) c += stage(1 << e, l);
-            for (int r : odd) c += stage(r, l);
-            if (c < best) {
-                best = c;
-                bestPow = cur;
-            }
+            comps.push_back(cur);
             return;
         }
         for (int e = std::min(left, maxPart); e >= 1; e--) {
@@ -297,18 +298,53 @@ std::vector<std::pair<int, int>> planStages(int n) {
         }
     };
     rec(p2, 4);
-    std::vector<std::pair<int, int>> out;
-    for (int e : bestPow) {
-        int l;
-        stage(1 << e, l);
-        out.push_back({1 << e, l});
-    }
-    for (int r : odd) {
-        int l;
-        stage(r, l);
-        out.push_back({r, l});
+    double best = 1e300;
+    *model = 0;
+    for (const auto &c : comps) {
+        std::vector<int> radices;
+        for (int e : c) radices.push_back(1 << e);
+        radices.insert(radices.end(), odd.begin(), odd.end());
+        // three mappings: the model's per-stage choice, all tasks, all lanes
+        for (int variant = 0; variant < 3; variant++) {
+            FftPlan pl;
+            double cost = 0.;
+            for (int R : radices) {
+                int lane = 0;
+                if (variant == 0) lane = laneCapable(R) && stageModel(n, R, 1) < stageModel(n, R, 0);
+                else if (variant == 2) lane = laneCapable(R);
+                pl.push_back({R, lane});
+                cost += stageModel(n, R, lane);
+            }
+            if (std::find(out.begin(), out.end(), pl) != out.end()) continue;
+            if (variant == 0 && cost < best) {
+                best = cost;
+                *model = (int)out.size();
+            }
+            out.push_back(pl);
+        }
     }
     return out;
+}
+
+// measured plans chosen so far in this process, per (Nr, size class)
+std::mutex g_fftTuneMutex;
+std::map<std::pair<int, int>, FftPlan> g_fftTuned;
+int g_fftPlanner = 0;  // 0 model, 1 measured
+
+void writePlan(const FftPlan &f, int n, int32_t *dsc0) {
+    int Ns = 1;
+    for (size_t st = 0; st < f.size(); st++) {
+        int32_t *dsc = dsc0 + st * FS_N;
+        const int R = f[st].first, nb = n / R;
+        dsc[FS_R] = R;
+        dsc[FS_NS] = Ns;
+        dsc[FS_NB] = nb;
+        dsc[FS_MNB] = (int32_t)hostMagic((uint32_t)nb);
+        dsc[FS_MNS] = (int32_t)hostMagic((uint32_t)Ns);
+        dsc[FS_WMUL] = nb / Ns;
+        dsc[FS_LANE] = f[st].second;
+        Ns *= R;
+    }
 }
 
 void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att,
@@ -330,7 +366,10 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
             throw std::runtime_error("FieldFFT::transformF2P || Nr = " + std::to_string(nr) +
                                      " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
         c->nr_used[nr] = 1;
-        byClass[w3dClass(((nr + 1) / 2 + 63) / 64)].push_back(e);
+        const int cls = w3dClass(((nr + 1) / 2 + 63) / 64);
+        if (c->nr_class.empty()) c->nr_class.assign(E3D_MAX_NR + 1, 0);
+        c->nr_class[nr] = std::max(c->nr_class[nr], cls);
+        byClass[cls].push_back(e);
         // material: descriptor [param][n][point] -> [point][param][n]
         int64_t off = (int64_t)c->h_mat3d.size();
         c->h_mat3d_off[e] = off;
@@ -363,6 +402,47 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
     }
 }
 
+// The measured planner: the candidate plans of a length-n transform timed
+// with the probe of the elements' size class; the fastest is kept for the
+// rest of the process (every later context reuses it).
+FftPlan tunedPlan(axg_ctx *c, int n, int cls, const std::vector<FftPlan> &cand, const std::vector<float2> &tw,
+                  int twoff) {
+    std::lock_guard<std::mutex> lock(g_fftTuneMutex);
+    auto key = std::make_pair(n, cls);
+    auto it = g_fftTuned.find(key);
+    if (it != g_fftTuned.end()) return it->second;
+    DevBuf<float2> dtw, sink;
+    dtw.upload(tw.data() + twoff, (size_t)n, c->stream);
+    sink.alloc(64);
+    DevBuf<int32_t> dplan;
+    std::vector<int32_t> h((size_t)E3D_MAX_STAGES * FS_N);
+    float best = 1e30f;
+    FftPlan bestPlan = cand[0];
+    for (const FftPlan &f : cand) {
+        if ((int)f.size() > E3D_MAX_STAGES) continue;
+        std::fill(h.begin(), h.end(), 0);
+        writePlan(f, n, h.data());
+        dplan.upload(h, c->stream);
+        float ms = 1e30f;
+        const int reps = 2;
+        switch (cls) {
+            case 1: ms = probeFFT<1>(dplan.p, (int)f.size(), dtw.p, n, reps, sink.p, c->stream, c->ev0, c->ev1); break;
+            case 2: ms = probeFFT<2>(dplan.p, (int)f.size(), dtw.p, n, reps, sink.p, c->stream, c->ev0, c->ev1); break;
+            case 3: ms = probeFFT<3>(dplan.p, (int)f.size(), dtw.p, n, reps, sink.p, c->stream, c->ev0, c->ev1); break;
+            case 5: ms = probeFFT<5>(dplan.p, (int)f.size(), dtw.p, n, reps, sink.p, c->stream, c->ev0, c->ev1); break;
+            default: ms = probeFFT<7>(dplan.p, (int)f.size(), dtw.p, n, reps, sink.p, c->stream, c->ev0, c->ev1); break;
+        }
+        HIPCHECK(hipGetLastError());
+        if (ms < best) {
+            best = ms;
+            bestPlan = f;
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    g_fftTuned[key] = bestPlan;
+    return bestPlan;
+}
+
 void finishBuckets3D(axg_ctx *c) {
     bool any = false;
     for (char u : c->nr_used) any = any || u;
@@ -381,22 +461,14 @@ void finishBuckets3D(axg_ctx *c) {
             double ang = -2. * 3.141592653589793238463 * t / n;
             tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
         }
-        const std::vector<std::pair<int, int>> f = planStages(n);
+        int model = 0;
+        const std::vector<FftPlan> cand = planCandidates(n, &model);
+        if (cand.empty()) throw std::runtime_error("FieldFFT || no radix plan for Nr = " + std::to_string(n) + ".");
+        FftPlan f = cand[model];
+        if (g_fftPlanner == 1 && c->nr_class[n] > 0) f = tunedPlan(c, n, c->nr_class[n], cand, tw, twoff[n]);
         if ((int)f.size() > E3D_MAX_STAGES) throw std::runtime_error("FieldFFT || too many FFT stages.");
         nstage[n] = (int32_t)f.size();
-        int Ns = 1;
-        for (size_t st = 0; st < f.size(); st++) {
-            int32_t *dsc = &plan[((size_t)n * E3D_MAX_STAGES + st) * FS_N];
-            const int R = f[st].first, nb = n / R;
-            dsc[FS_LANE] = f[st].second;
-            dsc[FS_R] = R;
-            dsc[FS_NS] = Ns;
-            dsc[FS_NB] = nb;
-            dsc[FS_MNB] = (int32_t)hostMagic((uint32_t)nb);
-            dsc[FS_MNS] = (int32_t)hostMagic((uint32_t)Ns);
-            dsc[FS_WMUL] = nb / Ns;
-            Ns *= R;
-        }
+        writePlan(f, n, &plan[(size_t)n * E3D_MAX_STAGES * FS_N]);
     }
     c->d_twiddle.upload(tw, s);
     c->d_tw_off.upload(twoff, s);
@@ -1629,6 +1701,13 @@ int axg_learn_wisdom(axg_ctx *c, float cutoff) {
         doLearn(c, clampCutoff(cutoff), 1, nullptr);
         HIPCHECK(hipGetLastError());
     })
+}
+
+int axg_set_fft_planner(int32_t measured) {
+    if (measured != 0 && measured != 1) return fail(AXG_EINVAL, "axg_set_fft_planner || 0 (model) or 1 (measured).");
+    std::lock_guard<std::mutex> lock(g_fftTuneMutex);
+    g_fftPlanner = measured;
+    return AXG_OK;
 }
 
 int axg_set_learning(axg_ctx *c, float cutoff, int32_t interval) {

@@ -57,4 +57,19 @@ void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att
     }
 }
 
+template <>
+float probeFFT<S>(const int32_t *plan, int nst, const float2 *tw, int N, int reps, float2 *sink, hipStream_t s,
+                  hipEvent_t e0, hipEvent_t e1) {
+    allowLds(&fft_probe_kernel<S>);
+    const size_t lds = (2 * (size_t)N + E3D_MAX_STAGES * FS_N / 2 + 15 * (size_t)N) * sizeof(float2);
+    const dim3 grid(256 * 4), block(W3D_THREADS);
+    hipLaunchKernelGGL(fft_probe_kernel<S>, grid, block, lds, s, plan, nst, tw, N, reps, sink);  // warm-up
+    (void)hipEventRecord(e0, s);
+    hipLaunchKernelGGL(fft_probe_kernel<S>, grid, block, lds, s, plan, nst, tw, N, reps, sink);
+    (void)hipEventRecord(e1, s);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return 1e30f;
+    return ms;
+}
+
 }  // namespace axg

@@ -937,4 +937,37 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 template <int S>
 void launchElem3D(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s);
 
+// The measured FFT planner's probe: every wave of a full grid of 5-wave
+// workgroups runs `reps` C2R + R2C pairs of length N on three sequences with
+// the class-S transform code and the stage table `plan`; returns the time of
+// the launch in ms (events e0, e1 on stream s).
+template <int S>
+__global__ __launch_bounds__(W3D_THREADS) void fft_probe_kernel(const int32_t *plan, int nst, const float2 *tw, int N,
+                                                                int reps, float2 *sink) {
+    extern __shared__ float2 lds[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    float2 *twf = lds, *twi = twf + N;
+    int32_t *pl = reinterpret_cast<int32_t *>(twi + N);
+    float2 *seq = twi + N + E3D_MAX_STAGES * FS_N / 2 + w * 3 * N;
+    for (int t = threadIdx.x; t < N; t += W3D_THREADS) {
+        const float2 v = tw[t];
+        twf[t] = v;
+        twi[t] = make_float2(v.x, -v.y);
+    }
+    for (int t = threadIdx.x; t < E3D_MAX_STAGES * FS_N; t += W3D_THREADS) pl[t] = plan[t];
+    for (int t = lane; t < 3 * N; t += 64) seq[t] = make_float2(1e-3f * (float)(t % 7), 1e-3f * (float)(t % 5));
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < reps; r++) {
+        wfft<128 * S>(seq, N, +1, pl, nst, twi, lane);
+        wfft<128 * S>(seq, N, -1, pl, nst, twf, lane);
+    }
+    if (seq[lane].x == 1.2345e30f) sink[lane] = seq[lane];  // keeps the transforms
+}
+
+template <int S>
+float probeFFT(const int32_t *plan, int nst, const float2 *tw, int N, int reps, float2 *sink, hipStream_t s,
+               hipEvent_t e0, hipEvent_t e1);
+
 }  // namespace axg

@@ -20,6 +20,14 @@ def _check(rc):
         raise RuntimeError(lib().axg_last_error().decode())
 
 
+def set_fft_planner(measured):
+    """FFT planning of the contexts created afterwards: False = instruction
+    model (deterministic), True = measured on the device once per Nr and
+    process (axg_set_fft_planner)."""
+    from . import lib
+    _check(lib().axg_set_fft_planner(1 if measured else 0))
+
+
 class Domain:
     """The whole mesh resident on one GPU (replaces Mesh::release -> Domain)."""
 

@@ -102,6 +102,7 @@ def probe(workload):
     torch.neg(a, out=b)  # one streaming kernel: reads and writes COPY_BYTES
     torch.cuda.synchronize()
     del a, b
+    ax.set_fft_planner(True)  # the bench's default planning
     mesh = ax.ShellMesh(which=workload)
     dom = ax.Domain(mesh)
     dom.resetZero()
@@ -265,6 +266,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--fft-plan", choices=("model", "measured"), default="measured",
+                    help="FFT planning (the analogue of FFTW_ESTIMATE / FFTW_PATIENT)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-sims", type=int, default=16)
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -283,6 +286,7 @@ def main():
     import axisem3d_amd as ax
 
     which = args.workload
+    ax.set_fft_planner(args.fft_plan == "measured")
     t0 = time.perf_counter()
     mesh = ax.ShellMesh(which=which)
     t_mesh = time.perf_counter() - t0
@@ -336,6 +340,7 @@ def main():
             "algorithmic_bytes_per_step": bytes_step,
             "dt_s": mesh.d.dt,
             "parallelism": "1 GPU, whole mesh in HBM",
+            "fft_plan": args.fft_plan,
         },
         "roofline": {
             "bound": "hbm",

@@ -252,6 +252,14 @@ int axg_create(const axg_mesh_desc *desc, int device, axg_ctx **ctx);
  * own); axg_run steps all of them.  Seismograms are [nstep][nrec_total][3] and
  * points / stations of simulation k start at point_off[k] / rec_off[k]. */
 int axg_create_batch(const axg_mesh_desc *const *descs, int32_t nsim, int device, axg_ctx **ctx);
+
+/* FFT planning for the contexts created afterwards, the analogue of the
+ * reference's FFTW planner flags (SolverFFTW.cpp:14-64, FFTW_ESTIMATE when
+ * wisdom is disabled, FFTW_PATIENT otherwise): 0 = instruction model
+ * (deterministic, the default), 1 = measured (each Nr's candidate plans timed
+ * on the device once per process and the fastest kept; results may then differ
+ * from a model-planned run in the last bits). */
+int axg_set_fft_planner(int32_t measured);
 int axg_batch_layout(axg_ctx *ctx, int32_t *point_off, int32_t *rec_off, int32_t n); /* n >= nsim + 1 */
 int axg_destroy(axg_ctx *ctx);
 

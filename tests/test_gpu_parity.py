@@ -531,3 +531,26 @@ def test_large_nr_size_classes():
         err = rel_l2(a, b)
         print("large-Nr field %d rel L2 %.3e" % (w, err), flush=True)
         assert err < 1e-5
+
+
+def test_measured_fft_planner_matches_oracle(small_layers):
+    """The measured FFT planner (every candidate plan timed on the device, the
+    analogue of FFTW_MEASURE) picks other stage plans than the instruction
+    model for some lengths; the stiffness must match the FP64 oracle all the
+    same."""
+    print("case measured fft planner", flush=True)
+    m = _mesh3d(small_layers, _abi.MAT_TRANSVERSELY_ISO, 40, 128, nstep=10, nelem_theta=8)
+    u = random_field(m, _abi.FIELD_SOLID_DISPL, seed=5)
+    r = RefDomain(m, double=True)
+    r.set_field(_abi.FIELD_SOLID_DISPL, u)
+    r.computeStiff()
+    ax.set_fft_planner(True)
+    try:
+        g = ax.Domain(m)
+    finally:
+        ax.set_fft_planner(False)
+    g.set_field(_abi.FIELD_SOLID_DISPL, u)
+    g.computeStiff()
+    err = rel_l2(g.get_field(_abi.FIELD_SOLID_STIFF), r.get_field(_abi.FIELD_SOLID_STIFF))
+    print("measured-plan stiff rel L2 %.3e" % err, flush=True)
+    assert err < 1e-5
