@@ -36,11 +36,14 @@ LEARN_LAYERS = [
 ]
 
 
-def model_config(seed, nstep, nelem_theta=40, nu=None, elem_nu=None):
+def model_config(seed, nstep, nelem_theta=40, nu=None, elem_nu=None, radial_scale=1):
     """One seeded model of the family: TI solid, GRF dvs/dvp of +-5 % with a
     500 km correlation length, seeded moment tensor and stations; uniform Nu
-    `nu` or the per-element Nu `elem_nu`."""
-    kw = dict(layers=LEARN_LAYERS, nelem_theta=nelem_theta, material=_abi.MAT_TRANSVERSELY_ISO, dv_amp=0.05,
+    `nu` or the per-element Nu `elem_nu`.  The mesh has nelem_theta x (50
+    radial_scale) quads; the model, source and stations depend only on the
+    seed, so meshes of one seed at several resolutions describe one model."""
+    layers = [dict(l, nelem_radial=l["nelem_radial"] * radial_scale) for l in LEARN_LAYERS]
+    kw = dict(layers=layers, nelem_theta=nelem_theta, material=_abi.MAT_TRANSVERSELY_ISO, dv_amp=0.05,
               dv_corr_len=500e3, seed=seed, nstep=nstep, nrec=20)
     if elem_nu is not None:
         kw["elem_nu"] = elem_nu
@@ -117,22 +120,24 @@ class LearnedNu:
         flagged = ((f < self.lo - tol) | (f > self.hi + tol)).any(1)
         return element_nu(mesh, nu, flagged, self.nu_full), int(flagged.sum())
 
-    def evaluate(self, seed):
-        """Full-Nu judge against the predicted-Nu solve of one new model:
-        wall-clock to solution of each (mesh set-up + solve; the learned path
-        also builds the feature mesh and runs the predictor) and the misfit."""
+    def evaluate(self, seed, nelem_theta=None, radial_scale=1):
+        """Full-Nu judge against the predicted-Nu solve of one new model
+        (optionally on a finer mesh than the training models'): wall-clock to
+        solution of each (mesh set-up + solve; the learned path also builds
+        the feature mesh and runs the predictor) and the misfit."""
+        nt = nelem_theta or self.nelem_theta
         t0 = time.perf_counter()
-        mf = ShellMesh(model_config(seed, self.nstep, self.nelem_theta, nu=self.nu_full))
+        mf = ShellMesh(model_config(seed, self.nstep, nt, nu=self.nu_full, radial_scale=radial_scale))
         sf, _, solve_full = solve(mf)
         wall_full = time.perf_counter() - t0
         t0 = time.perf_counter()
-        m1 = ShellMesh(model_config(seed, self.nstep, self.nelem_theta, nu=1))  # features only
+        m1 = ShellMesh(model_config(seed, self.nstep, nt, nu=1, radial_scale=radial_scale))  # features only
         enu, nflag = self.predict_element_nu(m1)
-        mp = ShellMesh(model_config(seed, self.nstep, self.nelem_theta, elem_nu=enu))
+        mp = ShellMesh(model_config(seed, self.nstep, nt, elem_nu=enu, radial_scale=radial_scale))
         sp, _, solve_pred = solve(mp)
         wall_pred = time.perf_counter() - t0
         misfit = float(np.linalg.norm(sp - sf) / np.linalg.norm(sf))
-        return dict(seed=seed, misfit=misfit, solve_full_s=solve_full, solve_pred_s=solve_pred,
+        return dict(seed=seed, quads=int(mf.d.nelem), misfit=misfit, solve_full_s=solve_full, solve_pred_s=solve_pred,
                     wall_full_s=wall_full, wall_pred_s=wall_pred, flagged_points=nflag,
                     mean_nu_full=float(point_nu(mf).mean()), mean_nu_pred=float(point_nu(mp).mean()),
                     updates_full=int(mf.stats["sum_nu1"]), updates_pred=int(mp.stats["sum_nu1"]))

@@ -225,9 +225,9 @@ def batched(nsim, steps, warmup=10):
 def learned_nu(n_train, nstep, n_test=2, nu_full=128, cutoff=1e-2):
     """The second half of the metric: wall-clock to solution at the Nu a
     trained predictor gives, against the full-Nu solve that judges its
-    seismogram misfit, on held-out seeded models (2000-quad solid + fluid TI
+    seismogram misfit, on held-out seeded models (8000-quad solid + fluid TI
     shells with a +-5 % random 3-D model).  Training data: the wisdom of
-    full-Nu runs of `n_train` other seeded models."""
+    full-Nu runs of `n_train` other seeded models on 2000-quad meshes."""
     from axisem3d_amd import nu_learning as nl
     t0 = time.perf_counter()
     L = nl.LearnedNu(nu_full=nu_full, nstep=nstep, cutoff=cutoff)
@@ -237,10 +237,11 @@ def learned_nu(n_train, nstep, n_test=2, nu_full=128, cutoff=1e-2):
     t0 = time.perf_counter()
     mse = L.train(epochs=40)
     t_train = time.perf_counter() - t0
-    runs = [L.evaluate(5000 + j) for j in range(n_test)]
+    runs = [L.evaluate(5000 + j, nelem_theta=80, radial_scale=2) for j in range(n_test)]
     L.mlp.close()
     return {
-        "models": {"train": n_train, "test": n_test, "quads": 2000, "steps": nstep, "nu_full": nu_full},
+        "models": {"train": n_train, "test": n_test, "train_quads": 2000, "test_quads": runs[0]["quads"],
+                   "steps": nstep, "nu_full": nu_full},
         "wisdom_cutoff": cutoff,
         "misfit_max": max(r["misfit"] for r in runs),
         "wall_speedup": sum(r["wall_full_s"] for r in runs) / sum(r["wall_pred_s"] for r in runs),
