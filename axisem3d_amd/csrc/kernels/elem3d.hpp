@@ -557,6 +557,20 @@ void solid3d_kernel(Elem3DArgs a) {
             tc2 = ldsuf(tab + kTabTiso + 75 + ip);
         }
 
+        // the step's column points (m, k) and row points (w, m): first rows,
+        // largest alphas and derivative weights, read once per step (scalar)
+        int rc[5], rr[5], ac[5], ar[5];
+        float gxw[5], gek[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            rc[m] = ldsu(tab + kTabRow + m * 5 + k);
+            rr[m] = ldsu(tab + kTabRow + w * 5 + m);
+            ac[m] = ldsu(tab + kTabAmax + m * 5 + k);
+            ar[m] = ldsu(tab + kTabAmax + w * 5 + m);
+            gxw[m] = Gx[m * 5 + w];
+            gek[m] = Ge[m * 5 + k];
+        }
+
         // ---------------- strain spectrum -> 3 Hermitian-extended sequences
 #pragma unroll 1
         for (int s = 0; s < S; s++) {
@@ -572,14 +586,13 @@ void solid3d_kernel(Elem3DArgs a) {
                 for (int c = 0; c < 3; c++) GU[c] = UG[c] = u[c] = czero();
 #pragma unroll
                 for (int m = 0; m < 5; m++) {
-                    // column k: point (m, k); row w: point (w, m)
-                    const int pc = m * 5 + k, pr = w * 5 + m;
-                    // rows above a point's largest alpha are read (the field
-                    // is padded) and weighted by zero
-                    const bool okc = al <= ldsu(tab + kTabAmax + pc), okr = al <= ldsu(tab + kTabAmax + pr);
-                    const float gx = okc ? Gx[m * 5 + w] : 0.f, ge = okr ? Ge[m * 5 + k] : 0.f;
-                    const float2 *uc = a.displ + (int64_t)ldsu(tab + kTabRow + pc) * 3;  // uniform row bases
-                    const float2 *ur = a.displ + (int64_t)ldsu(tab + kTabRow + pr) * 3;
+                    // column k: point (m, k); row w: point (w, m).  Rows above a
+                    // point's largest alpha are read (the field is padded) and
+                    // weighted by zero
+                    const bool okc = al <= ac[m], okr = al <= ar[m];
+                    const float gx = okc ? gxw[m] : 0.f, ge = okr ? gek[m] : 0.f;
+                    const float2 *uc = a.displ + (int64_t)rc[m] * 3;  // uniform row bases
+                    const float2 *ur = a.displ + (int64_t)rr[m] * 3;
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
                         float2 vc = uc[al * 3 + c];
@@ -737,10 +750,9 @@ void solid3d_kernel(Elem3DArgs a) {
         // G_eta(jp, k) Y(w, k), the point (w, k) also its xi and own terms
 #pragma unroll
         for (int jp = 0; jp < 5; jp++) {
-            const int pp = w * 5 + jp;
-            const int amax = ldsu(tab + kTabAmax + pp);
+            const int amax = ar[jp];
             const float ge = Ge[jp * 5 + k];
-            float2 *sp0 = a.stiff + (int64_t)ldsu(tab + kTabRow + pp) * 3;
+            float2 *sp0 = a.stiff + (int64_t)rr[jp] * 3;
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 const int al = lane + 64 * s;
