@@ -157,6 +157,11 @@ struct axg_ctx {
     std::vector<int32_t> sim_point_off, sim_rec_off;
     // receivers
     DevBuf<int32_t> d_rec_elem, d_rec_idx_s, d_rec_idx_f;
+    // stations in 3-D fluid elements: per station its element's launch info
+    int32_t nrec_f3 = 0, recf3_class = 1, recf3_maxNr = 0, recf3_maxM = 0;
+    DevBuf<int32_t> d_rec_idx_f3;
+    DevBuf<int4> d_recf3_info;
+    std::vector<int32_t> h_twoff, h_nstage;  // FFT table offsets and stage counts per Nr
     int32_t nrec_solid = 0, nrec_fluid = 0;
     DevBuf<float> d_rec_w, d_rec_phi, d_rec_cost, d_rec_sint, d_seis;
     int32_t rec_rtz = 1;
@@ -474,6 +479,8 @@ void finishBuckets3D(axg_ctx *c) {
     c->d_tw_off.upload(twoff, s);
     c->d_plan.upload(plan, s);
     c->d_nstage.upload(nstage, s);
+    c->h_twoff = twoff;
+    c->h_nstage = nstage;
     for (auto &b : c->buckets3d) {
         std::vector<int4> info(b->h_elems.size());
         for (size_t i = 0; i < info.size(); i++) {
@@ -823,6 +830,44 @@ void doCouple(axg_ctx *c) {
                        c->S.displ.p, c->S.stiff.p, c->F.stiff.p);
 }
 
+// FluidElement::computeGroundMotion of the stations in 3-D fluid elements
+// (the station mode of the 3-D fluid kernel, one workgroup per station)
+void recordFluid3D(axg_ctx *c, int tstep, const int *tstep_ptr) {
+    Elem3DArgs a;
+    std::memset(&a, 0, sizeof(a));
+    std::memcpy(a.G, c->G, sizeof(a.G));
+    a.einfo = c->d_recf3_info.p;
+    a.nelem = c->nrec_f3;
+    a.maxM = c->recf3_maxM;
+    a.maxNr = c->recf3_maxNr;
+    a.geom = c->d_geom.p;
+    a.mat3d = c->d_mat3d.p;
+    a.mat3d_off = c->d_mat3d_off.p;
+    a.ep_row = c->d_ep_row.p;
+    a.ep_amax = c->d_ep_amax.p;
+    a.displ = c->F.displ.p;
+    a.twiddle = c->d_twiddle.p;
+    a.plan = c->d_plan.p;
+    a.rec_idx = c->d_rec_idx_f3.p;
+    a.rec_w = c->d_rec_w.p;
+    a.rec_phi = c->d_rec_phi.p;
+    a.rec_cost = c->d_rec_cost.p;
+    a.rec_sint = c->d_rec_sint.p;
+    a.seis = c->d_seis.p;
+    a.nrec_total = c->nrec;
+    a.rtz = c->rec_rtz;
+    a.tstep = tstep;
+    a.tstep_ptr = tstep_ptr;
+    const size_t lds = (2 * (size_t)a.maxNr + 2 * W3D_WAVES * (size_t)a.maxNr + kTabSlots) * sizeof(float2);
+    switch (c->recf3_class) {
+        case 1: launchRecFluid3D<1>(a, lds, c->stream); break;
+        case 2: launchRecFluid3D<2>(a, lds, c->stream); break;
+        case 3: launchRecFluid3D<3>(a, lds, c->stream); break;
+        case 5: launchRecFluid3D<5>(a, lds, c->stream); break;
+        default: launchRecFluid3D<7>(a, lds, c->stream); break;
+    }
+}
+
 void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
     if (c->nrec == 0) return;
     if (tstep < 0 || tstep >= c->nstep) throw std::runtime_error("Domain::record || time step out of range.");
@@ -832,6 +877,7 @@ void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
                            c->d_rec_idx_s.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
                            c->d_rec_sint.p, c->d_ep_row.p, c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz,
                            tstep, tstep_ptr, c->d_seis.p);
+    if (c->nrec_f3) recordFluid3D(c, tstep, tstep_ptr);
     if (c->nrec_fluid)
         hipLaunchKernelGGL(record_fluid_kernel, dim3(c->nrec_fluid), dim3(256), 0, c->stream, c->nrec_fluid, c->nrec,
                            c->d_rec_idx_f.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
@@ -1165,14 +1211,13 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
     c->rec_rtz = d->rec_components == 0;
     if (c->nrec > 0) {
         std::vector<float> cost(c->nrec), sint(c->nrec);
-        std::vector<int32_t> idx_s, idx_f;
+        std::vector<int32_t> idx_s, idx_f, idx_f3;
         for (int i = 0; i < c->nrec; i++) {
             int e = d->rec_elem[i];
             if (e < 0 || e >= NE) throw std::runtime_error("PointwiseRecorder::addReceiver || bad element.");
             if (d->elem_flags[e] & AXG_EL_FLUID) {
-                if (d->elem_flags[e] & AXG_EL_3D)
-                    throw std::runtime_error("FluidElement::computeGroundMotion || receivers in azimuth-dependent fluid elements are not supported by this build.");
-                idx_f.push_back(i);
+                if (d->elem_flags[e] & AXG_EL_3D) idx_f3.push_back(i);
+                else idx_f.push_back(i);
             } else {
                 idx_s.push_back(i);
             }
@@ -1183,6 +1228,19 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
         c->nrec_fluid = (int32_t)idx_f.size();
         if (!idx_s.empty()) c->d_rec_idx_s.upload(idx_s, s);
         if (!idx_f.empty()) c->d_rec_idx_f.upload(idx_f, s);
+        c->nrec_f3 = (int32_t)idx_f3.size();
+        if (c->nrec_f3) {
+            std::vector<int4> info;
+            for (int i : idx_f3) {
+                const int e = d->rec_elem[i], nr = c->elem_nr[e];
+                info.push_back(make_int4(e, nr, c->elem_flags[e] | (c->h_nstage[nr] << 8), c->h_twoff[nr]));
+                c->recf3_class = std::max(c->recf3_class, w3dClass(((nr + 1) / 2 + 63) / 64));
+                c->recf3_maxNr = std::max(c->recf3_maxNr, nr);
+            }
+            c->recf3_maxM = std::min(c->recf3_maxNr / 2 + 1, 64 * c->recf3_class);
+            c->d_rec_idx_f3.upload(idx_f3, s);
+            c->d_recf3_info.upload(info, s);
+        }
         c->d_rec_elem.upload(d->rec_elem, c->nrec, s);
         c->d_rec_w.upload(d->rec_weights, (size_t)c->nrec * 25, s);
         c->d_rec_phi.upload(d->rec_phi, c->nrec, s);

@@ -45,8 +45,8 @@ void solidAtt(const Elem3DArgs &a, int att, size_t lds, hipStream_t s) {
 template <>
 void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s) {
     if (phase) {
-        allowLds(&fluid3d_kernel<S>);
-        hipLaunchKernelGGL((fluid3d_kernel<S>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+        allowLds(&fluid3d_kernel<S, false>);
+        hipLaunchKernelGGL((fluid3d_kernel<S, false>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
     } else if (mat == AXG_MAT_ISOTROPIC) {
         if (tiso) solidAtt<0, true>(a, att, lds, s);
         else solidAtt<0, false>(a, att, lds, s);
@@ -55,6 +55,12 @@ void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att
     } else {
         solidAtt<2, true>(a, att, lds, s);
     }
+}
+
+template <>
+void launchRecFluid3D<S>(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+    allowLds(&fluid3d_kernel<S, true>);
+    hipLaunchKernelGGL((fluid3d_kernel<S, true>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
 }
 
 template <>

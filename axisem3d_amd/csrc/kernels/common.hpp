@@ -40,6 +40,19 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 
 // geometry block of an element: q = 0 dsdxii, 1 dsdeta, 2 dzdxii, 3 dzdeta, 4 inv_s
+// station output: RTZ rotation by the colatitude (PointwiseRecorder.cpp:75-93)
+__device__ __forceinline__ void store_station(float us, float up, float uz, int rtz, float cost, float sint, float *o) {
+    if (rtz) {
+        o[0] = us * cost - uz * sint;  // ut
+        o[1] = up;
+        o[2] = us * sint + uz * cost;  // ur
+    } else {
+        o[0] = us;
+        o[1] = up;
+        o[2] = uz;
+    }
+}
+
 struct PointGeom {
     float dsdxii, dsdeta, dzdxii, dzdeta, inv_s;
 };

@@ -76,6 +76,13 @@ struct Elem3DArgs {
     int32_t nsls;
     float sls_alpha[8], sls_beta[8], sls_gamma[8];
     int32_t do_kappa;
+    // stations in 3-D fluid elements (the station mode of fluid3d_kernel):
+    // workgroup b records station rec_idx[b] of rec_w [25] weights at rec_phi
+    const int32_t *rec_idx;
+    const float *rec_w, *rec_phi, *rec_cost, *rec_sint;
+    float *seis;                 // [nstep][nrec_total][3]
+    int32_t nrec_total, rtz, tstep;
+    const int *tstep_ptr;        // device step counter (graph replay) or null
 };
 
 // Attenuation3D_{Full,CG4}::applyToStress + updateMemoryVariables on one
@@ -775,7 +782,11 @@ void solid3d_kernel(Elem3DArgs a) {
 // (Gradient::computeGrad, Gradient.cpp:26-56), C2R of its 3 columns packed in
 // 2 sequences, K(phi) grad chi (Acoustic3D::strainToStress, Acoustic3D.cpp:9-16),
 // R2C x 1/Nr, Gradient::computeQuad (Gradient.cpp:58-82).
-template <int S>
+// REC: FluidElement::computeGroundMotion (FluidElement.cpp:163-215) for the
+// station of each workgroup instead of the quadrature: the same gradient,
+// transforms and K(phi) on every point, then the Fourier sum of the stress
+// spectrum at the station's azimuth, weighted over the 25 points.
+template <int S, bool REC>
 __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -811,6 +822,9 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 
     const float *mbase = a.mat3d + uload64(a.mat3d_off + e);
     const float h = 0.5f / (float)Nr;
+    float racc[3] = {0.f, 0.f, 0.f};  // station mode: this lane's share of the motion
+    const int irec = REC ? __builtin_amdgcn_readfirstlane(a.rec_idx[blockIdx.x]) : 0;
+    const float rphi = REC ? a.rec_phi[irec] : 0.f;
 
 #pragma unroll 1
     for (int k = 0; k < 5; k++) {
@@ -885,6 +899,29 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
             wave_lds_fence();
         }
 
+        if constexpr (REC) {
+            // stress spectrum at the station's azimuth: alpha = 0 real part,
+            // then 2 Re(exp(i alpha phi) sigma_alpha) up to Nu - (Nr even)
+            const float wk = a.rec_w[irec * 25 + ip];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const int al = lane + 64 * s;
+                if (al >= M || (nyq && al == Nu)) continue;
+                float2 za = seq[2 * al], zb = seq[2 * (al == 0 ? 0 : Nr - al)];
+                float2 ya = seq[2 * al + 1], yb = seq[2 * (al == 0 ? 0 : Nr - al) + 1];
+                const float2 sg[3] = {make_float2(h * (za.x + zb.x), h * (za.y - zb.y)),
+                                      make_float2(h * (za.y + zb.y), h * (zb.x - za.x)),
+                                      make_float2(h * (ya.x + yb.x), h * (ya.y - yb.y))};
+                float sn = 0.f, cs = 1.f;
+                if (al > 0) sincosf((float)al * rphi, &sn, &cs);
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    racc[c] += wk * (al == 0 ? sg[c].x : 2.f * (cs * sg[c].x - sn * sg[c].y));
+            }
+            wave_lds_fence();  // the next step packs into the same area
+            continue;
+        }
+
         // ---------------- unpack x 1/Nr, quadrature operands
         float2 Yk[S], Ok[S], Xk[S];
 #pragma unroll
@@ -942,12 +979,39 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
             }
         }
     }
+    if constexpr (REC) {
+        // sum over the lanes of the 5 waves, then SPZ -> RTZ or SPZ (store_station)
+        __shared__ float red[3][W3D_THREADS];
+#pragma unroll
+        for (int c = 0; c < 3; c++) red[c][threadIdx.x] = racc[c];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            float v[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float t = 0.f;
+                for (int i = threadIdx.x; i < W3D_THREADS; i += 64) t += red[c][i];
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+                v[c] = t;
+            }
+            if (threadIdx.x == 0) {
+                const int tstep = a.tstep_ptr ? *a.tstep_ptr : a.tstep;
+                float *out = a.seis + ((int64_t)tstep * a.nrec_total + irec) * 3;
+                store_station(v[0], v[1], v[2], a.rtz, a.rec_cost[irec], a.rec_sint[irec], out);
+            }
+        }
+    }
 }
 
 // Launch of one 3-D bucket of class S (defined, one class per object file, in
 // elem3d_launch.hip): phase 0 solid (material, TI, attenuation kind), 1 fluid.
 template <int S>
 void launchElem3D(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s);
+
+// stations in 3-D fluid elements, one workgroup per station (a.nelem of them)
+template <int S>
+void launchRecFluid3D(const Elem3DArgs &a, size_t lds, hipStream_t s);
 
 // The measured FFT planner's probe: every wave of a full grid of 5-wave
 // workgroups runs `reps` C2R + R2C pairs of length N on three sequences with

@@ -190,19 +190,6 @@ __global__ __launch_bounds__(256) void stability_kernel(int32_t nrow, const int3
     if (!ok) atomicMin(bad, compact_to_point[row_point[r]]);
 }
 
-// station output: RTZ rotation by the colatitude (PointwiseRecorder.cpp:75-93)
-__device__ __forceinline__ void store_station(float us, float up, float uz, int rtz, float cost, float sint, float *o) {
-    if (rtz) {
-        o[0] = us * cost - uz * sint;  // ut
-        o[1] = up;
-        o[2] = us * sint + uz * cost;  // ur
-    } else {
-        o[0] = us;
-        o[1] = up;
-        o[2] = uz;
-    }
-}
-
 // sum over a 256-thread block of three values per thread; thread 0 gets the sums
 __device__ __forceinline__ void block_sum3(float acc[3], float (*red)[256]) {
     for (int c = 0; c < 3; c++) red[c][threadIdx.x] = acc[c];

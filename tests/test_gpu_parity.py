@@ -292,6 +292,28 @@ def test_fluid_receivers_seismograms(fluid_layers):
     assert rel_l2(sg, sr) < 1e-4
 
 
+def test_fluid_receivers_in_3d_fluid(fluid_layers):
+    """Stations inside azimuth-dependent fluid elements: gradient, transforms
+    and K(phi) on the element's points, then the Fourier sum of the stress
+    spectrum at the station's azimuth (FluidElement.cpp:163-215)."""
+    print("case fluid receivers 3-D", flush=True)
+    cfg = ax.shell_config(1, layers=fluid_layers, nelem_theta=12, nstep=300, nrec=6, rec_depth=6.371e6 - 2.5e6,
+                          nu_mode=1, nu_min=4, nu_max=30, dv_amp=0.05, dv_fluid=1)
+    m = ax.ShellMesh(cfg)
+    fl = m.array("elem_flags")[m.array("rec_elem")]
+    assert (fl & _abi.EL_FLUID).all() and (fl & _abi.EL_3D).all()
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, m.d.nstep, 100, use_graph=True)
+    r = RefDomain(m, double=False)
+    r.run(1, m.d.nstep, 100)
+    sg, sr = g.seismograms(), r.seismograms()
+    err = rel_l2(sg, sr)
+    print("3-D fluid stations rel L2 %.3e" % err, flush=True)
+    assert np.abs(sr).max() > 0
+    assert err < 1e-4
+
+
 def _decaying_field(m, which, seed):
     """Random point field whose modes decay like 0.6^alpha (a spectrum the
     wisdom cutoff cuts somewhere inside), masked like a displacement."""
