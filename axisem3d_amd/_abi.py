@@ -53,6 +53,8 @@ class MeshDesc(C.Structure):
         ("rec_phi", f32p),
         ("rec_theta", f32p),
         ("rec_components", C.c_int32),
+        ("ocean_off", i64p),
+        ("ocean", f32p),
     ]
 
 
@@ -98,6 +100,8 @@ class ShellConfig(C.Structure):
         ("seed", C.c_uint64),
         ("nthread", C.c_int32),
         ("elem_nu", C.POINTER(C.c_int32)),
+        ("ocean_depth", C.c_double),
+        ("ocean_var", C.c_double),
     ]
 
 

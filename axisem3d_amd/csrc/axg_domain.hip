@@ -89,6 +89,14 @@ struct Phase {
     DevBuf<int32_t> d_m3_q, d_m3_nr;
     DevBuf<int64_t> d_m3_off;
     DevBuf<float> d_m3_inv;
+    // ocean load (solid): MassOcean1D points with {1/(m+m_o), 1/m, sin, cos};
+    // MassOcean3D points with Nr, first float of their rows {1/m, scaled normal}
+    std::vector<int32_t> o1_q, o3_q, o3_nr;
+    std::vector<float> o1_coef, o3_coef;
+    std::vector<int64_t> o3_off;
+    DevBuf<int32_t> d_o1_q, d_o3_q, d_o3_nr;
+    DevBuf<float> d_o1_coef, d_o3_coef;
+    DevBuf<int64_t> d_o3_off;
     PhasePoints view() const {
         PhasePoints P;
         P.row_point = d_row_point.p;
@@ -583,6 +591,12 @@ void buildPhase(const axg_mesh_desc *d, Phase &P, bool solid) {
     P.m3_nr.clear();
     P.m3_off.clear();
     P.m3_inv.clear();
+    P.o1_q.clear();
+    P.o1_coef.clear();
+    P.o3_q.clear();
+    P.o3_nr.clear();
+    P.o3_off.clear();
+    P.o3_coef.clear();
     int32_t row = 0;
     const int64_t *moff = solid ? d->mass_solid_off : d->mass_fluid_off;
     const float *minv = solid ? d->inv_mass_solid : d->inv_mass_fluid;
@@ -604,7 +618,26 @@ void buildPhase(const axg_mesh_desc *d, Phase &P, bool solid) {
         if (nr % 2 == 0) f |= 2;
         if (!solid && (d->point_flags[p] & AXG_PT_FLUID_SURFACE)) f |= 4;
         P.pt_flags.push_back(f);
-        if (nm == 1) {
+        // ocean load: MassOcean1D / MassOcean3D replace the solid mass
+        // (GLLPoint.cpp:55-72); fluid points keep theirs
+        const int64_t no = (solid && d->ocean_off) ? d->ocean_off[p + 1] - d->ocean_off[p] : 0;
+        if (no != 0 && no != 1 && no != nr)
+            throw std::runtime_error("MassOcean3D::checkCompatibility || Incompatible size.");
+        if (no > 1 && nr > E3D_MAX_NR)
+            throw std::runtime_error("MassOcean3D::computeAccel || Nr = " + std::to_string(nr) +
+                                     " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
+        if (no == 1) {
+            // the accel is computed before the Newmark kernel, which then scales by 1
+            P.pt_invmass.push_back(1.f);
+            P.o1_q.push_back(P.compact[p]);
+            P.o1_coef.insert(P.o1_coef.end(), d->ocean + 4 * d->ocean_off[p], d->ocean + 4 * d->ocean_off[p + 1]);
+        } else if (no > 1) {
+            P.pt_invmass.push_back(1.f);
+            P.o3_q.push_back(P.compact[p]);
+            P.o3_nr.push_back(nr);
+            P.o3_off.push_back((int64_t)P.o3_coef.size());
+            P.o3_coef.insert(P.o3_coef.end(), d->ocean + 4 * d->ocean_off[p], d->ocean + 4 * d->ocean_off[p + 1]);
+        } else if (nm == 1) {
             P.pt_invmass.push_back(minv[moff[p]]);
         } else {
             // Mass3D: the accel is computed before the Newmark kernel, which then scales by 1
@@ -636,6 +669,16 @@ void uploadPhase(Phase &P, hipStream_t s) {
         P.d_m3_nr.upload(P.m3_nr, s);
         P.d_m3_off.upload(P.m3_off, s);
         P.d_m3_inv.upload(P.m3_inv, s);
+    }
+    if (!P.o1_q.empty()) {
+        P.d_o1_q.upload(P.o1_q, s);
+        P.d_o1_coef.upload(P.o1_coef, s);
+    }
+    if (!P.o3_q.empty()) {
+        P.d_o3_q.upload(P.o3_q, s);
+        P.d_o3_nr.upload(P.o3_nr, s);
+        P.d_o3_off.upload(P.o3_off, s);
+        P.d_o3_coef.upload(P.o3_coef, s);
     }
     size_t n = (size_t)P.nrow * P.ncomp;
     P.displ.alloc(n + kFieldPadRows * P.ncomp);
@@ -750,9 +793,30 @@ PointFFTArgs pointFFTArgs(axg_ctx *c) {
 
 size_t pointFFTLds(const PointFFTArgs &a) { return (size_t)PF_WAVES * a.wave_lds * sizeof(float2); }
 
-// Mass3D::computeAccel of the points whose mass varies in azimuth, before the
-// Newmark kernels (which then scale those points by 1)
+// Mass3D::computeAccel of the points whose mass varies in azimuth and the
+// ocean loads (MassOcean1D / MassOcean3D::computeAccel), before the Newmark
+// kernels (which then scale those points by 1)
 void doMass3D(axg_ctx *c) {
+    Phase &S = c->S;
+    if (!S.o1_q.empty()) {
+        const int n = (int)S.o1_q.size();
+        hipLaunchKernelGGL(ocean1d_kernel, dim3(std::min(1024, (n + 3) / 4)), dim3(256), 0, c->stream, n, S.d_o1_q.p,
+                           S.d_o1_coef.p, S.d_pt_row.p, S.d_pt_nu.p, S.d_pt_flags.p, S.stiff.p);
+    }
+    if (!S.o3_q.empty()) {
+        PointFFTArgs a = pointFFTArgs(c);
+        a.n = (int32_t)S.o3_q.size();
+        a.items = S.d_o3_q.p;
+        a.pt_row = S.d_pt_row.p;
+        a.pt_nu = S.d_pt_nu.p;
+        a.pt_flags = S.d_pt_flags.p;
+        a.pt_nr = S.d_o3_nr.p;
+        a.coef_off = S.d_o3_off.p;
+        a.coef = S.d_o3_coef.p;
+        a.field = S.stiff.p;
+        dim3 grid(std::min(1024, (a.n + PF_WAVES - 1) / PF_WAVES));
+        hipLaunchKernelGGL((mass3d_kernel<3, true>), grid, dim3(256), pointFFTLds(a), c->stream, a);
+    }
     for (int ph = 0; ph < 2; ph++) {
         Phase &P = ph ? c->F : c->S;
         if (P.m3_q.empty()) continue;
@@ -767,8 +831,8 @@ void doMass3D(axg_ctx *c) {
         a.coef = P.d_m3_inv.p;
         a.field = P.stiff.p;
         dim3 grid(std::min(1024, (a.n + PF_WAVES - 1) / PF_WAVES));
-        if (ph) hipLaunchKernelGGL(mass3d_kernel<1>, grid, dim3(256), pointFFTLds(a), c->stream, a);
-        else hipLaunchKernelGGL(mass3d_kernel<3>, grid, dim3(256), pointFFTLds(a), c->stream, a);
+        if (ph) hipLaunchKernelGGL((mass3d_kernel<1, false>), grid, dim3(256), pointFFTLds(a), c->stream, a);
+        else hipLaunchKernelGGL((mass3d_kernel<3, false>), grid, dim3(256), pointFFTLds(a), c->stream, a);
     }
 }
 
@@ -941,8 +1005,10 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
     }
     buildPhase(d, c->S, true);
     buildPhase(d, c->F, false);
-    for (Phase *P : {&c->S, &c->F})
+    for (Phase *P : {&c->S, &c->F}) {
         for (int nr : P->m3_nr) c->pf_max_nr = std::max(c->pf_max_nr, nr);
+        for (int nr : P->o3_nr) c->pf_max_nr = std::max(c->pf_max_nr, nr);
+    }
     uploadPhase(c->S, s);
     uploadPhase(c->F, s);
     if (bi) {
@@ -1092,10 +1158,13 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
             }
             if (!e3[key].empty()) createBuckets3D(c, d, key / NK, kk / 32, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
         }
-        // transforms of the points with Mass3D / SFCoupling3D use the same tables
+        // transforms of the points with Mass3D / MassOcean3D / SFCoupling3D use
+        // the same tables
         if (c->nr_used.empty()) c->nr_used.assign(E3D_MAX_NR + 1, 0);
-        for (Phase *P : {&c->S, &c->F})
+        for (Phase *P : {&c->S, &c->F}) {
             for (int nr : P->m3_nr) c->nr_used[nr] = 1;
+            for (int nr : P->o3_nr) c->nr_used[nr] = 1;
+        }
         for (int q = 0; q < d->nsf; q++) {
             int64_t rows = d->sf_off[q + 1] - d->sf_off[q];
             int nr = d->point_nr[d->sf_point[q]];
@@ -1111,6 +1180,10 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
                 if (fftFactors(nr).empty())
                     throw std::runtime_error("Mass3D::computeAccel || Nr = " + std::to_string(nr) +
                                              " has no radix plan (not a lucky number).");
+        for (int nr : c->S.o3_nr)
+            if (fftFactors(nr).empty())
+                throw std::runtime_error("MassOcean3D::computeAccel || Nr = " + std::to_string(nr) +
+                                         " has no radix plan (not a lucky number).");
         finishBuckets3D(c);
     }
 
@@ -1467,6 +1540,8 @@ struct MergedDesc {
     std::vector<int64_t> mass_solid_off, mass_fluid_off, mat_off, att_off, sf_off;
     std::vector<float> inv_mass_solid, inv_mass_fluid, mat_pool, elem_geom, elem_tiso, att_pool, sf_normal,
         rec_weights, rec_phi, rec_theta;
+    std::vector<int64_t> ocean_off;
+    std::vector<float> ocean;
 };
 
 template <class T>
@@ -1532,6 +1607,13 @@ void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo 
             appendOffsets(m.sf_off, d.sf_off, d.nsf);
             append(m.sf_normal, d.sf_normal + 6 * d.sf_off[0], (size_t)(6 * (d.sf_off[d.nsf] - d.sf_off[0])));
         }
+        if (d.ocean_off) {
+            appendOffsets(m.ocean_off, d.ocean_off, np);
+            append(m.ocean, d.ocean + 4 * d.ocean_off[0], (size_t)(4 * (d.ocean_off[np] - d.ocean_off[0])));
+        } else {
+            std::vector<int64_t> none(np + 1, 0);
+            appendOffsets(m.ocean_off, none.data(), np);
+        }
         for (int r = 0; r < d.nrec; r++) m.rec_elem.push_back(d.rec_elem[r] + nelem);
         append(m.rec_weights, d.rec_weights, 25 * (size_t)d.nrec);
         append(m.rec_phi, d.rec_phi, d.nrec);
@@ -1560,6 +1642,8 @@ void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo 
     d.inv_mass_solid = m.inv_mass_solid.data();
     d.mass_fluid_off = m.mass_fluid_off.data();
     d.inv_mass_fluid = m.inv_mass_fluid.data();
+    d.ocean_off = m.ocean_off.data();
+    d.ocean = m.ocean.data();
     d.nelem = nelem;
     d.elem_points = m.elem_points.data();
     d.elem_flags = m.elem_flags.data();

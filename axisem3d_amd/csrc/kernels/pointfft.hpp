@@ -91,8 +91,12 @@ __device__ __forceinline__ void pf_clear(float2 *seq, int Nr, int lane) {
 }
 
 // Mass3D: stiff -> mask -> C2R -> x 1/m(phi) -> R2C -> stiff (the Newmark
-// kernel then masks again and scales by 1, SolidPoint.cpp:23-38)
-template <int NCOMP>
+// kernel then masks again and scales by 1, SolidPoint.cpp:23-38).
+// OCEAN (MassOcean3D::computeAccel, MassOcean3D.cpp:19-53): the coefficients
+// are rows {1/m, n_s, n_phi, n_z} of each slice, n the unit surface normal times
+// sqrt(m_o / (m (m + m_o))), and the product is a = s / m - (s . n) n, which
+// divides the normal part by m + m_o and the tangential part by m.
+template <int NCOMP, bool OCEAN>
 __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
     extern __shared__ float2 lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -121,10 +125,17 @@ __global__ __launch_bounds__(256) void mass3d_kernel(PointFFTArgs a) {
         wfft<E3D_MAX_NR, 2>(seq, Nr, +1, plan, a.nstage[Nr], twi, lane);
         const float *im = a.coef + a.coef_off[it];
         for (int n = lane; n < Nr; n += 64) {
-            const float m = im[n];
             float2 z0 = seq[2 * n], z1 = seq[2 * n + 1];
-            seq[2 * n] = make_float2(m * z0.x, m * z0.y);
-            seq[2 * n + 1] = make_float2(m * z1.x, m * z1.y);
+            if (OCEAN) {
+                const float4 q = *reinterpret_cast<const float4 *>(im + 4 * n);
+                const float fn = z0.x * q.y + z0.y * q.z + z1.x * q.w;
+                seq[2 * n] = make_float2(z0.x * q.x - fn * q.y, z0.y * q.x - fn * q.z);
+                seq[2 * n + 1] = make_float2(z1.x * q.x - fn * q.w, 0.f);
+            } else {
+                const float m = im[n];
+                seq[2 * n] = make_float2(m * z0.x, m * z0.y);
+                seq[2 * n + 1] = make_float2(m * z1.x, m * z1.y);
+            }
         }
         wave_lds_fence();
         wfft<E3D_MAX_NR, 2>(seq, Nr, -1, plan, a.nstage[Nr], twf, lane);

@@ -130,6 +130,35 @@ __global__ __launch_bounds__(256) void newmark_fluid_kernel(PhasePoints P, float
     stiff[r] = czero();
 }
 
+// MassOcean1D::computeAccel (MassOcean1D.cpp:15-31) on the solid stiff of the
+// ocean-loaded points, after SolidPoint::maskField: the (s, z) components are
+// rotated to (normal Z, tangential R) by the colatitude theta, divided by
+// m + m_o and m, rotated back; phi is divided by m.  coef per point:
+// {1/(m+m_o), 1/m, sin theta, cos theta}.  One wave per point, lanes over the
+// modes; the Newmark kernel then masks again and scales by 1.
+__global__ __launch_bounds__(256) void ocean1d_kernel(int32_t n, const int32_t *__restrict__ items,
+                                                      const float *__restrict__ coef,
+                                                      const int32_t *__restrict__ pt_row,
+                                                      const int32_t *__restrict__ pt_nu,
+                                                      const int32_t *__restrict__ pt_flags, float2 *__restrict__ stiff) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int it = blockIdx.x * 4 + w; it < n; it += gridDim.x * 4) {
+        const int q = items[it];
+        const int nu = pt_nu[q], fl = pt_flags[q];
+        const float4 c = *reinterpret_cast<const float4 *>(coef + 4 * (int64_t)it);
+        float2 *st = stiff + (int64_t)pt_row[q] * 3;
+        for (int al = lane; al <= nu; al += 64) {
+            float2 v[3] = {st[al * 3], st[al * 3 + 1], st[al * 3 + 2]};
+            mask_solid_row(v, al, nu, fl);
+            const float2 Z = cscale(c.x, make_float2(v[0].x * c.z + v[2].x * c.w, v[0].y * c.z + v[2].y * c.w));
+            const float2 R = cscale(c.y, make_float2(v[0].x * c.w - v[2].x * c.z, v[0].y * c.w - v[2].y * c.z));
+            st[al * 3] = make_float2(Z.x * c.z + R.x * c.w, Z.y * c.z + R.y * c.w);
+            st[al * 3 + 1] = cscale(c.y, v[1]);
+            st[al * 3 + 2] = make_float2(Z.x * c.w - R.x * c.z, Z.y * c.w - R.y * c.z);
+        }
+    }
+}
+
 // SourceTerm::apply + Element::addSourceTerm (SourceTerm.cpp:29-35, Element.cpp:31-35)
 // one block per source (a batch of simulations has one each): source b reads
 // force + 450 b, src_row / src_nu + 25 b and stf + nstep b

@@ -368,6 +368,8 @@ struct axg_mesh {
     std::vector<int32_t> sf_point;
     std::vector<int64_t> sf_off;
     std::vector<float> sf_normal;
+    std::vector<int64_t> ocean_off;
+    std::vector<float> ocean;
     std::vector<float> src_force, stf;
     std::vector<int32_t> rec_elem;
     std::vector<float> rec_weights, rec_phi, rec_theta;
@@ -417,6 +419,8 @@ struct axg_mesh {
         d.rec_weights = rec_weights.data();
         d.rec_phi = rec_phi.data();
         d.rec_theta = rec_theta.data();
+        d.ocean_off = ocean_off.empty() ? nullptr : ocean_off.data();
+        d.ocean = ocean.data();
     }
 };
 
@@ -1234,6 +1238,37 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     }
     for (int p = 0; p < NPT; p++) m.sum_nu1 += pointNr[p] / 2 + 1;
 
+    // Quad::computeNormal (Quad.cpp:661-715) at GLL point ipol of side 0 or 2
+    // (jpol 0 or nPol) of a quad: spherical sides without undulation, so the
+    // normal is radial, times the boundary area of the point
+    auto sideNormal = [&](const QuadInfo &q, int side, int ipol, double nv[3]) {
+        const int jside = (side == 0) ? 0 : nPol;
+        double r0 = std::hypot(q.g.ns[side], q.g.nz[side]);
+        double t0 = std::atan2(q.g.ns[side], q.g.nz[side]);
+        int s1 = period0123(side + 1);
+        double r1 = std::hypot(q.g.ns[s1], q.g.nz[s1]);
+        double t1 = std::atan2(q.g.ns[s1], q.g.nz[s1]);
+        double rsf = .5 * (r0 + r1);
+        double half_r_dtheta = .5 * rsf * std::abs(t1 - t0);
+        double half_r2_dtheta = .5 * rsf * rsf * std::abs(t1 - t0);
+        int k = ipol * nP + jside;
+        double s = q.crd[k][0], z = q.crd[k][1];
+        double rr = std::hypot(s, z);
+        double sint = s / rr, cost = z / rr;
+        double wsf = q.wxi[ipol];  // side 0 / 2: xi weight
+        double fac;
+        if (q.axial) {
+            if (ipol == 0) fac = wsf * q.J[k][0][0] * half_r_dtheta;
+            else fac = wsf / (1. + q.xi[ipol]) * sint * half_r2_dtheta;
+        } else {
+            fac = wsf * sint * half_r2_dtheta;
+        }
+        if (side != q.g.curvedOuter) fac = -fac;
+        nv[0] = sint * fac;
+        nv[1] = 0.;
+        nv[2] = cost * fac;
+    };
+
     // ---------------- solid-fluid boundary ----------------
     {
         std::vector<std::array<double, 3>> sfn(NPT, {0., 0., 0.});
@@ -1251,33 +1286,14 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 if (updown == 0) side = q.south ? 2 : 0;
                 else side = q.south ? 0 : 2;
                 int jside = (side == 0) ? 0 : nPol;
-                // Quad::computeNormal (Quad.cpp:661-715)
-                double r0 = std::hypot(q.g.ns[side], q.g.nz[side]);
-                double t0 = std::atan2(q.g.ns[side], q.g.nz[side]);
-                int s1 = period0123(side + 1);
-                double r1 = std::hypot(q.g.ns[s1], q.g.nz[s1]);
-                double t1 = std::atan2(q.g.ns[s1], q.g.nz[s1]);
-                double rsf = .5 * (r0 + r1);
-                double half_r_dtheta = .5 * rsf * std::abs(t1 - t0);
-                double half_r2_dtheta = .5 * rsf * rsf * std::abs(t1 - t0);
                 for (int ipol = 0; ipol < nP; ipol++) {
-                    int k = ipol * nP + jside;
-                    double s = q.crd[k][0], z = q.crd[k][1];
-                    double rr = std::hypot(s, z);
-                    double sint = s / rr, cost = z / rr;
-                    double nvec[3] = {sint, 0., cost};
-                    double wsf = q.wxi[ipol];  // side 0 / 2: xi weight
-                    double fac;
-                    if (q.axial) {
-                        if (ipol == 0) fac = wsf * q.J[k][0][0] * half_r_dtheta;
-                        else fac = wsf / (1. + q.xi[ipol]) * sint * half_r2_dtheta;
-                    } else {
-                        fac = wsf * sint * half_r2_dtheta;
-                    }
-                    if (side != q.g.curvedOuter) fac = -fac;
-                    if (!q.fluid) fac = -fac;  // inverse in solid domain
-                    int p = q.gll[k];
-                    for (int d = 0; d < 3; d++) sfn[p][d] += nvec[d] * fac * .5;
+                    double nv[3];
+                    sideNormal(q, side, ipol, nv);
+                    // inverse in solid domain; both sides contribute, so halved
+                    // (Quad.cpp:359-366)
+                    const double f = q.fluid ? .5 : -.5;
+                    int p = q.gll[ipol * nP + jside];
+                    for (int d = 0; d < 3; d++) sfn[p][d] += nv[d] * f;
                     onSF[p] = 1;
                 }
             }
@@ -1301,6 +1317,77 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 m.sf_normal.push_back((float)(sfn[p][2] / mf));
             }
             m.sf_off.push_back(m.sf_off.back() + rows);
+        }
+    }
+
+    // ---------------- ocean load ----------------
+    // Quad::setupGLLPoints adds the normal of every surface point
+    // (Quad.cpp:368-373), Quad::setOceanLoad3D its depth on each of the point's
+    // slices (Quad.cpp:258-282); GLLPoint::release makes MassOcean1D where mass,
+    // depth and normal are the same on all slices, else MassOcean3D
+    // (GLLPoint.cpp:55-72).  The ocean rests on the solid points of the outer
+    // surface.
+    if (c.ocean_depth > 0.) {
+        if (c.ocean_var != 0. && !(c.dv_corr_len > 0.))
+            throw std::runtime_error("OceanLoad3D || a varying depth needs a correlation length.");
+        if (!(std::abs(c.ocean_var) < 1.)) throw std::runtime_error("OceanLoad3D || |ocean_var| must be below 1.");
+        const double kWater = 1027.;  // OceanLoad3D::mWaterDensity (OceanLoad3D.cpp:18)
+        std::vector<std::array<double, 3>> surf(NPT, {0., 0., 0.});
+        std::vector<char> onSurf(NPT, 0);
+        for (int e = 0; e < NQ; e++) {
+            const QuadInfo &q = B.quads[e];
+            if (q.ir != B.NR - 1 || q.fluid) continue;
+            const int side = q.south ? 0 : 2;  // outer side
+            const int jside = (side == 0) ? 0 : nPol;
+            for (int ipol = 0; ipol < nP; ipol++) {
+                double nv[3];
+                sideNormal(q, side, ipol, nv);
+                int p = q.gll[ipol * nP + jside];
+                for (int d = 0; d < 3; d++) surf[p][d] += nv[d];
+                onSurf[p] = 1;
+            }
+        }
+        GRF grfOcean;
+        if (c.ocean_var != 0.) grfOcean = GRF(c.seed * 31ULL + 4ULL, c.dv_corr_len);
+        m.ocean_off.assign(NPT + 1, 0);
+        std::vector<double> depth;
+        for (int p = 0; p < NPT; p++) {
+            m.ocean_off[p + 1] = m.ocean_off[p];
+            if (!onSurf[p] || !(pflags[p] & AXG_PT_SOLID)) continue;
+            const int nr = pointNr[p];
+            const double s = psz[2 * p], z = psz[2 * p + 1];
+            depth.assign(nr, c.ocean_depth);
+            if (c.ocean_var != 0.)
+                for (int n = 0; n < nr; n++) {
+                    const double phi = 2. * pi * n / nr;
+                    const double g = grfOcean.exact(s * std::cos(phi), s * std::sin(phi), z);
+                    depth[n] = c.ocean_depth * (1. + c.ocean_var * std::tanh(g / 2.));
+                }
+            // XMath::equalRows (XMath.h:67-74, tolerance 1e-10 relative); the
+            // solid mass and the normal of this mesh are the same on all slices
+            bool equal = true;
+            for (int n = 1; n < nr; n++) equal = equal && std::abs(depth[n] - depth[0]) <= 1e-10 * std::abs(depth[0]);
+            const double ms = massS[p];
+            const double area = std::sqrt(surf[p][0] * surf[p][0] + surf[p][1] * surf[p][1] + surf[p][2] * surf[p][2]);
+            if (equal) {
+                // MassOcean1D(mass, 1027 depth |n|, theta) (MassOcean1D.cpp:8-13)
+                const double mo = kWater * depth[0] * area;
+                const double theta = std::acos(z / std::hypot(s, z));  // Geodesy::theta
+                m.ocean.push_back((float)(1. / (ms + mo)));
+                m.ocean.push_back((float)(1. / ms));
+                m.ocean.push_back((float)std::sin(theta));
+                m.ocean.push_back((float)std::cos(theta));
+                m.ocean_off[p + 1] += 1;
+            } else {
+                // MassOcean3D(mass, 1027 depth area, normal / area) (MassOcean3D.cpp:9-17)
+                for (int n = 0; n < nr; n++) {
+                    const double mo = kWater * depth[n] * area;
+                    const double scal = std::sqrt(mo / (ms * (ms + mo)));
+                    m.ocean.push_back((float)(1. / ms));
+                    for (int d = 0; d < 3; d++) m.ocean.push_back((float)(surf[p][d] / area * scal));
+                }
+                m.ocean_off[p + 1] += nr;
+            }
         }
     }
 

@@ -142,6 +142,18 @@ typedef struct axg_mesh_desc {
     const float   *rec_phi;           /* [nrec] azimuth in the source frame        */
     const float   *rec_theta;         /* [nrec] colatitude in the source frame     */
     int32_t rec_components;           /* 0 = RTZ, 1 = SPZ                          */
+
+    /* ---- ocean load on solid points of the surface (MassOcean1D / MassOcean3D,
+     *      GLLPoint.cpp:55-72); null ocean_off: no ocean load ----
+     * ocean_off [npoint+1]: 0 (plain Mass1D / Mass3D), 1 (MassOcean1D) or nr rows
+     * (MassOcean3D) of 4 floats for the solid side of each point; rows replace the
+     * point's solid mass inverse (mass_solid_off still counts 1 or nr for it):
+     *   MassOcean1D  1/(m+m_o), 1/m, sin theta, cos theta    (MassOcean1D.cpp:8-13)
+     *   MassOcean3D  per slice 1/m, then the unit surface normal (s, phi, z)
+     *                times sqrt(m_o / (m (m + m_o)))          (MassOcean3D.cpp:9-17)
+     * m_o = 1027 kg/m^3 x ocean depth x |assembled surface normal| (OceanLoad3D.cpp:18). */
+    const int64_t *ocean_off;
+    const float   *ocean;
 } axg_mesh_desc;
 
 /* ===================================================================== *
@@ -195,6 +207,12 @@ typedef struct axg_shell_config {
     int32_t nthread;                  /* host threads for set-up (0 = all)         */
     const int32_t *elem_nu;           /* nu_mode 3: Nu of every element in build order
                                          (the order of a build with the same geometry) */
+    /* ocean load on the solid points of the outer surface (OceanLoad3D) */
+    double ocean_depth;               /* metres; 0: none (OceanLoad3D_const takes km)  */
+    double ocean_var;                 /* relative lateral variation of the depth: depth x
+                                         (1 + ocean_var tanh(g / 2)), g a unit Gaussian
+                                         random field of correlation length dv_corr_len;
+                                         0: laterally constant (MassOcean1D)          */
 } axg_shell_config;
 
 typedef struct axg_mesh axg_mesh;

@@ -163,6 +163,8 @@ struct Ref {
         Real invMass;
         std::vector<C> displ, veloc, accel, stiff;  // [(nu+1)*3], alpha-major
         std::vector<Real> invMass3;                 // Mass3D: 1/mass on the Nr slices
+        std::vector<Real> ocean1;                   // MassOcean1D: 1/(m+mo), 1/m, sin t, cos t
+        std::vector<Real> ocean3;                   // MassOcean3D: [Nr][1/m, scaled normal]
         double wmax[3] = {-1., -1., -1.};          // SolidPoint.h:78-79
         int wnu[3] = {0, 0, 0};
     };
@@ -273,6 +275,14 @@ struct Ref {
                 s.gidx = p;
                 s.axial = (fl & AXG_PT_AXIAL) != 0;
                 s.invMass = (Real)d->inv_mass_solid[d->mass_solid_off[p]];
+                // ocean load replaces the solid mass (GLLPoint.cpp:55-72)
+                const int64_t no = d->ocean_off ? d->ocean_off[p + 1] - d->ocean_off[p] : 0;
+                if (no != 0 && no != 1 && no != nr)
+                    throw std::runtime_error("MassOcean3D::checkCompatibility || Incompatible size.");
+                if (no > 0) {
+                    auto &o = (no == 1) ? s.ocean1 : s.ocean3;
+                    for (int64_t i = 4 * d->ocean_off[p]; i < 4 * d->ocean_off[p + 1]; i++) o.push_back((Real)d->ocean[i]);
+                }
                 s.displ.assign((s.nu + 1) * 3, C(0, 0));
                 s.veloc = s.accel = s.stiff = s.displ;
                 sIndex[p] = (int)S.size();
@@ -477,6 +487,40 @@ struct Ref {
         }
     }
 
+    // MassOcean1D::computeAccel (point/mass/MassOcean1D.cpp:15-31): (s, z) to
+    // (Z, R) by the colatitude, Z / (m + mo), R / m, back; phi / m
+    static void ocean1D(int nu, std::vector<C> &f, const std::vector<Real> &o) {
+        const Real invZ = o[0], invR = o[1], st = o[2], ct = o[3];
+        for (int a = 0; a <= nu; a++) {
+            C Z = (f[a * 3 + 0] * st + f[a * 3 + 2] * ct) * invZ;
+            C R = (f[a * 3 + 0] * ct - f[a * 3 + 2] * st) * invR;
+            f[a * 3 + 0] = Z * st + R * ct;
+            f[a * 3 + 2] = Z * ct - R * st;
+            f[a * 3 + 1] *= invR;
+        }
+    }
+    // MassOcean3D::computeAccel (point/mass/MassOcean3D.cpp:19-53): the three
+    // columns to the Nr slices, a = s / m - (s . n) n with the scaled normal n,
+    // back (x 1/Nr)
+    static void ocean3D(int nr, int nu, std::vector<C> &f, const std::vector<Real> &o) {
+        std::vector<C> X(nr / 2 + 1), wa, wb;
+        std::vector<Real> x[3];
+        for (int c = 0; c < 3; c++) {
+            x[c].resize(nr);
+            for (int a = 0; a <= nu; a++) X[a] = f[a * 3 + c];
+            c2r<Real>(nr, X.data(), x[c].data(), wa, wb);
+        }
+        for (int i = 0; i < nr; i++) {
+            const Real *q = &o[4 * i];
+            const Real fn = x[0][i] * q[1] + x[1][i] * q[2] + x[2][i] * q[3];
+            for (int c = 0; c < 3; c++) x[c][i] = x[c][i] * q[0] - fn * q[1 + c];
+        }
+        for (int c = 0; c < 3; c++) {
+            r2c<Real>(nr, x[c].data(), X.data(), wa, wb);
+            for (int a = 0; a <= nu; a++) f[a * 3 + c] = X[a];
+        }
+    }
+
     void updateNewmark(double dtt) {
         double half_dt = 0.5 * dtt;
         double half_dt_dt = half_dt * dtt;
@@ -485,7 +529,11 @@ struct Ref {
             for (int i = b; i < e; i++) {
                 SolidPt &p = S[i];
                 maskSolid(p, p.stiff);
-                if (p.invMass3.empty())
+                if (!p.ocean1.empty())
+                    ocean1D(p.nu, p.stiff, p.ocean1);
+                else if (!p.ocean3.empty())
+                    ocean3D(p.nr, p.nu, p.stiff, p.ocean3);
+                else if (p.invMass3.empty())
                     for (auto &v : p.stiff) v *= p.invMass;  // Mass1D::computeAccel
                 else
                     mass3D(p.nr, p.nu, 3, p.stiff, p.invMass3);

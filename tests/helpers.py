@@ -150,3 +150,56 @@ class LiftedMass3D:
 
     def field_size(self, which):
         return self.mesh.field_size(which)
+
+
+class OceanRows:
+    """The same mesh with the ocean rows replaced: ocean_off [npoint+1] and rows
+    [n][4] in the descriptor's layout (MassOcean1D / MassOcean3D)."""
+
+    def __init__(self, mesh, off, rows):
+        import ctypes as C
+        import axisem3d_amd._abi as abi
+        self.mesh = mesh
+        d = abi.MeshDesc.from_buffer_copy(mesh.d)
+        self._off = np.ascontiguousarray(off, dtype=np.int64)
+        self._rows = np.ascontiguousarray(np.asarray(rows, dtype=np.float32).reshape(-1), dtype=np.float32)
+        if self._rows.size == 0:
+            self._rows = np.zeros(4, np.float32)
+        d.ocean_off = self._off.ctypes.data_as(abi.i64p)
+        d.ocean = self._rows.ctypes.data_as(abi.f32p)
+        self._d = d
+        self.desc = C.pointer(d)
+
+    @property
+    def d(self):
+        return self._d
+
+    def field_size(self, which):
+        return self.mesh.field_size(which)
+
+    def array(self, name):
+        if name == "ocean_off":
+            return self._off
+        if name == "ocean":
+            return self._rows[:4 * int(self._off[-1])]
+        return self.mesh.array(name)
+
+
+def lift_ocean3d(mesh):
+    """MassOcean1D rows {1/(m+mo), 1/m, sin t, cos t} of a mesh rewritten as
+    MassOcean3D rows {1/m, (sin t, 0, cos t) sqrt(mo / (m (m + mo)))} on each of
+    the point's Nr slices (points of Nr 1 keep their one row): the same
+    operator, through the FFT path."""
+    off, rows = mesh.array("ocean_off"), mesh.array("ocean").reshape(-1, 4).astype(np.float64)
+    nr = mesh.array("point_nr")
+    new_off, new_rows = [0], []
+    for p in range(mesh.d.npoint):
+        blk = rows[off[p]:off[p + 1]]
+        if blk.shape[0] == 1 and nr[p] > 1:
+            iz, ir, st, ct = blk[0]
+            m, mo = 1. / ir, 1. / iz - 1. / ir
+            sc = np.sqrt(mo / (m * (m + mo)))
+            blk = np.repeat([[ir, st * sc, 0., ct * sc]], nr[p], axis=0)
+        new_rows.append(blk)
+        new_off.append(new_off[-1] + blk.shape[0])
+    return OceanRows(mesh, new_off, np.concatenate(new_rows) if new_rows else np.zeros((0, 4)))

@@ -1,7 +1,8 @@
 """Independent simulations batched in one context (axg_create_batch): every
 simulation's stations and learned Nu equal its own single-context run bit for
 bit (each keeps its dt, source and STF; colouring and launch order per point
-are unchanged by the merge)."""
+are unchanged by the merge).  One simulation carries an ocean load
+(MassOcean1D and MassOcean3D points), the others none."""
 import numpy as np
 import pytest
 
@@ -16,7 +17,7 @@ def _meshes(small_layers, fluid_layers, nstep):
                                      dv_amp=0.05, nu_mode=1, nu_min=4, nu_max=40, nstep=nstep, seed=4)),
         ax.ShellMesh(ax.shell_config(1, layers=small_layers, nelem_theta=8, material=_abi.MAT_ISOTROPIC,
                                      dv_amp=0.05, nu_mode=1, nu_min=6, nu_max=20, nstep=nstep, seed=5,
-                                     src_depth=300e3)),
+                                     src_depth=300e3, ocean_depth=4000., ocean_var=0.3)),
     ]
 
 
