@@ -41,3 +41,4 @@ from .domain import Domain, DomainBatch, Newmark, set_fft_planner  # noqa: E402
 from .nu_mlp import NuMLP  # noqa: E402
 
 __all__ = ["lib", "last_error", "ShellMesh", "shell_config", "Domain", "DomainBatch", "Newmark", "NuMLP", "set_fft_planner"]
+from .files import read_wisdom, wisdom_nu, wisdom_rows, write_stations, write_wisdom  # noqa: E402

@@ -100,6 +100,9 @@ class ShellConfig(C.Structure):
         ("seed", C.c_uint64),
         ("nthread", C.c_int32),
         ("elem_nu", C.POINTER(C.c_int32)),
+        ("wisdom", C.POINTER(C.c_double)),
+        ("nwisdom", C.c_int64),
+        ("wisdom_factor", C.c_double),
         ("ocean_depth", C.c_double),
         ("ocean_var", C.c_double),
     ]
@@ -134,6 +137,16 @@ def declare_mesh_api(lib):
     lib.axg_mesh_point_features.restype = C.c_int
     lib.axg_last_error.argtypes = []
     lib.axg_last_error.restype = C.c_char_p
+    f64p = C.POINTER(C.c_double)
+    lib.axg_wisdom_write.argtypes = [C.c_char_p, C.c_int64, f64p]
+    lib.axg_wisdom_write.restype = C.c_int
+    lib.axg_wisdom_read.argtypes = [C.c_char_p, f64p, C.c_int64, i64p]
+    lib.axg_wisdom_read.restype = C.c_int
+    lib.axg_wisdom_get_nu.argtypes = [C.c_int64, f64p, C.c_int32, C.c_int64, f64p, i32p]
+    lib.axg_wisdom_get_nu.restype = C.c_int
+    lib.axg_write_station_ascii.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_double, C.c_double, f32p,
+                                            C.c_int64]
+    lib.axg_write_station_ascii.restype = C.c_int
     lib.axg_field_size.argtypes = [C.POINTER(MeshDesc), C.c_int32]
     lib.axg_field_size.restype = C.c_int64
 

@@ -1,0 +1,267 @@
+// nu_wisdom.cpp -- Nu wisdom rows, their ascii file and the Nu of a new point.
+#include "nu_wisdom.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+
+#include "axg.h"
+
+namespace axg {
+void set_last_error(const std::string &msg);
+
+namespace {
+const char *kVar = "axisem3d_wisdom";
+
+std::string trimLower(const std::string &s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) a++;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) b--;
+    std::string t = s.substr(a, b - a);
+    for (auto &c : t) c = (char)std::tolower((unsigned char)c);
+    return t;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The grid holds about four rows per bucket; a query visits rings of buckets
+// around its own until it holds k rows and the next ring lies farther than the
+// k-th nearest of them.
+NuWisdomIndex::NuWisdomIndex(std::vector<double> rows) {
+    const int64_t n = (int64_t)rows.size() / 4;
+    if (n <= 0) throw std::runtime_error("NuWisdom::getNu || Wisdom is empty.");
+    s_.resize(n);
+    z_.resize(n);
+    nu_.resize(n);
+    double hi_s = -1e300, hi_z = -1e300;
+    lo_s_ = lo_z_ = 1e300;
+    for (int64_t i = 0; i < n; i++) {
+        s_[i] = rows[4 * i];
+        z_[i] = rows[4 * i + 1];
+        // Domain::dumpWisdom rounds; readFromFile rounds again (NuWisdom.cpp:75-76)
+        nu_[i] = (int)std::lround(rows[4 * i + 2]);
+        lo_s_ = std::min(lo_s_, s_[i]);
+        lo_z_ = std::min(lo_z_, z_[i]);
+        hi_s = std::max(hi_s, s_[i]);
+        hi_z = std::max(hi_z, z_[i]);
+    }
+    const double ws = std::max(hi_s - lo_s_, 1e-9), wz = std::max(hi_z - lo_z_, 1e-9);
+    h_ = std::sqrt(ws * wz * 4. / (double)n);
+    h_ = std::max(h_, std::max(ws, wz) / 2048.);
+    ns_ = (int)(ws / h_) + 1;
+    nz_ = (int)(wz / h_) + 1;
+    start_.assign((size_t)ns_ * nz_ + 1, 0);
+    auto cell = [&](int64_t i) {
+        int a = std::min(ns_ - 1, (int)((s_[i] - lo_s_) / h_));
+        int b = std::min(nz_ - 1, (int)((z_[i] - lo_z_) / h_));
+        return (int64_t)b * ns_ + a;
+    };
+    for (int64_t i = 0; i < n; i++) start_[cell(i) + 1]++;
+    for (size_t c = 0; c + 1 < start_.size(); c++) start_[c + 1] += start_[c];
+    item_.resize(n);
+    std::vector<int64_t> fill(start_.begin(), start_.end() - 1);
+    for (int64_t i = 0; i < n; i++) item_[fill[cell(i)]++] = i;
+}
+
+int NuWisdomIndex::getNu(double s, double z, int k) const {
+    const int64_t n = (int64_t)nu_.size();
+    k = (int)std::min<int64_t>(k, n);
+    // candidates: (distance, row), the k nearest kept sorted
+    std::vector<std::pair<double, int64_t>> best;
+    best.reserve(k + 1);
+    auto offer = [&](int64_t i) {
+        const double d = std::hypot(s - s_[i], z - z_[i]);
+        if ((int)best.size() == k && d >= best.back().first) return;
+        auto it = std::upper_bound(best.begin(), best.end(), std::make_pair(d, i));
+        best.insert(it, {d, i});
+        if ((int)best.size() > k) best.pop_back();
+    };
+    const int a0 = (int)std::floor((s - lo_s_) / h_), b0 = (int)std::floor((z - lo_z_) / h_);
+    for (int r = 0;; r++) {
+        // the nearest a row of ring r can be (the query may lie outside the grid)
+        const double ca = std::max(0., std::max((double)(-a0), (double)(a0 - (ns_ - 1))));
+        const double cb = std::max(0., std::max((double)(-b0), (double)(b0 - (nz_ - 1))));
+        const double ringMin = std::max(0., (double)(r - 1)) * h_;
+        if ((int)best.size() == k && ringMin > best.back().first && r > 0) break;
+        bool any = false;
+        for (int b = b0 - r; b <= b0 + r; b++) {
+            if (b < 0 || b >= nz_) continue;
+            for (int a = a0 - r; a <= a0 + r; a++) {
+                if (a < 0 || a >= ns_) continue;
+                if (std::max(std::abs(a - a0), std::abs(b - b0)) != r) continue;  // ring only
+                any = true;
+                const int64_t c = (int64_t)b * ns_ + a;
+                for (int64_t t = start_[c]; t < start_[c + 1]; t++) offer(item_[t]);
+            }
+        }
+        // past every bucket of the grid
+        if (!any && r > ns_ + nz_ + ca + cb) break;
+    }
+    // NuWisdom::getNu (NuWisdom.cpp:82-98)
+    double nuTarget = 0., distTotal = 0.;
+    for (const auto &c : best) {
+        if (c.first < 1e-10) return nu_[c.second];
+        distTotal += 1. / c.first;
+        nuTarget += nu_[c.second] / c.first;
+    }
+    return (int)std::round(nuTarget / distTotal);
+}
+
+// ---------------------------------------------------------------------------
+void writeWisdomAscii(const std::string &path, const double *rows, int64_t n) {
+    std::FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("NuWisdom::writeToFile || Error opening file: || " + path);
+    std::fprintf(f, "@@ NETCDF ASCII FORMAT @@\n");
+    std::fprintf(f, "Nu wisdom: s, z, learned Nu, original Nu of every GLL point\n\n");
+    std::fprintf(f, "@VAR_START %s\n%lld 4\n", kVar, (long long)n);
+    for (int64_t i = 0; i < n; i++)
+        std::fprintf(f, "%.17g %.17g %.17g %.17g\n", rows[4 * i], rows[4 * i + 1], rows[4 * i + 2], rows[4 * i + 3]);
+    std::fprintf(f, "@VAR_END %s\n", kVar);
+    if (std::fclose(f) != 0) throw std::runtime_error("NuWisdom::writeToFile || Error writing file: || " + path);
+}
+
+std::vector<double> readWisdomAscii(const std::string &path) {
+    std::ifstream fs(path);
+    if (!fs) throw std::runtime_error("NetCDF_ReaderAscii::open || Error opening NetCDF-alternative ascii file: || " + path);
+    std::string line;
+    if (!std::getline(fs, line) || trimLower(line) != "@@ netcdf ascii format @@")
+        throw std::runtime_error("NetCDF_ReaderAscii::checkNetCDFAscii || Error identifying NetCDF or "
+                                 "NetCDF-alternative ascii file: || " + path);
+    while (std::getline(fs, line)) {
+        std::stringstream ss(line);
+        std::string flag, name;
+        if (!(ss >> flag) || flag != "@VAR_START") continue;
+        if (!(ss >> name))
+            throw std::runtime_error("NetCDF_ReaderAscii::checkVarStart || No variable name provided after "
+                                     "@VAR_START || NetCDF-alternative ascii file: " + path);
+        if (name != kVar) continue;
+        if (!std::getline(fs, line))
+            throw std::runtime_error("NetCDF_ReaderAscii::readMetaData || No dimension line after @VAR_START "
+                                     "line, varaible: " + std::string(kVar) + " || NetCDF-alternative ascii file: " + path);
+        std::vector<size_t> dims;
+        {
+            std::stringstream ds(line);
+            size_t d;
+            while (ds >> d) dims.push_back(d);
+        }
+        size_t total = 1;
+        for (size_t d : dims) total *= d;
+        std::vector<double> data(total);
+        for (size_t i = 0; i < total; i++)
+            if (!(fs >> data[i]))
+                throw std::runtime_error("NetCDF_ReaderAscii::readMetaData || Insufficient data or invalid number "
+                                         "format, Variable = " + std::string(kVar) + " || NetCDF-alternative ascii file: " + path);
+        std::string endFlag, endName;
+        if (!(fs >> endFlag) || endFlag != "@VAR_END" || !(fs >> endName) || endName != kVar)
+            throw std::runtime_error("NetCDF_ReaderAscii::readMetaData || Error detecting @VAR_END line, Variable = " +
+                                     std::string(kVar) + " || Too many data or invalid number format or missing "
+                                     "@VAR_END || NetCDF-alternative ascii file: " + path);
+        if (dims.size() != 2)
+            throw std::runtime_error("NetCDF_ReaderAscii::read2D || Variable is not 2D, Variable = " +
+                                     std::string(kVar) + " || NetCDF file: " + path);
+        const size_t nrow = dims[0], ncol = dims[1];
+        if (ncol != 3 && ncol != 4)
+            throw std::runtime_error("NuWisdom::readFromFile || Inconsistent dimensions for wisdom data, must be a "
+                                     "matrix of 3 or 4 columns || File = " + path);
+        std::vector<double> rows(4 * nrow);
+        for (size_t i = 0; i < nrow; i++) {
+            for (size_t c = 0; c < ncol; c++) rows[4 * i + c] = data[i * ncol + c];
+            if (ncol == 3) rows[4 * i + 3] = rows[4 * i + 2];
+        }
+        return rows;
+    }
+    throw std::runtime_error("NetCDF_ReaderAscii::readMetaData || Error detecting @VAR_START line, || Variable not "
+                             "found, Variable = " + std::string(kVar) + " || NetCDF-alternative ascii file: " + path);
+}
+
+}  // namespace axg
+
+// ===================================================================== C ABI
+extern "C" int axg_wisdom_write(const char *path, int64_t n, const double *rows) {
+    if (!path || n < 0 || (n > 0 && !rows)) {
+        axg::set_last_error("axg_wisdom_write || null argument.");
+        return AXG_EINVAL;
+    }
+    try {
+        axg::writeWisdomAscii(path, rows, n);
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
+}
+
+extern "C" int axg_wisdom_read(const char *path, double *rows, int64_t cap, int64_t *n) {
+    if (!path || !n) {
+        axg::set_last_error("axg_wisdom_read || null argument.");
+        return AXG_EINVAL;
+    }
+    try {
+        const std::vector<double> r = axg::readWisdomAscii(path);
+        *n = (int64_t)r.size() / 4;
+        if (rows) {
+            if (cap < *n) {
+                axg::set_last_error("axg_wisdom_read || buffer holds fewer rows than the file.");
+                return AXG_EINVAL;
+            }
+            std::copy(r.begin(), r.end(), rows);
+        }
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
+}
+
+extern "C" int axg_wisdom_get_nu(int64_t n, const double *rows, int32_t num_samples, int64_t nq, const double *sz,
+                                 int32_t *nu) {
+    if (n <= 0 || !rows || num_samples <= 0 || nq < 0 || (nq > 0 && (!sz || !nu))) {
+        axg::set_last_error("axg_wisdom_get_nu || invalid argument.");
+        return AXG_EINVAL;
+    }
+    try {
+        axg::NuWisdomIndex idx(std::vector<double>(rows, rows + 4 * n));
+        for (int64_t q = 0; q < nq; q++) nu[q] = idx.getNu(sz[2 * q], sz[2 * q + 1], num_samples);
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PointwiseIOAscii (SOLVER/src/core/output/pointwise/PointwiseIOAscii.cpp:25-35,
+// 96-102): one file per station, each recorded step a line "t c0 c1 c2" at full
+// precision; steps recorded where (step index) % interval == 0
+// (PointwiseRecorder.cpp:62-68), t accumulated from -shift by dt as
+// Newmark::solve does (Newmark.cpp:27, 64-65).
+extern "C" int axg_write_station_ascii(const char *path, int64_t nstep, int32_t interval, double t_shift, double dt,
+                                       const float *trace, int64_t row_stride) {
+    if (!path || nstep < 0 || interval <= 0 || (nstep > 0 && !trace) || row_stride < 3) {
+        axg::set_last_error("axg_write_station_ascii || invalid argument.");
+        return AXG_EINVAL;
+    }
+    std::FILE *f = std::fopen(path, "w");
+    if (!f) {
+        axg::set_last_error(std::string("PointwiseIOAscii::initialize || Error opening ascii output file: || ") + path);
+        return AXG_EINVAL;
+    }
+    double t = 0. - t_shift;
+    for (int64_t i = 0; i < nstep; i++) {
+        if (i % interval == 0) {
+            const float *r = trace + i * row_stride;
+            std::fprintf(f, "%.17g %.17g %.17g %.17g\n", t, (double)r[0], (double)r[1], (double)r[2]);
+        }
+        t += dt;
+    }
+    if (std::fclose(f) != 0) {
+        axg::set_last_error(std::string("PointwiseIOAscii::dumpToFile || Error writing file: || ") + path);
+        return AXG_EINVAL;
+    }
+    return AXG_OK;
+}

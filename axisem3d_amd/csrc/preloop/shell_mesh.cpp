@@ -28,6 +28,7 @@
 // the seeded Gaussian-random-field perturbations and the seeded Nu / Q / moment
 // tensor draws are this project's own synthetic workloads.
 #include "axg.h"
+#include "nu_wisdom.hpp"
 #include "spectral.hpp"
 
 #include <sys/mman.h>
@@ -696,10 +697,31 @@ struct Builder {
         }
         // Nu of a point = max over the elements sharing it; Nr = 2 Nu + 1 (ConstNrField.cpp:13-16)
         std::vector<int> baseNr(npoint, 0);
-        for (int p = 0; p < npoint; p++) {
-            int nu = 0;
-            for (auto &pe : pointElems[p]) nu = std::max(nu, elemNu[pe.first]);
-            baseNr[p] = 2 * nu + 1;
+        if (c.nu_mode == 4) {
+            // WisdomNrField::getNrAtPoint (WisdomNrField.cpp:20-24) at the point's
+            // coordinates with numerical noise erased (Quad.cpp:553-558)
+            if (!c.wisdom || c.nwisdom <= 0) throw std::runtime_error("NrField || nu_mode 4 needs wisdom rows.");
+            const NuWisdomIndex idx(std::vector<double>(c.wisdom, c.wisdom + 4 * c.nwisdom));
+            const double factor = c.wisdom_factor > 0. ? c.wisdom_factor : 1.;
+            const double offset = 1e-8;
+            for (int p = 0; p < npoint; p++) {
+                const auto &pe = pointElems[p].front();
+                const double s = std::round(quads[pe.first].crd[pe.second][0] / offset) * offset;
+                const double z = std::round(quads[pe.first].crd[pe.second][1] / offset) * offset;
+                const int nu = (int)std::lround(idx.getNu(s, z, 4) * factor);
+                baseNr[p] = 2 * nu + 1;
+            }
+            for (size_t e = 0; e < quads.size(); e++) {
+                int nu = 0;
+                for (int k = 0; k < nP * nP; k++) nu = std::max(nu, baseNr[quads[e].gll[k]] / 2);
+                elemNu[e] = nu;
+            }
+        } else {
+            for (int p = 0; p < npoint; p++) {
+                int nu = 0;
+                for (auto &pe : pointElems[p]) nu = std::max(nu, elemNu[pe.first]);
+                baseNr[p] = 2 * nu + 1;
+            }
         }
         std::vector<int> pointNr(npoint, -1);
         for (auto &q : quads) {

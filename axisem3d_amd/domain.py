@@ -99,6 +99,12 @@ class Domain:
         _check(self._lib.axg_get_wisdom(self._h, out.ctypes.data_as(_abi.i32p)))
         return out
 
+    def dumpWisdom(self, path):
+        """Domain::dumpWisdom (Domain.cpp:404-438): (s, z, learned Nu, Nu) of every
+        point as the wisdom file that the NU_TYPE wisdom Nr field reads."""
+        from .files import wisdom_rows, write_wisdom
+        write_wisdom(path, wisdom_rows(self.mesh, self.wisdom()))
+
     # ---- batched loop and data exchange ----
     def run(self, first, count, stab_interval=0, use_graph=False):
         """Newmark.cpp:47-93 for tstep = first .. first+count-1 on the device."""
@@ -193,6 +199,11 @@ class DomainBatch(Domain):
         out = np.zeros(self.point_off[-1], dtype=np.int32)
         _check(self._lib.axg_get_wisdom(self._h, out.ctypes.data_as(_abi.i32p)))
         return out if k is None else out[self.point_off[k]:self.point_off[k + 1]]
+
+    def dumpWisdom(self, path, k=0):
+        """Domain::dumpWisdom of simulation k of the batch."""
+        from .files import wisdom_rows, write_wisdom
+        write_wisdom(path, wisdom_rows(self.meshes[k], self.wisdom(k)))
 
 
 class Newmark:

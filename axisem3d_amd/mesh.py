@@ -39,6 +39,12 @@ def shell_config(which=0, layers=None, **overrides):
             keep_nu = np.ascontiguousarray(v, dtype=np.int32)
             cfg.elem_nu = keep_nu.ctypes.data_as(C.POINTER(C.c_int32))
             cfg.nu_mode = 3
+        elif k == "wisdom":
+            # nu_mode 4: Nu from wisdom rows [n][4] (s, z, learned Nu, original Nu)
+            keep_nu = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1, 4))
+            cfg.wisdom = keep_nu.ctypes.data_as(C.POINTER(C.c_double))
+            cfg.nwisdom = keep_nu.shape[0]
+            cfg.nu_mode = 4
         else:
             setattr(cfg, k, v)
     cfg._keep = keep  # keep the layer array alive with the config

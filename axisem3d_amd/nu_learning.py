@@ -36,10 +36,10 @@ LEARN_LAYERS = [
 ]
 
 
-def model_config(seed, nstep, nelem_theta=40, nu=None, elem_nu=None, radial_scale=1):
+def model_config(seed, nstep, nelem_theta=40, nu=None, elem_nu=None, radial_scale=1, wisdom=None):
     """One seeded model of the family: TI solid, GRF dvs/dvp of +-5 % with a
     500 km correlation length, seeded moment tensor and stations; uniform Nu
-    `nu` or the per-element Nu `elem_nu`.  The mesh has nelem_theta x (50
+    `nu`, the per-element Nu `elem_nu` or the Nr field of wisdom rows `wisdom`.  The mesh has nelem_theta x (50
     radial_scale) quads; the model, source and stations depend only on the
     seed, so meshes of one seed at several resolutions describe one model."""
     layers = [dict(l, nelem_radial=l["nelem_radial"] * radial_scale) for l in LEARN_LAYERS]
@@ -47,6 +47,8 @@ def model_config(seed, nstep, nelem_theta=40, nu=None, elem_nu=None, radial_scal
               dv_corr_len=500e3, seed=seed, nstep=nstep, nrec=20)
     if elem_nu is not None:
         kw["elem_nu"] = elem_nu
+    elif wisdom is not None:
+        kw["wisdom"] = wisdom
     else:
         kw.update(nu_mode=0, nu_const=int(nu))
     return shell_config(1, **kw)
@@ -112,13 +114,30 @@ class LearnedNu:
         self.mlp.train(epochs, batch, lr, under_weight, seed)
         return float(np.mean((self.mlp.predict(x) - y) ** 2))
 
-    def predict_element_nu(self, mesh):
+    def predict_point_nu(self, mesh):
+        """Predicted Nu of every point (with margin and pad) and the points whose
+        features lie outside the training range."""
         f = mesh.point_features()
         y = np.clip(self.mlp.predict(f), 0.0, 1.0)
         nu = np.ceil(self.margin * y * self.nu_full).astype(np.int32) + self.pad
         tol = self.range_tol * np.maximum(self.hi - self.lo, 1e-6)
         flagged = ((f < self.lo - tol) | (f > self.hi + tol)).any(1)
+        return nu, flagged
+
+    def predict_element_nu(self, mesh):
+        nu, flagged = self.predict_point_nu(mesh)
         return element_nu(mesh, nu, flagged, self.nu_full), int(flagged.sum())
+
+    def write_wisdom(self, mesh, path):
+        """The prediction for `mesh` as a Nu wisdom file (s, z, predicted Nu, full
+        Nu per point; flagged points keep the full Nu), which the reference's
+        NU_TYPE wisdom Nr field reads (WisdomNrField.cpp:20-24) and so does
+        shell_config(wisdom=read_wisdom(path))."""
+        from .files import write_wisdom
+        nu, flagged = self.predict_point_nu(mesh)
+        nu = np.where(flagged, self.nu_full, np.clip(nu, 1, self.nu_full))
+        sz = mesh.array("point_sz").reshape(-1, 2)
+        write_wisdom(path, np.column_stack([sz[:, 0], sz[:, 1], nu, np.full(nu.shape, self.nu_full)]))
 
     def evaluate(self, seed, nelem_theta=None, radial_scale=1):
         """Full-Nu judge against the predicted-Nu solve of one new model

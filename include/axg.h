@@ -183,7 +183,8 @@ typedef struct axg_shell_config {
     int32_t dv_fluid;                 /* perturb vp in fluid too                   */
     /* Fourier order */
     int32_t nu_mode;                  /* 0 const, 1 uniform per element, 2 log-normal per element,
-                                         3 given per element (elem_nu)             */
+                                         3 given per element (elem_nu), 4 wisdom (NU_TYPE
+                                         wisdom: WisdomNrField.cpp:20-24)          */
     int32_t nu_const, nu_min, nu_max;
     double nu_median, nu_sigma;
     int32_t lucky;                    /* FFTW_LUCKY_NUMBER                         */
@@ -207,6 +208,11 @@ typedef struct axg_shell_config {
     int32_t nthread;                  /* host threads for set-up (0 = all)         */
     const int32_t *elem_nu;           /* nu_mode 3: Nu of every element in build order
                                          (the order of a build with the same geometry) */
+    /* nu_mode 4: Nu of a point = round(factor x inverse-distance mean of the
+       learned Nu of the 4 nearest wisdom rows) (NuWisdom::getNu, NuWisdom.cpp:82-98) */
+    const double *wisdom;             /* [nwisdom][4] s, z, learned Nu, original Nu */
+    int64_t nwisdom;
+    double wisdom_factor;             /* NU_WISDOM_REUSE_FACTOR; 0 reads as 1       */
     /* ocean load on the solid points of the outer surface (OceanLoad3D) */
     double ocean_depth;               /* metres; 0: none (OceanLoad3D_const takes km)  */
     double ocean_var;                 /* relative lateral variation of the depth: depth x
@@ -222,6 +228,27 @@ int axg_mesh_build_shell(const axg_shell_config *cfg, axg_mesh **mesh);
 const axg_mesh_desc *axg_mesh_get_desc(const axg_mesh *mesh);
 int axg_mesh_stats(const axg_mesh *mesh, int64_t *stats, int nstats);
 void axg_mesh_free(axg_mesh *mesh);
+
+/* ---- Nu wisdom files ------------------------------------------------------
+ * Rows (s, z, learned Nu, original Nu) of every GLL point, as Domain::dumpWisdom
+ * gathers them (Domain.cpp:404-438), in the reference's "NetCDF ascii format"
+ * alternative that NuWisdom::readFromFile accepts (NuWisdom.cpp:47-79,
+ * NetCDF_ReaderAscii.cpp): variable axisem3d_wisdom [n][4] (or [n][3]: the
+ * third column is taken as the original Nu too).  axg_wisdom_read with rows
+ * null returns the row count in *n.  axg_wisdom_get_nu evaluates NuWisdom::getNu
+ * (NuWisdom.cpp:82-98) at nq points (s, z). */
+int axg_wisdom_write(const char *path, int64_t n, const double *rows);
+int axg_wisdom_read(const char *path, double *rows, int64_t cap, int64_t *n);
+int axg_wisdom_get_nu(int64_t n, const double *rows, int32_t num_samples, int64_t nq, const double *sz,
+                      int32_t *nu);
+
+/* One station's ascii file (PointwiseIOAscii.cpp:25-35, 96-102): a line
+ * "t c0 c1 c2" at full precision for every step index i with i % interval == 0
+ * (PointwiseRecorder.cpp:62-68); t starts at -t_shift and grows by dt
+ * (Newmark.cpp:27, 64-65).  trace: [nstep] rows of 3 floats, row_stride floats
+ * apart (3 nrec for the station-interleaved seismograms of axg_get_seismograms). */
+int axg_write_station_ascii(const char *path, int64_t nstep, int32_t interval, double t_shift, double dt,
+                            const float *trace, int64_t row_stride);
 
 /* Features of every GLL point for the learned Nu predictor (no counterpart in
  * the reference; they describe what the reference's NuWisdom learns from):

@@ -351,6 +351,15 @@ def test_wisdom_matches_oracle(fluid_layers):
         wg, wr = g.wisdom(), r.wisdom()
         assert (wg >= 0).all()
         assert (wg == wr).all(), (it, np.flatnonzero(wg != wr)[:10])
+    # Domain::dumpWisdom: the file holds (s, z, learned Nu, Nu) of every point
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "wisdom.txt")
+        g.dumpWisdom(path)
+        rows = ax.read_wisdom(path)
+    sz = m.array("point_sz").reshape(-1, 2)
+    assert np.array_equal(rows[:, :2], sz)
+    assert np.array_equal(rows[:, 2], wg) and np.array_equal(rows[:, 3], m.array("point_nr") // 2)
 
 
 def test_wisdom_learned_in_graph_run_matches_eager(fluid_layers):

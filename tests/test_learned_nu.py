@@ -67,3 +67,21 @@ def test_learned_nu_round_trip():
     print(r, flush=True)
     assert np.isfinite(r["misfit"]) and r["misfit"] < 0.05
     assert r["mean_nu_pred"] <= r["mean_nu_full"] + 1e-9
+    # the prediction as a wisdom file, solved through the NU_TYPE wisdom Nr field
+    import os
+    import tempfile
+    m1 = ax.ShellMesh(nl.model_config(900, L.nstep, L.nelem_theta, nu=1))
+    nu, flagged = L.predict_point_nu(m1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "predicted_wisdom.txt")
+        L.write_wisdom(m1, path)
+        rows = ax.read_wisdom(path)
+    assert np.array_equal(rows[:, 2], np.where(flagged, 24, np.clip(nu, 1, 24)))
+    mw = ax.ShellMesh(nl.model_config(900, L.nstep, L.nelem_theta, wisdom=rows))
+    mf = ax.ShellMesh(nl.model_config(900, L.nstep, L.nelem_theta, nu=24))
+    assert mw.stats["sum_nu1"] <= mf.stats["sum_nu1"]
+    sw, _, _ = nl.solve(mw)
+    sf, _, _ = nl.solve(mf)
+    misfit = float(np.linalg.norm(sw - sf) / np.linalg.norm(sf))
+    print("wisdom-file path: misfit %.3e, updates %d of %d" % (misfit, mw.stats["sum_nu1"], mf.stats["sum_nu1"]))
+    assert misfit < 0.1
