@@ -103,6 +103,13 @@ class ShellConfig(C.Structure):
         ("wisdom", C.POINTER(C.c_double)),
         ("nwisdom", C.c_int64),
         ("wisdom_factor", C.c_double),
+        ("emp_scale_axis", C.c_int32), ("emp_scale_theta", C.c_int32), ("emp_scale_depth", C.c_int32),
+        ("emp_pow_axis", C.c_double),
+        ("emp_factor_pi", C.c_double),
+        ("emp_theta_start", C.c_double),
+        ("emp_pow_theta", C.c_double),
+        ("emp_factor_surf", C.c_double),
+        ("emp_depth_start", C.c_double), ("emp_depth_end", C.c_double),
         ("ocean_depth", C.c_double),
         ("ocean_var", C.c_double),
     ]

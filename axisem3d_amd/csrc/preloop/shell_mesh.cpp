@@ -702,7 +702,7 @@ struct Builder {
             // coordinates with numerical noise erased (Quad.cpp:553-558)
             if (!c.wisdom || c.nwisdom <= 0) throw std::runtime_error("NrField || nu_mode 4 needs wisdom rows.");
             const NuWisdomIndex idx(std::vector<double>(c.wisdom, c.wisdom + 4 * c.nwisdom));
-            const double factor = c.wisdom_factor > 0. ? c.wisdom_factor : 1.;
+            const double factor = c.wisdom_factor > 1e-10 ? c.wisdom_factor : 1.;  // NrField.cpp:43-46
             const double offset = 1e-8;
             for (int p = 0; p < npoint; p++) {
                 const auto &pe = pointElems[p].front();
@@ -711,6 +711,31 @@ struct Builder {
                 const int nu = (int)std::lround(idx.getNu(s, z, 4) * factor);
                 baseNr[p] = 2 * nu + 1;
             }
+        } else if (c.nu_mode == 5) {
+            // EmpNrField::getNrAtPoint (EmpNrField.cpp:20-48); R = outer radius
+            const double R = rnode.back(), offset = 1e-8;
+            const double startT = c.emp_theta_start * pi / 180.;
+            for (int p = 0; p < npoint; p++) {
+                const auto &pe = pointElems[p].front();
+                const double s = std::round(quads[pe.first].crd[pe.second][0] / offset) * offset;
+                const double z = std::round(quads[pe.first].crd[pe.second][1] / offset) * offset;
+                const double r = std::hypot(s, z);
+                const double theta = r < 1e-10 ? 0. : std::acos(z / r);  // Geodesy::rtheta
+                const double d = R - r;
+                double nu = (double)c.nu_const;
+                if (c.emp_scale_axis) nu *= std::pow(s / R, c.emp_pow_axis);
+                if (c.emp_scale_theta && theta > startT)
+                    nu *= 1. + (c.emp_factor_pi - 1.) * std::pow((theta - startT) / (pi - startT), c.emp_pow_theta);
+                if (c.emp_scale_depth && d <= c.emp_depth_end) {
+                    if (d <= c.emp_depth_start) nu *= c.emp_factor_surf;
+                    else nu *= 1. + (c.emp_factor_surf - 1.) / (c.emp_depth_start - c.emp_depth_end) * (d - c.emp_depth_end);
+                }
+                const int nr = 2 * std::max(c.nu_min, (int)std::ceil(nu)) + 1;
+                if (nr <= 0) throw std::runtime_error("EmpNrField::getNrAtPoint || Non-positive Nr.");
+                baseNr[p] = nr;
+            }
+        }
+        if (c.nu_mode == 4 || c.nu_mode == 5) {
             for (size_t e = 0; e < quads.size(); e++) {
                 int nu = 0;
                 for (int k = 0; k < nP * nP; k++) nu = std::max(nu, baseNr[quads[e].gll[k]] / 2);

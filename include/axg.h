@@ -184,7 +184,8 @@ typedef struct axg_shell_config {
     /* Fourier order */
     int32_t nu_mode;                  /* 0 const, 1 uniform per element, 2 log-normal per element,
                                          3 given per element (elem_nu), 4 wisdom (NU_TYPE
-                                         wisdom: WisdomNrField.cpp:20-24)          */
+                                         wisdom: WisdomNrField.cpp:20-24), 5 empirical
+                                         (NU_TYPE empirical: EmpNrField.cpp:20-48) */
     int32_t nu_const, nu_min, nu_max;
     double nu_median, nu_sigma;
     int32_t lucky;                    /* FFTW_LUCKY_NUMBER                         */
@@ -213,6 +214,15 @@ typedef struct axg_shell_config {
     const double *wisdom;             /* [nwisdom][4] s, z, learned Nu, original Nu */
     int64_t nwisdom;
     double wisdom_factor;             /* NU_WISDOM_REUSE_FACTOR; 0 reads as 1       */
+    /* nu_mode 5: nu_const = NU_EMP_REFERENCE, nu_min = NU_EMP_MIN, and */
+    int32_t emp_scale_axis, emp_scale_theta, emp_scale_depth; /* NU_EMP_SCALE_AXIS/THETA/DEPTH */
+    double emp_pow_axis;              /* NU_EMP_POW_AXIS                           */
+    double emp_factor_pi;             /* NU_EMP_FACTOR_PI                          */
+    double emp_theta_start;           /* NU_EMP_THETA_START, degrees               */
+    double emp_pow_theta;             /* NU_EMP_POW_THETA                          */
+    double emp_factor_surf;           /* NU_EMP_FACTOR_SURF                        */
+    double emp_depth_start, emp_depth_end; /* NU_EMP_DEPTH_START/END, metres (the
+                                         parameter file gives km)                  */
     /* ocean load on the solid points of the outer surface (OceanLoad3D) */
     double ocean_depth;               /* metres; 0: none (OceanLoad3D_const takes km)  */
     double ocean_var;                 /* relative lateral variation of the depth: depth x

@@ -1,0 +1,66 @@
+"""The empirical Nr field (NU_TYPE empirical, EmpNrField.cpp:20-48) of the
+shell builder, pinned at points where its factors take closed-form values:
+the reference order at the surface of the equator, the axis factor (s/R)^p,
+the epicentral factor reaching NU_EMP_FACTOR_PI at theta = pi, the surface
+factor above NU_EMP_DEPTH_START and the linear ramp to 1 at NU_EMP_DEPTH_END,
+and the NU_EMP_MIN floor.  Lucky numbers are off, so that Nr = 2 Nu + 1 up to
+the upper limit of the GLL spacing (Quad.cpp:560-565), which the points
+checked here stay below."""
+import numpy as np
+
+import axisem3d_amd as ax
+
+
+def _mesh(layers, **kw):
+    cfg = dict(layers=layers, nelem_theta=16, nstep=10, nu_mode=5, lucky=0, nu_const=10, nu_min=2)
+    cfg.update(kw)
+    m = ax.ShellMesh(ax.shell_config(0, **cfg))
+    sz = m.array("point_sz").reshape(-1, 2)
+    r = np.hypot(sz[:, 0], sz[:, 1])
+    theta = np.arccos(np.clip(sz[:, 1] / r, -1, 1))
+    return m, m.array("point_nr"), sz[:, 0], r, theta
+
+
+def test_empirical_constant_and_min(small_layers):
+    _, nr, s, r, th = _mesh(small_layers)
+    # no scaling: Nr = 2 NU_EMP_REFERENCE + 1 away from the axis
+    far = s > 0.5 * r.max()
+    assert far.sum() > 100 and (nr[far] == 21).all()
+    _, nr, _, _, _ = _mesh(small_layers, nu_const=1, nu_min=6)
+    assert (nr[far] == 13).all()
+
+
+def test_empirical_axis_and_theta_factors(small_layers):
+    R = small_layers[-1]["r_top"]
+    _, nr, s, r, th = _mesh(small_layers, emp_scale_axis=1, emp_pow_axis=1.0)
+    eq = (np.abs(th - np.pi / 2) < 1e-9) & (np.abs(r - R) < 1e-3)
+    assert eq.sum() == 1 and nr[eq][0] == 21  # s = R
+    sel = s > 0.3 * R
+    exp = 2 * np.maximum(2, np.ceil(10 * s[sel] / R - 1e-12)).astype(int) + 1
+    assert np.abs(nr[sel] - exp).max() <= 2  # ceil at an exact integer may go either way
+    assert (np.diff(nr[sel][np.argsort(s[sel])]) >= -2).all()
+    # epicentral factor: 1 up to theta_start, NU_EMP_FACTOR_PI at theta = pi
+    _, nr, s, r, th = _mesh(small_layers, emp_scale_theta=1, emp_factor_pi=0.25, emp_theta_start=90.,
+                            emp_pow_theta=1.0)
+    north = (th < np.pi / 2 - 1e-9) & (s > 0.5 * R)
+    assert north.any() and (nr[north] == 21).all()
+    # linear in theta past 90 degrees: at 135 degrees the factor is 0.625, Nu 6.25 -> 7
+    mid = (np.abs(th - 0.75 * np.pi) < 1e-9) & (s > 0)
+    assert mid.any() and (nr[mid] == 2 * 7 + 1).all()
+    south = (np.abs(th - np.pi) < 1e-9)
+    assert south.any() and (nr[south] == 3).all()  # on the axis the upper limit is 3 (Quad.cpp:560-565)
+
+
+def test_empirical_surface_factor(small_layers):
+    R = small_layers[-1]["r_top"]
+    _, nr, s, r, th = _mesh(small_layers, emp_scale_depth=1, emp_factor_surf=2.0, emp_depth_start=100e3,
+                            emp_depth_end=300e3)
+    far = s > 0.5 * R
+    d = R - r
+    assert (nr[far & (d <= 100e3)] == 41).all()
+    assert (nr[far & (d > 300e3 + 1)] == 21).all()
+    ramp = far & (d > 100e3 + 1) & (d < 300e3 - 1)
+    assert ramp.any()
+    fac = 1. + (2. - 1.) / (100e3 - 300e3) * (d[ramp] - 300e3)
+    assert (np.abs(nr[ramp] - (2 * np.ceil(10 * fac) + 1)) <= 2).all()
+    assert (nr[ramp] > 21).all() and (nr[ramp] <= 41).all()
