@@ -55,6 +55,8 @@ class MeshDesc(C.Structure):
         ("rec_components", C.c_int32),
         ("ocean_off", i64p),
         ("ocean", f32p),
+        ("prt_off", i64p),
+        ("prt_pool", f32p),
     ]
 
 

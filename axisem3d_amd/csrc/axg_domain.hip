@@ -45,6 +45,7 @@ struct Bucket1D {
     int material = 0;
     bool tiso = false;
     int att = 0;
+    bool prt = false;   // PRT_1D particle relabelling
     DevBuf<int32_t> item_elem, item_alpha;
     int32_t nitem = 0;
 };
@@ -126,7 +127,9 @@ struct axg_ctx {
     // elements
     std::vector<int32_t> elem_flags, elem_nr;
     DevBuf<int32_t> d_elem_flags, d_elem_nr, d_ep_row, d_ep_amax;
-    DevBuf<float> d_geom, d_tiso, d_mat_pool, d_att_pool;
+    DevBuf<float> d_geom, d_tiso, d_mat_pool, d_att_pool, d_prt_pool;
+    DevBuf<int64_t> d_prt_off;
+    std::vector<char> elem_prt;  // element carries a PRT_1D relabelling
     DevBuf<int64_t> d_mat_off, d_att_off, d_att_mem_off;
     DevBuf<float2> d_att_memvar, d_att_stressR;
     int64_t att_mem_stride = 0;
@@ -722,15 +725,17 @@ Elem1DArgs makeArgs1D(axg_ctx *c, const Bucket1D &b) {
     }
     a.do_kappa = c->do_kappa;
     a.att_mem_stride = c->att_mem_stride;
+    a.prt_pool = c->d_prt_pool.p;
+    a.prt_off = c->d_prt_off.p;
     return a;
 }
 
-template <int MAT, bool TISO>
+template <int MAT, bool TISO, bool PRT>
 void launchSolid1DAtt(const Elem1DArgs &a, int att, hipStream_t s) {
     dim3 grid((a.nitem + E1D_ITEMS - 1) / E1D_ITEMS);
-    if (att == AXG_ATT_NONE) hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 0>), grid, dim3(256), 0, s, a);
-    else if (att == AXG_ATT_FULL) hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 1>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 2>), grid, dim3(256), 0, s, a);
+    if (att == AXG_ATT_NONE) hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 0, PRT>), grid, dim3(256), 0, s, a);
+    else if (att == AXG_ATT_FULL) hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 1, PRT>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((solid1d_kernel<MAT, TISO, 2, PRT>), grid, dim3(256), 0, s, a);
 }
 
 void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
@@ -739,16 +744,25 @@ void launchBucket1D(axg_ctx *c, const Bucket1D &b) {
     Elem1DArgs a = makeArgs1D(c, b);
     hipStream_t s = c->stream;
     if (b.phase == 1) {
-        hipLaunchKernelGGL(fluid1d_kernel, dim3((a.nitem + E1D_ITEMS - 1) / E1D_ITEMS), dim3(256), 0, s, a);
+        const dim3 grid((a.nitem + E1D_ITEMS - 1) / E1D_ITEMS);
+        if (b.prt) hipLaunchKernelGGL(fluid1d_kernel<true>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(fluid1d_kernel<false>, grid, dim3(256), 0, s, a);
+        return;
+    }
+    if (b.prt) {
+        // a relabelled element always rotates to (R, T, Z) (SolidElement.cpp:21)
+        if (b.material == AXG_MAT_ISOTROPIC) launchSolid1DAtt<0, true, true>(a, b.att, s);
+        else if (b.material == AXG_MAT_TRANSVERSELY_ISO) launchSolid1DAtt<1, true, true>(a, b.att, s);
+        else launchSolid1DAtt<2, true, true>(a, b.att, s);
         return;
     }
     if (b.material == AXG_MAT_ISOTROPIC) {
-        if (b.tiso) launchSolid1DAtt<0, true>(a, b.att, s);
-        else launchSolid1DAtt<0, false>(a, b.att, s);
+        if (b.tiso) launchSolid1DAtt<0, true, false>(a, b.att, s);
+        else launchSolid1DAtt<0, false, false>(a, b.att, s);
     } else if (b.material == AXG_MAT_TRANSVERSELY_ISO) {
-        launchSolid1DAtt<1, true>(a, b.att, s);
+        launchSolid1DAtt<1, true, false>(a, b.att, s);
     } else {
-        launchSolid1DAtt<2, true>(a, b.att, s);
+        launchSolid1DAtt<2, true, false>(a, b.att, s);
     }
 }
 
@@ -946,7 +960,8 @@ void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
         hipLaunchKernelGGL(record_fluid_kernel, dim3(c->nrec_fluid), dim3(256), 0, c->stream, c->nrec_fluid, c->nrec,
                            c->d_rec_idx_f.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
                            c->d_rec_sint.p, c->d_ep_row.p, c->d_ep_amax.p, c->d_elem_nr.p, c->d_elem_flags.p,
-                           c->d_geom.p, c->d_mat_pool.p, c->d_mat_off.p, c->F.displ.p, c->rec_rtz, tstep, tstep_ptr,
+                           c->d_geom.p, c->d_mat_pool.p, c->d_mat_off.p, c->d_tiso.p, c->d_prt_pool.p,
+                           c->d_prt_pool.p ? c->d_prt_off.p : nullptr, c->F.displ.p, c->rec_rtz, tstep, tstep_ptr,
                            c->d_seis.p);
 }
 
@@ -1061,6 +1076,35 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
     c->d_tiso.upload(d->elem_tiso, (size_t)NE * 100, s);
     c->d_mat_pool.upload(d->mat_pool, (size_t)d->mat_off[NE], s);
     c->d_mat_off.upload(d->mat_off, (size_t)NE + 1, s);
+    // particle relabelling: PRT_1D (X0..X3 of the 25 points) on 1-D elements
+    // (Relabelling::createPRT, Relabelling.cpp:153-170)
+    c->elem_prt.assign(NE, 0);
+    if (d->prt_off) {
+        std::vector<int64_t> off(NE + 1, 0);
+        std::vector<float> pool;
+        for (int e = 0; e < NE; e++) {
+            const int64_t n = d->prt_off[e + 1] - d->prt_off[e];
+            off[e] = (int64_t)pool.size();  // the element's first float
+            if (n == 0) continue;
+            if (n == 100 * (int64_t)c->elem_nr[e] && n != 100)
+                throw std::runtime_error("PRT_3D || particle relabelling varying in azimuth is not supported by this "
+                                         "build.");
+            if (n != 100) throw std::runtime_error("PRT_1D || Incompatible size.");
+            if (d->elem_flags[e] & AXG_EL_3D)
+                throw std::runtime_error("PRT_3D || particle relabelling on a 3-D element is not supported by this "
+                                         "build.");
+            if (!(d->elem_flags[e] & AXG_EL_TISO))
+                throw std::runtime_error("SolidElement::SolidElement || a relabelled element needs the TIso "
+                                         "rotation.");
+            pool.insert(pool.end(), d->prt_pool + d->prt_off[e], d->prt_pool + d->prt_off[e + 1]);
+            c->elem_prt[e] = 1;
+        }
+        off[NE] = (int64_t)pool.size();
+        if (!pool.empty()) {
+            c->d_prt_pool.upload(pool, s);
+            c->d_prt_off.upload(off, s);
+        }
+    }
     // attenuation
     c->nsls = d->nsls;
     c->do_kappa = d->att_do_kappa;
@@ -1115,7 +1159,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
             }
         }
         c->ncolour = 1 + *std::max_element(colour.begin(), colour.end());
-        const int NK = 64;
+        const int NK = 128;
         std::vector<std::vector<int32_t>> ie((size_t)c->ncolour * NK), ia((size_t)c->ncolour * NK),
             e3((size_t)c->ncolour * NK);
         for (int e = 0; e < NE; e++) {
@@ -1128,7 +1172,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
             if ((mat == AXG_MAT_TRANSVERSELY_ISO || mat == AXG_MAT_ANISOTROPIC) && !tiso)
                 throw std::runtime_error("SolidElement::SolidElement || material needs the TIso rotation.");
             int att = d->elem_att[e];
-            int key = colour[e] * NK + (fluid ? 1 : 0) * 32 + mat * 8 + (tiso ? 4 : 0) + att;
+            int key = colour[e] * NK + (c->elem_prt[e] ? 64 : 0) + (fluid ? 1 : 0) * 32 + mat * 8 + (tiso ? 4 : 0) + att;
             if (fl & AXG_EL_3D) {
                 e3[key].push_back(e);
                 continue;
@@ -1146,7 +1190,8 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
             if (!ie[key].empty()) {
                 auto b = std::make_unique<Bucket1D>();
                 b->colour = key / NK;
-                b->phase = kk / 32;
+                b->prt = kk >= 64;
+                b->phase = (kk / 32) % 2;
                 b->material = (kk / 8) % 4;
                 b->tiso = (kk / 4) % 2;
                 b->att = kk % 4;
@@ -1156,7 +1201,8 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
                 HIPCHECK(hipStreamSynchronize(s));
                 c->buckets1d.push_back(std::move(b));
             }
-            if (!e3[key].empty()) createBuckets3D(c, d, key / NK, kk / 32, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
+            if (!e3[key].empty())
+                createBuckets3D(c, d, key / NK, (kk / 32) % 2, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
         }
         // transforms of the points with Mass3D / MassOcean3D / SFCoupling3D use
         // the same tables
@@ -1540,8 +1586,8 @@ struct MergedDesc {
     std::vector<int64_t> mass_solid_off, mass_fluid_off, mat_off, att_off, sf_off;
     std::vector<float> inv_mass_solid, inv_mass_fluid, mat_pool, elem_geom, elem_tiso, att_pool, sf_normal,
         rec_weights, rec_phi, rec_theta;
-    std::vector<int64_t> ocean_off;
-    std::vector<float> ocean;
+    std::vector<int64_t> ocean_off, prt_off;
+    std::vector<float> ocean, prt_pool;
 };
 
 template <class T>
@@ -1607,6 +1653,13 @@ void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo 
             appendOffsets(m.sf_off, d.sf_off, d.nsf);
             append(m.sf_normal, d.sf_normal + 6 * d.sf_off[0], (size_t)(6 * (d.sf_off[d.nsf] - d.sf_off[0])));
         }
+        if (d.prt_off) {
+            appendOffsets(m.prt_off, d.prt_off, ne);
+            append(m.prt_pool, d.prt_pool + d.prt_off[0], (size_t)(d.prt_off[ne] - d.prt_off[0]));
+        } else {
+            std::vector<int64_t> none(ne + 1, 0);
+            appendOffsets(m.prt_off, none.data(), ne);
+        }
         if (d.ocean_off) {
             appendOffsets(m.ocean_off, d.ocean_off, np);
             append(m.ocean, d.ocean + 4 * d.ocean_off[0], (size_t)(4 * (d.ocean_off[np] - d.ocean_off[0])));
@@ -1644,6 +1697,8 @@ void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo 
     d.inv_mass_fluid = m.inv_mass_fluid.data();
     d.ocean_off = m.ocean_off.data();
     d.ocean = m.ocean.data();
+    d.prt_off = m.prt_off.data();
+    d.prt_pool = m.prt_pool.data();
     d.nelem = nelem;
     d.elem_points = m.elem_points.data();
     d.elem_flags = m.elem_flags.data();

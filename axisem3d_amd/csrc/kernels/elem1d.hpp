@@ -50,7 +50,97 @@ struct Elem1DArgs {
     float sls_alpha[8], sls_beta[8], sls_gamma[8];
     int32_t do_kappa;
     int64_t att_mem_stride;   // complex entries per SLS
+    // particle relabelling (PRT_1D): X0..X3 per GLL point, [4][25] per element
+    const float *prt_pool;
+    const int64_t *prt_off;   // [nelem+1]
 };
+
+// Gradient::computeGrad9 (Gradient.cpp:84-152) at one GLL point: the 3 x 3
+// displacement gradient g[3c + d] = d u_c / d x_d, c and d over (s, phi, z),
+// with the axial row-0 terms
+__device__ __forceinline__ void grad9_at(const float2 (*U)[25], int i, int j, float alpha, bool axial,
+                                         const PointGeom &g, float2 o[9]) {
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    float2 GU[3], UG[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float2 a = czero(), b = czero();
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            float gx = Gx[m * 5 + i];
+            float ge = Ge[m * 5 + j];
+            float2 um = U[c][m * 5 + j];
+            float2 un = U[c][i * 5 + m];
+            a.x += gx * um.x;
+            a.y += gx * um.y;
+            b.x += un.x * ge;
+            b.y += un.y * ge;
+        }
+        GU[c] = a;
+        UG[c] = b;
+    }
+    float2 u0 = U[0][i * 5 + j], u1 = U[1][i * 5 + j], u2 = U[2][i * 5 + j];
+    float2 v0 = cadd(u0, cimul(alpha, u1));
+    float2 v1 = csub(cimul(alpha, u0), u1);
+    float2 v2 = cimul(alpha, u2);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o[3 * c + 0] = cadd(cscale(g.dzdeta, GU[c]), cscale(g.dzdxii, UG[c]));
+        o[3 * c + 2] = cadd(cscale(g.dsdeta, GU[c]), cscale(g.dsdxii, UG[c]));
+    }
+    o[1] = cscale(g.inv_s, v1);
+    o[4] = cscale(g.inv_s, v0);
+    o[7] = cscale(g.inv_s, v2);
+    if (axial && i == 0) {
+        o[4] = cadd(o[4], cscale(g.dzdeta, cadd(GU[0], cimul(alpha, GU[1]))));
+        o[1] = cadd(o[1], cscale(g.dzdeta, csub(cimul(alpha, GU[0]), GU[1])));
+        o[7] = cadd(o[7], cscale(g.dzdeta, cimul(alpha, GU[2])));
+        if (alpha == 1.f) {
+            o[4] = cadd(o[4], cscale(g.dzdxii, cadd(UG[0], cimul(alpha, UG[1]))));
+            o[1] = cadd(o[1], cscale(g.dzdxii, csub(cimul(alpha, UG[0]), UG[1])));
+        }
+    }
+}
+
+// CrdTransTIsoSolid::transformSPZ_RTZ / transformRTZ_SPZ of a 3 x 3 tensor
+// (CrdTransTIsoSolid.cpp:44-84); sg = -1 for SPZ -> RTZ, +1 back
+__device__ __forceinline__ void tiso9(float2 u[9], float s1, float c1, float s2, float c2, float sg) {
+    const float2 sum08 = cadd(u[0], u[8]), dif08 = csub(u[0], u[8]);
+    const float2 sum26 = cadd(u[2], u[6]), dif26 = csub(u[2], u[6]);
+    const float2 u1 = u[1], u3 = u[3];
+    u[0] = cscale(0.5f, cadd(cadd(sum08, cscale(c2, dif08)), cscale(sg * s2, sum26)));
+    u[2] = cscale(0.5f, csub(cadd(dif26, cscale(c2, sum26)), cscale(sg * s2, dif08)));
+    u[6] = csub(u[2], dif26);
+    u[8] = csub(sum08, u[0]);
+    u[1] = cadd(cscale(c1, u1), cscale(sg * s1, u[7]));
+    u[7] = csub(cscale(c1, u[7]), cscale(sg * s1, u1));
+    u[3] = cadd(cscale(c1, u3), cscale(sg * s1, u[5]));
+    u[5] = csub(cscale(c1, u[5]), cscale(sg * s1, u3));
+}
+
+// PRT_1D::sphericalToUndulated (PRT_1D.cpp:28-49): gradient -> undulated strain
+__device__ __forceinline__ void prt_sph_und(const float2 g[9], float X0, float X1, float X2, float X3, float2 e[6]) {
+    e[0] = cadd(cscale(X0, g[0]), cscale(X1, g[2]));
+    e[1] = cadd(cscale(X0, g[4]), cscale(X2, g[5]));
+    e[2] = cscale(X3, g[8]);
+    e[3] = cadd(cadd(cscale(X0, g[7]), cscale(X2, g[8])), cscale(X3, g[5]));
+    e[4] = cadd(cadd(cscale(X0, g[6]), cscale(X1, g[8])), cscale(X3, g[2]));
+    e[5] = cadd(cadd(cscale(X0, cadd(g[3], g[1])), cscale(X1, g[5])), cscale(X2, g[2]));
+}
+
+// PRT_1D::undulatedToSpherical (PRT_1D.cpp:51-71): undulated stress -> 3 x 3
+__device__ __forceinline__ void prt_und_sph(const float2 t[6], float X0, float X1, float X2, float X3, float2 o[9]) {
+    o[0] = cscale(X0, t[0]);
+    o[1] = cscale(X0, t[5]);
+    o[2] = cadd(cadd(cscale(X1, t[0]), cscale(X3, t[4])), cscale(X2, t[5]));
+    o[3] = o[1];
+    o[4] = cscale(X0, t[1]);
+    o[5] = cadd(cadd(cscale(X2, t[1]), cscale(X3, t[3])), cscale(X1, t[5]));
+    o[6] = cscale(X0, t[4]);
+    o[7] = cscale(X0, t[3]);
+    o[8] = cadd(cadd(cscale(X3, t[2]), cscale(X2, t[3])), cscale(X1, t[4]));
+}
 
 // strain of one GLL point (i, j) of mode alpha, from the LDS displacement
 // U[c][m*5+n] of the item.  Gradient::computeGrad6 with the axial row-0 terms.
@@ -208,7 +298,10 @@ __device__ __forceinline__ void attenuation1d(const Elem1DArgs &a, int64_t e, in
     for (int c = 0; c < 6; c++) SR[base + c * np] = nsr[c];
 }
 
-template <int MAT, bool TISO, int ATT>
+// PRT: the element carries a particle relabelling (SolidElement.cpp:404-441):
+// 3 x 3 gradient, rotation, PRT_1D to the undulated strain, stress, back to a
+// 3 x 3 stress, rotation back, Gradient::computeQuad9 (Gradient.cpp:154-204)
+template <int MAT, bool TISO, int ATT, bool PRT>
 __global__ __launch_bounds__(256) void solid1d_kernel(Elem1DArgs a) {
     __shared__ float2 sU[E1D_ITEMS][3][25];
     __shared__ float2 sX[E1D_ITEMS][3][25];
@@ -243,7 +336,6 @@ __global__ __launch_bounds__(256) void solid1d_kernel(Elem1DArgs a) {
         gm = load_geom(a.geom, e, k);
         float fa = (float)alpha;
         float2 e6[6], s6[6];
-        strain6_at(sU[it], i, j, fa, axial, gm, e6);
         float ts1 = 0.f, tc1 = 0.f, ts2 = 0.f, tc2 = 0.f;
         if (TISO) {
             const float *tp = a.tiso + e * 100;
@@ -251,23 +343,53 @@ __global__ __launch_bounds__(256) void solid1d_kernel(Elem1DArgs a) {
             tc1 = tp[25 + k];
             ts2 = tp[50 + k];
             tc2 = tp[75 + k];
-            tiso_spz_rtz(e6, ts1, tc1, ts2, tc2);
+        }
+        float X0 = 1.f, X1 = 0.f, X2 = 0.f, X3 = 1.f;
+        if (PRT) {
+            const float *xp = a.prt_pool + a.prt_off[e];
+            X0 = xp[k];
+            X1 = xp[25 + k];
+            X2 = xp[50 + k];
+            X3 = xp[75 + k];
+            float2 g9[9];
+            grad9_at(sU[it], i, j, fa, axial, gm, g9);
+            tiso9(g9, ts1, tc1, ts2, tc2, -1.f);
+            prt_sph_und(g9, X0, X1, X2, X3, e6);
+        } else {
+            strain6_at(sU[it], i, j, fa, axial, gm, e6);
+            if (TISO) tiso_spz_rtz(e6, ts1, tc1, ts2, tc2);
         }
         stress1d<MAT>(a.mat_pool + a.mat_off[e], k, e6, s6);
         if (ATT != 0) attenuation1d<ATT>(a, e, alpha, k, e6, s6);
-        if (TISO) tiso_rtz_spz(s6, ts1, tc1, ts2, tc2);
-        // Gradient::computeQuad6 operands at this point
-        sX[it][0][k] = cadd(cscale(gm.dzdeta, s6[0]), cscale(gm.dsdeta, s6[4]));
-        sX[it][1][k] = cadd(cscale(gm.dzdeta, s6[5]), cscale(gm.dsdeta, s6[3]));
-        sX[it][2][k] = cadd(cscale(gm.dzdeta, s6[4]), cscale(gm.dsdeta, s6[2]));
-        sY[it][0][k] = cadd(cscale(gm.dzdxii, s6[0]), cscale(gm.dsdxii, s6[4]));
-        sY[it][1][k] = cadd(cscale(gm.dzdxii, s6[5]), cscale(gm.dsdxii, s6[3]));
-        sY[it][2][k] = cadd(cscale(gm.dzdxii, s6[4]), cscale(gm.dsdxii, s6[2]));
-        // g = (s1 - i a s5, -i a s1 - s5, -i a s3)
         float nb = -fa;
-        g[0] = cadd(s6[1], cimul(nb, s6[5]));
-        g[1] = csub(cimul(nb, s6[1]), s6[5]);
-        g[2] = cimul(nb, s6[3]);
+        if (PRT) {
+            // Gradient::computeQuad9 operands at this point
+            float2 t9[9];
+            prt_und_sph(s6, X0, X1, X2, X3, t9);
+            tiso9(t9, ts1, tc1, ts2, tc2, 1.f);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                sX[it][c][k] = cadd(cscale(gm.dzdeta, t9[3 * c]), cscale(gm.dsdeta, t9[3 * c + 2]));
+                sY[it][c][k] = cadd(cscale(gm.dzdxii, t9[3 * c]), cscale(gm.dsdxii, t9[3 * c + 2]));
+            }
+            // g = (t4 - i a t1, -i a t4 - t1, -i a t7)
+            g[0] = cadd(t9[4], cimul(nb, t9[1]));
+            g[1] = csub(cimul(nb, t9[4]), t9[1]);
+            g[2] = cimul(nb, t9[7]);
+        } else {
+            if (TISO) tiso_rtz_spz(s6, ts1, tc1, ts2, tc2);
+            // Gradient::computeQuad6 operands at this point
+            sX[it][0][k] = cadd(cscale(gm.dzdeta, s6[0]), cscale(gm.dsdeta, s6[4]));
+            sX[it][1][k] = cadd(cscale(gm.dzdeta, s6[5]), cscale(gm.dsdeta, s6[3]));
+            sX[it][2][k] = cadd(cscale(gm.dzdeta, s6[4]), cscale(gm.dsdeta, s6[2]));
+            sY[it][0][k] = cadd(cscale(gm.dzdxii, s6[0]), cscale(gm.dsdxii, s6[4]));
+            sY[it][1][k] = cadd(cscale(gm.dzdxii, s6[5]), cscale(gm.dsdxii, s6[3]));
+            sY[it][2][k] = cadd(cscale(gm.dzdxii, s6[4]), cscale(gm.dsdxii, s6[2]));
+            // g = (s1 - i a s5, -i a s1 - s5, -i a s3)
+            g[0] = cadd(s6[1], cimul(nb, s6[5]));
+            g[1] = csub(cimul(nb, s6[1]), s6[5]);
+            g[2] = cimul(nb, s6[3]);
+        }
         if (axial && i == 0) {
 #pragma unroll
             for (int c = 0; c < 3; c++) sW[it][c][j] = cscale(gm.dzdeta, g[c]);
@@ -318,6 +440,10 @@ __global__ __launch_bounds__(256) void solid1d_kernel(Elem1DArgs a) {
     }
 }
 
+// PRT: FluidElement::displToStiff with a relabelling (FluidElement.cpp:333-355):
+// gradient rotated to (R, T, Z) (CrdTransTIsoFluid.cpp:14-32), PRT_1D to the
+// undulated gradient and back (PRT_1D.cpp:7-26)
+template <bool PRT>
 __global__ __launch_bounds__(256) void fluid1d_kernel(Elem1DArgs a) {
     __shared__ float2 sU[E1D_ITEMS][25];
     __shared__ float2 sX[E1D_ITEMS][25];
@@ -366,9 +492,33 @@ __global__ __launch_bounds__(256) void fluid1d_kernel(Elem1DArgs a) {
         float2 e1 = cscale(gm.inv_s, v);
         float2 e2 = cadd(cscale(gm.dsdeta, GU), cscale(gm.dsdxii, UG));
         if (axial && i == 0) e1 = cadd(e1, cscale(gm.dzdeta, cimul(fa, GU)));
+        float tc1 = 1.f, ts1 = 0.f, X0 = 1.f, X1 = 0.f, X2 = 0.f, X3 = 1.f;
+        if (PRT) {
+            ts1 = a.tiso[e * 100 + k];
+            tc1 = a.tiso[e * 100 + 25 + k];
+            const float2 r0 = csub(cscale(tc1, e0), cscale(ts1, e2));
+            e2 = cadd(cscale(tc1, e2), cscale(ts1, e0));
+            e0 = r0;
+            const float *xp = a.prt_pool + a.prt_off[e];
+            X0 = xp[k];
+            X1 = xp[25 + k];
+            X2 = xp[50 + k];
+            X3 = xp[75 + k];
+            e0 = cadd(cscale(X0, e0), cscale(X1, e2));
+            e1 = cadd(cscale(X0, e1), cscale(X2, e2));
+            e2 = cscale(X3, e2);
+        }
         // Acoustic1D::strainToStress (Acoustic1D.cpp:8-16)
         float K = a.mat_pool[a.mat_off[e] + k];
         float2 s0 = cscale(K, e0), s1 = cscale(K, e1), s2 = cscale(K, e2);
+        if (PRT) {
+            s2 = cadd(cadd(cscale(X1, s0), cscale(X2, s1)), cscale(X3, s2));
+            s0 = cscale(X0, s0);
+            s1 = cscale(X0, s1);
+            const float2 r0 = cadd(cscale(tc1, s0), cscale(ts1, s2));
+            s2 = csub(cscale(tc1, s2), cscale(ts1, s0));
+            s0 = r0;
+        }
         // Gradient::computeQuad (Gradient.cpp:58-82)
         sX[it][k] = cadd(cscale(gm.dzdeta, s0), cscale(gm.dsdeta, s2));
         sY[it][k] = cadd(cscale(gm.dzdxii, s0), cscale(gm.dsdxii, s2));

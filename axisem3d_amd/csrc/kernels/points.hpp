@@ -298,6 +298,9 @@ __global__ __launch_bounds__(256) void record_fluid_kernel(int32_t nrec, int32_t
                                                            const float *__restrict__ geom,
                                                            const float *__restrict__ mat_pool,
                                                            const int64_t *__restrict__ mat_off,
+                                                           const float *__restrict__ tiso,
+                                                           const float *__restrict__ prt_pool,
+                                                           const int64_t *__restrict__ prt_off,
                                                            const float2 *__restrict__ chi, int rtz, int tstep,
                                                            const int *__restrict__ tstep_ptr,
                                                            float *__restrict__ seis) {
@@ -335,6 +338,23 @@ __global__ __launch_bounds__(256) void record_fluid_kernel(int32_t nrec, int32_t
         gr[1] = cscale(g.inv_s, cimul((float)al, X(k)));
         gr[2] = cadd(cscale(g.dsdeta, GU), cscale(g.dsdxii, UG));
         if (axial && i == 0) gr[1] = cadd(gr[1], cscale(g.dzdeta, cimul((float)al, GU)));
+        if (prt_off && prt_off[e + 1] > prt_off[e]) {
+            // relabelled element (FluidElement.cpp:176-194): rotate, PRT_1D there
+            // and back, rotate back; K is a factor per point
+            const float s1 = tiso[e * 100 + k], c1 = tiso[e * 100 + 25 + k];
+            const float *xp = prt_pool + prt_off[e];
+            const float X0 = xp[k], X1 = xp[25 + k], X2 = xp[50 + k], X3 = xp[75 + k];
+            float2 r0 = csub(cscale(c1, gr[0]), cscale(s1, gr[2]));
+            float2 r2 = cadd(cscale(c1, gr[2]), cscale(s1, gr[0]));
+            float2 u0 = cadd(cscale(X0, r0), cscale(X1, r2));
+            float2 u1 = cadd(cscale(X0, gr[1]), cscale(X2, r2));
+            float2 u2 = cscale(X3, r2);
+            r2 = cadd(cadd(cscale(X1, u0), cscale(X2, u1)), cscale(X3, u2));
+            r0 = cscale(X0, u0);
+            gr[1] = cscale(X0, u1);
+            gr[0] = cadd(cscale(c1, r0), cscale(s1, r2));
+            gr[2] = csub(cscale(c1, r2), cscale(s1, r0));
+        }
         const float kk = K[k] * wt;
         if (al == 0) {
             acc[0] += kk * gr[0].x;

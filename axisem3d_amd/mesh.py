@@ -144,6 +144,11 @@ class ShellMesh:
             return self._view(name, int(self.array("att_off")[-1]), np.float32)
         if name == "sf_normal":
             return self._view(name, 6 * int(self.array("sf_off")[-1]), np.float32)
+        if name == "prt_off":
+            return self._view(name, ne + 1, np.int64)
+        if name == "prt_pool":
+            off = self.array("prt_off")
+            return self._view(name, int(off[-1]) if off.size else 0, np.float32)
         if name == "ocean_off":
             return self._view(name, npt + 1, np.int64)
         if name == "ocean":

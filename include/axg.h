@@ -154,6 +154,16 @@ typedef struct axg_mesh_desc {
      * m_o = 1027 kg/m^3 x ocean depth x |assembled surface normal| (OceanLoad3D.cpp:18). */
     const int64_t *ocean_off;
     const float   *ocean;
+
+    /* ---- particle relabelling of undulated elements (PRT, Relabelling::createPRT,
+     *      Relabelling.cpp:153-170); null prt_off: none ----
+     * prt_off [nelem+1]: 0 or 100 floats per element: X0, X1, X2, X3 of the 25
+     * GLL points ([4][25], Relabelling::getStiffX, Relabelling.cpp:84-109) for
+     * PRT_1D; such an element needs AXG_EL_TISO (SolidElement.cpp:21,
+     * FluidElement.cpp:21), and its material (and mass) already carry the
+     * relabelling Jacobians.  PRT_3D (nr x 100) is not supported by this build. */
+    const int64_t *prt_off;
+    const float   *prt_pool;
 } axg_mesh_desc;
 
 /* ===================================================================== *

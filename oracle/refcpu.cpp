@@ -184,6 +184,8 @@ struct Ref {
         RMat dsdxii, dsdeta, dzdxii, dzdeta, inv_s;
         RMat sin1t, cos1t, sin2t, cos2t;
         std::vector<Real> matv;  // as in the descriptor (Real)
+        bool prt = false;        // PRT_1D (Relabelling::createPRT)
+        std::array<Real, 100> X; // X0..X3 of the 25 points
         // attenuation
         std::vector<Real> dkappa3, dmu, dmu2;
         std::vector<C> stressR1;                // 1-D: [alpha][6][np]
@@ -238,6 +240,7 @@ struct Ref {
     struct Work {
         std::vector<std::array<CMat, 3>> displ, stiff;
         std::vector<std::array<CMat, 6>> strain6, stress6;
+        std::vector<std::array<CMat, 9>> strain9, stress9;
         std::vector<CMat> fdispl, fstiff;
         std::vector<std::array<CMat, 3>> fstrain, fstress;
         std::vector<C> flatC;     // [alpha][150]
@@ -352,6 +355,12 @@ struct Ref {
                 el.cos2t[k] = (Real)t[75 + k];
             }
             el.matv.assign(d->mat_pool + d->mat_off[e], d->mat_pool + d->mat_off[e + 1]);
+            if (d->prt_off && d->prt_off[e + 1] > d->prt_off[e]) {
+                if (d->prt_off[e + 1] - d->prt_off[e] != 100 || el.is3D)
+                    throw std::runtime_error("PRT_3D || not restated by this oracle.");
+                el.prt = true;
+                for (int i = 0; i < 100; i++) el.X[i] = (Real)d->prt_pool[d->prt_off[e] + i];
+            }
             if (el.att != AXG_ATT_NONE) {
                 int np = el.att == AXG_ATT_CG4 ? 4 : 25;
                 int rows = el.is3D ? el.nr : 1;
@@ -766,6 +775,174 @@ struct Ref {
             for (int c = 0; c < 3; c++) fi[Nu][c].fill(C(0, 0));
     }
 
+    // Gradient::computeGrad9 (element/grad/Gradient.cpp:84-152): ui_j[3c+d] =
+    // d u_c / d x_d over (s, phi, z)
+    void grad9(const Elem &el, const std::vector<std::array<CMat, 3>> &ui, std::vector<std::array<CMat, 9>> &uij,
+               int Nu, int nyq) {
+        const RMat &GTx = el.axial ? GT_GLJ : GT_GLL;
+        const RMat &Ge = G_GLL;
+        C ii(zero, one);
+        for (int alpha = 0; alpha <= Nu - nyq; alpha++) {
+            const auto &u = ui[alpha];
+            auto &o = uij[alpha];
+            C ia = (Real)alpha * ii;
+            CMat v[3], GU[3], UG[3];
+            for (int k = 0; k < 25; k++) {
+                v[0][k] = u[0][k] + ia * u[1][k];
+                v[1][k] = ia * u[0][k] - u[1][k];
+                v[2][k] = ia * u[2][k];
+            }
+            for (int c = 0; c < 3; c++)
+                for (int i = 0; i < 5; i++)
+                    for (int j = 0; j < 5; j++) {
+                        C a(0, 0), b(0, 0);
+                        for (int m = 0; m < 5; m++) {
+                            a += GTx[i * 5 + m] * u[c][m * 5 + j];
+                            b += u[c][i * 5 + m] * Ge[m * 5 + j];
+                        }
+                        GU[c][i * 5 + j] = a;
+                        UG[c][i * 5 + j] = b;
+                    }
+            for (int k = 0; k < 25; k++) {
+                for (int c = 0; c < 3; c++) {
+                    o[3 * c][k] = el.dzdeta[k] * GU[c][k] + el.dzdxii[k] * UG[c][k];
+                    o[3 * c + 2][k] = el.dsdeta[k] * GU[c][k] + el.dsdxii[k] * UG[c][k];
+                }
+                o[1][k] = el.inv_s[k] * v[1][k];
+                o[4][k] = el.inv_s[k] * v[0][k];
+                o[7][k] = el.inv_s[k] * v[2][k];
+            }
+            if (el.axial) {
+                // row 0: the 1/s terms become derivatives along xi, at alpha = 1
+                // also along eta
+                for (int j = 0; j < 5; j++) {
+                    C d[3] = {C(0, 0), C(0, 0), C(0, 0)}, f[2] = {C(0, 0), C(0, 0)};
+                    for (int m = 0; m < 5; m++)
+                        for (int c = 0; c < 3; c++) d[c] += GTx[0 * 5 + m] * v[c][m * 5 + j];
+                    o[4][j] += el.dzdeta[j] * d[0];
+                    o[1][j] += el.dzdeta[j] * d[1];
+                    o[7][j] += el.dzdeta[j] * d[2];
+                    if (alpha == 1) {
+                        for (int m = 0; m < 5; m++) {
+                            f[0] += v[0][0 * 5 + m] * Ge[m * 5 + j];
+                            f[1] += v[1][0 * 5 + m] * Ge[m * 5 + j];
+                        }
+                        o[4][j] += el.dzdxii[j] * f[0];
+                        o[1][j] += el.dzdxii[j] * f[1];
+                    }
+                }
+            }
+            if (alpha == 0)
+                for (int c = 0; c < 9; c++)
+                    for (int k = 0; k < 25; k++) o[c][k] = C(o[c][k].real(), zero);
+        }
+        if (nyq)
+            for (int c = 0; c < 9; c++) uij[Nu][c].fill(C(0, 0));
+    }
+
+    // Gradient::computeQuad9 (Gradient.cpp:154-204)
+    void quad9(const Elem &el, std::vector<std::array<CMat, 3>> &fi, const std::vector<std::array<CMat, 9>> &sij,
+               int Nu, int nyq) {
+        const RMat &Gx = el.axial ? G_GLJ : G_GLL;
+        const RMat &GTe = GT_GLL;
+        C ii(zero, one);
+        for (int b = 0; b <= Nu - nyq; b++) {
+            const auto &s = sij[b];
+            auto &f = fi[b];
+            C ib = -(Real)b * ii;
+            CMat g[3], X[3], Y[3];
+            for (int k = 0; k < 25; k++) {
+                g[0][k] = s[4][k] + ib * s[1][k];
+                g[1][k] = ib * s[4][k] - s[1][k];
+                g[2][k] = ib * s[7][k];
+                for (int c = 0; c < 3; c++) {
+                    X[c][k] = el.dzdeta[k] * s[3 * c][k] + el.dsdeta[k] * s[3 * c + 2][k];
+                    Y[c][k] = el.dzdxii[k] * s[3 * c][k] + el.dsdxii[k] * s[3 * c + 2][k];
+                }
+            }
+            for (int c = 0; c < 3; c++)
+                for (int i = 0; i < 5; i++)
+                    for (int j = 0; j < 5; j++) {
+                        C a(0, 0);
+                        for (int m = 0; m < 5; m++) a += Gx[i * 5 + m] * X[c][m * 5 + j] + Y[c][i * 5 + m] * GTe[m * 5 + j];
+                        f[c][i * 5 + j] = a + el.inv_s[i * 5 + j] * g[c][i * 5 + j];
+                    }
+            if (el.axial) {
+                for (int c = 0; c < 3; c++)
+                    for (int i = 0; i < 5; i++)
+                        for (int j = 0; j < 5; j++) f[c][i * 5 + j] += Gx[i * 5 + 0] * (el.dzdeta[j] * g[c][j]);
+                if (b == 1)
+                    for (int c = 0; c < 2; c++)
+                        for (int j = 0; j < 5; j++) {
+                            C h(0, 0);
+                            for (int m = 0; m < 5; m++) h += el.dzdxii[m] * g[c][m] * GTe[m * 5 + j];
+                            f[c][j] += h;
+                        }
+            }
+            if (b == 0)
+                for (int c = 0; c < 3; c++)
+                    for (int k = 0; k < 25; k++) f[c][k] = C(f[c][k].real(), zero);
+        }
+        if (nyq)
+            for (int c = 0; c < 3; c++) fi[Nu][c].fill(C(0, 0));
+    }
+
+    // CrdTransTIsoSolid::transformSPZ_RTZ / RTZ_SPZ of a 3 x 3 tensor
+    // (CrdTransTIsoSolid.cpp:44-84); sg = -1 SPZ -> RTZ, +1 back
+    void rot9(const Elem &el, std::vector<std::array<CMat, 9>> &u, int Nu, Real sg) {
+        for (int a = 0; a <= Nu; a++) {
+            auto &ua = u[a];
+            for (int k = 0; k < 25; k++) {
+                const Real c1 = el.cos1t[k], s1 = sg * el.sin1t[k], c2 = el.cos2t[k], s2 = sg * el.sin2t[k];
+                C sum08 = ua[0][k] + ua[8][k], dif08 = ua[0][k] - ua[8][k];
+                C sum26 = ua[2][k] + ua[6][k], dif26 = ua[2][k] - ua[6][k];
+                C u1 = ua[1][k], u3 = ua[3][k];
+                ua[0][k] = half * (sum08 + c2 * dif08 + s2 * sum26);
+                ua[2][k] = half * (dif26 + c2 * sum26 - s2 * dif08);
+                ua[6][k] = ua[2][k] - dif26;
+                ua[8][k] = sum08 - ua[0][k];
+                ua[1][k] = c1 * u1 + s1 * ua[7][k];
+                ua[7][k] = c1 * ua[7][k] - s1 * u1;
+                ua[3][k] = c1 * u3 + s1 * ua[5][k];
+                ua[5][k] = c1 * ua[5][k] - s1 * u3;
+            }
+        }
+    }
+
+    // PRT_1D::sphericalToUndulated / undulatedToSpherical (element/prt/PRT_1D.cpp:28-71)
+    void prtSphUnd(const Elem &el, const std::vector<std::array<CMat, 9>> &g, std::vector<std::array<CMat, 6>> &e,
+                   int Nu) {
+        for (int a = 0; a <= Nu; a++)
+            for (int k = 0; k < 25; k++) {
+                const Real X0 = el.X[k], X1 = el.X[25 + k], X2 = el.X[50 + k], X3 = el.X[75 + k];
+                const auto &s = g[a];
+                e[a][0][k] = X0 * s[0][k] + X1 * s[2][k];
+                e[a][1][k] = X0 * s[4][k] + X2 * s[5][k];
+                e[a][2][k] = X3 * s[8][k];
+                e[a][3][k] = X0 * s[7][k] + X2 * s[8][k] + X3 * s[5][k];
+                e[a][4][k] = X0 * s[6][k] + X1 * s[8][k] + X3 * s[2][k];
+                e[a][5][k] = X0 * (s[3][k] + s[1][k]) + X1 * s[5][k] + X2 * s[2][k];
+            }
+    }
+    void prtUndSph(const Elem &el, const std::vector<std::array<CMat, 6>> &t, std::vector<std::array<CMat, 9>> &o,
+                   int Nu) {
+        for (int a = 0; a <= Nu; a++)
+            for (int k = 0; k < 25; k++) {
+                const Real X0 = el.X[k], X1 = el.X[25 + k], X2 = el.X[50 + k], X3 = el.X[75 + k];
+                const auto &u = t[a];
+                auto &s = o[a];
+                s[0][k] = X0 * u[0][k];
+                s[1][k] = X0 * u[5][k];
+                s[2][k] = X1 * u[0][k] + X3 * u[4][k] + X2 * u[5][k];
+                s[3][k] = s[1][k];
+                s[4][k] = X0 * u[1][k];
+                s[5][k] = X2 * u[1][k] + X3 * u[3][k] + X1 * u[5][k];
+                s[6][k] = X0 * u[4][k];
+                s[7][k] = X0 * u[3][k];
+                s[8][k] = X3 * u[2][k] + X2 * u[3][k] + X1 * u[4][k];
+            }
+    }
+
     // CrdTransTIsoSolid (element/crd/CrdTransTIsoSolid.cpp:14-42)
     void spz2rtz(const Elem &el, std::vector<std::array<CMat, 6>> &u, int Nu) {
         for (int a = 0; a <= Nu; a++) {
@@ -920,6 +1097,8 @@ struct Ref {
             w.stiff.resize(nu + 1);
             w.strain6.resize(nu + 1);
             w.stress6.resize(nu + 1);
+            w.strain9.resize(nu + 1);
+            w.stress9.resize(nu + 1);
             w.fdispl.resize(nu + 1);
             w.fstiff.resize(nu + 1);
             w.fstrain.resize(nu + 1);
@@ -947,8 +1126,15 @@ struct Ref {
                 for (int c = 0; c < 3; c++)
                     w.displ[a][c][k] = (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0);
         }
-        grad6(el, w.displ, w.strain6, Nu, nyq);
-        if (el.tiso) spz2rtz(el, w.strain6, Nu);
+        if (el.prt) {
+            // SolidElement::displToStiff with PRT (SolidElement.cpp:404-441)
+            grad9(el, w.displ, w.strain9, Nu, nyq);
+            rot9(el, w.strain9, Nu, -one);
+            prtSphUnd(el, w.strain9, w.strain6, Nu);
+        } else {
+            grad6(el, w.displ, w.strain6, Nu, nyq);
+            if (el.tiso) spz2rtz(el, w.strain6, Nu);
+        }
         if (!el.is3D) {
             for (int a = 0; a <= Nu; a++)
                 for (int k = 0; k < 25; k++) {
@@ -983,8 +1169,14 @@ struct Ref {
                     for (int a = 0; a <= Nu; a++) w.stress6[a][c][k] = w.colC[a];
                 }
         }
-        if (el.tiso) rtz2spz(el, w.stress6, Nu);
-        quad6(el, w.stiff, w.stress6, Nu, nyq);
+        if (el.prt) {
+            prtUndSph(el, w.stress6, w.stress9, Nu);
+            rot9(el, w.stress9, Nu, one);
+            quad9(el, w.stiff, w.stress9, Nu, nyq);
+        } else {
+            if (el.tiso) rtz2spz(el, w.stress6, Nu);
+            quad6(el, w.stiff, w.stress6, Nu, nyq);
+        }
         // SolidPoint::gatherStiffFromElement (SolidPoint.cpp:197-209)
         for (int k = 0; k < 25; k++) {
             SolidPt &p = S[el.pts[k]];
@@ -1046,7 +1238,26 @@ struct Ref {
         }
         if (nyq)
             for (int c = 0; c < 3; c++) w.fstrain[Nu][c].fill(C(0, 0));
-        if (!el.is3D) {
+        if (!el.is3D && el.prt) {
+            // FluidElement::displToStiff with PRT (FluidElement.cpp:333-355):
+            // CrdTransTIsoFluid (CrdTransTIsoFluid.cpp:14-32), PRT_1D (PRT_1D.cpp:7-26)
+            for (int a = 0; a <= Nu; a++)
+                for (int k = 0; k < 25; k++) {
+                    const Real c1 = el.cos1t[k], s1 = el.sin1t[k];
+                    const Real X0 = el.X[k], X1 = el.X[25 + k], X2 = el.X[50 + k], X3 = el.X[75 + k];
+                    auto &e = w.fstrain[a];
+                    C r0 = c1 * e[0][k] - s1 * e[2][k], r2 = c1 * e[2][k] + s1 * e[0][k], r1 = e[1][k];
+                    C u0 = X0 * r0 + X1 * r2, u1 = X0 * r1 + X2 * r2, u2 = X3 * r2;
+                    u0 *= el.matv[k];
+                    u1 *= el.matv[k];
+                    u2 *= el.matv[k];
+                    C t2 = X1 * u0 + X2 * u1 + X3 * u2, t0 = X0 * u0, t1 = X0 * u1;
+                    auto &o = w.fstress[a];
+                    o[0][k] = c1 * t0 + s1 * t2;
+                    o[1][k] = t1;
+                    o[2][k] = c1 * t2 - s1 * t0;
+                }
+        } else if (!el.is3D) {
             // Acoustic1D::strainToStress (acoustic/Acoustic1D.cpp:8-16)
             for (int a = 0; a <= Nu; a++)
                 for (int c = 0; c < 3; c++)

@@ -203,3 +203,60 @@ def lift_ocean3d(mesh):
         new_rows.append(blk)
         new_off.append(new_off[-1] + blk.shape[0])
     return OceanRows(mesh, new_off, np.concatenate(new_rows) if new_rows else np.zeros((0, 4)))
+
+
+class WithPRT:
+    """The same mesh with a particle relabelling (PRT_1D) on the elements of
+    `elems`: X [len(elems)][4][25] (X0..X3 per GLL point).  Those elements get
+    the TIso flag, as a relabelled element always rotates (SolidElement.cpp:21)."""
+
+    def __init__(self, mesh, elems, X):
+        import ctypes as C
+        import axisem3d_amd._abi as abi
+        self.mesh = mesh
+        d = abi.MeshDesc.from_buffer_copy(mesh.d)
+        ne = mesh.d.nelem
+        X = np.asarray(X, dtype=np.float32).reshape(len(elems), 100)
+        off = np.zeros(ne + 1, dtype=np.int64)
+        cnt = np.zeros(ne, dtype=np.int64)
+        cnt[np.asarray(elems, dtype=np.int64)] = 100
+        off[1:] = np.cumsum(cnt)
+        pool = np.zeros(max(int(off[-1]), 1), dtype=np.float32)
+        for i, e in enumerate(elems):
+            pool[off[e]:off[e] + 100] = X[i]
+        flags = mesh.array("elem_flags").copy()
+        flags[np.asarray(elems, dtype=np.int64)] |= abi.EL_TISO
+        self._keep = [off, pool, flags]
+        d.prt_off = off.ctypes.data_as(abi.i64p)
+        d.prt_pool = pool.ctypes.data_as(abi.f32p)
+        d.elem_flags = flags.ctypes.data_as(abi.i32p)
+        self._d = d
+        self.desc = C.pointer(d)
+
+    @property
+    def d(self):
+        return self._d
+
+    def field_size(self, which):
+        return self.mesh.field_size(which)
+
+    def array(self, name):
+        if name == "elem_flags":
+            return self._keep[2]
+        if name == "prt_off":
+            return self._keep[0]
+        if name == "prt_pool":
+            return self._keep[1]
+        return self.mesh.array(name)
+
+
+def random_prt(n, seed, amp=0.1):
+    """X0..X3 of n elements: X0, X3 near 1, X1, X2 small (a smooth undulation's
+    inverse Jacobian factors), [n][4][25]."""
+    rng = np.random.default_rng(seed)
+    X = np.empty((n, 4, 25), dtype=np.float32)
+    X[:, 0] = 1. + amp * rng.uniform(-1, 1, (n, 25))
+    X[:, 1] = amp * rng.uniform(-1, 1, (n, 25))
+    X[:, 2] = amp * rng.uniform(-1, 1, (n, 25))
+    X[:, 3] = 1. + amp * rng.uniform(-1, 1, (n, 25))
+    return X
