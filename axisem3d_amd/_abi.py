@@ -114,6 +114,10 @@ class ShellConfig(C.Structure):
         ("emp_depth_start", C.c_double), ("emp_depth_end", C.c_double),
         ("ocean_depth", C.c_double),
         ("ocean_var", C.c_double),
+        ("undulation", C.c_int32),
+        ("undulation_order", C.c_int32),
+        ("undulation_amp", C.c_double),
+        ("undulation_depth", C.c_double),
     ]
 
 

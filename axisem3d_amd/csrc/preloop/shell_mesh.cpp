@@ -269,6 +269,12 @@ struct QuadInfo {
     double iFact[nP * nP];
     int pointNr[nP * nP];
     int nr;
+    // particle relabelling (PRT_1D): stiffness/mass Jacobian, X0..X3, the
+    // undulation and its radial / colatitudinal derivatives at the points
+    bool prt = false;
+    double jac[nP * nP];
+    double X[4][nP * nP];
+    double dZ[nP * nP], dZdR[nP * nP];
 };
 
 struct Layer {
@@ -369,8 +375,8 @@ struct axg_mesh {
     std::vector<int32_t> sf_point;
     std::vector<int64_t> sf_off;
     std::vector<float> sf_normal;
-    std::vector<int64_t> ocean_off;
-    std::vector<float> ocean;
+    std::vector<int64_t> ocean_off, prt_off;
+    std::vector<float> ocean, prt_pool;
     std::vector<float> src_force, stf;
     std::vector<int32_t> rec_elem;
     std::vector<float> rec_weights, rec_phi, rec_theta;
@@ -422,6 +428,8 @@ struct axg_mesh {
         d.rec_theta = rec_theta.data();
         d.ocean_off = ocean_off.empty() ? nullptr : ocean_off.data();
         d.ocean = ocean.data();
+        d.prt_off = prt_off.empty() ? nullptr : prt_off.data();
+        d.prt_pool = prt_pool.data();
     }
 };
 
@@ -675,6 +683,71 @@ struct Builder {
         });
     }
 
+    // Relabelling::addUndulation / formGradientUndulation / getStiffX /
+    // getStiffJacobian (Relabelling.cpp:29-109, 190-253) for an undulation that
+    // is the same on every azimuth: deltaR at the points, its gradient in the
+    // meridian (the element's own operator at alpha = 0), rotated to (R, Z),
+    // then J0 = 1 + dZ / Z (1 + dZ/dZ at the centre), J1 = dZ/dR, J3 = 1 + dZ/dZ,
+    // X = (1 / J0, -J1 / (J0 J3), 0, 1 / J3), Jacobian J0^2 J3 (J3^3 at the centre)
+    void buildRelabelling() {
+        for (auto &q : quads)
+            for (int k = 0; k < nP * nP; k++) q.jac[k] = 1.;
+        if (c.undulation == 0) return;
+        if (c.dv_amp > 0.)
+            throw std::runtime_error("PRT_3D || relabelling of a 3-D model is not supported by this build.");
+        if (c.undulation != 1 && c.undulation != 2) throw std::runtime_error("Geometric3D || unknown undulation.");
+        const double R = rnode.back();
+        auto deltaR = [&](double r, double th) {
+            if (c.undulation == 1) return c.undulation_amp * r;
+            const double d = R - r;
+            if (!(c.undulation_depth > 0.) || d >= c.undulation_depth) return 0.;
+            return c.undulation_amp * std::cos(c.undulation_order * th) * (1. - d / c.undulation_depth);
+        };
+        const auto &GL = spectral::gll().G;
+        const auto &GJ = spectral::glj().G;
+        parallelFor((int)quads.size(), nthread, [&](int b, int e) {
+            for (int iq = b; iq < e; iq++) {
+                QuadInfo &q = quads[iq];
+                double th[nP * nP], rr[nP * nP], amax = 0.;
+                for (int k = 0; k < nP * nP; k++) {
+                    rr[k] = std::hypot(q.crd[k][0], q.crd[k][1]);
+                    th[k] = rr[k] < tinyDouble ? 0. : std::acos(q.crd[k][1] / rr[k]);  // Geodesy::theta
+                    q.dZ[k] = deltaR(rr[k], th[k]);
+                    amax = std::max(amax, std::abs(q.dZ[k]));
+                }
+                if (amax < tinyDouble) continue;  // Relabelling::isZero
+                const auto &Gx = q.axial ? GJ : GL;
+                for (int i = 0; i < nP; i++)
+                    for (int j = 0; j < nP; j++) {
+                        const int k = i * nP + j;
+                        double GU = 0., UG = 0.;
+                        for (int m = 0; m < nP; m++) {
+                            GU += Gx[m][i] * q.dZ[m * nP + j];
+                            UG += q.dZ[i * nP + m] * GL[m][j];
+                        }
+                        const double(*J)[2] = q.J[k];
+                        const double det = det2(J);
+                        const double dsdxii = J[0][0] / det, dsdeta = -J[0][1] / det;
+                        const double dzdxii = -J[1][0] / det, dzdeta = J[1][1] / det;
+                        const double drds = dzdeta * GU + dzdxii * UG;
+                        const double drdz = dsdeta * GU + dsdxii * UG;
+                        const double dZdZ = drdz * std::cos(th[k]) + drds * std::sin(th[k]);
+                        const double dZdR = -drdz * std::sin(th[k]) + drds * std::cos(th[k]);
+                        const double J0 = rr[k] < tinyDouble ? 1. + dZdZ : 1. + q.dZ[k] / rr[k];
+                        const double J3 = 1. + dZdZ;
+                        q.X[0][k] = 1. / J0;
+                        q.X[1][k] = -dZdR / (J0 * J3);
+                        q.X[2][k] = 0.;
+                        q.X[3][k] = 1. / J3;
+                        q.jac[k] = rr[k] < tinyDouble ? J3 * J3 * J3 : J0 * J0 * J3;
+                        q.dZdR[k] = dZdR;
+                        if (q.jac[k] <= 0.) throw std::runtime_error("Relabelling::getStiffJacobian || Negative Jacobian.");
+                    }
+                q.prt = true;
+            }
+        });
+    }
+
     // Nr of each point: Quad::formNrField (Quad.cpp:549-616)
     std::vector<int> buildNrField() {
         std::mt19937_64 rng(c.seed * 7919ULL + 17ULL);
@@ -820,6 +893,7 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     B.nthread = c.nthread > 0 ? c.nthread : (int)std::max(1u, std::thread::hardware_concurrency());
     B.buildTopology();
     B.buildGeometry();
+    B.buildRelabelling();
     // coordinates of the points, with the GLLPoint::setup distance check
     std::vector<double> psz(2 * (size_t)B.npoint, 0.);
     std::vector<char> pset(B.npoint, 0);
@@ -937,10 +1011,21 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             sampleElem(e, v, q.nr);
             o.is3D = !rowsEqual(v, q.nr);
             // Quad::getDeltaT with courant given (Quad.cpp:292-307)
-            double hmin = DBL_MAX;
+            // on the deformed element when relabelled (Quad::getHminSlices, Quad.cpp:633-657)
+            double hmin = DBL_MAX, cd[nP * nP][2];
+            for (int i = 0; i < nP * nP; i++) {
+                cd[i][0] = q.crd[i][0];
+                cd[i][1] = q.crd[i][1];
+                if (q.prt) {
+                    const double r = std::hypot(q.crd[i][0], q.crd[i][1]);
+                    const double t = r < tinyDouble ? 0. : std::acos(q.crd[i][1] / r);
+                    cd[i][0] += q.dZ[i] * std::sin(t);
+                    cd[i][1] += q.dZ[i] * std::cos(t);
+                }
+            }
             for (int i = 0; i < nP * nP; i++)
                 for (int j = i + 1; j < nP * nP; j++)
-                    hmin = std::min(hmin, std::hypot(q.crd[i][0] - q.crd[j][0], q.crd[i][1] - q.crd[j][1]));
+                    hmin = std::min(hmin, std::hypot(cd[i][0] - cd[j][0], cd[i][1] - cd[j][1]));
             for (int row = 0; row < q.nr; row++) {
                 double vmax = 0.;
                 for (int k = 0; k < nP * nP; k++)
@@ -952,7 +1037,9 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             for (int k = 0; k < nP * nP; k++) {
                 double rho = v[k][4];
                 o.rhoPt[k] = rho;
-                o.mass[k] = q.fluid ? q.iFact[k] / (rho * v[k][0] * v[k][0]) : rho * q.iFact[k];
+                // the relabelling's mass Jacobian (Material.cpp:233-253)
+                const double fj = q.iFact[k] * q.jac[k];
+                o.mass[k] = q.fluid ? fj / (rho * v[k][0] * v[k][0]) : rho * fj;
             }
             if (q.fluid && threeD && c.dv_fluid) {
                 const Layer &L = B.layers[q.layer];
@@ -964,7 +1051,7 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                         const double phi = 2. * pi * n / nrp;
                         const double dvp = c.dv_amp * std::tanh(B.grfVp(sk * std::cos(phi), sk * std::sin(phi), zk) / 2.);
                         const double vp = L.vp(r) * (1. + dvp), rho = L.rho(r);
-                        o.massv[k][n] = q.iFact[k] / (rho * vp * vp);
+                        o.massv[k][n] = q.iFact[k] * q.jac[k] / (rho * vp * vp);
                     }
                 }
             }
@@ -1011,24 +1098,27 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             if (q.fluid) {
                 // Material::createAcoustic (Material.cpp:257-278): K = iFact / rho
                 o.material = AXG_MAT_ACOUSTIC;
+                o.tiso = q.prt;  // FluidElement.cpp:21
                 for (int row = 0; row < nrows; row++)
                     for (int k = 0; k < NPE; k++)
-                        mat[(size_t)row * NPE + k] = (float)(q.iFact[k] / v[(size_t)row * NPE + k][4]);
+                        mat[(size_t)row * NPE + k] = (float)(q.iFact[k] * q.jac[k] / v[(size_t)row * NPE + k][4]);
                 continue;
             }
             o.material = c.material;
-            o.tiso = c.material != AXG_MAT_ISOTROPIC;
-            // A C F L N scaled by the integral factor (Material.cpp:313-330)
+            o.tiso = c.material != AXG_MAT_ISOTROPIC || q.prt;
+            // A C F L N scaled by the integral factor and the relabelling's
+            // stiffness Jacobian, before attenuation (Material.cpp:313-331)
             size_t n = (size_t)nrows * NPE;
             for (auto *x : {&A, &C, &F, &L, &N, &kp, &mu}) x->assign(n, 0.);
             for (int row = 0; row < nrows; row++) {
                 for (int k = 0; k < NPE; k++) {
                     size_t i = (size_t)row * NPE + k;
                     const auto &a = v[i];
-                    A[i] = a[4] * a[1] * a[1] * q.iFact[k];
-                    C[i] = a[4] * a[0] * a[0] * q.iFact[k];
-                    L[i] = a[4] * a[2] * a[2] * q.iFact[k];
-                    N[i] = a[4] * a[3] * a[3] * q.iFact[k];
+                    const double fj = q.iFact[k] * q.jac[k];
+                    A[i] = a[4] * a[1] * a[1] * fj;
+                    C[i] = a[4] * a[0] * a[0] * fj;
+                    L[i] = a[4] * a[2] * a[2] * fj;
+                    N[i] = a[4] * a[3] * a[3] * fj;
                     F[i] = a[5] * (A[i] - 2. * L[i]);
                 }
             }
@@ -1250,6 +1340,14 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         if (o.tiso) f |= AXG_EL_TISO;
         m.elem_flags[e] = f;
         m.elem_material[e] = o.material;
+        // Relabelling::createPRT (Relabelling.cpp:153-170): PRT_1D, X0..X3 of the points
+        if (B.c.undulation) {
+            if (m.prt_off.empty()) m.prt_off.assign(NQ + 1, 0);
+            m.prt_off[e + 1] = m.prt_off[e] + (q.prt ? 100 : 0);
+            if (q.prt)
+                for (int t = 0; t < 4; t++)
+                    for (int k = 0; k < 25; k++) m.prt_pool.push_back((float)q.X[t][k]);
+        }
         // Quad::createGraident (Quad.cpp:509-525)
         for (int k = 0; k < 25; k++) {
             const double(*J)[2] = q.J[k];
@@ -1311,9 +1409,17 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             fac = wsf * sint * half_r2_dtheta;
         }
         if (side != q.g.curvedOuter) fac = -fac;
-        nv[0] = sint * fac;
+        // normal of the undulated boundary in (R, T, Z), rotated to (s, phi, z):
+        // Relabelling::getSFNormalRTZ (Relabelling.cpp:130-147), radial if plain
+        double nR = 0., nZ = 1.;
+        if (q.prt) {
+            const double J0 = rr < tinyDouble ? 1. / q.X[3][k] : 1. + q.dZ[k] / rr;
+            nR = -q.dZdR[k] * J0;
+            nZ = J0 * J0;
+        }
+        nv[0] = (nR * cost + nZ * sint) * fac;
         nv[1] = 0.;
-        nv[2] = cost * fac;
+        nv[2] = (-nR * sint + nZ * cost) * fac;
     };
 
     // ---------------- solid-fluid boundary ----------------
@@ -1481,7 +1587,8 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         const QuadInfo &q = B.quads[srcElem];
         std::array<double, nP> interpZ;
         spectral::interpLagrange(etaSrc, spectral::gll().x, interpZ);
-        // Earthquake::computeSourceFourier (Earthquake.cpp:20-88), no relabelling
+        // Earthquake::computeSourceFourier (Earthquake.cpp:20-88), with the
+        // relabelling's X on the axis points of a relabelled source element
         double axJ[nP][2][2];
         for (int j = 0; j < nP; j++) {
             sphJacobian(q.g, spectral::glj().x[0], spectral::gll().x[j], axJ[j]);
@@ -1504,20 +1611,28 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                     double UG = (ipol == 0) ? GL[jpol][js] : 0.;
                     double dwds = J[1][1] * GU - J[1][0] * UG;
                     double dwdz = J[0][0] * UG - J[0][1] * GU;
+                    double X0 = 1., X1 = 0., X2 = 0., X3 = 1.;
+                    if (q.prt) {
+                        X0 = q.X[0][js];  // point (0, js)
+                        X1 = q.X[1][js];
+                        X2 = q.X[2][js];
+                        X3 = q.X[3][js];
+                    }
                     // monopole
-                    f[0][0][0] += fact * dwds * (Mxx + Myy) / (2. * pi);
-                    f[0][2][0] += fact * dwdz * Mzz / (2. * pi);
-                    // dipole: (Mxz - i Myz)
+                    f[0][0][0] += fact * dwds * (Mxx + Myy) * X0 / (2. * pi);
+                    f[0][2][0] += fact * dwdz * (Mxz * X1 + Myz * X2 + Mzz * X3) / (2. * pi);
+                    // dipole: (Mxx - i Mxy) X1 + (Mxy - i Myy) X2 + (Mxz - i Myz) X3
+                    const double dr = Mxx * X1 + Mxy * X2 + Mxz * X3, di = -(Mxy * X1 + Myy * X2 + Myz * X3);
                     double dre = fact * dwdz / (4. * pi);
-                    f[1][0][0] += dre * Mxz;
-                    f[1][0][1] += dre * (-Myz);
+                    f[1][0][0] += dre * dr;
+                    f[1][0][1] += dre * di;
                     // * i
-                    f[1][1][0] += dre * Myz;
-                    f[1][1][1] += dre * Mxz;
-                    f[1][2][0] += fact * dwds * Mxz / (2. * pi);
-                    f[1][2][1] += fact * dwds * (-Myz) / (2. * pi);
+                    f[1][1][0] += dre * (-di);
+                    f[1][1][1] += dre * dr;
+                    f[1][2][0] += fact * dwds * Mxz * X0 / (2. * pi);
+                    f[1][2][1] += fact * dwds * (-Myz) * X0 / (2. * pi);
                     // quadrupole: ((Mxx - Myy)/2 - i Mxy)
-                    double qre = fact * dwds / (2. * pi);
+                    double qre = fact * dwds * X0 / (2. * pi);
                     f[2][0][0] += qre * (Mxx - Myy) / 2.;
                     f[2][0][1] += qre * (-Mxy);
                     f[2][1][0] += qre * Mxy;

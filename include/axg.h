@@ -239,6 +239,14 @@ typedef struct axg_shell_config {
                                          (1 + ocean_var tanh(g / 2)), g a unit Gaussian
                                          random field of correlation length dv_corr_len;
                                          0: laterally constant (MassOcean1D)          */
+    /* particle relabelling (Relabelling.cpp): an undulation deltaR(r, theta) of the
+       radius, the same on every azimuth (PRT_1D); needs a 1-D model (dv_amp 0) */
+    int32_t undulation;               /* 0 none; 1 deltaR = undulation_amp r (a uniform
+                                         stretch); 2 undulation_amp cos(undulation_order
+                                         theta) at the surface, tapering linearly to 0
+                                         at undulation_depth below it                */
+    int32_t undulation_order;
+    double undulation_amp, undulation_depth;
 } axg_shell_config;
 
 typedef struct axg_mesh axg_mesh;

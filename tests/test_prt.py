@@ -174,3 +174,122 @@ def test_prt_seismograms_gpu(fluid_layers, refcpu):
     sr = r.seismograms()
     assert np.abs(sr).max() > 0
     assert rel_l2(sg, sr) < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# the relabelling built by the shell builder (Relabelling.cpp)
+# ---------------------------------------------------------------------------
+
+def _scaled(layers, f):
+    return [dict(l, r_bottom=l["r_bottom"] * f, r_top=l["r_top"] * f) for l in layers]
+
+
+def test_uniform_stretch_relabelling_equals_a_larger_sphere(fluid_layers, refcpu):
+    """deltaR = c r relabels the shell into the shell of radii (1 + c) r: X0 = X3 =
+    1 / (1 + c), X1 = 0, Jacobian (1 + c)^3.  Masses, time step, solid-fluid
+    normals, source and the stiffness operator of the relabelled mesh equal those
+    of the mesh built (1 + c) larger, and so do the station traces (oracle)."""
+    c = 0.05
+    kw = dict(nelem_theta=8, material=_abi.MAT_ISOTROPIC, nu_mode=1, nu_min=2, nu_max=6, nstep=150, nrec=5,
+              rec_depth=0., nthread=1)
+    b = ax.ShellMesh(ax.shell_config(0, layers=fluid_layers, undulation=1, undulation_amp=c, src_depth=200e3, **kw))
+    a = ax.ShellMesh(ax.shell_config(0, layers=_scaled(fluid_layers, 1 + c), src_depth=200e3 * (1 + c), **kw))
+    assert b.d.nelem == a.d.nelem and (np.diff(b.array("prt_off")) == 100).all()
+    X = b.array("prt_pool").reshape(-1, 4, 25).astype(np.float64)
+    np.testing.assert_allclose(X[:, 0], 1 / (1 + c), rtol=1e-6)
+    np.testing.assert_allclose(X[:, 3], 1 / (1 + c), rtol=1e-6)
+    assert np.abs(X[:, 1:3]).max() < 1e-6
+    assert abs(b.d.dt / a.d.dt - 1) < 1e-9
+    for name in ("inv_mass_solid", "inv_mass_fluid"):
+        np.testing.assert_allclose(b.array(name), a.array(name), rtol=1e-6)
+    np.testing.assert_allclose(b.array("sf_normal"), a.array("sf_normal"), rtol=1e-5, atol=1e-6 * np.abs(a.array("sf_normal")).max())
+    np.testing.assert_allclose(b.array("src_force"), a.array("src_force"), rtol=1e-5,
+                               atol=1e-6 * np.abs(a.array("src_force")).max())
+    u = random_field(a, _abi.FIELD_SOLID_DISPL, 21, scale=1.)
+    uf = random_field(a, _abi.FIELD_FLUID_DISPL, 22, scale=1.)
+    ka = _stiff(refcpu.RefDomain(a, double=True), a, u, uf)
+    kb = _stiff(refcpu.RefDomain(b, double=True), b, u, uf)
+    assert rel_l2(kb[0], ka[0]) < 1e-6 and rel_l2(kb[1], ka[1]) < 1e-6
+    s = []
+    for m in (a, b):
+        d = refcpu.RefDomain(m, double=True)
+        d.run(1, 150, 0)
+        s.append(d.seismograms())
+    assert np.abs(s[0]).max() > 0
+    assert rel_l2(s[1], s[0]) < 1e-5
+
+
+def test_topography_relabelling(small_layers, refcpu):
+    """A surface topography tapering with depth relabels the elements above the
+    taper depth only; X1 follows the colatitude derivative of the undulation
+    (X1 = -dZ/dR / (J0 J3)), X2 = 0; the relabelled operator stays
+    self-adjoint and keeps rigid translations in its null space."""
+    R = small_layers[-1]["r_top"]
+    m = ax.ShellMesh(ax.shell_config(0, layers=small_layers, nelem_theta=8, nu_mode=1, nu_min=2, nu_max=6, nstep=10,
+                                     undulation=2, undulation_amp=8e3, undulation_order=3, undulation_depth=250e3,
+                                     nthread=1))
+    off = m.array("prt_off")
+    has = np.diff(off) == 100
+    ep = m.array("elem_points").reshape(-1, 25)
+    sz = m.array("point_sz").reshape(-1, 2)
+    rmax = np.hypot(sz[ep, 0], sz[ep, 1]).max(1)
+    assert has.any() and (~has).any()
+    assert (rmax[has] > R - 250e3).all() and (rmax[~has] <= R - 250e3 + 1.).all()
+    X = m.array("prt_pool").reshape(-1, 4, 25).astype(np.float64)
+    assert np.abs(X[:, 2]).max() == 0 and np.abs(X[:, 1]).max() > 1e-4
+    # at the surface, dZ/dR = (1/r) d/dtheta (A cos(3 theta)) = -3 A sin(3 theta) / R
+    pts = ep[has]
+    r = np.hypot(sz[pts, 0], sz[pts, 1])
+    th = np.arccos(np.clip(sz[pts, 1] / r, -1, 1))
+    top = np.abs(r - R) < 1.
+    dZ = 8e3 * np.cos(3 * th)
+    J0 = 1 + dZ / r
+    dZdZ = 8e3 * np.cos(3 * th) / 250e3  # linear taper in r
+    J3 = 1 + dZdZ
+    J1 = -3 * 8e3 * np.sin(3 * th) / r
+    np.testing.assert_allclose(X[:, 0][top], (1 / J0)[top], rtol=1e-5)
+    np.testing.assert_allclose(X[:, 3][top], (1 / J3)[top], rtol=2e-3)
+    np.testing.assert_allclose(X[:, 1][top], (-J1 / (J0 * J3))[top], rtol=0.05, atol=1e-5)
+    dom = refcpu.RefDomain(m, double=True)
+    w = _weights(m)
+    u = random_field(m, _abi.FIELD_SOLID_DISPL, 1, scale=1.)
+    v = random_field(m, _abi.FIELD_SOLID_DISPL, 2, scale=1.)
+    Ku, Kv = -_stiff(dom, m, u)[0], -_stiff(dom, m, v)[0]
+    assert abs(np.dot(w * u, Kv) - np.dot(w * v, Ku)) < 1e-10 * np.linalg.norm(Ku) * np.linalg.norm(v)
+    assert np.dot(w * u, Ku) > 0
+
+
+@pytest.mark.gpu
+def test_topography_seismograms_gpu(fluid_layers, refcpu):
+    """Relabelled solid and fluid elements (the taper reaches through the fluid
+    layer, so the solid-fluid boundaries are undulated too): GPU traces against
+    the oracle's, and a uniform stretch against the larger sphere on the GPU."""
+    print("case topography seismograms", flush=True)
+    m = ax.ShellMesh(ax.shell_config(0, layers=fluid_layers, nelem_theta=10, nu_mode=1, nu_min=2, nu_max=8,
+                                     nstep=300, nrec=6, undulation=2, undulation_amp=20e3, undulation_order=2,
+                                     undulation_depth=5500e3))
+    fl = m.array("elem_flags")
+    has = np.diff(m.array("prt_off")) == 100
+    assert has[(fl & _abi.EL_FLUID) != 0].any()
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, 300, 100, use_graph=True)
+    sg = g.seismograms()
+    g.close()
+    r = refcpu.RefDomain(m, double=False)
+    r.run(1, 300, 100)
+    sr = r.seismograms()
+    assert np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4
+    c = 0.05
+    kw = dict(nelem_theta=8, nu_mode=1, nu_min=2, nu_max=6, nstep=200, nrec=5, rec_depth=0.)
+    b = ax.ShellMesh(ax.shell_config(0, layers=fluid_layers, undulation=1, undulation_amp=c, src_depth=200e3, **kw))
+    a = ax.ShellMesh(ax.shell_config(0, layers=_scaled(fluid_layers, 1 + c), src_depth=200e3 * (1 + c), **kw))
+    s = []
+    for mm in (a, b):
+        d = ax.Domain(mm)
+        d.resetZero()
+        d.run(1, 200, 0, use_graph=True)
+        s.append(d.seismograms())
+        d.close()
+    assert rel_l2(s[1], s[0]) < 1e-4
