@@ -57,6 +57,7 @@ class MeshDesc(C.Structure):
         ("ocean", f32p),
         ("prt_off", i64p),
         ("prt_pool", f32p),
+        ("rec_dump", i32p),
     ]
 
 
@@ -118,6 +119,7 @@ class ShellConfig(C.Structure):
         ("undulation_order", C.c_int32),
         ("undulation_amp", C.c_double),
         ("undulation_depth", C.c_double),
+        ("rec_dump", C.c_int32),
     ]
 
 
@@ -157,8 +159,8 @@ def declare_mesh_api(lib):
     lib.axg_wisdom_read.restype = C.c_int
     lib.axg_wisdom_get_nu.argtypes = [C.c_int64, f64p, C.c_int32, C.c_int64, f64p, i32p]
     lib.axg_wisdom_get_nu.restype = C.c_int
-    lib.axg_write_station_ascii.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_double, C.c_double, f32p,
-                                            C.c_int64]
+    lib.axg_write_station_ascii.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                            f32p, C.c_int64]
     lib.axg_write_station_ascii.restype = C.c_int
     lib.axg_field_size.argtypes = [C.POINTER(MeshDesc), C.c_int32]
     lib.axg_field_size.restype = C.c_int64
@@ -186,6 +188,7 @@ def declare_domain_api(lib):
         "axg_set_learning": ([vp, C.c_float, C.c_int32], C.c_int),
         "axg_run": ([vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "axg_get_seismograms": ([vp, f32p, C.c_int64], C.c_int),
+        "axg_get_strain_curl": ([vp, f32p, C.c_int64, f32p, C.c_int64, i32p, i32p], C.c_int),
         "axg_get_field": ([vp, C.c_int32, f32p, C.c_int64], C.c_int),
         "axg_set_field": ([vp, C.c_int32, f32p, C.c_int64], C.c_int),
         "axg_last_run_timing": ([vp, f64p, f64p, C.c_int32], C.c_int),

@@ -174,6 +174,11 @@ struct axg_ctx {
     DevBuf<int4> d_recf3_info;
     std::vector<int32_t> h_twoff, h_nstage;  // FFT table offsets and stage counts per Nr
     int32_t nrec_solid = 0, nrec_fluid = 0;
+    // stations that dump strain / curl: block -> station, (strain slot, curl slot)
+    int32_t nrec_sc = 0, nstrain = 0, ncurl = 0;
+    DevBuf<int32_t> d_sc_idx;
+    DevBuf<int2> d_sc_slot;
+    DevBuf<float> d_strain, d_curl;
     DevBuf<float> d_rec_w, d_rec_phi, d_rec_cost, d_rec_sint, d_seis;
     int32_t rec_rtz = 1;
     // stability
@@ -956,6 +961,12 @@ void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
                            c->d_rec_sint.p, c->d_ep_row.p, c->d_ep_amax.p, c->d_elem_nr.p, c->S.displ.p, c->rec_rtz,
                            tstep, tstep_ptr, c->d_seis.p);
     if (c->nrec_f3) recordFluid3D(c, tstep, tstep_ptr);
+    if (c->nrec_sc)
+        hipLaunchKernelGGL(record_strain_kernel, dim3(c->nrec_sc), dim3(256), 0, c->stream, c->nrec_sc, c->d_sc_idx.p,
+                           c->d_sc_slot.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_ep_row.p,
+                           c->d_ep_amax.p, c->d_elem_nr.p, c->d_elem_flags.p, c->d_geom.p, c->d_tiso.p,
+                           c->d_prt_pool.p, c->d_prt_pool.p ? c->d_prt_off.p : nullptr, c->S.displ.p, tstep,
+                           tstep_ptr, c->nstrain, c->ncurl, c->d_strain.p, c->d_curl.p);
     if (c->nrec_fluid)
         hipLaunchKernelGGL(record_fluid_kernel, dim3(c->nrec_fluid), dim3(256), 0, c->stream, c->nrec_fluid, c->nrec,
                            c->d_rec_idx_f.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
@@ -1367,6 +1378,28 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
         c->d_rec_sint.upload(sint, s);
         c->d_seis.alloc((size_t)std::max(1, c->nstep) * c->nrec * 3);
         c->d_seis.zero(s);
+        if (d->rec_dump) {
+            std::vector<int32_t> sc;
+            std::vector<int2> slot;
+            for (int i = 0; i < c->nrec; i++) {
+                const int f = d->rec_dump[i];
+                if (!(f & 3)) continue;
+                if (d->elem_flags[d->rec_elem[i]] & AXG_EL_FLUID)
+                    throw std::runtime_error("FluidElement::computeStrain || strain at a fluid station is not "
+                                             "supported by this build.");
+                sc.push_back(i);
+                slot.push_back(make_int2((f & 1) ? c->nstrain++ : -1, (f & 2) ? c->ncurl++ : -1));
+            }
+            c->nrec_sc = (int32_t)sc.size();
+            if (c->nrec_sc) {
+                c->d_sc_idx.upload(sc, s);
+                c->d_sc_slot.upload(slot, s);
+                c->d_strain.alloc((size_t)std::max(1, c->nstep) * std::max(1, c->nstrain) * 6);
+                c->d_curl.alloc((size_t)std::max(1, c->nstep) * std::max(1, c->ncurl) * 3);
+                c->d_strain.zero(s);
+                c->d_curl.zero(s);
+            }
+        }
         HIPCHECK(hipStreamSynchronize(s));
     }
     c->d_bad.alloc(1);
@@ -1588,6 +1621,7 @@ struct MergedDesc {
         rec_weights, rec_phi, rec_theta;
     std::vector<int64_t> ocean_off, prt_off;
     std::vector<float> ocean, prt_pool;
+    std::vector<int32_t> rec_dump;
 };
 
 template <class T>
@@ -1668,6 +1702,7 @@ void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo 
             appendOffsets(m.ocean_off, none.data(), np);
         }
         for (int r = 0; r < d.nrec; r++) m.rec_elem.push_back(d.rec_elem[r] + nelem);
+        for (int r = 0; r < d.nrec; r++) m.rec_dump.push_back(d.rec_dump ? d.rec_dump[r] : 0);
         append(m.rec_weights, d.rec_weights, 25 * (size_t)d.nrec);
         append(m.rec_phi, d.rec_phi, d.nrec);
         append(m.rec_theta, d.rec_theta, d.nrec);
@@ -1699,6 +1734,7 @@ void mergeDescs(const axg_mesh_desc *const *ds, int n, MergedDesc &m, BatchInfo 
     d.ocean = m.ocean.data();
     d.prt_off = m.prt_off.data();
     d.prt_pool = m.prt_pool.data();
+    d.rec_dump = m.rec_dump.empty() ? nullptr : m.rec_dump.data();
     d.nelem = nelem;
     d.elem_points = m.elem_points.data();
     d.elem_flags = m.elem_flags.data();
@@ -1824,6 +1860,10 @@ int axg_reset_zero(axg_ctx *c) {
         c->d_att_memvar.zero(c->stream);
         c->d_att_stressR.zero(c->stream);
         c->d_seis.zero(c->stream);
+        if (c->nrec_sc) {
+            c->d_strain.zero(c->stream);
+            c->d_curl.zero(c->stream);
+        }
         HIPCHECK(hipStreamSynchronize(c->stream));
     })
 }
@@ -1971,6 +2011,21 @@ int axg_run(axg_ctx *c, int32_t first, int32_t count, int32_t stab_interval, int
         std::string m = e.what();
         return fail(m.find("blew up") != std::string::npos ? AXG_EUNSTABLE : AXG_EINVAL, m);
     }
+}
+
+int axg_get_strain_curl(axg_ctx *c, float *strain, int64_t nstrain_float, float *curl, int64_t ncurl_float,
+                        int32_t *nstrain, int32_t *ncurl) {
+    CTX_CHECK(c);
+    if (nstrain) *nstrain = c->nstrain;
+    if (ncurl) *ncurl = c->ncurl;
+    const int64_t ns = (int64_t)c->nstep * c->nstrain * 6, nc = (int64_t)c->nstep * c->ncurl * 3;
+    if ((strain && nstrain_float < ns) || (curl && ncurl_float < nc))
+        return fail(AXG_EINVAL, "axg_get_strain_curl || output buffer too small.");
+    API_GUARD({
+        if (strain && ns) HIPCHECK(hipMemcpyAsync(strain, c->d_strain.p, ns * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        if (curl && nc) HIPCHECK(hipMemcpyAsync(curl, c->d_curl.p, nc * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    })
 }
 
 int axg_get_seismograms(axg_ctx *c, float *out, int64_t nfloat) {

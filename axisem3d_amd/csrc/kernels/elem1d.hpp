@@ -194,6 +194,139 @@ __device__ __forceinline__ void strain6_at(const float2 (*U)[25], int i, int j, 
     }
 }
 
+// PRT_1D::sphericalToUndulated9 (PRT_1D.cpp:73-100), for the curl; its output
+// order is that of the reference (the transpose of the input's [3c + d] for
+// the off-diagonal terms), which the curl below then reads as written there
+__device__ __forceinline__ void prt_sph_und9(float2 g[9], float X0, float X1, float X2, float X3) {
+    float2 u[9];
+    u[0] = cadd(cscale(X0, g[0]), cscale(X1, g[2]));
+    u[1] = cadd(cscale(X0, g[3]), cscale(X1, g[5]));
+    u[2] = cadd(cscale(X0, g[6]), cscale(X1, g[8]));
+    u[3] = cadd(cscale(X0, g[1]), cscale(X2, g[2]));
+    u[4] = cadd(cscale(X0, g[4]), cscale(X2, g[5]));
+    u[5] = cadd(cscale(X0, g[7]), cscale(X2, g[8]));
+    u[6] = cscale(X3, g[2]);
+    u[7] = cscale(X3, g[5]);
+    u[8] = cscale(X3, g[8]);
+#pragma unroll
+    for (int i = 0; i < 9; i++) g[i] = u[i];
+}
+
+// SolidElement::computeStrain / computeCurl (SolidElement.cpp:219-345): one
+// block per station that dumps either; work items (mode, GLL point) as in the
+// fluid station kernel.  The 3 x 3 gradient in Fourier space (every element,
+// 1-D or 3-D: the gradient is linear), rotated to (R, T, Z); with a PRT_1D
+// relabelling the undulated strain (sphericalToUndulated) and gradient
+// (sphericalToUndulated9); then the weighted Fourier sum at the station's
+// azimuth over alpha <= Nu - nyquist of the element.  Strain in Voigt order
+// RR TT ZZ 2TZ 2RZ 2RT, curl (dUZ/dT - dUT/dZ, dUR/dZ - dUZ/dR, dUT/dR - dUR/dT).
+__global__ __launch_bounds__(256) void record_strain_kernel(
+    int32_t n, const int32_t *__restrict__ rec_idx, const int2 *__restrict__ slots, const int32_t *__restrict__ rec_elem,
+    const float *__restrict__ rec_w, const float *__restrict__ rec_phi, const int32_t *__restrict__ ep_row,
+    const int32_t *__restrict__ ep_amax, const int32_t *__restrict__ elem_nr, const int32_t *__restrict__ elem_flags,
+    const float *__restrict__ geom, const float *__restrict__ tiso, const float *__restrict__ prt_pool,
+    const int64_t *__restrict__ prt_off, const float2 *__restrict__ displ, int tstep, const int *__restrict__ tstep_ptr,
+    int32_t nstrain, int32_t ncurl, float *__restrict__ strain_out, float *__restrict__ curl_out) {
+    __shared__ float red[9][256];
+    if ((int)blockIdx.x >= n) return;
+    const int irec = rec_idx[blockIdx.x];
+    const int2 slot = slots[blockIdx.x];
+    if (tstep_ptr) tstep = *tstep_ptr;
+    const int64_t e = rec_elem[irec];
+    const int nr = elem_nr[e];
+    const int amaxe = nr / 2 - ((nr % 2 == 0) ? 1 : 0);
+    const bool axial = (elem_flags[e] & 2) != 0;
+    const bool prt = prt_off && prt_off[e + 1] > prt_off[e];
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    const float phi = rec_phi[irec];
+    float acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) acc[c] = 0.f;
+    for (int t = threadIdx.x; t < 25 * (amaxe + 1); t += blockDim.x) {
+        const int k = t % 25, al = t / 25;
+        const float wt = rec_w[irec * 25 + k];
+        if (fabsf(wt) < 1e-10f) continue;
+        const int i = k / 5, j = k - 5 * i;
+        auto U = [&](int p, int c) {
+            return al <= ep_amax[e * 25 + p] ? displ[((int64_t)ep_row[e * 25 + p] + al) * 3 + c] : czero();
+        };
+        float2 GU[3], UG[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            GU[c] = UG[c] = czero();
+            for (int m = 0; m < 5; m++) {
+                GU[c] = cadd(GU[c], cscale(Gx[m * 5 + i], U(m * 5 + j, c)));
+                UG[c] = cadd(UG[c], cscale(Ge[m * 5 + j], U(i * 5 + m, c)));
+            }
+        }
+        const PointGeom g = load_geom(geom, e, k);
+        const float fa = (float)al;
+        const float2 u0 = U(k, 0), u1 = U(k, 1), u2 = U(k, 2);
+        const float2 v0 = cadd(u0, cimul(fa, u1)), v1 = csub(cimul(fa, u0), u1), v2 = cimul(fa, u2);
+        float2 g9[9];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            g9[3 * c + 0] = cadd(cscale(g.dzdeta, GU[c]), cscale(g.dzdxii, UG[c]));
+            g9[3 * c + 2] = cadd(cscale(g.dsdeta, GU[c]), cscale(g.dsdxii, UG[c]));
+        }
+        g9[1] = cscale(g.inv_s, v1);
+        g9[4] = cscale(g.inv_s, v0);
+        g9[7] = cscale(g.inv_s, v2);
+        if (axial && i == 0) {
+            g9[4] = cadd(g9[4], cscale(g.dzdeta, cadd(GU[0], cimul(fa, GU[1]))));
+            g9[1] = cadd(g9[1], cscale(g.dzdeta, csub(cimul(fa, GU[0]), GU[1])));
+            g9[7] = cadd(g9[7], cscale(g.dzdeta, cimul(fa, GU[2])));
+            if (al == 1) {
+                g9[4] = cadd(g9[4], cscale(g.dzdxii, cadd(UG[0], cimul(fa, UG[1]))));
+                g9[1] = cadd(g9[1], cscale(g.dzdxii, csub(cimul(fa, UG[0]), UG[1])));
+            }
+        }
+        const float *tp = tiso + e * 100;
+        tiso9(g9, tp[k], tp[25 + k], tp[50 + k], tp[75 + k], -1.f);
+        float2 v[9];
+        if (prt) {
+            const float *xp = prt_pool + prt_off[e];
+            const float X0 = xp[k], X1 = xp[25 + k], X2 = xp[50 + k], X3 = xp[75 + k];
+            prt_sph_und(g9, X0, X1, X2, X3, v);
+            prt_sph_und9(g9, X0, X1, X2, X3);
+        } else {
+            v[0] = g9[0];
+            v[1] = g9[4];
+            v[2] = g9[8];
+            v[3] = cadd(g9[5], g9[7]);
+            v[4] = cadd(g9[2], g9[6]);
+            v[5] = cadd(g9[1], g9[3]);
+        }
+        v[6] = csub(g9[7], g9[5]);
+        v[7] = csub(g9[2], g9[6]);
+        v[8] = csub(g9[3], g9[1]);
+        if (al == 0) {
+#pragma unroll
+            for (int c = 0; c < 9; c++) acc[c] += wt * v[c].x;
+        } else {
+            float sn, cs;
+            sincosf(fa * phi, &sn, &cs);
+#pragma unroll
+            for (int c = 0; c < 9; c++) acc[c] += wt * 2.f * (cs * v[c].x - sn * v[c].y);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 9; c++) red[c][threadIdx.x] += red[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (slot.x >= 0)
+            for (int c = 0; c < 6; c++) strain_out[((int64_t)tstep * nstrain + slot.x) * 6 + c] = red[c][0];
+        if (slot.y >= 0)
+            for (int c = 0; c < 3; c++) curl_out[((int64_t)tstep * ncurl + slot.y) * 3 + c] = red[6 + c][0];
+    }
+}
+
 template <int MAT>
 __device__ __forceinline__ void stress1d(const float *__restrict__ mp, int k, const float2 e[6], float2 s[6]) {
     if (MAT == 0) {

@@ -236,13 +236,13 @@ extern "C" int axg_wisdom_get_nu(int64_t n, const double *rows, int32_t num_samp
 
 // ---------------------------------------------------------------------------
 // PointwiseIOAscii (SOLVER/src/core/output/pointwise/PointwiseIOAscii.cpp:25-35,
-// 96-102): one file per station, each recorded step a line "t c0 c1 c2" at full
-// precision; steps recorded where (step index) % interval == 0
+// 96-102): one file per station, each recorded step a line "t c0 .. c(ncol-1)"
+// at full precision (3 displacement, 6 strain or 3 curl columns); steps recorded where (step index) % interval == 0
 // (PointwiseRecorder.cpp:62-68), t accumulated from -shift by dt as
 // Newmark::solve does (Newmark.cpp:27, 64-65).
 extern "C" int axg_write_station_ascii(const char *path, int64_t nstep, int32_t interval, double t_shift, double dt,
-                                       const float *trace, int64_t row_stride) {
-    if (!path || nstep < 0 || interval <= 0 || (nstep > 0 && !trace) || row_stride < 3) {
+                                       int32_t ncol, const float *trace, int64_t row_stride) {
+    if (!path || nstep < 0 || interval <= 0 || ncol < 1 || (nstep > 0 && !trace) || row_stride < ncol) {
         axg::set_last_error("axg_write_station_ascii || invalid argument.");
         return AXG_EINVAL;
     }
@@ -255,7 +255,9 @@ extern "C" int axg_write_station_ascii(const char *path, int64_t nstep, int32_t 
     for (int64_t i = 0; i < nstep; i++) {
         if (i % interval == 0) {
             const float *r = trace + i * row_stride;
-            std::fprintf(f, "%.17g %.17g %.17g %.17g\n", t, (double)r[0], (double)r[1], (double)r[2]);
+            std::fprintf(f, "%.17g", t);
+            for (int c = 0; c < ncol; c++) std::fprintf(f, " %.17g", (double)r[c]);
+            std::fprintf(f, "\n");
         }
         t += dt;
     }

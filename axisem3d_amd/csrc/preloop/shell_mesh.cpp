@@ -377,6 +377,7 @@ struct axg_mesh {
     std::vector<float> sf_normal;
     std::vector<int64_t> ocean_off, prt_off;
     std::vector<float> ocean, prt_pool;
+    std::vector<int32_t> rec_dump;
     std::vector<float> src_force, stf;
     std::vector<int32_t> rec_elem;
     std::vector<float> rec_weights, rec_phi, rec_theta;
@@ -430,6 +431,9 @@ struct axg_mesh {
         d.ocean = ocean.data();
         d.prt_off = prt_off.empty() ? nullptr : prt_off.data();
         d.prt_pool = prt_pool.data();
+        bool dumps = false;
+        for (int32_t f : rec_dump) dumps = dumps || f != 0;
+        d.rec_dump = dumps ? rec_dump.data() : nullptr;
     }
 };
 
@@ -1695,6 +1699,11 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 }
             m.rec_phi.push_back((float)phi);
             m.rec_theta.push_back((float)theta);
+            // strain / curl at solid stations; their element rotates to (R, T, Z)
+            // (Receiver::release, Receiver.cpp:47-54)
+            const int dump = q.fluid ? 0 : (c.rec_dump & 3);
+            m.rec_dump.push_back(dump);
+            if (dump) m.elem_flags[found] |= AXG_EL_TISO;
         }
     }
     m.bind();

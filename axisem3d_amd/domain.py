@@ -150,6 +150,18 @@ class Domain:
         _check(self._lib.axg_get_seismograms(self._h, out.ctypes.data_as(_abi.f32p), out.size))
         return out[:n].reshape(self.nstep, self.nrec, 3)
 
+    def strain_curl(self):
+        """Strain [nstep][nstrain][6] (RR TT ZZ 2TZ 2RZ 2RT) and curl [nstep][ncurl][3]
+        of the stations that dump them (SolidElement::computeStrain / computeCurl)."""
+        ns, nc = C.c_int32(0), C.c_int32(0)
+        _check(self._lib.axg_get_strain_curl(self._h, None, 0, None, 0, C.byref(ns), C.byref(nc)))
+        a = np.zeros(max(self.nstep * ns.value * 6, 1), dtype=np.float32)
+        b = np.zeros(max(self.nstep * nc.value * 3, 1), dtype=np.float32)
+        _check(self._lib.axg_get_strain_curl(self._h, a.ctypes.data_as(_abi.f32p), a.size,
+                                             b.ctypes.data_as(_abi.f32p), b.size, C.byref(ns), C.byref(nc)))
+        return (a[:self.nstep * ns.value * 6].reshape(self.nstep, ns.value, 6),
+                b[:self.nstep * nc.value * 3].reshape(self.nstep, nc.value, 3))
+
     def get_field(self, which):
         n = self.mesh.field_size(which)
         out = np.zeros(n, dtype=np.float32)

@@ -64,12 +64,16 @@ def wisdom_rows(mesh, learned_nu):
     return np.column_stack([sz[:, 0], sz[:, 1], learned.astype(np.float64), nu.astype(np.float64)])
 
 
-def write_stations(outdir, seismograms, dt, t_shift, names=None, components="RTZ", interval=1):
-    """One ascii file per station from seismograms [nstep][nrec][3] (the layout of
-    Domain.seismograms()); names are "NET.STA" strings (default AX.S0000, ...).
-    Returns the paths."""
+def write_stations(outdir, seismograms, dt, t_shift, names=None, components="RTZ", interval=1, kind=""):
+    """One ascii file per station from seismograms [nstep][nrec][ncol] (the
+    layout of Domain.seismograms(), or of Domain.strain_curl() with kind
+    "strain" / "curl", whose files the reference names NET.STA.RTZ.strain.ascii
+    and NET.STA.RTZ.curl.ascii); names are "NET.STA" strings (default AX.S0000,
+    ...).  Returns the paths."""
     s = np.ascontiguousarray(seismograms, dtype=np.float32)
-    nstep, nrec = s.shape[0], s.shape[1]
+    nstep, nrec, ncol = s.shape
+    if kind:
+        components = "RTZ." + kind
     if names is None:
         names = ["AX.S%04d" % i for i in range(nrec)]
     if len(names) != nrec:
@@ -79,7 +83,8 @@ def write_stations(outdir, seismograms, dt, t_shift, names=None, components="RTZ
     base = s.ctypes.data_as(_abi.f32p)
     for r, name in enumerate(names):
         path = os.path.join(outdir, "%s.%s.ascii" % (name, components))
-        ptr = C.cast(C.addressof(base.contents) + 12 * r, _abi.f32p)
-        _check(_lib().axg_write_station_ascii(os.fsencode(path), nstep, interval, t_shift, dt, ptr, 3 * nrec))
+        ptr = C.cast(C.addressof(base.contents) + 4 * ncol * r, _abi.f32p)
+        _check(_lib().axg_write_station_ascii(os.fsencode(path), nstep, interval, t_shift, dt, ncol, ptr,
+                                              ncol * nrec))
         paths.append(path)
     return paths

@@ -164,6 +164,14 @@ typedef struct axg_mesh_desc {
      * relabelling Jacobians.  PRT_3D (nr x 100) is not supported by this build. */
     const int64_t *prt_off;
     const float   *prt_pool;
+
+    /* ---- strain and curl at stations (PointwiseRecorder mDumpStrain / mDumpCurl;
+     *      SolidElement::computeStrain / computeCurl, SolidElement.cpp:219-345);
+     *      null: none ----
+     * rec_dump [nrec]: bit 0 strain (RR TT ZZ 2TZ 2RZ 2RT, RTZ), bit 1 curl (R, T, Z);
+     * solid stations only.  The reference forces the TIso rotation on the
+     * elements of such stations (Receiver.cpp:49-51): set AXG_EL_TISO there. */
+    const int32_t *rec_dump;
 } axg_mesh_desc;
 
 /* ===================================================================== *
@@ -247,6 +255,7 @@ typedef struct axg_shell_config {
                                          at undulation_depth below it                */
     int32_t undulation_order;
     double undulation_amp, undulation_depth;
+    int32_t rec_dump;                 /* solid stations dump strain (bit 0) and curl (bit 1) */
 } axg_shell_config;
 
 typedef struct axg_mesh axg_mesh;
@@ -271,12 +280,13 @@ int axg_wisdom_get_nu(int64_t n, const double *rows, int32_t num_samples, int64_
                       int32_t *nu);
 
 /* One station's ascii file (PointwiseIOAscii.cpp:25-35, 96-102): a line
- * "t c0 c1 c2" at full precision for every step index i with i % interval == 0
- * (PointwiseRecorder.cpp:62-68); t starts at -t_shift and grows by dt
- * (Newmark.cpp:27, 64-65).  trace: [nstep] rows of 3 floats, row_stride floats
- * apart (3 nrec for the station-interleaved seismograms of axg_get_seismograms). */
+ * "t c0 .. c(ncol-1)" at full precision for every step index i with
+ * i % interval == 0 (PointwiseRecorder.cpp:62-68); t starts at -t_shift and
+ * grows by dt (Newmark.cpp:27, 64-65).  trace: [nstep] rows of ncol floats
+ * (3 displacement, 6 strain, 3 curl), row_stride floats apart (3 nrec for the
+ * station-interleaved seismograms of axg_get_seismograms). */
 int axg_write_station_ascii(const char *path, int64_t nstep, int32_t interval, double t_shift, double dt,
-                            const float *trace, int64_t row_stride);
+                            int32_t ncol, const float *trace, int64_t row_stride);
 
 /* Features of every GLL point for the learned Nu predictor (no counterpart in
  * the reference; they describe what the reference's NuWisdom learns from):
@@ -379,6 +389,11 @@ int axg_run(axg_ctx *ctx, int32_t first, int32_t count, int32_t stab_interval, i
 
 /* Seismograms recorded so far: [nstep][nrec][3] floats (RTZ or SPZ) */
 int axg_get_seismograms(axg_ctx *ctx, float *out, int64_t nfloat);
+/* strain [nstep][nstrain][6] and curl [nstep][ncurl][3] of the stations that dump
+ * them, in station order (PointwiseRecorder.cpp:96-125); n*: counts, out may be
+ * null to query them */
+int axg_get_strain_curl(axg_ctx *ctx, float *strain, int64_t nstrain_float, float *curl, int64_t ncurl_float,
+                        int32_t *nstrain, int32_t *ncurl);
 
 /* canonical-layout field access; which = AXG_FIELD_* */
 #define AXG_FIELD_SOLID_DISPL 0

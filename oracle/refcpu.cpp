@@ -226,7 +226,10 @@ struct Ref {
         RMat w;
         Real phi;
         double theta;
+        int strain = -1, curl = -1;  // slots of the stations that dump them
     };
+    std::vector<double> strainOut, curlOut;  // [nstep][nstrain][6], [nstep][ncurl][3]
+    int nstrain = 0, ncurl = 0;
     std::vector<Rec> recs;
     int rtz = 1;
     std::vector<double> seis;
@@ -441,9 +444,17 @@ struct Ref {
             for (int k = 0; k < 25; k++) r.w[k] = (Real)d->rec_weights[i * 25 + k];
             r.phi = (Real)d->rec_phi[i];
             r.theta = (double)d->rec_theta[i];
+            if (d->rec_dump && (d->rec_dump[i] & 3)) {
+                if (d->elem_flags[r.elem] & AXG_EL_FLUID)
+                    throw std::runtime_error("FluidElement::computeStrain || not restated by this oracle.");
+                if (d->rec_dump[i] & 1) r.strain = nstrain++;
+                if (d->rec_dump[i] & 2) r.curl = ncurl++;
+            }
             recs.push_back(r);
         }
         seis.assign((size_t)nstep * recs.size() * 3, 0.);
+        strainOut.assign((size_t)nstep * nstrain * 6, 0.);
+        curlOut.assign((size_t)nstep * ncurl * 3, 0.);
         setThreads(1);
         (void)maxNu;
     }
@@ -1353,6 +1364,83 @@ struct Ref {
 
     // SolidElement::computeGroundMotion + PointwiseRecorder::record
     // (SolidElement.cpp:189-216, output/pointwise/PointwiseRecorder.cpp:62-93)
+    // SolidElement::computeStrain / computeCurl (element/SolidElement.cpp:219-345),
+    // Fourier sums at the station as PointwiseRecorder::record takes them
+    // (PointwiseRecorder.cpp:96-125)
+    void prtSphUnd9(const Elem &el, std::vector<std::array<CMat, 9>> &g, int Nu) {
+        // PRT_1D::sphericalToUndulated9 (PRT_1D.cpp:73-100)
+        for (int a = 0; a <= Nu; a++)
+            for (int k = 0; k < 25; k++) {
+                const Real X0 = el.X[k], X1 = el.X[25 + k], X2 = el.X[50 + k], X3 = el.X[75 + k];
+                auto &s = g[a];
+                C u[9];
+                u[0] = X0 * s[0][k] + X1 * s[2][k];
+                u[1] = X0 * s[3][k] + X1 * s[5][k];
+                u[2] = X0 * s[6][k] + X1 * s[8][k];
+                u[3] = X0 * s[1][k] + X2 * s[2][k];
+                u[4] = X0 * s[4][k] + X2 * s[5][k];
+                u[5] = X0 * s[7][k] + X2 * s[8][k];
+                u[6] = X3 * s[2][k];
+                u[7] = X3 * s[5][k];
+                u[8] = X3 * s[8][k];
+                for (int c = 0; c < 9; c++) s[c][k] = u[c];
+            }
+    }
+    void strainCurl(const Rec &rc, int tstep) {
+        Elem &el = E[rc.elem];
+        Work &w = works[0];
+        const int Nr = el.nr, Nu = el.nu, nyq = (Nr % 2 == 0) ? 1 : 0;
+        ensureWork(w, Nu, Nr);
+        for (int k = 0; k < 25; k++) {
+            const SolidPt &p = S[el.pts[k]];
+            int pn = (p.nr % 2 == 0) ? 1 : 0;
+            for (int a = 0; a <= el.maxNu; a++)
+                for (int c = 0; c < 3; c++) w.displ[a][c][k] = (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0);
+        }
+        C ii(zero, one);
+        auto fsum = [&](auto &&val, int k) {
+            Real v = val(0, k).real();
+            for (int a = 1; a <= el.maxNu - nyq; a++) v += (two * std::exp((Real)a * rc.phi * ii) * val(a, k)).real();
+            return v;
+        };
+        if (rc.strain >= 0) {
+            if (el.prt) {
+                grad9(el, w.displ, w.strain9, Nu, nyq);
+                rot9(el, w.strain9, Nu, -one);
+                prtSphUnd(el, w.strain9, w.strain6, Nu);
+            } else {
+                grad6(el, w.displ, w.strain6, Nu, nyq);
+                spz2rtz(el, w.strain6, Nu);
+            }
+            double *o = &strainOut[((size_t)tstep * nstrain + rc.strain) * 6];
+            for (int c = 0; c < 6; c++) {
+                Real acc = 0;
+                for (int k = 0; k < 25; k++) {
+                    if (std::abs(rc.w[k]) < 1e-10) continue;
+                    acc += rc.w[k] * fsum([&](int a, int kk) { return w.strain6[a][c][kk]; }, k);
+                }
+                o[c] = acc;
+            }
+        }
+        if (rc.curl >= 0) {
+            grad9(el, w.displ, w.strain9, Nu, nyq);
+            rot9(el, w.strain9, Nu, -one);
+            if (el.prt) prtSphUnd9(el, w.strain9, Nu);
+            double *o = &curlOut[((size_t)tstep * ncurl + rc.curl) * 3];
+            const int pairs[3][2] = {{7, 5}, {2, 6}, {3, 1}};
+            for (int c = 0; c < 3; c++) {
+                Real acc = 0;
+                for (int k = 0; k < 25; k++) {
+                    if (std::abs(rc.w[k]) < 1e-10) continue;
+                    Real a = fsum([&](int al, int kk) { return w.strain9[al][pairs[c][0]][kk]; }, k);
+                    Real b = fsum([&](int al, int kk) { return w.strain9[al][pairs[c][1]][kk]; }, k);
+                    acc += rc.w[k] * (a - b);
+                }
+                o[c] = acc;
+            }
+        }
+    }
+
     void record(int tstep) {
         if (recs.empty()) return;
         if (tstep < 0 || tstep >= nstep) throw std::runtime_error("Domain::record || time step out of range.");
@@ -1387,6 +1475,7 @@ struct Ref {
                 }
                 for (int c = 0; c < 3; c++) u[c] += rc.w[k] * up[c];
             }
+            if (rc.strain >= 0 || rc.curl >= 0) strainCurl(rc, tstep);
             double *o = &seis[((size_t)tstep * recs.size() + ir) * 3];
             if (rtz) {
                 Real cost = (Real)std::cos(rc.theta), sint = (Real)std::sin(rc.theta);
@@ -1473,6 +1562,8 @@ struct Ref {
                 for (auto &v : m) v = 0;
         }
         std::fill(seis.begin(), seis.end(), 0.);
+        std::fill(strainOut.begin(), strainOut.end(), 0.);
+        std::fill(curlOut.begin(), curlOut.end(), 0.);
     }
 
     // Newmark::solve loop body (core/newmark/Newmark.cpp:47-93)
@@ -1601,6 +1692,21 @@ int refcpu_get_seismograms(refcpu *h, double *out, int64_t n) {
     GUARD(DISPATCH(h, {
         if ((int64_t)R.seis.size() > n) throw std::runtime_error("refcpu || buffer too small");
         std::copy(R.seis.begin(), R.seis.end(), out);
+    }))
+}
+int refcpu_get_strain_curl(refcpu *h, double *strain, int64_t ns, double *curl, int64_t nc, int32_t *nstrain,
+                           int32_t *ncurl) {
+    GUARD(DISPATCH(h, {
+        if (nstrain) *nstrain = R.nstrain;
+        if (ncurl) *ncurl = R.ncurl;
+        if (strain) {
+            if ((int64_t)R.strainOut.size() > ns) throw std::runtime_error("refcpu || buffer too small");
+            std::copy(R.strainOut.begin(), R.strainOut.end(), strain);
+        }
+        if (curl) {
+            if ((int64_t)R.curlOut.size() > nc) throw std::runtime_error("refcpu || buffer too small");
+            std::copy(R.curlOut.begin(), R.curlOut.end(), curl);
+        }
     }))
 }
 int refcpu_get_field(refcpu *h, int32_t which, double *out, int64_t n) {

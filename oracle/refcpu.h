@@ -36,6 +36,10 @@ int refcpu_check_stability(refcpu *h, int64_t *bad_point);
 int refcpu_run(refcpu *h, int32_t first, int32_t count, int32_t stab_interval);
 /* [nstep][nrec][3] */
 int refcpu_get_seismograms(refcpu *h, double *out, int64_t n);
+/* [nstep][nstrain][6] and [nstep][ncurl][3] of the stations that dump them; null
+ * buffers: counts only */
+int refcpu_get_strain_curl(refcpu *h, double *strain, int64_t ns, double *curl, int64_t nc, int32_t *nstrain,
+                           int32_t *ncurl);
 int refcpu_get_field(refcpu *h, int32_t which, double *out, int64_t n);
 int refcpu_set_field(refcpu *h, int32_t which, const double *in, int64_t n);
 /* wall time (s) of the last refcpu_run split into the reference's cost buckets

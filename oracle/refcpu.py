@@ -33,6 +33,8 @@ def lib():
             "refcpu_check_stability": ([vp, C.POINTER(C.c_int64)], C.c_int),
             "refcpu_run": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
             "refcpu_get_seismograms": ([vp, f64p, C.c_int64], C.c_int),
+            "refcpu_get_strain_curl": ([vp, f64p, C.c_int64, f64p, C.c_int64, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)], C.c_int),
             "refcpu_get_field": ([vp, C.c_int32, f64p, C.c_int64], C.c_int),
             "refcpu_set_field": ([vp, C.c_int32, f64p, C.c_int64], C.c_int),
             "refcpu_last_run_cost": ([vp, f64p], C.c_int),
@@ -118,6 +120,17 @@ class RefDomain:
         out = np.zeros(self.mesh.d.npoint, dtype=np.int32)
         _check(lib().refcpu_get_wisdom(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size))
         return out
+
+    def strain_curl(self):
+        """[nstep][nstrain][6] strain and [nstep][ncurl][3] curl of the stations that dump them."""
+        ns, nc = C.c_int32(0), C.c_int32(0)
+        _check(lib().refcpu_get_strain_curl(self._h, None, 0, None, 0, C.byref(ns), C.byref(nc)))
+        a = np.zeros(max(self.nstep * ns.value * 6, 1))
+        b = np.zeros(max(self.nstep * nc.value * 3, 1))
+        _check(lib().refcpu_get_strain_curl(self._h, a.ctypes.data_as(f64p), a.size, b.ctypes.data_as(f64p), b.size,
+                                            C.byref(ns), C.byref(nc)))
+        return (a[:self.nstep * ns.value * 6].reshape(self.nstep, ns.value, 6),
+                b[:self.nstep * nc.value * 3].reshape(self.nstep, nc.value, 3))
 
     def seismograms(self):
         n = self.nstep * self.nrec * 3

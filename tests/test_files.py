@@ -130,3 +130,10 @@ def test_station_ascii_files(tmp_path):
     np.testing.assert_array_equal(d[:, 0], t[::3])
     np.testing.assert_array_equal(d[:, 1:].astype(np.float32), s[::3, 1, :])
     assert p3[1].endswith("AX.S0001.ENZ.ascii")
+    # strain (6 columns) and curl files
+    st = rng.standard_normal((nstep, 2, 6)).astype(np.float32)
+    ps = ax.write_stations(str(tmp_path / "st"), st, dt, shift, kind="strain")
+    assert ps[0].endswith("AX.S0000.RTZ.strain.ascii")
+    d = np.loadtxt(ps[1])
+    assert d.shape == (nstep, 7)
+    np.testing.assert_array_equal(d[:, 1:].astype(np.float32), st[:, 1, :])

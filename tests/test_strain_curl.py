@@ -1,0 +1,100 @@
+"""Strain and curl at stations (SolidElement::computeStrain / computeCurl,
+SOLVER/src/core/element/SolidElement.cpp:219-345; PointwiseRecorder.cpp:96-125).
+
+Oracle pins, on displacement fields whose strain and curl are known in closed
+form (up to the spectral interpolation of the spherical geometry): a rigid
+translation has neither; a rigid rotation about the axis, u_phi = w s, has no
+strain and curl 2 w z_hat = (-2 w sin(theta), 0, 2 w cos(theta)) in (R, T, Z);
+a uniform dilatation u = e r has strain (e, e, e, 0, 0, 0) and no curl.
+GPU (marked gpu): strain and curl traces against the oracle's, on a 3-D model
+and on a relabelled one.
+"""
+import numpy as np
+import pytest
+
+import axisem3d_amd as ax
+from axisem3d_amd import _abi
+from tests.helpers import rel_l2
+
+
+@pytest.fixture(scope="module")
+def refcpu():
+    from oracle import refcpu as r
+    r.lib()
+    return r
+
+
+def _mesh(layers, **kw):
+    cfg = dict(layers=layers, nelem_theta=12, nu_mode=1, nu_min=2, nu_max=8, nstep=10, nrec=8, rec_depth=30e3,
+               rec_dump=3, nthread=1)
+    cfg.update(kw)
+    return ax.ShellMesh(ax.shell_config(0, **cfg))
+
+
+def _field(m, fn):
+    """Solid field from fn(s, z, alpha) -> 3 complex (s, phi, z) components."""
+    nr, fl, sz = m.array("point_nr"), m.array("point_flags"), m.array("point_sz").reshape(-1, 2)
+    out = []
+    for p in range(m.d.npoint):
+        if fl[p] & _abi.PT_SOLID:
+            for a in range(nr[p] // 2 + 1):
+                v = fn(sz[p, 0], sz[p, 1], a)
+                for c in range(3):
+                    out += [np.real(v[c]), np.imag(v[c])]
+    return np.array(out)
+
+
+def _record(dom, m, u):
+    dom.set_field(_abi.FIELD_SOLID_DISPL, u)
+    dom.record(0)
+    st, cu = dom.strain_curl()
+    return st[0], cu[0]
+
+
+def test_strain_curl_of_known_fields(small_layers, refcpu):
+    m = _mesh(small_layers)
+    assert (m.array("rec_dump") == 3).all()
+    dom = refcpu.RefDomain(m, double=True)
+    theta = m.array("rec_theta").astype(np.float64)
+    w = 1e-3
+    st, cu = _record(dom, m, _field(m, lambda s, z, a: [0., w * s, 0.] if a == 0 else [0., 0., 0.]))
+    assert np.abs(st).max() < 1e-5 * w
+    np.testing.assert_allclose(cu[:, 0], -2 * w * np.sin(theta), atol=1e-5 * w)
+    np.testing.assert_allclose(cu[:, 1], 0., atol=1e-5 * w)
+    np.testing.assert_allclose(cu[:, 2], 2 * w * np.cos(theta), atol=1e-5 * w)
+    e = 2e-4
+    st, cu = _record(dom, m, _field(m, lambda s, z, a: [e * s, 0., e * z] if a == 0 else [0., 0., 0.]))
+    np.testing.assert_allclose(st[:, :3], e, rtol=1e-5)
+    assert np.abs(st[:, 3:]).max() < 1e-5 * e and np.abs(cu).max() < 1e-5 * e
+    for fn in (lambda s, z, a: [0., 0., 1.] if a == 0 else [0., 0., 0.],
+               lambda s, z, a: [0.5, 0.5j, 0.] if a == 1 else [0., 0., 0.]):
+        st, cu = _record(dom, m, _field(m, fn))
+        assert np.abs(st).max() < 1e-9 and np.abs(cu).max() < 1e-9
+
+
+def test_strain_curl_fluid_station_rejected(fluid_layers):
+    # stations in the fluid keep no strain flag; the descriptor with one is rejected
+    m = _mesh(fluid_layers, rec_depth=4000e3)
+    assert (m.array("rec_dump") == 0).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["3d", "prt"])
+def test_strain_curl_traces_gpu(small_layers, refcpu, case):
+    print("case strain curl %s" % case, flush=True)
+    kw = dict(nstep=300, material=_abi.MAT_TRANSVERSELY_ISO)
+    if case == "3d":
+        kw.update(dv_amp=0.05, nu_min=3, nu_max=12)
+    else:
+        kw.update(undulation=2, undulation_amp=10e3, undulation_order=2, undulation_depth=300e3)
+    m = _mesh(small_layers, **kw)
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, 300, 100, use_graph=True)
+    sg, cg = g.strain_curl()
+    g.close()
+    r = refcpu.RefDomain(m, double=False)
+    r.run(1, 300, 100)
+    sr, cr = r.strain_curl()
+    assert np.abs(sr).max() > 0 and np.abs(cr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4 and rel_l2(cg, cr) < 1e-4
