@@ -53,6 +53,7 @@ struct Bucket1D {
 struct Bucket3D {
     int colour = 0, phase = 0, material = 0, att = 0;
     bool tiso = false;
+    bool prt = false;  // PRT_3D relabelling
     int32_t nelem = 0, maxM = 0, maxNr = 0, slots = 1;
     std::vector<int32_t> h_elems;
     DevBuf<int4> einfo;  // per element: id, Nr, flags | stages << 8, twiddle offset
@@ -62,7 +63,7 @@ struct Bucket3D {
 // sequences [maxNr][nseq] (solid 3, fluid 2; they also carry the xi operands
 // [ncomp][maxM] of the exchange), the element table
 inline size_t lds3D(const Bucket3D &b) {
-    const size_t ns = b.phase ? 2 : 3;
+    const size_t ns = b.phase ? 2 : b.prt ? 5 : 3;  // a relabelled solid transforms 9 columns
     return (2 * (size_t)b.maxNr + ns * W3D_WAVES * (size_t)b.maxNr + kTabSlots) * sizeof(float2);
 }
 
@@ -119,6 +120,7 @@ struct axg_ctx {
     int device = 0;
     float G[2][25] = {};  // GLL / GLJ derivative matrices (kernel arguments of the 3-D kernels)
     int pf_max_nr = 0;    // largest Nr of the Mass3D / SFCoupling3D points
+    bool recf3_prt = false;  // a 3-D fluid station sits in a relabelled element
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int32_t npoint = 0, nelem = 0, nstep = 0, nrec = 0;
@@ -129,7 +131,7 @@ struct axg_ctx {
     DevBuf<int32_t> d_elem_flags, d_elem_nr, d_ep_row, d_ep_amax;
     DevBuf<float> d_geom, d_tiso, d_mat_pool, d_att_pool, d_prt_pool;
     DevBuf<int64_t> d_prt_off;
-    std::vector<char> elem_prt;  // element carries a PRT_1D relabelling
+    std::vector<char> elem_prt;  // 1: PRT_1D, 2: PRT_3D relabelling
     DevBuf<int64_t> d_mat_off, d_att_off, d_att_mem_off;
     DevBuf<float2> d_att_memvar, d_att_stressR;
     int64_t att_mem_stride = 0;
@@ -368,9 +370,9 @@ void writePlan(const FftPlan &f, int n, int32_t *dsc0) {
     }
 }
 
-void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att,
+void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, int mat, int tiso, int att, bool prt,
                      const std::vector<int32_t> &elems) {
-    if (phase != 0 && tiso)
+    if (phase != 0 && tiso && !prt)
         throw std::runtime_error("CrdTransTIsoFluid || transversely isotropic rotation of fluid elements is not supported by this build.");
     if (phase != 0 && att != AXG_ATT_NONE)
         throw std::runtime_error("Attenuation3D::applyToStress || attenuation of fluid elements is not defined.");
@@ -388,6 +390,9 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
                                      " is not supported by this build (max " + std::to_string(E3D_MAX_NR) + ").");
         c->nr_used[nr] = 1;
         const int cls = w3dClass(((nr + 1) / 2 + 63) / 64);
+        if (prt && cls == W3D_MAX_S)
+            throw std::runtime_error("PRT_3D || Nr = " + std::to_string(nr) + " is not supported by this build (max " +
+                                     std::to_string(2 * 64 * 5) + ").");
         if (c->nr_class.empty()) c->nr_class.assign(E3D_MAX_NR + 1, 0);
         c->nr_class[nr] = std::max(c->nr_class[nr], cls);
         byClass[cls].push_back(e);
@@ -414,6 +419,7 @@ void createBuckets3D(axg_ctx *c, const axg_mesh_desc *d, int colour, int phase, 
         b->material = mat;
         b->tiso = tiso != 0;
         b->att = att;
+        b->prt = prt;
         b->nelem = (int32_t)v.size();
         b->slots = k;
         b->maxNr = c->elem_nr[v.front()];
@@ -561,13 +567,15 @@ void launchBucket3D(axg_ctx *c, const Bucket3D &b) {
         a.sls_gamma[i] = c->sls_g[i];
     }
     a.do_kappa = c->do_kappa;
+    a.prt_pool = c->d_prt_pool.p;
+    a.prt_off = c->d_prt_off.p;
     const size_t lds = lds3D(b);
     switch (b.slots) {
-        case 1: launchElem3D<1>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
-        case 2: launchElem3D<2>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
-        case 3: launchElem3D<3>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
-        case 5: launchElem3D<5>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
-        default: launchElem3D<7>(a, b.phase, b.material, b.tiso, b.att, lds, c->stream); break;
+        case 1: launchElem3D<1>(a, b.phase, b.material, b.tiso, b.att, b.prt, lds, c->stream); break;
+        case 2: launchElem3D<2>(a, b.phase, b.material, b.tiso, b.att, b.prt, lds, c->stream); break;
+        case 3: launchElem3D<3>(a, b.phase, b.material, b.tiso, b.att, b.prt, lds, c->stream); break;
+        case 5: launchElem3D<5>(a, b.phase, b.material, b.tiso, b.att, b.prt, lds, c->stream); break;
+        default: launchElem3D<7>(a, b.phase, b.material, b.tiso, b.att, b.prt, lds, c->stream); break;
     }
 }
 
@@ -941,13 +949,17 @@ void recordFluid3D(axg_ctx *c, int tstep, const int *tstep_ptr) {
     a.rtz = c->rec_rtz;
     a.tstep = tstep;
     a.tstep_ptr = tstep_ptr;
+    a.tiso = c->d_tiso.p;
+    a.prt_pool = c->d_prt_pool.p;
+    a.prt_off = c->d_prt_off.p;
     const size_t lds = (2 * (size_t)a.maxNr + 2 * W3D_WAVES * (size_t)a.maxNr + kTabSlots) * sizeof(float2);
+    const bool prt = c->recf3_prt;
     switch (c->recf3_class) {
-        case 1: launchRecFluid3D<1>(a, lds, c->stream); break;
-        case 2: launchRecFluid3D<2>(a, lds, c->stream); break;
-        case 3: launchRecFluid3D<3>(a, lds, c->stream); break;
-        case 5: launchRecFluid3D<5>(a, lds, c->stream); break;
-        default: launchRecFluid3D<7>(a, lds, c->stream); break;
+        case 1: launchRecFluid3D<1>(a, prt, lds, c->stream); break;
+        case 2: launchRecFluid3D<2>(a, prt, lds, c->stream); break;
+        case 3: launchRecFluid3D<3>(a, prt, lds, c->stream); break;
+        case 5: launchRecFluid3D<5>(a, prt, lds, c->stream); break;
+        default: launchRecFluid3D<7>(a, prt, lds, c->stream); break;
     }
 }
 
@@ -1097,18 +1109,25 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
             const int64_t n = d->prt_off[e + 1] - d->prt_off[e];
             off[e] = (int64_t)pool.size();  // the element's first float
             if (n == 0) continue;
-            if (n == 100 * (int64_t)c->elem_nr[e] && n != 100)
-                throw std::runtime_error("PRT_3D || particle relabelling varying in azimuth is not supported by this "
-                                         "build.");
-            if (n != 100) throw std::runtime_error("PRT_1D || Incompatible size.");
-            if (d->elem_flags[e] & AXG_EL_3D)
-                throw std::runtime_error("PRT_3D || particle relabelling on a 3-D element is not supported by this "
-                                         "build.");
+            const bool is3 = (d->elem_flags[e] & AXG_EL_3D) != 0;
+            const int64_t nr = c->elem_nr[e];
             if (!(d->elem_flags[e] & AXG_EL_TISO))
                 throw std::runtime_error("SolidElement::SolidElement || a relabelled element needs the TIso "
                                          "rotation.");
-            pool.insert(pool.end(), d->prt_pool + d->prt_off[e], d->prt_pool + d->prt_off[e + 1]);
-            c->elem_prt[e] = 1;
+            if (!is3) {
+                // PRT_1D (Relabelling::createPRT with elem1D)
+                if (n != 100) throw std::runtime_error("PRT_1D || Incompatible size.");
+                pool.insert(pool.end(), d->prt_pool + d->prt_off[e], d->prt_pool + d->prt_off[e + 1]);
+                c->elem_prt[e] = 1;
+            } else {
+                // PRT_3D: X of every slice [Nr][4][25]; one row stands for all slices
+                if (n != 100 && n != 100 * nr) throw std::runtime_error("PRT_3D || Incompatible size.");
+                for (int64_t r = 0; r < nr; r++) {
+                    const float *src = d->prt_pool + d->prt_off[e] + (n == 100 ? 0 : 100 * r);
+                    pool.insert(pool.end(), src, src + 100);
+                }
+                c->elem_prt[e] = 2;
+            }
         }
         off[NE] = (int64_t)pool.size();
         if (!pool.empty()) {
@@ -1213,7 +1232,7 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
                 c->buckets1d.push_back(std::move(b));
             }
             if (!e3[key].empty())
-                createBuckets3D(c, d, key / NK, (kk / 32) % 2, (kk / 8) % 4, (kk / 4) % 2, kk % 4, e3[key]);
+                createBuckets3D(c, d, key / NK, (kk / 32) % 2, (kk / 8) % 4, (kk / 4) % 2, kk % 4, kk >= 64, e3[key]);
         }
         // transforms of the points with Mass3D / MassOcean3D / SFCoupling3D use
         // the same tables
@@ -1365,9 +1384,13 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
                 const int e = d->rec_elem[i], nr = c->elem_nr[e];
                 info.push_back(make_int4(e, nr, c->elem_flags[e] | (c->h_nstage[nr] << 8), c->h_twoff[nr]));
                 c->recf3_class = std::max(c->recf3_class, w3dClass(((nr + 1) / 2 + 63) / 64));
+                c->recf3_prt = c->recf3_prt || c->elem_prt[e] != 0;
                 c->recf3_maxNr = std::max(c->recf3_maxNr, nr);
             }
             c->recf3_maxM = std::min(c->recf3_maxNr / 2 + 1, 64 * c->recf3_class);
+            if (c->recf3_prt && c->recf3_class == W3D_MAX_S)
+                throw std::runtime_error("PRT_3D || stations in relabelled fluid elements of Nr above 640 are not "
+                                         "supported by this build.");
             c->d_rec_idx_f3.upload(idx_f3, s);
             c->d_recf3_info.upload(info, s);
         }
@@ -1386,6 +1409,9 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
                 if (!(f & 3)) continue;
                 if (d->elem_flags[d->rec_elem[i]] & AXG_EL_FLUID)
                     throw std::runtime_error("FluidElement::computeStrain || strain at a fluid station is not "
+                                             "supported by this build.");
+                if (c->elem_prt[d->rec_elem[i]] == 2)
+                    throw std::runtime_error("SolidElement::computeStrain || strain in a PRT_3D element is not "
                                              "supported by this build.");
                 sc.push_back(i);
                 slot.push_back(make_int2((f & 1) ? c->nstrain++ : -1, (f & 2) ? c->ncurl++ : -1));

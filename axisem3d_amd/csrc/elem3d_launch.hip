@@ -27,26 +27,43 @@ void allowLds(K kernel) {
     }
 }
 
-template <int MAT, bool TISO, int ATT>
+template <int MAT, bool TISO, int ATT, bool PRT = false>
 void solid(const Elem3DArgs &a, size_t lds, hipStream_t s) {
-    allowLds(&solid3d_kernel<MAT, TISO, S, ATT>);
-    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S, ATT>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+    allowLds(&solid3d_kernel<MAT, TISO, S, ATT, PRT>);
+    hipLaunchKernelGGL((solid3d_kernel<MAT, TISO, S, ATT, PRT>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
 }
 
-template <int MAT, bool TISO>
+template <int MAT, bool TISO, bool PRT = false>
 void solidAtt(const Elem3DArgs &a, int att, size_t lds, hipStream_t s) {
-    if (att == AXG_ATT_NONE) solid<MAT, TISO, 0>(a, lds, s);
-    else if (att == AXG_ATT_FULL) solid<MAT, TISO, 1>(a, lds, s);
-    else solid<MAT, TISO, 2>(a, lds, s);
+    if (att == AXG_ATT_NONE) solid<MAT, TISO, 0, PRT>(a, lds, s);
+    else if (att == AXG_ATT_FULL) solid<MAT, TISO, 1, PRT>(a, lds, s);
+    else solid<MAT, TISO, 2, PRT>(a, lds, s);
+}
+
+template <bool REC, bool PRT>
+void fluid(const Elem3DArgs &a, size_t lds, hipStream_t s) {
+    allowLds(&fluid3d_kernel<S, REC, PRT>);
+    hipLaunchKernelGGL((fluid3d_kernel<S, REC, PRT>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
 }
 
 }  // namespace
 
 template <>
-void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s) {
+void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, bool prt, size_t lds,
+                     hipStream_t s) {
+    if (prt) {
+        // a relabelled element rotates to (R, T, Z) (SolidElement.cpp:21); the
+        // largest size class holds no 9-column transform in LDS
+        if constexpr (S < 7) {
+            if (phase) fluid<false, true>(a, lds, s);
+            else if (mat == AXG_MAT_ISOTROPIC) solidAtt<0, true, true>(a, att, lds, s);
+            else if (mat == AXG_MAT_TRANSVERSELY_ISO) solidAtt<1, true, true>(a, att, lds, s);
+            else solidAtt<2, true, true>(a, att, lds, s);
+        }
+        return;
+    }
     if (phase) {
-        allowLds(&fluid3d_kernel<S, false>);
-        hipLaunchKernelGGL((fluid3d_kernel<S, false>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+        fluid<false, false>(a, lds, s);
     } else if (mat == AXG_MAT_ISOTROPIC) {
         if (tiso) solidAtt<0, true>(a, att, lds, s);
         else solidAtt<0, false>(a, att, lds, s);
@@ -58,9 +75,12 @@ void launchElem3D<S>(const Elem3DArgs &a, int phase, int mat, bool tiso, int att
 }
 
 template <>
-void launchRecFluid3D<S>(const Elem3DArgs &a, size_t lds, hipStream_t s) {
-    allowLds(&fluid3d_kernel<S, true>);
-    hipLaunchKernelGGL((fluid3d_kernel<S, true>), dim3(a.nelem), dim3(W3D_THREADS), lds, s, a);
+void launchRecFluid3D<S>(const Elem3DArgs &a, bool prt, size_t lds, hipStream_t s) {
+    if (prt) {
+        if constexpr (S < 7) fluid<true, true>(a, lds, s);
+    } else {
+        fluid<true, false>(a, lds, s);
+    }
 }
 
 template <>

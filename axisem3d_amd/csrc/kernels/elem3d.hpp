@@ -83,7 +83,48 @@ struct Elem3DArgs {
     float *seis;                 // [nstep][nrec_total][3]
     int32_t nrec_total, rtz, tstep;
     const int *tstep_ptr;        // device step counter (graph replay) or null
+    // particle relabelling PRT_3D: per element [Nr][4][25] X0..X3 of every slice
+    const float *prt_pool;
+    const int64_t *prt_off;
 };
+
+// PRT_3D::sphericalToUndulated / undulatedToSpherical on one sample
+// (PRT_3D.cpp:36-80): 3 x 3 gradient -> undulated strain, undulated stress -> 3 x 3
+__device__ __forceinline__ void prt3_sph_und(const float g[9], float X0, float X1, float X2, float X3, float e[6]) {
+    e[0] = X0 * g[0] + X1 * g[2];
+    e[1] = X0 * g[4] + X2 * g[5];
+    e[2] = X3 * g[8];
+    e[3] = X0 * g[7] + X2 * g[8] + X3 * g[5];
+    e[4] = X0 * g[6] + X1 * g[8] + X3 * g[2];
+    e[5] = X0 * (g[3] + g[1]) + X1 * g[5] + X2 * g[2];
+}
+__device__ __forceinline__ void prt3_und_sph(const float t[6], float X0, float X1, float X2, float X3, float o[9]) {
+    o[0] = X0 * t[0];
+    o[1] = X0 * t[5];
+    o[2] = X1 * t[0] + X3 * t[4] + X2 * t[5];
+    o[3] = o[1];
+    o[4] = X0 * t[1];
+    o[5] = X2 * t[1] + X3 * t[3] + X1 * t[5];
+    o[6] = X0 * t[4];
+    o[7] = X0 * t[3];
+    o[8] = X3 * t[2] + X2 * t[3] + X1 * t[4];
+}
+
+// CrdTransTIsoSolid on a 3 x 3 tensor (CrdTransTIsoSolid.cpp:44-84); sg = -1
+// SPZ -> RTZ, +1 back
+__device__ __forceinline__ void tiso9_3d(float2 u[9], float s1, float c1, float s2, float c2, float sg) {
+    const float2 sum08 = cadd(u[0], u[8]), dif08 = csub(u[0], u[8]);
+    const float2 sum26 = cadd(u[2], u[6]), dif26 = csub(u[2], u[6]);
+    const float2 u1 = u[1], u3 = u[3];
+    u[0] = cscale(0.5f, cadd(cadd(sum08, cscale(c2, dif08)), cscale(sg * s2, sum26)));
+    u[2] = cscale(0.5f, csub(cadd(dif26, cscale(c2, sum26)), cscale(sg * s2, dif08)));
+    u[6] = csub(u[2], dif26);
+    u[8] = csub(sum08, u[0]);
+    u[1] = cadd(cscale(c1, u1), cscale(sg * s1, u[7]));
+    u[7] = csub(cscale(c1, u[7]), cscale(sg * s1, u1));
+    u[3] = cadd(cscale(c1, u3), cscale(sg * s1, u[5]));
+    u[5] = csub(cscale(c1, u[5]), cscale(sg * s1, u3));
+}
 
 // Attenuation3D_{Full,CG4}::applyToStress + updateMemoryVariables on one
 // sample n of one attenuating GLL point i (Attenuation3D_Full.cpp,
@@ -502,13 +543,16 @@ __device__ __forceinline__ void stress3d(ParFn P, const float e[6], float s[6]) 
 }
 
 // ---------------------------------------------------------------- kernel
-template <int MAT, bool TISO, int S, int ATT>
+template <int MAT, bool TISO, int S, int ATT, bool PRT = false>
 // classes 1 and 2 at 96 VGPRs (five waves per SIMD: four workgroups per CU),
-// class 3 at 128 (three workgroups)
-__global__ __launch_bounds__(W3D_THREADS) __attribute__((amdgpu_waves_per_eu(S <= 2 ? 5 : S == 3 ? 4 : 1)))
+// class 3 at 128 (three workgroups).  PRT: the element carries a PRT_3D
+// relabelling (SolidElement.cpp:404-441): the 3 x 3 gradient (9 columns in 5
+// sequences) goes through the transforms, the relabelling acts per sample
+__global__ __launch_bounds__(W3D_THREADS) __attribute__((amdgpu_waves_per_eu(PRT ? 1 : S <= 2 ? 5 : S == 3 ? 4 : 1)))
 void solid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     constexpr int NPAR = MatParams<MAT>::v;
+    constexpr int NS = PRT ? 5 : 3;  // complex sequences per point (real columns 9 or 6)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // row ipol of this wave
     const int4 info = a.einfo[blockIdx.x];  // one load: no dependent chain at the start
     const int64_t e = __builtin_amdgcn_readfirstlane(info.x);
@@ -527,9 +571,9 @@ void solid3d_kernel(Elem3DArgs a) {
     float2 *twf = lds;                                   // W_N^t, t < Nr
     float2 *twi = twf + a.maxNr;                         // W_N^{-t}
     float2 *seq0 = twi + a.maxNr;                        // the waves' sequence areas
-    float2 *seq = seq0 + w * 3 * a.maxNr;                // this wave's [Nr][3]; after the
+    float2 *seq = seq0 + w * NS * a.maxNr;               // this wave's [Nr][NS]; after the
                                                          // R2C its xi operands [3][maxM]
-    int32_t *tab = reinterpret_cast<int32_t *>(seq0 + 15 * a.maxNr);  // element table
+    int32_t *tab = reinterpret_cast<int32_t *>(seq0 + W3D_WAVES * NS * a.maxNr);  // element table
     for (int t = threadIdx.x; t < Nr; t += W3D_THREADS) {
         float2 v = a.twiddle[twoff + t];
         twf[t] = v;
@@ -541,6 +585,7 @@ void solid3d_kernel(Elem3DArgs a) {
     plan = tab + kTabPlan;
 
     const float *mbase = a.mat3d + uload64(a.mat3d_off + e);
+    const float *xbase = PRT ? a.prt_pool + uload64(a.prt_off + e) : nullptr;
     const float *att_par = nullptr;
     float2 *att_mv = nullptr, *att_sr = nullptr;
     if (ATT != 0) {
@@ -583,10 +628,10 @@ void solid3d_kernel(Elem3DArgs a) {
         for (int s = 0; s < S; s++) {
             const int al = lane + 64 * s;
             if (al >= M) continue;
-            float2 e6[6];
+            float2 e6[2 * NS];
             if (nyq && al == Nu) {
 #pragma unroll
-                for (int c = 0; c < 6; c++) e6[c] = czero();
+                for (int c = 0; c < 2 * NS; c++) e6[c] = czero();
             } else {
                 float2 GU[3], UG[3], u[3];
 #pragma unroll
@@ -613,6 +658,28 @@ void solid3d_kernel(Elem3DArgs a) {
                 }
                 const float fa = (float)al;
                 float2 v0 = cadd(u[0], cimul(fa, u[1])), v1 = csub(cimul(fa, u[0]), u[1]), v2 = cimul(fa, u[2]);
+                if constexpr (PRT) {
+                    // Gradient::computeGrad9 (Gradient.cpp:84-152), rotated to (R, T, Z)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        e6[3 * c + 0] = cadd(cscale(g.dzdeta, GU[c]), cscale(g.dzdxii, UG[c]));
+                        e6[3 * c + 2] = cadd(cscale(g.dsdeta, GU[c]), cscale(g.dsdxii, UG[c]));
+                    }
+                    e6[1] = cscale(g.inv_s, v1);
+                    e6[4] = cscale(g.inv_s, v0);
+                    e6[7] = cscale(g.inv_s, v2);
+                    e6[9] = czero();
+                    if (axial && w == 0) {
+                        e6[4] = cadd(e6[4], cscale(g.dzdeta, cadd(GU[0], cimul(fa, GU[1]))));
+                        e6[1] = cadd(e6[1], cscale(g.dzdeta, csub(cimul(fa, GU[0]), GU[1])));
+                        e6[7] = cadd(e6[7], cscale(g.dzdeta, cimul(fa, GU[2])));
+                        if (al == 1) {
+                            e6[4] = cadd(e6[4], cscale(g.dzdxii, cadd(UG[0], cimul(fa, UG[1]))));
+                            e6[1] = cadd(e6[1], cscale(g.dzdxii, csub(cimul(fa, UG[0]), UG[1])));
+                        }
+                    }
+                    tiso9_3d(e6, ts1, tc1, ts2, tc2, -1.f);
+                } else {
                 e6[0] = cadd(cscale(g.dzdeta, GU[0]), cscale(g.dzdxii, UG[0]));
                 e6[1] = cscale(g.inv_s, v0);
                 e6[2] = cadd(cscale(g.dsdeta, GU[2]), cscale(g.dsdxii, UG[2]));
@@ -631,24 +698,25 @@ void solid3d_kernel(Elem3DArgs a) {
                     }
                 }
                 if (TISO) tiso_spz_rtz(e6, ts1, tc1, ts2, tc2);
+                }
             }
             // Z_b = X_{2b} + i X_{2b+1}; Z_{N-k} = conj(X_k) + i conj(Y_k); the
             // imaginary parts of DC and Nyquist are dropped (FFTW c2r)
 #pragma unroll
-            for (int b = 0; b < 3; b++) {
+            for (int b = 0; b < NS; b++) {
                 float2 X = e6[2 * b], Y = e6[2 * b + 1];
                 float2 *z = seq + b;
                 if (al == 0 || 2 * al == Nr) {
-                    z[3 * al] = make_float2(X.x, Y.x);
+                    z[NS * al] = make_float2(X.x, Y.x);
                 } else {
-                    z[3 * al] = make_float2(X.x - Y.y, X.y + Y.x);
-                    z[3 * (Nr - al)] = make_float2(X.x + Y.y, -X.y + Y.x);
+                    z[NS * al] = make_float2(X.x - Y.y, X.y + Y.x);
+                    z[NS * (Nr - al)] = make_float2(X.x + Y.y, -X.y + Y.x);
                 }
             }
         }
         if (nyq && lane == 0) {
 #pragma unroll
-            for (int b = 0; b < 3; b++) seq[3 * Nu + b] = czero();  // Nyquist: no lane when Nu = 64 S
+            for (int b = 0; b < NS; b++) seq[NS * Nu + b] = czero();  // Nyquist: no lane when Nu = 64 S
         }
         wave_lds_fence();
         // C2R (pass 0), stress on the samples, R2C (pass 1): one copy of the
@@ -656,24 +724,37 @@ void solid3d_kernel(Elem3DArgs a) {
         const float *mp = mbase + (int64_t)ip * NPAR * Nr;
 #pragma unroll 1
         for (int pass = 0; pass < 2; pass++) {
-            wfft<128 * S>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, lane);
+            wfft<128 * S, NS>(seq, Nr, pass == 0 ? +1 : -1, plan, nst, pass == 0 ? twi : twf, lane);
             if (pass == 1) break;
             // ---------------- stress at every sample n of the point
             // attenuating point: all 25 (Full) or (1,1),(1,3),(3,1),(3,3) (CG4)
             const int np = (ATT == 2) ? 4 : 25;
             const int iatt = (ATT == 1) ? ip : (ATT == 2) ? (ip == 6 ? 0 : ip == 8 ? 1 : ip == 16 ? 2 : ip == 18 ? 3 : -1) : -1;
             auto sample = [&](int n, auto P) {
-                float ev[6], sv[6];
+                float ev[2 * NS], sv[2 * NS];
 #pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    float2 z = seq[3 * n + b];
+                for (int b = 0; b < NS; b++) {
+                    float2 z = seq[NS * n + b];
                     ev[2 * b] = z.x;
                     ev[2 * b + 1] = z.y;
                 }
-                stress3d<MAT>(P, ev, sv);
-                if (ATT != 0 && iatt >= 0) attenuate3d(a, att_par, att_mv, att_sr, np, iatt, Nr, n, ev, sv);
+                if constexpr (PRT) {
+                    // PRT_3D of this sample (PRT_3D.cpp:36-80); attenuation on the
+                    // undulated strain
+                    const float *xp = xbase + (int64_t)n * 100 + ip;
+                    const float X0 = xp[0], X1 = xp[25], X2 = xp[50], X3 = xp[75];
+                    float und[6], st[6];
+                    prt3_sph_und(ev, X0, X1, X2, X3, und);
+                    stress3d<MAT>(P, und, st);
+                    if (ATT != 0 && iatt >= 0) attenuate3d(a, att_par, att_mv, att_sr, np, iatt, Nr, n, und, st);
+                    prt3_und_sph(st, X0, X1, X2, X3, sv);
+                    sv[9] = 0.f;
+                } else {
+                    stress3d<MAT>(P, ev, sv);
+                    if (ATT != 0 && iatt >= 0) attenuate3d(a, att_par, att_mv, att_sr, np, iatt, Nr, n, ev, sv);
+                }
 #pragma unroll
-                for (int b = 0; b < 3; b++) seq[3 * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
+                for (int b = 0; b < NS; b++) seq[NS * n + b] = make_float2(sv[2 * b], sv[2 * b + 1]);
             };
 #pragma unroll 1
             for (int n = lane; n < Nr; n += 64) sample(n, [&](int p) { return mp[p * Nr + n]; });
@@ -693,17 +774,29 @@ void solid3d_kernel(Elem3DArgs a) {
             if (al >= M) continue;
             float2 *X = Xk[s];
             if (!(nyq && al == Nu)) {
-                float2 s6[6];
+                float2 s6[2 * NS];
 #pragma unroll
-                for (int b = 0; b < 3; b++) {
+                for (int b = 0; b < NS; b++) {
                     const float2 *z = seq + b;
-                    float2 za = z[3 * al], zb = z[3 * (al == 0 ? 0 : Nr - al)];
+                    float2 za = z[NS * al], zb = z[NS * (al == 0 ? 0 : Nr - al)];
                     s6[2 * b] = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
                     s6[2 * b + 1] = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
                 }
-                if (TISO) tiso_rtz_spz(s6, ts1, tc1, ts2, tc2);
                 const float nb = -(float)al;
                 float2 gg[3];
+                if constexpr (PRT) {
+                    // rotated back, Gradient::computeQuad9 operands (Gradient.cpp:154-204)
+                    tiso9_3d(s6, ts1, tc1, ts2, tc2, 1.f);
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        X[c] = cadd(cscale(g.dzdeta, s6[3 * c]), cscale(g.dsdeta, s6[3 * c + 2]));
+                        Yk[s][c] = cadd(cscale(g.dzdxii, s6[3 * c]), cscale(g.dsdxii, s6[3 * c + 2]));
+                    }
+                    gg[0] = cadd(s6[4], cimul(nb, s6[1]));
+                    gg[1] = csub(cimul(nb, s6[4]), s6[1]);
+                    gg[2] = cimul(nb, s6[7]);
+                } else {
+                if (TISO) tiso_rtz_spz(s6, ts1, tc1, ts2, tc2);
                 X[0] = cadd(cscale(g.dzdeta, s6[0]), cscale(g.dsdeta, s6[4]));
                 X[1] = cadd(cscale(g.dzdeta, s6[5]), cscale(g.dsdeta, s6[3]));
                 X[2] = cadd(cscale(g.dzdeta, s6[4]), cscale(g.dsdeta, s6[2]));
@@ -713,6 +806,7 @@ void solid3d_kernel(Elem3DArgs a) {
                 gg[0] = cadd(s6[1], cimul(nb, s6[5]));
                 gg[1] = csub(cimul(nb, s6[1]), s6[5]);
                 gg[2] = cimul(nb, s6[3]);
+                }
                 if (axial && w == 0) {
                     // axis row: G_xi(:,0) dzdeta g joins the xi term, (dzdxii g)
                     // G_eta^T the eta term at alpha = 1 (Gradient.cpp:291-315)
@@ -746,7 +840,7 @@ void solid3d_kernel(Elem3DArgs a) {
                 const float gx = Gx[w * 5 + i];
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    float2 x = seq0[i * 3 * a.maxNr + c * a.maxM + al];
+                    float2 x = seq0[i * NS * a.maxNr + c * a.maxM + al];
                     Ok[s][c].x += gx * x.x;
                     Ok[s][c].y += gx * x.y;
                 }
@@ -786,7 +880,10 @@ void solid3d_kernel(Elem3DArgs a) {
 // station of each workgroup instead of the quadrature: the same gradient,
 // transforms and K(phi) on every point, then the Fourier sum of the stress
 // spectrum at the station's azimuth, weighted over the 25 points.
-template <int S, bool REC>
+// PRT: a PRT_3D relabelling (FluidElement.cpp:333-355): the gradient rotated
+// to (R, T, Z) in Fourier space (CrdTransTIsoFluid.cpp:14-32), relabelled and
+// back per sample around K(phi), rotated back after the R2C.
+template <int S, bool REC, bool PRT = false>
 __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
     extern __shared__ float2 lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -815,12 +912,15 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
         twf[t] = v;
         twi[t] = make_float2(v.x, -v.y);
     }
-    load_elem_table(tab, a.plan + Nr * E3D_MAX_STAGES * FS_N, nst, a.ep_row, a.ep_amax, a.geom, nullptr, e,
-                    threadIdx.x, W3D_THREADS);
+    load_elem_table(tab, a.plan + Nr * E3D_MAX_STAGES * FS_N, nst, a.ep_row, a.ep_amax, a.geom,
+                    PRT ? a.tiso : nullptr, e, threadIdx.x, W3D_THREADS);
     __syncthreads();
     plan = tab + kTabPlan;
 
     const float *mbase = a.mat3d + uload64(a.mat3d_off + e);
+    const float *xbase = PRT ? a.prt_pool + uload64(a.prt_off + e) : nullptr;
+    // station launches mix relabelled and plain elements: decided per workgroup
+    const bool eprt = PRT && (!REC || uload64(a.prt_off + e + 1) > uload64(a.prt_off + e));
     const float h = 0.5f / (float)Nr;
     float racc[3] = {0.f, 0.f, 0.f};  // station mode: this lane's share of the motion
     const int irec = REC ? __builtin_amdgcn_readfirstlane(a.rec_idx[blockIdx.x]) : 0;
@@ -831,6 +931,7 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
         const int lane = opaque((int)(threadIdx.x & 63));
         const int ip = w * 5 + k;
         PointGeom g = tab_geom(tab, ip);
+        const float ts1 = PRT ? ldsuf(tab + kTabTiso + ip) : 0.f, tc1 = PRT ? ldsuf(tab + kTabTiso + 25 + ip) : 1.f;
 
         // ---------------- gradient spectrum -> 2 Hermitian-extended sequences
 #pragma unroll 1
@@ -860,6 +961,11 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
                 g3[1] = cscale(g.inv_s, cimul(fa, u));
                 g3[2] = cadd(cscale(g.dsdeta, GU), cscale(g.dsdxii, UG));
                 if (axial && w == 0) g3[1] = cadd(g3[1], cscale(g.dzdeta, cimul(fa, GU)));
+                if (eprt) {
+                    const float2 r0 = csub(cscale(tc1, g3[0]), cscale(ts1, g3[2]));
+                    g3[2] = cadd(cscale(tc1, g3[2]), cscale(ts1, g3[0]));
+                    g3[0] = r0;
+                }
             }
             // Z_0 = X_0 + i X_1, Z_1 = X_2 (+ i 0)
             float2 *z0 = seq, *z1 = seq + 1;
@@ -892,8 +998,18 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
                 const int n = lane + 64 * t;
                 if (n < Nr) {
                     float2 a0 = seq[2 * n], a1 = seq[2 * n + 1];
-                    seq[2 * n] = make_float2(kv[t] * a0.x, kv[t] * a0.y);
-                    seq[2 * n + 1] = make_float2(kv[t] * a1.x, 0.f);
+                    if (eprt) {
+                        // PRT_3D (PRT_3D.cpp:17-34): relabel, K, back
+                        const float *xp = xbase + (int64_t)n * 100 + ip;
+                        const float X0 = xp[0], X1 = xp[25], X2 = xp[50], X3 = xp[75];
+                        const float u0 = kv[t] * (X0 * a0.x + X1 * a1.x), u1 = kv[t] * (X0 * a0.y + X2 * a1.x);
+                        const float u2 = kv[t] * (X3 * a1.x);
+                        seq[2 * n] = make_float2(X0 * u0, X0 * u1);
+                        seq[2 * n + 1] = make_float2(X1 * u0 + X2 * u1 + X3 * u2, 0.f);
+                    } else {
+                        seq[2 * n] = make_float2(kv[t] * a0.x, kv[t] * a0.y);
+                        seq[2 * n + 1] = make_float2(kv[t] * a1.x, 0.f);
+                    }
                 }
             }
             wave_lds_fence();
@@ -909,9 +1025,14 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
                 if (al >= M || (nyq && al == Nu)) continue;
                 float2 za = seq[2 * al], zb = seq[2 * (al == 0 ? 0 : Nr - al)];
                 float2 ya = seq[2 * al + 1], yb = seq[2 * (al == 0 ? 0 : Nr - al) + 1];
-                const float2 sg[3] = {make_float2(h * (za.x + zb.x), h * (za.y - zb.y)),
-                                      make_float2(h * (za.y + zb.y), h * (zb.x - za.x)),
-                                      make_float2(h * (ya.x + yb.x), h * (ya.y - yb.y))};
+                float2 sg[3] = {make_float2(h * (za.x + zb.x), h * (za.y - zb.y)),
+                                make_float2(h * (za.y + zb.y), h * (zb.x - za.x)),
+                                make_float2(h * (ya.x + yb.x), h * (ya.y - yb.y))};
+                if (eprt) {
+                    const float2 r0 = cadd(cscale(tc1, sg[0]), cscale(ts1, sg[2]));
+                    sg[2] = csub(cscale(tc1, sg[2]), cscale(ts1, sg[0]));
+                    sg[0] = r0;
+                }
                 float sn = 0.f, cs = 1.f;
                 if (al > 0) sincosf((float)al * rphi, &sn, &cs);
 #pragma unroll
@@ -936,6 +1057,11 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
                 float2 s0 = make_float2(h * (za.x + zb.x), h * (za.y - zb.y));
                 float2 s1 = make_float2(h * (za.y + zb.y), h * (zb.x - za.x));
                 float2 s2 = make_float2(h * (ya.x + yb.x), h * (ya.y - yb.y));
+                if (eprt) {
+                    const float2 r0 = cadd(cscale(tc1, s0), cscale(ts1, s2));
+                    s2 = csub(cscale(tc1, s2), cscale(ts1, s0));
+                    s0 = r0;
+                }
                 float2 gg = cimul(-(float)al, s1);
                 X = cadd(cscale(g.dzdeta, s0), cscale(g.dsdeta, s2));
                 Yk[s] = cadd(cscale(g.dzdxii, s0), cscale(g.dsdxii, s2));
@@ -1007,11 +1133,11 @@ __global__ __launch_bounds__(W3D_THREADS) void fluid3d_kernel(Elem3DArgs a) {
 // Launch of one 3-D bucket of class S (defined, one class per object file, in
 // elem3d_launch.hip): phase 0 solid (material, TI, attenuation kind), 1 fluid.
 template <int S>
-void launchElem3D(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, size_t lds, hipStream_t s);
+void launchElem3D(const Elem3DArgs &a, int phase, int mat, bool tiso, int att, bool prt, size_t lds, hipStream_t s);
 
 // stations in 3-D fluid elements, one workgroup per station (a.nelem of them)
 template <int S>
-void launchRecFluid3D(const Elem3DArgs &a, size_t lds, hipStream_t s);
+void launchRecFluid3D(const Elem3DArgs &a, bool prt, size_t lds, hipStream_t s);
 
 // The measured FFT planner's probe: every wave of a full grid of 5-wave
 // workgroups runs `reps` C2R + R2C pairs of length N on three sequences with

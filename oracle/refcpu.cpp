@@ -184,8 +184,9 @@ struct Ref {
         RMat dsdxii, dsdeta, dzdxii, dzdeta, inv_s;
         RMat sin1t, cos1t, sin2t, cos2t;
         std::vector<Real> matv;  // as in the descriptor (Real)
-        bool prt = false;        // PRT_1D (Relabelling::createPRT)
-        std::array<Real, 100> X; // X0..X3 of the 25 points
+        bool prt = false;        // PRT_1D or PRT_3D (Relabelling::createPRT)
+        std::array<Real, 100> X; // PRT_1D: X0..X3 of the 25 points
+        std::vector<Real> X3;    // PRT_3D: [Nr][4][25]
         // attenuation
         std::vector<Real> dkappa3, dmu, dmu2;
         std::vector<C> stressR1;                // 1-D: [alpha][6][np]
@@ -249,6 +250,7 @@ struct Ref {
         std::vector<C> flatC;     // [alpha][150]
         std::vector<Real> flatR;  // [row][150]
         std::vector<Real> flatS;  // [row][150]
+        std::vector<Real> flat9;  // [row][225] (PRT_3D)
         std::vector<C> colC;
         std::vector<Real> colR;
         std::vector<C> wa, wb;
@@ -359,10 +361,17 @@ struct Ref {
             }
             el.matv.assign(d->mat_pool + d->mat_off[e], d->mat_pool + d->mat_off[e + 1]);
             if (d->prt_off && d->prt_off[e + 1] > d->prt_off[e]) {
-                if (d->prt_off[e + 1] - d->prt_off[e] != 100 || el.is3D)
-                    throw std::runtime_error("PRT_3D || not restated by this oracle.");
+                const int64_t n = d->prt_off[e + 1] - d->prt_off[e];
                 el.prt = true;
-                for (int i = 0; i < 100; i++) el.X[i] = (Real)d->prt_pool[d->prt_off[e] + i];
+                if (!el.is3D) {
+                    if (n != 100) throw std::runtime_error("PRT_1D || Incompatible size.");
+                    for (int i = 0; i < 100; i++) el.X[i] = (Real)d->prt_pool[d->prt_off[e] + i];
+                } else {
+                    if (n != 100 && n != 100 * (int64_t)el.nr) throw std::runtime_error("PRT_3D || Incompatible size.");
+                    for (int r = 0; r < el.nr; r++)
+                        for (int i = 0; i < 100; i++)
+                            el.X3.push_back((Real)d->prt_pool[d->prt_off[e] + (n == 100 ? 0 : 100 * r) + i]);
+                }
             }
             if (el.att != AXG_ATT_NONE) {
                 int np = el.att == AXG_ATT_CG4 ? 4 : 25;
@@ -1118,6 +1127,7 @@ struct Ref {
         if ((int)w.flatR.size() < nr * 150) {
             w.flatR.resize((size_t)nr * 150);
             w.flatS.resize((size_t)nr * 150);
+            w.flat9.resize((size_t)nr * 225);
         }
         if ((int)w.colC.size() < nr) {
             w.colC.resize(nr);
@@ -1136,6 +1146,71 @@ struct Ref {
             for (int a = 0; a <= el.maxNu; a++)
                 for (int c = 0; c < 3; c++)
                     w.displ[a][c][k] = (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0);
+        }
+        if (el.prt && el.is3D) {
+            // SolidElement::displToStiff with PRT_3D (SolidElement.cpp:404-441):
+            // 9 columns to the slices, PRT_3D per sample (PRT_3D.cpp:36-80)
+            grad9(el, w.displ, w.strain9, Nu, nyq);
+            rot9(el, w.strain9, Nu, -one);
+            for (int c = 0; c < 9; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int a = 0; a <= Nu; a++) w.colC[a] = w.strain9[a][c][k];
+                    c2r<Real>(Nr, w.colC.data(), w.colR.data(), w.wa, w.wb);
+                    for (int r = 0; r < Nr; r++) w.flat9[r * 225 + c * 25 + k] = w.colR[r];
+                }
+            for (int r = 0; r < Nr; r++)
+                for (int k = 0; k < 25; k++) {
+                    const Real *xr = &el.X3[(size_t)r * 100];
+                    const Real X0 = xr[k], X1 = xr[25 + k], X2 = xr[50 + k], X3 = xr[75 + k];
+                    auto g = [&](int c) { return w.flat9[r * 225 + c * 25 + k]; };
+                    Real e[6], st[6];
+                    e[0] = X0 * g(0) + X1 * g(2);
+                    e[1] = X0 * g(4) + X2 * g(5);
+                    e[2] = X3 * g(8);
+                    e[3] = X0 * g(7) + X2 * g(8) + X3 * g(5);
+                    e[4] = X0 * g(6) + X1 * g(8) + X3 * g(2);
+                    e[5] = X0 * (g(3) + g(1)) + X1 * g(5) + X2 * g(2);
+                    hooke<Real>(el, el.matv.data() + (size_t)r * 25, Nr * 25, k, e, st);
+                    for (int c = 0; c < 6; c++) {
+                        w.flatR[r * 150 + c * 25 + k] = e[c];
+                        w.flatS[r * 150 + c * 25 + k] = st[c];
+                    }
+                }
+            if (el.att != AXG_ATT_NONE) att3D(el, w.flatS.data(), w.flatR.data(), Nr);
+            for (int r = 0; r < Nr; r++)
+                for (int k = 0; k < 25; k++) {
+                    const Real *xr = &el.X3[(size_t)r * 100];
+                    const Real X0 = xr[k], X1 = xr[25 + k], X2 = xr[50 + k], X3 = xr[75 + k];
+                    auto t = [&](int c) { return w.flatS[r * 150 + c * 25 + k]; };
+                    Real o[9];
+                    o[0] = X0 * t(0);
+                    o[1] = X0 * t(5);
+                    o[2] = X1 * t(0) + X3 * t(4) + X2 * t(5);
+                    o[3] = o[1];
+                    o[4] = X0 * t(1);
+                    o[5] = X2 * t(1) + X3 * t(3) + X1 * t(5);
+                    o[6] = X0 * t(4);
+                    o[7] = X0 * t(3);
+                    o[8] = X3 * t(2) + X2 * t(3) + X1 * t(4);
+                    for (int c = 0; c < 9; c++) w.flat9[r * 225 + c * 25 + k] = o[c];
+                }
+            for (int c = 0; c < 9; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int r = 0; r < Nr; r++) w.colR[r] = w.flat9[r * 225 + c * 25 + k];
+                    r2c<Real>(Nr, w.colR.data(), w.colC.data(), w.wa, w.wb);
+                    for (int a = 0; a <= Nu; a++) w.stress9[a][c][k] = w.colC[a];
+                }
+            rot9(el, w.stress9, Nu, one);
+            quad9(el, w.stiff, w.stress9, Nu, nyq);
+            for (int k = 0; k < 25; k++) {
+                SolidPt &p = S[el.pts[k]];
+                int pn = (p.nr % 2 == 0) ? 1 : 0;
+                for (int a = 0; a <= p.nu - pn; a++)
+                    for (int c = 0; c < 3; c++) p.stiff[a * 3 + c] -= w.stiff[a][c][k];
+                if (pn)
+                    for (int c = 0; c < 3; c++) p.stiff[p.nu * 3 + c] = C(0, 0);
+            }
+            return;
         }
         if (el.prt) {
             // SolidElement::displToStiff with PRT (SolidElement.cpp:404-441)
@@ -1273,6 +1348,48 @@ struct Ref {
             for (int a = 0; a <= Nu; a++)
                 for (int c = 0; c < 3; c++)
                     for (int k = 0; k < 25; k++) w.fstress[a][c][k] = el.matv[k] * w.fstrain[a][c][k];
+        } else if (el.prt) {
+            // FluidElement::displToStiff with PRT_3D (FluidElement.cpp:333-355):
+            // CrdTransTIsoFluid, the 3 columns to the slices, PRT_3D (PRT_3D.cpp:17-34)
+            // with K per sample, back
+            for (int a = 0; a <= Nu; a++)
+                for (int k = 0; k < 25; k++) {
+                    auto &e = w.fstrain[a];
+                    C r0 = el.cos1t[k] * e[0][k] - el.sin1t[k] * e[2][k];
+                    e[2][k] = el.cos1t[k] * e[2][k] + el.sin1t[k] * e[0][k];
+                    e[0][k] = r0;
+                }
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int a = 0; a <= Nu; a++) w.colC[a] = w.fstrain[a][c][k];
+                    c2r<Real>(Nr, w.colC.data(), w.colR.data(), w.wa, w.wb);
+                    for (int r = 0; r < Nr; r++) w.flatR[r * 150 + c * 25 + k] = w.colR[r];
+                }
+            for (int r = 0; r < Nr; r++)
+                for (int k = 0; k < 25; k++) {
+                    const Real *xr = &el.X3[(size_t)r * 100];
+                    const Real X0 = xr[k], X1 = xr[25 + k], X2 = xr[50 + k], X3 = xr[75 + k];
+                    Real *f = &w.flatR[r * 150];
+                    const Real K = el.matv[(size_t)r * 25 + k];
+                    const Real u0 = K * (X0 * f[k] + X1 * f[50 + k]), u1 = K * (X0 * f[25 + k] + X2 * f[50 + k]);
+                    const Real u2 = K * (X3 * f[50 + k]);
+                    f[k] = X0 * u0;
+                    f[25 + k] = X0 * u1;
+                    f[50 + k] = X1 * u0 + X2 * u1 + X3 * u2;
+                }
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < 25; k++) {
+                    for (int r = 0; r < Nr; r++) w.colR[r] = w.flatR[r * 150 + c * 25 + k];
+                    r2c<Real>(Nr, w.colR.data(), w.colC.data(), w.wa, w.wb);
+                    for (int a = 0; a <= Nu; a++) w.fstress[a][c][k] = w.colC[a];
+                }
+            for (int a = 0; a <= Nu; a++)
+                for (int k = 0; k < 25; k++) {
+                    auto &o = w.fstress[a];
+                    C r0 = el.cos1t[k] * o[0][k] + el.sin1t[k] * o[2][k];
+                    o[2][k] = el.cos1t[k] * o[2][k] - el.sin1t[k] * o[0][k];
+                    o[0][k] = r0;
+                }
         } else {
             // FieldFFT (N3) + Acoustic3D::strainToStress (acoustic/Acoustic3D.cpp:9-16)
             for (int c = 0; c < 3; c++)

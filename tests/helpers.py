@@ -216,14 +216,16 @@ class WithPRT:
         self.mesh = mesh
         d = abi.MeshDesc.from_buffer_copy(mesh.d)
         ne = mesh.d.nelem
-        X = np.asarray(X, dtype=np.float32).reshape(len(elems), 100)
+        # one block per element: [4][25] (PRT_1D) or [rows][4][25] (PRT_3D)
+        blocks = [np.asarray(x, dtype=np.float32).ravel() for x in X]
         off = np.zeros(ne + 1, dtype=np.int64)
         cnt = np.zeros(ne, dtype=np.int64)
-        cnt[np.asarray(elems, dtype=np.int64)] = 100
+        for e, b in zip(elems, blocks):
+            cnt[e] = b.size
         off[1:] = np.cumsum(cnt)
         pool = np.zeros(max(int(off[-1]), 1), dtype=np.float32)
-        for i, e in enumerate(elems):
-            pool[off[e]:off[e] + 100] = X[i]
+        for e, b in zip(elems, blocks):
+            pool[off[e]:off[e] + b.size] = b
         flags = mesh.array("elem_flags").copy()
         flags[np.asarray(elems, dtype=np.int64)] |= abi.EL_TISO
         self._keep = [off, pool, flags]
@@ -250,13 +252,13 @@ class WithPRT:
         return self.mesh.array(name)
 
 
-def random_prt(n, seed, amp=0.1):
+def random_prt(n, seed, amp=0.1, rows=1):
     """X0..X3 of n elements: X0, X3 near 1, X1, X2 small (a smooth undulation's
-    inverse Jacobian factors), [n][4][25]."""
+    inverse Jacobian factors), [n][4][25], or [n][rows][4][25] per slice."""
     rng = np.random.default_rng(seed)
-    X = np.empty((n, 4, 25), dtype=np.float32)
-    X[:, 0] = 1. + amp * rng.uniform(-1, 1, (n, 25))
-    X[:, 1] = amp * rng.uniform(-1, 1, (n, 25))
-    X[:, 2] = amp * rng.uniform(-1, 1, (n, 25))
-    X[:, 3] = 1. + amp * rng.uniform(-1, 1, (n, 25))
-    return X
+    X = np.empty((n, rows, 4, 25), dtype=np.float32)
+    X[:, :, 0] = 1. + amp * rng.uniform(-1, 1, (n, rows, 25))
+    X[:, :, 1] = amp * rng.uniform(-1, 1, (n, rows, 25))
+    X[:, :, 2] = amp * rng.uniform(-1, 1, (n, rows, 25))
+    X[:, :, 3] = 1. + amp * rng.uniform(-1, 1, (n, rows, 25))
+    return X[:, 0] if rows == 1 else X

@@ -293,3 +293,98 @@ def test_topography_seismograms_gpu(fluid_layers, refcpu):
         s.append(d.seismograms())
         d.close()
     assert rel_l2(s[1], s[0]) < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# PRT_3D: relabelling on 3-D elements, X per slice (PRT_3D.cpp)
+# ---------------------------------------------------------------------------
+
+def _mesh3d(layers, material=_abi.MAT_TRANSVERSELY_ISO, nstep=10, fluid3d=True, **kw):
+    cfg = ax.shell_config(0, layers=layers, nelem_theta=kw.pop("nelem_theta", 8), material=material, nu_mode=1,
+                          nu_min=2, nu_max=9, nstep=nstep, nrec=6, dv_amp=0.05, dv_fluid=1 if fluid3d else 0,
+                          nthread=1, **kw)
+    return ax.ShellMesh(cfg)
+
+
+def _x3d(m, elems, seed=None, amp=0.1):
+    """Per-slice X of each element: identity, or random near identity."""
+    nr = m.array("point_nr")
+    ep = m.array("elem_points").reshape(-1, 25)
+    out = []
+    for i, e in enumerate(elems):
+        rows = int(nr[ep[e]].max())
+        if seed is None:
+            X = np.zeros((rows, 4, 25), np.float32)
+            X[:, 0] = 1.
+            X[:, 3] = 1.
+        else:
+            X = random_prt(1, seed + i, amp, rows=max(rows, 2))[0][:rows]
+        out.append(X)
+    return out
+
+
+@pytest.mark.parametrize("material", [_abi.MAT_ISOTROPIC, _abi.MAT_TRANSVERSELY_ISO])
+def test_identity_prt3d_is_the_plain_path(fluid_layers, refcpu, material):
+    m = _mesh3d(fluid_layers, material=material)
+    fl = m.array("elem_flags")
+    el = list(np.flatnonzero(fl & _abi.EL_3D))
+    assert (fl[el] & _abi.EL_FLUID).any() and (~fl[el] & _abi.EL_FLUID).any()
+    mp = WithPRT(m, el, _x3d(m, el))
+    us = random_field(m, _abi.FIELD_SOLID_DISPL, 3, scale=1.)
+    uf = random_field(m, _abi.FIELD_FLUID_DISPL, 4, scale=1.)
+    a = _stiff(refcpu.RefDomain(m, double=True), m, us, uf)
+    b = _stiff(refcpu.RefDomain(mp, double=True), mp, us, uf)
+    tol = 1e-7 if material == _abi.MAT_ISOTROPIC else 1e-12
+    assert rel_l2(b[0], a[0]) < tol and rel_l2(b[1], a[1]) < 1e-7
+
+
+def test_prt3d_operator_self_adjoint(small_layers, refcpu):
+    m = _mesh3d(small_layers, material=_abi.MAT_ANISOTROPIC)
+    el = list(range(m.d.nelem))
+    mp = WithPRT(m, el, _x3d(m, el, seed=30))
+    dom = refcpu.RefDomain(mp, double=True)
+    w = _weights(m)
+    u = random_field(m, _abi.FIELD_SOLID_DISPL, 1, scale=1.)
+    v = random_field(m, _abi.FIELD_SOLID_DISPL, 2, scale=1.)
+    Ku, Kv = -_stiff(dom, mp, u)[0], -_stiff(dom, mp, v)[0]
+    assert abs(np.dot(w * u, Kv) - np.dot(w * v, Ku)) < 1e-10 * np.linalg.norm(Ku) * np.linalg.norm(v)
+    assert np.dot(w * u, Ku) > 0
+    K0 = -_stiff(refcpu.RefDomain(m, double=True), m, u)[0]
+    assert rel_l2(Ku, K0) > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("att", [_abi.ATT_NONE, _abi.ATT_FULL])
+def test_prt3d_operator_gpu(fluid_layers, refcpu, att):
+    print("case prt3d operator att=%d" % att, flush=True)
+    m = _mesh3d(fluid_layers, attenuation=att)
+    fl = m.array("elem_flags")
+    el = list(np.flatnonzero(fl & _abi.EL_3D))[::2]
+    mp = WithPRT(m, el, _x3d(m, el, seed=40))
+    g = ax.Domain(mp)
+    r = refcpu.RefDomain(mp, double=True)
+    us = random_field(m, _abi.FIELD_SOLID_DISPL, 11)
+    uf = random_field(m, _abi.FIELD_FLUID_DISPL, 12)
+    sg, fg = _stiff(g, mp, us, uf)
+    sr, fr = _stiff(r, mp, us, uf)
+    g.close()
+    assert rel_l2(sg, sr) < 1e-5 and rel_l2(fg, fr) < 1e-5
+
+
+@pytest.mark.gpu
+def test_prt3d_seismograms_gpu(fluid_layers, refcpu):
+    print("case prt3d seismograms", flush=True)
+    m = _mesh3d(fluid_layers, nstep=300, nelem_theta=10, rec_depth=4000e3)
+    fl = m.array("elem_flags")
+    el = list(np.flatnonzero(fl & _abi.EL_3D))
+    mp = WithPRT(m, el, _x3d(m, el, seed=50, amp=0.05))
+    g = ax.Domain(mp)
+    g.resetZero()
+    g.run(1, 300, 100, use_graph=True)
+    sg = g.seismograms()
+    g.close()
+    r = refcpu.RefDomain(mp, double=False)
+    r.run(1, 300, 100)
+    sr = r.seismograms()
+    assert np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4
