@@ -689,7 +689,8 @@ struct Builder {
 
     // Relabelling::addUndulation / formGradientUndulation / getStiffX /
     // getStiffJacobian (Relabelling.cpp:29-109, 190-253) for an undulation that
-    // is the same on every azimuth: deltaR at the points, its gradient in the
+    // is the same on every azimuth (PRT_1D on 1-D elements, PRT_3D with equal
+    // slices on elements whose material varies): deltaR at the points, its gradient in the
     // meridian (the element's own operator at alpha = 0), rotated to (R, Z),
     // then J0 = 1 + dZ / Z (1 + dZ/dZ at the centre), J1 = dZ/dR, J3 = 1 + dZ/dZ,
     // X = (1 / J0, -J1 / (J0 J3), 0, 1 / J3), Jacobian J0^2 J3 (J3^3 at the centre)
@@ -697,8 +698,6 @@ struct Builder {
         for (auto &q : quads)
             for (int k = 0; k < nP * nP; k++) q.jac[k] = 1.;
         if (c.undulation == 0) return;
-        if (c.dv_amp > 0.)
-            throw std::runtime_error("PRT_3D || relabelling of a 3-D model is not supported by this build.");
         if (c.undulation != 1 && c.undulation != 2) throw std::runtime_error("Geometric3D || unknown undulation.");
         const double R = rnode.back();
         auto deltaR = [&](double r, double th) {
@@ -1347,8 +1346,10 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         // Relabelling::createPRT (Relabelling.cpp:153-170): PRT_1D, X0..X3 of the points
         if (B.c.undulation) {
             if (m.prt_off.empty()) m.prt_off.assign(NQ + 1, 0);
-            m.prt_off[e + 1] = m.prt_off[e] + (q.prt ? 100 : 0);
-            if (q.prt)
+            // PRT_3D on a 3-D element: the same X on each of its Nr slices
+            const int rows = q.prt ? (o.is3D ? q.nr : 1) : 0;
+            m.prt_off[e + 1] = m.prt_off[e] + 100 * rows;
+            for (int r = 0; r < rows; r++)
                 for (int t = 0; t < 4; t++)
                     for (int k = 0; k < 25; k++) m.prt_pool.push_back((float)q.X[t][k]);
         }
@@ -1701,7 +1702,8 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             m.rec_theta.push_back((float)theta);
             // strain / curl at solid stations; their element rotates to (R, T, Z)
             // (Receiver::release, Receiver.cpp:47-54)
-            const int dump = q.fluid ? 0 : (c.rec_dump & 3);
+            // (not in a PRT_3D element, whose strain this build does not compute)
+            const int dump = (q.fluid || (q.prt && out[found].is3D)) ? 0 : (c.rec_dump & 3);
             m.rec_dump.push_back(dump);
             if (dump) m.elem_flags[found] |= AXG_EL_TISO;
         }

@@ -388,3 +388,44 @@ def test_prt3d_seismograms_gpu(fluid_layers, refcpu):
     sr = r.seismograms()
     assert np.abs(sr).max() > 0
     assert rel_l2(sg, sr) < 1e-4
+
+
+def test_builder_prt3d_on_a_3d_model(small_layers, refcpu):
+    """An undulated 3-D model: relabelled 3-D elements carry PRT_3D rows (the
+    same X on every slice of an azimuth-independent undulation), which equal
+    the PRT_1D rows of the same undulation on the 1-D model."""
+    kw = dict(layers=small_layers, nelem_theta=8, nu_mode=1, nu_min=2, nu_max=6, nstep=10, nthread=1,
+              undulation=2, undulation_amp=8e3, undulation_order=3, undulation_depth=250e3)
+    m3 = ax.ShellMesh(ax.shell_config(0, dv_amp=0.05, **kw))
+    m1 = ax.ShellMesh(ax.shell_config(0, **kw))
+    f3, off3, off1 = m3.array("elem_flags"), m3.array("prt_off"), m1.array("prt_off")
+    nr = m3.array("point_nr")[m3.array("elem_points").reshape(-1, 25)].max(1)
+    rows = np.diff(off3) // 100
+    has = rows > 0
+    assert (has == (np.diff(off1) > 0)).all() and (f3[has] & _abi.EL_3D).any()
+    assert (rows[has & ((f3 & _abi.EL_3D) != 0)] == nr[has & ((f3 & _abi.EL_3D) != 0)]).all()
+    p3, p1 = m3.array("prt_pool"), m1.array("prt_pool")
+    for e in np.flatnonzero(has)[:10]:
+        x1 = p1[off1[e]:off1[e] + 100]
+        x3 = p3[off3[e]:off3[e + 1]].reshape(-1, 100)
+        assert (x3 == x1).all()
+
+
+@pytest.mark.gpu
+def test_builder_prt3d_seismograms_gpu(fluid_layers, refcpu):
+    print("case builder prt3d seismograms", flush=True)
+    m = ax.ShellMesh(ax.shell_config(0, layers=fluid_layers, nelem_theta=10, nu_mode=1, nu_min=2, nu_max=8,
+                                     nstep=300, nrec=6, dv_amp=0.05, dv_fluid=1, material=_abi.MAT_TRANSVERSELY_ISO,
+                                     undulation=2, undulation_amp=20e3, undulation_order=2, undulation_depth=5500e3))
+    rows = np.diff(m.array("prt_off")) // 100
+    assert (rows > 1).any()
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, 300, 100, use_graph=True)
+    sg = g.seismograms()
+    g.close()
+    r = refcpu.RefDomain(m, double=False)
+    r.run(1, 300, 100)
+    sr = r.seismograms()
+    assert np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4
