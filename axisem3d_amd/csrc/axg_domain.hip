@@ -181,6 +181,11 @@ struct axg_ctx {
     DevBuf<int32_t> d_sc_idx;
     DevBuf<int2> d_sc_slot;
     DevBuf<float> d_strain, d_curl;
+    // fluid stations that dump strain: block -> station, strain slot, scratch
+    int32_t nrec_fsc = 0;
+    int64_t fsc_stride = 0;
+    DevBuf<int32_t> d_fsc_idx, d_fsc_slot;
+    DevBuf<float2> d_fsc_scratch;
     DevBuf<float> d_rec_w, d_rec_phi, d_rec_cost, d_rec_sint, d_seis;
     int32_t rec_rtz = 1;
     // stability
@@ -979,6 +984,13 @@ void doRecord(axg_ctx *c, int tstep, const int *tstep_ptr = nullptr) {
                            c->d_ep_amax.p, c->d_elem_nr.p, c->d_elem_flags.p, c->d_geom.p, c->d_tiso.p,
                            c->d_prt_pool.p, c->d_prt_pool.p ? c->d_prt_off.p : nullptr, c->S.displ.p, tstep,
                            tstep_ptr, c->nstrain, c->ncurl, c->d_strain.p, c->d_curl.p);
+    if (c->nrec_fsc)
+        hipLaunchKernelGGL(record_fluid_strain_kernel, dim3(c->nrec_fsc), dim3(256), 0, c->stream, c->nrec_fsc,
+                           c->d_fsc_idx.p, c->d_fsc_slot.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p,
+                           c->d_ep_row.p, c->d_ep_amax.p, c->d_elem_nr.p, c->d_elem_flags.p, c->d_geom.p, c->d_tiso.p,
+                           c->d_mat_pool.p, c->d_mat_off.p, c->d_prt_pool.p,
+                           c->d_prt_pool.p ? c->d_prt_off.p : nullptr, c->F.displ.p, c->d_fsc_scratch.p,
+                           c->fsc_stride, tstep, tstep_ptr, c->nstrain, c->d_strain.p);
     if (c->nrec_fluid)
         hipLaunchKernelGGL(record_fluid_kernel, dim3(c->nrec_fluid), dim3(256), 0, c->stream, c->nrec_fluid, c->nrec,
                            c->d_rec_idx_f.p, c->d_rec_elem.p, c->d_rec_w.p, c->d_rec_phi.p, c->d_rec_cost.p,
@@ -1402,24 +1414,45 @@ void createCtx(const axg_mesh_desc *d, int device, axg_ctx *c, const BatchInfo *
         c->d_seis.alloc((size_t)std::max(1, c->nstep) * c->nrec * 3);
         c->d_seis.zero(s);
         if (d->rec_dump) {
-            std::vector<int32_t> sc;
+            std::vector<int32_t> sc, fsc, fslot;
             std::vector<int2> slot;
+            int maxM = 1;
             for (int i = 0; i < c->nrec; i++) {
                 const int f = d->rec_dump[i];
                 if (!(f & 3)) continue;
-                if (d->elem_flags[d->rec_elem[i]] & AXG_EL_FLUID)
-                    throw std::runtime_error("FluidElement::computeStrain || strain at a fluid station is not "
-                                             "supported by this build.");
-                if (c->elem_prt[d->rec_elem[i]] == 2)
+                const int e = d->rec_elem[i];
+                if (c->elem_prt[e] == 2)
                     throw std::runtime_error("SolidElement::computeStrain || strain in a PRT_3D element is not "
                                              "supported by this build.");
+                if (d->elem_flags[e] & AXG_EL_FLUID) {
+                    if (d->elem_flags[e] & AXG_EL_3D)
+                        throw std::runtime_error("FluidElement::computeStrain || strain in a 3-D fluid element is "
+                                                 "not supported by this build.");
+                    if (f & 1) {
+                        fsc.push_back(i);
+                        fslot.push_back(c->nstrain++);
+                        maxM = std::max(maxM, c->elem_nr[e] / 2 + 1);
+                    }
+                    // no curl in a fluid: its slot stays zero (FluidElement.cpp:308-311)
+                    if (f & 2) c->ncurl++;
+                    continue;
+                }
                 sc.push_back(i);
                 slot.push_back(make_int2((f & 1) ? c->nstrain++ : -1, (f & 2) ? c->ncurl++ : -1));
             }
             c->nrec_sc = (int32_t)sc.size();
-            if (c->nrec_sc) {
-                c->d_sc_idx.upload(sc, s);
-                c->d_sc_slot.upload(slot, s);
+            c->nrec_fsc = (int32_t)fsc.size();
+            if (c->nrec_sc || c->nrec_fsc || c->ncurl) {
+                if (c->nrec_sc) {
+                    c->d_sc_idx.upload(sc, s);
+                    c->d_sc_slot.upload(slot, s);
+                }
+                if (c->nrec_fsc) {
+                    c->d_fsc_idx.upload(fsc, s);
+                    c->d_fsc_slot.upload(fslot, s);
+                    c->fsc_stride = (int64_t)maxM * 75;
+                    c->d_fsc_scratch.alloc((size_t)c->fsc_stride * c->nrec_fsc);
+                }
                 c->d_strain.alloc((size_t)std::max(1, c->nstep) * std::max(1, c->nstrain) * 6);
                 c->d_curl.alloc((size_t)std::max(1, c->nstep) * std::max(1, c->ncurl) * 3);
                 c->d_strain.zero(s);
@@ -1886,7 +1919,7 @@ int axg_reset_zero(axg_ctx *c) {
         c->d_att_memvar.zero(c->stream);
         c->d_att_stressR.zero(c->stream);
         c->d_seis.zero(c->stream);
-        if (c->nrec_sc) {
+        if (c->d_strain.p) {
             c->d_strain.zero(c->stream);
             c->d_curl.zero(c->stream);
         }

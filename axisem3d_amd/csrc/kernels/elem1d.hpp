@@ -327,6 +327,145 @@ __global__ __launch_bounds__(256) void record_strain_kernel(
     }
 }
 
+// FluidElement::computeStrain (FluidElement.cpp:219-306) for stations in 1-D
+// fluid elements: the fluid "stress" K grad(chi) (with the rotation and PRT_1D
+// of a relabelled element, as the stiffness path) of every point and mode goes
+// to the station's scratch, then Gradient::computeGrad6 of that 3-component
+// field (grad9 and PRT_1D's sphericalToUndulated on a relabelled element),
+// rotated to (R, T, Z), summed at the station with the fluid weights (w /
+// iFact).  No curl in a fluid (FluidElement.cpp:308-311).
+__global__ __launch_bounds__(256) void record_fluid_strain_kernel(
+    int32_t n, const int32_t *__restrict__ rec_idx, const int32_t *__restrict__ slots,
+    const int32_t *__restrict__ rec_elem, const float *__restrict__ rec_w, const float *__restrict__ rec_phi,
+    const int32_t *__restrict__ ep_row, const int32_t *__restrict__ ep_amax, const int32_t *__restrict__ elem_nr,
+    const int32_t *__restrict__ elem_flags, const float *__restrict__ geom, const float *__restrict__ tiso,
+    const float *__restrict__ mat_pool, const int64_t *__restrict__ mat_off, const float *__restrict__ prt_pool,
+    const int64_t *__restrict__ prt_off, const float2 *__restrict__ chi, float2 *__restrict__ scratch,
+    int64_t scratch_stride, int tstep, const int *__restrict__ tstep_ptr, int32_t nstrain,
+    float *__restrict__ strain_out) {
+    __shared__ float red[6][256];
+    if ((int)blockIdx.x >= n) return;
+    const int irec = rec_idx[blockIdx.x];
+    const int slot = slots[blockIdx.x];
+    if (tstep_ptr) tstep = *tstep_ptr;
+    const int64_t e = rec_elem[irec];
+    const int nr = elem_nr[e];
+    const int amaxe = nr / 2 - ((nr % 2 == 0) ? 1 : 0);
+    const bool axial = (elem_flags[e] & 2) != 0;
+    const bool prt = prt_off && prt_off[e + 1] > prt_off[e];
+    const float *Gx = cG[axial ? 1 : 0];
+    const float *Ge = cG[0];
+    const float *K = mat_pool + mat_off[e];
+    const float *tp = tiso + e * 100;
+    const float *xp = prt ? prt_pool + prt_off[e] : nullptr;
+    float2 *sig = scratch + (int64_t)blockIdx.x * scratch_stride;  // [alpha][25][3]
+    // ---- the fluid stress of every (mode, point)
+    for (int t = threadIdx.x; t < 25 * (amaxe + 1); t += blockDim.x) {
+        const int k = t % 25, al = t / 25;
+        const int i = k / 5, j = k - 5 * i;
+        auto X = [&](int p) {
+            return al <= ep_amax[e * 25 + p] ? chi[(int64_t)ep_row[e * 25 + p] + al] : czero();
+        };
+        float2 GU = czero(), UG = czero();
+        for (int m = 0; m < 5; m++) {
+            GU = cadd(GU, cscale(Gx[m * 5 + i], X(m * 5 + j)));
+            UG = cadd(UG, cscale(Ge[m * 5 + j], X(i * 5 + m)));
+        }
+        const PointGeom g = load_geom(geom, e, k);
+        float2 gr[3];
+        gr[0] = cadd(cscale(g.dzdeta, GU), cscale(g.dzdxii, UG));
+        gr[1] = cscale(g.inv_s, cimul((float)al, X(k)));
+        gr[2] = cadd(cscale(g.dsdeta, GU), cscale(g.dsdxii, UG));
+        if (axial && i == 0) gr[1] = cadd(gr[1], cscale(g.dzdeta, cimul((float)al, GU)));
+        if (prt) {
+            const float s1 = tp[k], c1 = tp[25 + k];
+            const float X0 = xp[k], X1 = xp[25 + k], X2 = xp[50 + k], X3 = xp[75 + k];
+            float2 r0 = csub(cscale(c1, gr[0]), cscale(s1, gr[2]));
+            float2 r2 = cadd(cscale(c1, gr[2]), cscale(s1, gr[0]));
+            float2 u0 = cadd(cscale(X0, r0), cscale(X1, r2));
+            float2 u1 = cadd(cscale(X0, gr[1]), cscale(X2, r2));
+            float2 u2 = cscale(X3, r2);
+            r2 = cadd(cadd(cscale(X1, u0), cscale(X2, u1)), cscale(X3, u2));
+            r0 = cscale(X0, u0);
+            gr[1] = cscale(X0, u1);
+            gr[0] = cadd(cscale(c1, r0), cscale(s1, r2));
+            gr[2] = csub(cscale(c1, r2), cscale(s1, r0));
+        }
+        for (int c = 0; c < 3; c++) {
+            float2 v = cscale(K[k], gr[c]);
+            if (al == 0) v.y = 0.f;  // the reference's alpha = 0 branch keeps the real part
+            sig[((int64_t)al * 25 + k) * 3 + c] = v;
+        }
+    }
+    __syncthreads();
+    // ---- strain of the stress field at the weighted points
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int t = threadIdx.x; t < 25 * (amaxe + 1); t += blockDim.x) {
+        const int k = t % 25, al = t / 25;
+        const float wt = rec_w[irec * 25 + k];
+        if (fabsf(wt) < 1e-10f) continue;
+        const int i = k / 5, j = k - 5 * i;
+        const float2 *sa = sig + (int64_t)al * 75;
+        float2 GU[3], UG[3];
+        for (int c = 0; c < 3; c++) {
+            GU[c] = UG[c] = czero();
+            for (int m = 0; m < 5; m++) {
+                GU[c] = cadd(GU[c], cscale(Gx[m * 5 + i], sa[(m * 5 + j) * 3 + c]));
+                UG[c] = cadd(UG[c], cscale(Ge[m * 5 + j], sa[(i * 5 + m) * 3 + c]));
+            }
+        }
+        const PointGeom g = load_geom(geom, e, k);
+        const float fa = (float)al;
+        const float2 u0 = sa[k * 3], u1 = sa[k * 3 + 1], u2 = sa[k * 3 + 2];
+        const float2 v0 = cadd(u0, cimul(fa, u1)), v1 = csub(cimul(fa, u0), u1), v2 = cimul(fa, u2);
+        float2 g9[9];
+        for (int c = 0; c < 3; c++) {
+            g9[3 * c + 0] = cadd(cscale(g.dzdeta, GU[c]), cscale(g.dzdxii, UG[c]));
+            g9[3 * c + 2] = cadd(cscale(g.dsdeta, GU[c]), cscale(g.dsdxii, UG[c]));
+        }
+        g9[1] = cscale(g.inv_s, v1);
+        g9[4] = cscale(g.inv_s, v0);
+        g9[7] = cscale(g.inv_s, v2);
+        if (axial && i == 0) {
+            g9[4] = cadd(g9[4], cscale(g.dzdeta, cadd(GU[0], cimul(fa, GU[1]))));
+            g9[1] = cadd(g9[1], cscale(g.dzdeta, csub(cimul(fa, GU[0]), GU[1])));
+            g9[7] = cadd(g9[7], cscale(g.dzdeta, cimul(fa, GU[2])));
+            if (al == 1) {
+                g9[4] = cadd(g9[4], cscale(g.dzdxii, cadd(UG[0], cimul(fa, UG[1]))));
+                g9[1] = cadd(g9[1], cscale(g.dzdxii, csub(cimul(fa, UG[0]), UG[1])));
+            }
+        }
+        tiso9(g9, tp[k], tp[25 + k], tp[50 + k], tp[75 + k], -1.f);
+        float2 v[6];
+        if (prt) {
+            prt_sph_und(g9, xp[k], xp[25 + k], xp[50 + k], xp[75 + k], v);
+        } else {
+            v[0] = g9[0];
+            v[1] = g9[4];
+            v[2] = g9[8];
+            v[3] = cadd(g9[5], g9[7]);
+            v[4] = cadd(g9[2], g9[6]);
+            v[5] = cadd(g9[1], g9[3]);
+        }
+        if (al == 0) {
+            for (int c = 0; c < 6; c++) acc[c] += wt * v[c].x;
+        } else {
+            float sn, cs;
+            sincosf(fa * rec_phi[irec], &sn, &cs);
+            for (int c = 0; c < 6; c++) acc[c] += wt * 2.f * (cs * v[c].x - sn * v[c].y);
+        }
+    }
+    for (int c = 0; c < 6; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 6; c++) red[c][threadIdx.x] += red[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; c++) strain_out[((int64_t)tstep * nstrain + slot) * 6 + c] = red[c][0];
+}
+
 template <int MAT>
 __device__ __forceinline__ void stress1d(const float *__restrict__ mp, int k, const float2 e[6], float2 s[6]) {
     if (MAT == 0) {

@@ -454,8 +454,8 @@ struct Ref {
             r.phi = (Real)d->rec_phi[i];
             r.theta = (double)d->rec_theta[i];
             if (d->rec_dump && (d->rec_dump[i] & 3)) {
-                if (d->elem_flags[r.elem] & AXG_EL_FLUID)
-                    throw std::runtime_error("FluidElement::computeStrain || not restated by this oracle.");
+                if ((d->elem_flags[r.elem] & AXG_EL_FLUID) && (d->elem_flags[r.elem] & AXG_EL_3D))
+                    throw std::runtime_error("FluidElement::computeStrain || 3-D fluid: not restated by this oracle.");
                 if (d->rec_dump[i] & 1) r.strain = nstrain++;
                 if (d->rec_dump[i] & 2) r.curl = ncurl++;
             }
@@ -1508,11 +1508,21 @@ struct Ref {
         Work &w = works[0];
         const int Nr = el.nr, Nu = el.nu, nyq = (Nr % 2 == 0) ? 1 : 0;
         ensureWork(w, Nu, Nr);
-        for (int k = 0; k < 25; k++) {
-            const SolidPt &p = S[el.pts[k]];
-            int pn = (p.nr % 2 == 0) ? 1 : 0;
-            for (int a = 0; a <= el.maxNu; a++)
-                for (int c = 0; c < 3; c++) w.displ[a][c][k] = (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0);
+        if (el.fluid) {
+            // FluidElement::computeStrain (FluidElement.cpp:219-306): the strain of
+            // the fluid stress field K grad(chi); no curl in a fluid
+            if (rc.strain < 0) return;
+            fluidGradStress(el, w);
+            for (int a = 0; a <= Nu; a++)
+                for (int c = 0; c < 3; c++)
+                    for (int k = 0; k < 25; k++) w.displ[a][c][k] = w.fstress[a][c][k];
+        } else {
+            for (int k = 0; k < 25; k++) {
+                const SolidPt &p = S[el.pts[k]];
+                int pn = (p.nr % 2 == 0) ? 1 : 0;
+                for (int a = 0; a <= el.maxNu; a++)
+                    for (int c = 0; c < 3; c++) w.displ[a][c][k] = (a <= p.nu - pn) ? p.displ[a * 3 + c] : C(0, 0);
+            }
         }
         C ii(zero, one);
         auto fsum = [&](auto &&val, int k) {
@@ -1539,7 +1549,7 @@ struct Ref {
                 o[c] = acc;
             }
         }
-        if (rc.curl >= 0) {
+        if (rc.curl >= 0 && !el.fluid) {
             grad9(el, w.displ, w.strain9, Nu, nyq);
             rot9(el, w.strain9, Nu, -one);
             if (el.prt) prtSphUnd9(el, w.strain9, Nu);

@@ -6,6 +6,9 @@ form (up to the spectral interpolation of the spherical geometry): a rigid
 translation has neither; a rigid rotation about the axis, u_phi = w s, has no
 strain and curl 2 w z_hat = (-2 w sin(theta), 0, 2 w cos(theta)) in (R, T, Z);
 a uniform dilatation u = e r has strain (e, e, e, 0, 0, 0) and no curl.
+Fluid stations (FluidElement::computeStrain) take the strain of the fluid
+stress field K grad(chi), summed with weights that the reference's own 1e-10
+threshold removes: zero, as in the reference.
 GPU (marked gpu): strain and curl traces against the oracle's, on a 3-D model
 and on a relabelled one.
 """
@@ -72,10 +75,25 @@ def test_strain_curl_of_known_fields(small_layers, refcpu):
         assert np.abs(st).max() < 1e-9 and np.abs(cu).max() < 1e-9
 
 
-def test_strain_curl_fluid_station_rejected(fluid_layers):
-    # stations in the fluid keep no strain flag; the descriptor with one is rejected
-    m = _mesh(fluid_layers, rec_depth=4000e3)
-    assert (m.array("rec_dump") == 0).all()
+def test_fluid_station_strain(fluid_layers, refcpu):
+    """FluidElement::computeStrain: the strain of the fluid stress field K
+    grad(chi) (FluidElement.cpp:219-306); no curl in a fluid.  It skips points
+    whose weight is below 1e-10 (tinyDouble), and a fluid station's weights
+    carry 1 / iFact (Receiver.cpp:107-114, ~1e-16 here), so the reference's
+    fluid-station strain is zero; restated as it is.  Stations in 3-D fluid
+    elements get no flag."""
+    from tests.helpers import random_field
+    m = _mesh(fluid_layers, rec_depth=4000e3, nstep=10)
+    assert (m.array("rec_dump") == 3).all()
+    d = refcpu.RefDomain(m, double=True)
+    d.set_field(_abi.FIELD_FLUID_DISPL, random_field(m, _abi.FIELD_FLUID_DISPL, 3, scale=1.))
+    d.record(0)
+    st, cu = d.strain_curl()
+    w = m.array("rec_weights")
+    assert np.abs(w).max() < 1e-10
+    assert st.shape[1] == m.d.nrec and np.abs(st).max() == 0 and np.abs(cu).max() == 0
+    m3 = _mesh(fluid_layers, rec_depth=4000e3, dv_amp=0.05, dv_fluid=1)
+    assert (m3.array("rec_dump") == 0).all()
 
 
 @pytest.mark.gpu
@@ -98,3 +116,21 @@ def test_strain_curl_traces_gpu(small_layers, refcpu, case):
     sr, cr = r.strain_curl()
     assert np.abs(sr).max() > 0 and np.abs(cr).max() > 0
     assert rel_l2(sg, sr) < 1e-4 and rel_l2(cg, cr) < 1e-4
+
+
+@pytest.mark.gpu
+def test_fluid_strain_traces_gpu(fluid_layers, refcpu):
+    print("case fluid strain", flush=True)
+    m = _mesh(fluid_layers, rec_depth=4000e3, nstep=200, undulation=2, undulation_amp=20e3, undulation_order=2,
+              undulation_depth=5500e3)
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, 200, 100, use_graph=True)
+    sg, cg = g.strain_curl()
+    g.close()
+    r = refcpu.RefDomain(m, double=False)
+    r.run(1, 200, 100)
+    sr, cr = r.strain_curl()
+    # both zero, as the reference (see test_fluid_station_strain); the solid
+    # stations' traces are compared in test_strain_curl_traces_gpu
+    assert np.abs(sr).max() == 0 and np.abs(sg).max() == 0 and np.abs(cg).max() == 0
