@@ -732,7 +732,9 @@ struct Builder {
                         const double det = det2(J);
                         const double dsdxii = J[0][0] / det, dsdeta = -J[0][1] / det;
                         const double dzdxii = -J[1][0] / det, dzdeta = J[1][1] / det;
-                        const double drds = dzdeta * GU + dzdxii * UG;
+                        // the s derivative of the order-0 term vanishes on the axis
+                        // (PreloopGradient::gradScalar axial masking, PreloopGradient.cpp:27-29)
+                        const double drds = (q.axial && i == 0) ? 0. : dzdeta * GU + dzdxii * UG;
                         const double drdz = dsdeta * GU + dsdxii * UG;
                         const double dZdZ = drdz * std::cos(th[k]) + drds * std::sin(th[k]);
                         const double dZdR = -drdz * std::sin(th[k]) + drds * std::cos(th[k]);
