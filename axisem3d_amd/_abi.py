@@ -72,6 +72,23 @@ class ShellLayer(C.Structure):
     ]
 
 
+VOL3D_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
+
+
+class Volumetric3D(C.Structure):
+    _fields_ = [
+        ("nprop", C.c_int32),
+        ("props", i32p),
+        ("ref_types", i32p),
+        ("fn", VOL3D_FN),
+        ("user", C.c_void_p),
+    ]
+
+
+PROP_VP, PROP_VS, PROP_RHO, PROP_VPV, PROP_VPH, PROP_VSV, PROP_VSH, PROP_ETA = range(8)
+REF_ABSOLUTE, REF_REFERENCE_1D, REF_REFERENCE_3D, REF_REFERENCE_PERT = range(4)
+
+
 class ShellConfig(C.Structure):
     _fields_ = [
         ("nelem_theta", C.c_int32),
@@ -120,6 +137,8 @@ class ShellConfig(C.Structure):
         ("undulation_amp", C.c_double),
         ("undulation_depth", C.c_double),
         ("rec_dump", C.c_int32),
+        ("nvol3d", C.c_int32),
+        ("vol3d", C.POINTER(Volumetric3D)),
     ]
 
 

@@ -960,35 +960,77 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     };
     std::vector<ElemOut> out(NQ);
 
+    // the properties vpv, vph, vsv, vsh, rho, eta of GLL point k of a quad at
+    // azimuth phi: the layer's 1-D values, the built-in random perturbation,
+    // then the volumetric models with their reference types (Material.cpp:145-209)
+    if (c.nvol3d < 0 || (c.nvol3d > 0 && !c.vol3d)) throw std::runtime_error("Volumetric3D || bad model list.");
+    const bool vol3d = c.nvol3d > 0;
+    auto sampleAt = [&](const QuadInfo &q, int k, double phi) {
+        const Layer &L = B.layers[q.layer];
+        const double s = q.crd[k][0], z = q.crd[k][1];
+        const double r = std::hypot(s, z);
+        const double vp = L.vp(r), vs = L.vs(r), rho = L.rho(r);
+        double dvs = 0., dvp = 0.;
+        if (threeD && (!q.fluid || c.dv_fluid)) {
+            const double x = s * std::cos(phi), y = s * std::sin(phi);
+            dvs = c.dv_amp * std::tanh(B.grfVs(x, y, z) / 2.);
+            dvp = c.dv_amp * std::tanh(B.grfVp(x, y, z) / 2.);
+        }
+        const bool ti = (!q.fluid) && c.material != AXG_MAT_ISOTROPIC;
+        auto props = [&](double vpr, double vsr) {
+            std::array<double, 6> a;
+            a[0] = ti ? vpr * std::sqrt(c.tiso_phi) : vpr;  // vpv
+            a[1] = vpr;                                     // vph
+            a[2] = vsr;                                     // vsv
+            a[3] = ti ? vsr * std::sqrt(c.tiso_xi) : vsr;   // vsh
+            a[4] = rho;
+            a[5] = ti ? c.tiso_eta : 1.;
+            return a;
+        };
+        std::array<double, 6> a = props(vp * (1. + dvp), q.fluid ? 0. : vs * (1. + dvs));
+        if (vol3d) {
+            const std::array<double, 6> a1 = props(vp, q.fluid ? 0. : vs);
+            const double th = r < tinyDouble ? 0. : std::acos(z / r);
+            double val[32];
+            for (int m = 0; m < c.nvol3d; m++) {
+                const axg_volumetric3d &md = c.vol3d[m];
+                if (md.nprop < 0 || md.nprop > 32 || !md.fn) throw std::runtime_error("Volumetric3D || bad model.");
+                if (!md.fn(md.user, r, th, phi, val)) continue;
+                for (int ip = 0; ip < md.nprop; ip++) {
+                    const int pr = md.props[ip], rt = md.ref_types[ip];
+                    int tg[2] = {-1, -1};
+                    switch (pr) {
+                        case AXG_PROP_VP: tg[0] = 0; tg[1] = 1; break;
+                        case AXG_PROP_VS: tg[0] = 2; tg[1] = 3; break;
+                        case AXG_PROP_RHO: tg[0] = 4; break;
+                        case AXG_PROP_VPV: tg[0] = 0; break;
+                        case AXG_PROP_VPH: tg[0] = 1; break;
+                        case AXG_PROP_VSV: tg[0] = 2; break;
+                        case AXG_PROP_VSH: tg[0] = 3; break;
+                        case AXG_PROP_ETA: tg[0] = 5; break;
+                        default: throw std::runtime_error("Volumetric3D || unknown property.");
+                    }
+                    for (int t : tg) {
+                        if (t < 0) continue;
+                        if (q.fluid && (t == 2 || t == 3 || t == 5)) continue;  // no shear in a fluid
+                        if (rt == AXG_REF_ABSOLUTE) a[t] = val[ip];
+                        else if (rt == AXG_REF_REFERENCE_1D) a[t] = a1[t] * (1. + val[ip]);
+                        else if (rt == AXG_REF_REFERENCE_3D) a[t] *= 1. + val[ip];
+                        else if (rt == AXG_REF_REFERENCE_PERT) a[t] = (a[t] - a1[t]) * (1. + val[ip]) + a1[t];
+                        else throw std::runtime_error("Volumetric3D || unknown reference type.");
+                    }
+                }
+            }
+        }
+        return a;
+    };
     // slice values of one element: vpv, vph, vsv, vsh, rho, eta at [row][ipnt]
     auto sampleElem = [&](int e, std::vector<std::array<double, 6>> &v, int nrows) {
         const QuadInfo &q = B.quads[e];
-        const Layer &L = B.layers[q.layer];
         v.resize((size_t)nrows * nP * nP);
-        for (int k = 0; k < nP * nP; k++) {
-            double s = q.crd[k][0], z = q.crd[k][1];
-            double r = std::hypot(s, z);
-            double vp = L.vp(r), vs = L.vs(r), rho = L.rho(r);
-            for (int row = 0; row < nrows; row++) {
-                double dvs = 0., dvp = 0.;
-                if (threeD && (!q.fluid || c.dv_fluid)) {
-                    double phi = 2. * pi * row / nrows;
-                    double x = s * std::cos(phi), y = s * std::sin(phi);
-                    dvs = c.dv_amp * std::tanh(B.grfVs(x, y, z) / 2.);
-                    dvp = c.dv_amp * std::tanh(B.grfVp(x, y, z) / 2.);
-                }
-                double vpr = vp * (1. + dvp), vsr = q.fluid ? 0. : vs * (1. + dvs);
-                std::array<double, 6> a;
-                bool ti = (!q.fluid) && c.material != AXG_MAT_ISOTROPIC;
-                a[0] = ti ? vpr * std::sqrt(c.tiso_phi) : vpr;  // vpv
-                a[1] = vpr;                                     // vph
-                a[2] = vsr;                                     // vsv
-                a[3] = ti ? vsr * std::sqrt(c.tiso_xi) : vsr;   // vsh
-                a[4] = rho;
-                a[5] = ti ? c.tiso_eta : 1.;
-                v[(size_t)row * nP * nP + k] = a;
-            }
-        }
+        for (int k = 0; k < nP * nP; k++)
+            for (int row = 0; row < nrows; row++)
+                v[(size_t)row * nP * nP + k] = sampleAt(q, k, 2. * pi * row / nrows);
     };
     // XMath::equalRows over all parameters (Material::isSolidPar1D / isFluidPar1D)
     auto rowsEqual = [&](const std::vector<std::array<double, 6>> &v, int nrows) {
@@ -1037,8 +1079,8 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                     vmax = std::max(vmax, std::max(v[(size_t)row * nP * nP + k][0], v[(size_t)row * nP * nP + k][1]));
                 o.dtElem = std::min(o.dtElem, c.courant * hmin / vmax);
             }
-            // Material::computeElementalMass (rho and fluid vp taken from the first slice:
-            // the synthetic models never perturb rho, and fluid vp only with dv_fluid)
+            // Material::computeElementalMass (from the first slice unless the mass
+            // may vary in azimuth: then on the point's own slices, below)
             for (int k = 0; k < nP * nP; k++) {
                 double rho = v[k][4];
                 o.rhoPt[k] = rho;
@@ -1046,17 +1088,16 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 const double fj = q.iFact[k] * q.jac[k];
                 o.mass[k] = q.fluid ? fj / (rho * v[k][0] * v[k][0]) : rho * fj;
             }
-            if (q.fluid && threeD && c.dv_fluid) {
-                const Layer &L = B.layers[q.layer];
+            // the mass on each point's own slices where it may vary in azimuth
+            // (fluid vp, or any property of a volumetric model)
+            if (o.is3D && ((q.fluid && threeD && c.dv_fluid) || vol3d)) {
                 for (int k = 0; k < nP * nP; k++) {
                     const int nrp = pointNr[q.gll[k]];
-                    const double sk = q.crd[k][0], zk = q.crd[k][1], r = std::hypot(sk, zk);
                     o.massv[k].resize(nrp);
                     for (int n = 0; n < nrp; n++) {
-                        const double phi = 2. * pi * n / nrp;
-                        const double dvp = c.dv_amp * std::tanh(B.grfVp(sk * std::cos(phi), sk * std::sin(phi), zk) / 2.);
-                        const double vp = L.vp(r) * (1. + dvp), rho = L.rho(r);
-                        o.massv[k][n] = q.iFact[k] * q.jac[k] / (rho * vp * vp);
+                        const auto a = sampleAt(q, k, 2. * pi * n / nrp);
+                        const double fj = q.iFact[k] * q.jac[k];
+                        o.massv[k][n] = q.fluid ? fj / (a[4] * a[0] * a[0]) : a[4] * fj;
                     }
                 }
             }
@@ -1277,6 +1318,9 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     const int NPT = B.npoint;
     std::vector<double> massS(NPT, 0.), massF(NPT, 0.);
     std::vector<std::vector<double>> massFv(NPT);  // azimuth-dependent fluid mass
+    std::vector<std::vector<double>> massSv(NPT);  // azimuth-dependent solid mass
+    // the part of each from elements whose mass is the same on all slices
+    std::vector<double> massSc(NPT, 0.), massFc(NPT, 0.);
     std::vector<int> pflags(NPT, 0);
     for (int e = 0; e < NQ; e++) {
         const QuadInfo &q = B.quads[e];
@@ -1288,10 +1332,19 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 if (!mv.empty()) {
                     if (massFv[p].empty()) massFv[p].assign(mv.size(), 0.);
                     for (size_t n = 0; n < mv.size(); n++) massFv[p][n] += mv[n];
+                } else {
+                    massFc[p] += out[e].mass[k];
                 }
                 pflags[p] |= AXG_PT_FLUID;
             } else {
                 massS[p] += out[e].mass[k];
+                const auto &mv = out[e].massv[k];
+                if (!mv.empty()) {
+                    if (massSv[p].empty()) massSv[p].assign(mv.size(), 0.);
+                    for (size_t n = 0; n < mv.size(); n++) massSv[p][n] += mv[n];
+                } else {
+                    massSc[p] += out[e].mass[k];
+                }
                 pflags[p] |= AXG_PT_SOLID;
             }
             int ipol = k / nP;
@@ -1304,9 +1357,26 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     m.mass_solid_off.assign(NPT + 1, 0);
     m.mass_fluid_off.assign(NPT + 1, 0);
     for (int p = 0; p < NPT; p++) {
-        // GLLPoint::release: Mass1D = 1 / mass (GLLPoint.cpp:74-79)
-        m.mass_solid_off[p + 1] = m.mass_solid_off[p] + ((pflags[p] & AXG_PT_SOLID) ? 1 : 0);
-        if (pflags[p] & AXG_PT_SOLID) m.inv_mass_solid.push_back((float)(1. / massS[p]));
+        // GLLPoint::release: Mass1D = 1 / mass (GLLPoint.cpp:74-79), Mass3D when
+        // the assembled mass differs between slices (GLLPoint.cpp:80-84)
+        for (auto &x : massSv[p]) x += massSc[p];
+        for (auto &x : massFv[p]) x += massFc[p];
+        bool solid3 = false;
+        if ((pflags[p] & AXG_PT_SOLID) && !massSv[p].empty()) {
+            const auto &mv = massSv[p];
+            double mx = 0., dif = 0.;
+            for (double x : mv) mx = std::max(mx, std::abs(x));
+            for (double x : mv) dif = std::max(dif, std::abs(x - mv[0]));
+            solid3 = dif > 1e-12 * mx;
+        }
+        if (solid3) {
+            m.mass_solid_off[p + 1] = m.mass_solid_off[p] + (int64_t)massSv[p].size();
+            for (double x : massSv[p]) m.inv_mass_solid.push_back((float)(1. / x));
+        } else {
+            m.mass_solid_off[p + 1] = m.mass_solid_off[p] + ((pflags[p] & AXG_PT_SOLID) ? 1 : 0);
+            if (pflags[p] & AXG_PT_SOLID) m.inv_mass_solid.push_back((float)(1. / massS[p]));
+            massSv[p].clear();
+        }
         // Mass1D when the assembled mass is the same on every slice
         // (XMath::equalRows), otherwise Mass3D with the point's Nr values
         // (GLLPoint.cpp:86-95)
@@ -1525,9 +1595,10 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
                 }
             // XMath::equalRows (XMath.h:67-74, tolerance 1e-10 relative); the
             // solid mass and the normal of this mesh are the same on all slices
-            bool equal = true;
+            bool equal = massSv[p].empty();
             for (int n = 1; n < nr; n++) equal = equal && std::abs(depth[n] - depth[0]) <= 1e-10 * std::abs(depth[0]);
             const double ms = massS[p];
+            auto msAt = [&](int n) { return massSv[p].empty() ? ms : massSv[p][n]; };
             const double area = std::sqrt(surf[p][0] * surf[p][0] + surf[p][1] * surf[p][1] + surf[p][2] * surf[p][2]);
             if (equal) {
                 // MassOcean1D(mass, 1027 depth |n|, theta) (MassOcean1D.cpp:8-13)
@@ -1541,9 +1612,9 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
             } else {
                 // MassOcean3D(mass, 1027 depth area, normal / area) (MassOcean3D.cpp:9-17)
                 for (int n = 0; n < nr; n++) {
-                    const double mo = kWater * depth[n] * area;
-                    const double scal = std::sqrt(mo / (ms * (ms + mo)));
-                    m.ocean.push_back((float)(1. / ms));
+                    const double mo = kWater * depth[n] * area, mn = msAt(n);
+                    const double scal = std::sqrt(mo / (mn * (mn + mo)));
+                    m.ocean.push_back((float)(1. / mn));
                     for (int d = 0; d < 3; d++) m.ocean.push_back((float)(surf[p][d] / area * scal));
                 }
                 m.ocean_off[p + 1] += nr;

@@ -39,6 +39,32 @@ def shell_config(which=0, layers=None, **overrides):
             keep_nu = np.ascontiguousarray(v, dtype=np.int32)
             cfg.elem_nu = keep_nu.ctypes.data_as(C.POINTER(C.c_int32))
             cfg.nu_mode = 3
+        elif k == "vol3d":
+            # 3-D volumetric models: [(props, ref_types, fn)], fn(r, theta, phi) ->
+            # sequence of values, or None where the model does not apply
+            arr = (_abi.Volumetric3D * len(v))()
+            keep_v = [arr]
+            for i, (props, refs, fn) in enumerate(v):
+                pa = np.ascontiguousarray(props, dtype=np.int32)
+                ra = np.ascontiguousarray(refs, dtype=np.int32)
+                n = len(pa)
+
+                def cb(user, r, t, p, out, fn=fn, n=n):
+                    vals = fn(r, t, p)
+                    if vals is None:
+                        return 0
+                    for j in range(n):
+                        out[j] = float(vals[j])
+                    return 1
+                f = _abi.VOL3D_FN(cb)
+                arr[i].nprop = n
+                arr[i].props = pa.ctypes.data_as(_abi.i32p)
+                arr[i].ref_types = ra.ctypes.data_as(_abi.i32p)
+                arr[i].fn = f
+                keep_v += [pa, ra, f]
+            cfg.nvol3d = len(v)
+            cfg.vol3d = C.cast(arr, C.POINTER(_abi.Volumetric3D))
+            cfg._keep_vol3d = keep_v
         elif k == "wisdom":
             # nu_mode 4: Nu from wisdom rows [n][4] (s, z, learned Nu, original Nu)
             keep_nu = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1, 4))

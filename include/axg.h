@@ -188,6 +188,39 @@ typedef struct axg_shell_layer {
     double q_mu, q_kappa;             /* used when attenuation is on; <=0: draw in [q_min,q_max] */
 } axg_shell_layer;
 
+/* A 3-D volumetric model (Volumetric3D::get3dProperties, Volumetric3D.h):
+ * fn(user, r, theta, phi, values) gives the values of its nprop properties at
+ * radius r (m) and colatitude / azimuth theta, phi (rad, source-centred frame)
+ * and returns nonzero where the model applies there (0: no change).  Each value
+ * acts on the current 3-D value v of its property by its reference type
+ * (Material.cpp:194-209), with v1 the layer's 1-D value:
+ *   ABSOLUTE        v = value
+ *   REFERENCE_1D    v = v1 (1 + value)
+ *   REFERENCE_3D    v = v (1 + value)
+ *   REFERENCE_PERT  v = (v - v1)(1 + value) + v1
+ * VP acts on VPV and VPH, VS on VSV and VSH (Material.cpp:153-176).  Models act
+ * in the order given, after the built-in random perturbation (dv_amp). */
+#define AXG_PROP_VP 0
+#define AXG_PROP_VS 1
+#define AXG_PROP_RHO 2
+#define AXG_PROP_VPV 3
+#define AXG_PROP_VPH 4
+#define AXG_PROP_VSV 5
+#define AXG_PROP_VSH 6
+#define AXG_PROP_ETA 7
+#define AXG_REF_ABSOLUTE 0
+#define AXG_REF_REFERENCE_1D 1
+#define AXG_REF_REFERENCE_3D 2
+#define AXG_REF_REFERENCE_PERT 3
+typedef int (*axg_vol3d_fn)(void *user, double r, double theta, double phi, double *values);
+typedef struct axg_volumetric3d {
+    int32_t nprop;
+    const int32_t *props;             /* [nprop] AXG_PROP_*                         */
+    const int32_t *ref_types;         /* [nprop] AXG_REF_*                          */
+    axg_vol3d_fn fn;
+    void *user;
+} axg_volumetric3d;
+
 typedef struct axg_shell_config {
     int32_t nelem_theta;              /* elements along colatitude [0, pi]        */
     int32_t nlayer;
@@ -256,6 +289,8 @@ typedef struct axg_shell_config {
     int32_t undulation_order;
     double undulation_amp, undulation_depth;
     int32_t rec_dump;                 /* solid stations dump strain (bit 0) and curl (bit 1) */
+    int32_t nvol3d;                   /* 3-D volumetric models, applied in order  */
+    const axg_volumetric3d *vol3d;
 } axg_shell_config;
 
 typedef struct axg_mesh axg_mesh;
