@@ -41,4 +41,5 @@ from .domain import Domain, DomainBatch, Newmark, set_fft_planner  # noqa: E402
 from .nu_mlp import NuMLP  # noqa: E402
 
 __all__ = ["lib", "last_error", "ShellMesh", "shell_config", "Domain", "DomainBatch", "Newmark", "NuMLP", "set_fft_planner"]
-from .files import read_wisdom, wisdom_nu, wisdom_rows, write_stations, write_wisdom  # noqa: E402
+from .files import (read_netcdf, read_wisdom, wisdom_nu, wisdom_rows, write_stations,  # noqa: E402
+                    write_stations_netcdf, write_wisdom)

@@ -181,6 +181,14 @@ def declare_mesh_api(lib):
     lib.axg_write_station_ascii.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                             f32p, C.c_int64]
     lib.axg_write_station_ascii.restype = C.c_int
+    lib.axg_wisdom_write_netcdf.argtypes = [C.c_char_p, C.c_int64, f64p]
+    lib.axg_wisdom_write_netcdf.restype = C.c_int
+    lib.axg_write_stations_netcdf.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                              C.c_char_p, C.c_char_p, f64p, f32p, C.c_int32, i32p, f32p, C.c_int32,
+                                              i32p, f32p, f64p]
+    lib.axg_write_stations_netcdf.restype = C.c_int
+    lib.axg_netcdf_read.argtypes = [C.c_char_p, C.c_char_p, i64p, C.c_int32, i32p, f64p, C.c_int64, i64p]
+    lib.axg_netcdf_read.restype = C.c_int
     lib.axg_field_size.argtypes = [C.POINTER(MeshDesc), C.c_int32]
     lib.axg_field_size.restype = C.c_int64
 

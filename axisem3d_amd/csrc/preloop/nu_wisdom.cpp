@@ -11,6 +11,7 @@
 #include <stdexcept>
 
 #include "axg.h"
+#include "netcdf_classic.hpp"
 
 namespace axg {
 void set_last_error(const std::string &msg);
@@ -125,7 +126,56 @@ void writeWisdomAscii(const std::string &path, const double *rows, int64_t n) {
     if (std::fclose(f) != 0) throw std::runtime_error("NuWisdom::writeToFile || Error writing file: || " + path);
 }
 
+void writeWisdomNetCDF(const std::string &path, const double *rows, int64_t n) {
+    netcdf::File f;
+    netcdf::Var v;
+    v.name = kVar;
+    v.dims = {f.dim(n), f.dim(4)};
+    v.data.assign(rows, rows + 4 * n);
+    f.vars.push_back(std::move(v));
+    netcdf::write(path, f);
+}
+
+namespace {
+// the rows of a 2-D variable of 3 or 4 columns (NuWisdom::readFromFile,
+// NuWisdom.cpp:47-72)
+std::vector<double> wisdomRows(const std::vector<size_t> &dims, const std::vector<double> &data,
+                               const std::string &path) {
+    if (dims.size() != 2)
+        throw std::runtime_error("NetCDF_Reader::read2D || Variable is not 2D, Variable = " + std::string(kVar) +
+                                 " || NetCDF file: " + path);
+    const size_t nrow = dims[0], ncol = dims[1];
+    if (ncol != 3 && ncol != 4)
+        throw std::runtime_error("NuWisdom::readFromFile || Inconsistent dimensions for wisdom data, must be a "
+                                 "matrix of 3 or 4 columns || File = " + path);
+    std::vector<double> rows(4 * nrow);
+    for (size_t i = 0; i < nrow; i++) {
+        for (size_t c = 0; c < ncol; c++) rows[4 * i + c] = data[i * ncol + c];
+        if (ncol == 3) rows[4 * i + 3] = rows[4 * i + 2];
+    }
+    return rows;
+}
+}  // namespace
+
 std::vector<double> readWisdomAscii(const std::string &path) {
+    {
+        // a binary NetCDF file (NetCDF_Reader::checkNetCDF_isAscii decides the
+        // same way, by the first line)
+        std::ifstream bs(path, std::ios::binary);
+        char magic[4] = {0, 0, 0, 0};
+        bs.read(magic, 4);
+        if (bs && magic[0] == 'C' && magic[1] == 'D' && magic[2] == 'F') {
+            std::vector<int64_t> shape;
+            std::vector<double> data;
+            if (!netcdf::readVar(path, kVar, shape, data))
+                throw std::runtime_error("NetCDF_Reader::getVariableID || Error inquiring variable ID, variable: " +
+                                         std::string(kVar) + " || NetCDF file: " + path);
+            return wisdomRows(std::vector<size_t>(shape.begin(), shape.end()), data, path);
+        }
+        if (bs && magic[0] == '\x89' && magic[1] == 'H' && magic[2] == 'D' && magic[3] == 'F')
+            throw std::runtime_error("NuWisdom::readFromFile || NetCDF-4 (HDF5) files need the NetCDF library; "
+                                     "write the wisdom as NetCDF classic or ascii || File = " + path);
+    }
     std::ifstream fs(path);
     if (!fs) throw std::runtime_error("NetCDF_ReaderAscii::open || Error opening NetCDF-alternative ascii file: || " + path);
     std::string line;
@@ -161,19 +211,7 @@ std::vector<double> readWisdomAscii(const std::string &path) {
             throw std::runtime_error("NetCDF_ReaderAscii::readMetaData || Error detecting @VAR_END line, Variable = " +
                                      std::string(kVar) + " || Too many data or invalid number format or missing "
                                      "@VAR_END || NetCDF-alternative ascii file: " + path);
-        if (dims.size() != 2)
-            throw std::runtime_error("NetCDF_ReaderAscii::read2D || Variable is not 2D, Variable = " +
-                                     std::string(kVar) + " || NetCDF file: " + path);
-        const size_t nrow = dims[0], ncol = dims[1];
-        if (ncol != 3 && ncol != 4)
-            throw std::runtime_error("NuWisdom::readFromFile || Inconsistent dimensions for wisdom data, must be a "
-                                     "matrix of 3 or 4 columns || File = " + path);
-        std::vector<double> rows(4 * nrow);
-        for (size_t i = 0; i < nrow; i++) {
-            for (size_t c = 0; c < ncol; c++) rows[4 * i + c] = data[i * ncol + c];
-            if (ncol == 3) rows[4 * i + 3] = rows[4 * i + 2];
-        }
-        return rows;
+        return wisdomRows(dims, data, path);
     }
     throw std::runtime_error("NetCDF_ReaderAscii::readMetaData || Error detecting @VAR_START line, || Variable not "
                              "found, Variable = " + std::string(kVar) + " || NetCDF-alternative ascii file: " + path);
@@ -266,4 +304,147 @@ extern "C" int axg_write_station_ascii(const char *path, int64_t nstep, int32_t 
         return AXG_EINVAL;
     }
     return AXG_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int axg_wisdom_write_netcdf(const char *path, int64_t n, const double *rows) {
+    if (!path || n < 0 || (n > 0 && !rows)) {
+        axg::set_last_error("axg_wisdom_write_netcdf || null argument.");
+        return AXG_EINVAL;
+    }
+    try {
+        axg::writeWisdomNetCDF(path, rows, n);
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
+}
+
+// The assembled station file of PointwiseIONetCDF::finalize
+// (SOLVER/src/core/output/pointwise/PointwiseIONetCDF.cpp:272-380): double
+// time_points [nrow], then float "NET.STA.<components>" [nrow][3] of every
+// station, "NET.STA.RTZ.strain" [nrow][6] and "NET.STA.RTZ.curl" [nrow][3] of
+// the stations that dump them, each with double attributes latitude,
+// longitude and depth; global double attributes source_latitude,
+// source_longitude and source_depth; dimensions "ncdim_<n>".  nrow counts the
+// step indices i with i % interval == 0, t as in axg_write_station_ascii.
+extern "C" int axg_write_stations_netcdf(const char *path, int64_t nstep, int32_t interval, double t_shift,
+                                         double dt, int32_t nrec, const char *names, const char *components,
+                                         const double *rec_geo, const float *disp, int32_t nstrain,
+                                         const int32_t *strain_rec, const float *strain, int32_t ncurl,
+                                         const int32_t *curl_rec, const float *curl, const double *src_geo) {
+    if (!path || nstep < 0 || interval <= 0 || nrec < 0 || nstrain < 0 || ncurl < 0 || !components ||
+        (nrec > 0 && (!names || !rec_geo || (nstep > 0 && !disp))) ||
+        (nstrain > 0 && (!strain_rec || (nstep > 0 && !strain))) ||
+        (ncurl > 0 && (!curl_rec || (nstep > 0 && !curl))) || !src_geo) {
+        axg::set_last_error("axg_write_stations_netcdf || invalid argument.");
+        return AXG_EINVAL;
+    }
+    try {
+        std::vector<std::string> key;
+        {
+            std::stringstream ss(names ? names : "");
+            std::string line;
+            while (std::getline(ss, line, '\n')) key.push_back(line);
+        }
+        if ((int32_t)key.size() != nrec) {
+            axg::set_last_error("axg_write_stations_netcdf || one \"NET.STA\" name per station expected.");
+            return AXG_EINVAL;
+        }
+        for (int32_t k = 0; k < nstrain; k++)
+            if (strain_rec[k] < 0 || strain_rec[k] >= nrec) throw std::runtime_error("bad strain station index");
+        for (int32_t k = 0; k < ncurl; k++)
+            if (curl_rec[k] < 0 || curl_rec[k] >= nrec) throw std::runtime_error("bad curl station index");
+        std::vector<int64_t> rows;
+        std::vector<double> times;
+        double t = 0. - t_shift;
+        for (int64_t i = 0; i < nstep; i++) {
+            if (i % interval == 0) {
+                rows.push_back(i);
+                times.push_back(t);
+            }
+            t += dt;
+        }
+        const int64_t nrow = (int64_t)rows.size();
+        namespace nc = axg::netcdf;
+        nc::File f;
+        const int dT = f.dim(nrow), d3 = f.dim(3), d6 = f.dim(6);
+        auto geo = [&](int r) {
+            std::vector<nc::Attr> a(3);
+            const char *nm[3] = {"latitude", "longitude", "depth"};
+            for (int k = 0; k < 3; k++) {
+                a[k].name = nm[k];
+                a[k].values = {rec_geo[3 * r + k]};
+            }
+            return a;
+        };
+        auto series = [&](const std::string &name, int r, int dc, int ncol, const float *src, int64_t nsta,
+                          int64_t col0) {
+            nc::Var v;
+            v.name = name;
+            v.type = nc::FLOAT;
+            v.dims = {dT, dc};
+            v.attrs = geo(r);
+            v.data.resize(nrow * ncol);
+            for (int64_t j = 0; j < nrow; j++)
+                for (int c = 0; c < ncol; c++) v.data[j * ncol + c] = src[(rows[j] * nsta + col0) * ncol + c];
+            f.vars.push_back(std::move(v));
+        };
+        {
+            nc::Var v;
+            v.name = "time_points";
+            v.dims = {dT};
+            v.data = times;
+            f.vars.push_back(std::move(v));
+        }
+        for (int32_t r = 0; r < nrec; r++) series(key[r] + "." + components, r, d3, 3, disp, nrec, r);
+        for (int32_t k = 0; k < nstrain; k++) series(key[strain_rec[k]] + ".RTZ.strain", strain_rec[k], d6, 6, strain, nstrain, k);
+        for (int32_t k = 0; k < ncurl; k++) series(key[curl_rec[k]] + ".RTZ.curl", curl_rec[k], d3, 3, curl, ncurl, k);
+        const char *gn[3] = {"source_latitude", "source_longitude", "source_depth"};
+        for (int k = 0; k < 3; k++) {
+            nc::Attr a;
+            a.name = gn[k];
+            a.values = {src_geo[k]};
+            f.attrs.push_back(a);
+        }
+        nc::write(path, f);
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(std::string("axg_write_stations_netcdf || ") + e.what());
+        return AXG_EINVAL;
+    }
+}
+
+// One variable of a NetCDF classic file (CDF-1 or CDF-2), any numeric type,
+// as double: with values null, only its shape and element count.
+extern "C" int axg_netcdf_read(const char *path, const char *name, int64_t *shape, int32_t maxdims,
+                               int32_t *ndims, double *values, int64_t cap, int64_t *n) {
+    if (!path || !name || !ndims || !n || maxdims < 0 || (maxdims > 0 && !shape)) {
+        axg::set_last_error("axg_netcdf_read || invalid argument.");
+        return AXG_EINVAL;
+    }
+    try {
+        std::vector<int64_t> sh;
+        std::vector<double> v;
+        if (!axg::netcdf::readVar(path, name, sh, v)) {
+            axg::set_last_error(std::string("NetCDF_Reader::getVariableID || Error inquiring variable ID, variable: ") +
+                                name + " || NetCDF file: " + path);
+            return AXG_EINVAL;
+        }
+        *ndims = (int32_t)sh.size();
+        *n = (int64_t)v.size();
+        for (int32_t d = 0; d < std::min<int32_t>(maxdims, (int32_t)sh.size()); d++) shape[d] = sh[d];
+        if (values) {
+            if (cap < *n) {
+                axg::set_last_error("axg_netcdf_read || buffer holds fewer values than the variable.");
+                return AXG_EINVAL;
+            }
+            std::copy(v.begin(), v.end(), values);
+        }
+        return AXG_OK;
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
 }

@@ -32,8 +32,10 @@ struct NuWisdomIndex {
 
 // the ascii wisdom file: variable axisem3d_wisdom [n][4]
 void writeWisdomAscii(const std::string &path, const double *rows, int64_t n);
-// rows [n][4] of a wisdom file; a 3-column variable takes its third column as
-// the original Nu too (NuWisdom.cpp:60-71)
+// the same variable in a NetCDF classic file (NuWisdom.cpp:19-44)
+void writeWisdomNetCDF(const std::string &path, const double *rows, int64_t n);
+// rows [n][4] of a wisdom file, ascii or NetCDF classic; a 3-column variable
+// takes its third column as the original Nu too (NuWisdom.cpp:60-71)
 std::vector<double> readWisdomAscii(const std::string &path);
 
 }  // namespace axg

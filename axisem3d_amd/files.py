@@ -5,8 +5,14 @@
   the input of the NU_TYPE wisdom Nr field (WisdomNrField.cpp:20-24), in the
   reference's "NetCDF ascii format" alternative that NuWisdom::readFromFile
   accepts (NuWisdom.cpp:47-79).
+  The same variable as a NetCDF classic file (NuWisdom::writeToFile,
+  NuWisdom.cpp:19-44); read_wisdom takes either format.
 * Station traces: one ascii file NET.STA.<components>.ascii per station with a
-  line "t c0 c1 c2" per recorded step (PointwiseIOAscii.cpp:25-35, 96-102).
+  line "t c0 c1 c2" per recorded step (PointwiseIOAscii.cpp:25-35, 96-102), or
+  the assembled NetCDF file axisem3d_synthetics.nc (PointwiseIONetCDF.cpp:
+  272-380).  The reference creates NetCDF-4 files; these are NetCDF classic
+  (64-bit offset) files with the same variables, dimensions and attributes,
+  which nc_open and every NetCDF reader detect and read.
 """
 import ctypes as C
 import os
@@ -28,10 +34,12 @@ def _check(rc):
         raise RuntimeError(_lib().axg_last_error().decode())
 
 
-def write_wisdom(path, rows):
-    """Write rows [n][4] (s, z, learned Nu, original Nu) as a wisdom file."""
+def write_wisdom(path, rows, netcdf=False):
+    """Write rows [n][4] (s, z, learned Nu, original Nu) as a wisdom file,
+    ascii or (netcdf=True) NetCDF classic."""
     rows = np.ascontiguousarray(np.asarray(rows, dtype=np.float64).reshape(-1, 4))
-    _check(_lib().axg_wisdom_write(os.fsencode(path), rows.shape[0], rows.ctypes.data_as(_f64p)))
+    fn = _lib().axg_wisdom_write_netcdf if netcdf else _lib().axg_wisdom_write
+    _check(fn(os.fsencode(path), rows.shape[0], rows.ctypes.data_as(_f64p)))
 
 
 def read_wisdom(path):
@@ -88,3 +96,56 @@ def write_stations(outdir, seismograms, dt, t_shift, names=None, components="RTZ
                                               ncol * nrec))
         paths.append(path)
     return paths
+
+
+def write_stations_netcdf(path, seismograms, dt, t_shift, names=None, components="RTZ", interval=1,
+                          rec_geo=None, src_geo=(0., 0., 0.), strain=None, strain_rec=None, curl=None,
+                          curl_rec=None):
+    """The assembled station file (axisem3d_synthetics.nc) from seismograms
+    [nstep][nrec][3], and optionally strain [nstep][nstrain][6] / curl
+    [nstep][ncurl][3] of the stations strain_rec / curl_rec (indices into
+    names; default: the stations Domain.strain_curl() returns, in order);
+    rec_geo [nrec][3] latitude, longitude, depth; src_geo the source's."""
+    s = np.ascontiguousarray(seismograms, dtype=np.float32)
+    nstep, nrec, ncol = s.shape
+    if ncol != 3:
+        raise ValueError("write_stations_netcdf: seismograms [nstep][nrec][3] expected")
+    if names is None:
+        names = ["AX.S%04d" % i for i in range(nrec)]
+    if len(names) != nrec or any("\n" in x for x in names):
+        raise ValueError("write_stations_netcdf: one name per station expected")
+    geo = np.zeros((nrec, 3)) if rec_geo is None else np.asarray(rec_geo, dtype=np.float64).reshape(nrec, 3)
+    geo = np.ascontiguousarray(geo)
+    src = np.ascontiguousarray(np.asarray(src_geo, dtype=np.float64).reshape(3))
+
+    def part(a, idx, w):
+        if a is None:
+            return 0, None, None, None
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.ndim != 3 or a.shape[0] != nstep or a.shape[2] != w:
+            raise ValueError("write_stations_netcdf: [nstep][n][%d] expected" % w)
+        idx = np.arange(a.shape[1]) if idx is None else np.asarray(idx)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        if idx.shape != (a.shape[1],):
+            raise ValueError("write_stations_netcdf: one station index per series expected")
+        return a.shape[1], idx, a, (idx.ctypes.data_as(_abi.i32p), a.ctypes.data_as(_abi.f32p))
+
+    ns, si, sa, sp = part(strain, strain_rec, 6)
+    nc, ci, ca, cp = part(curl, curl_rec, 3)
+    _check(_lib().axg_write_stations_netcdf(
+        os.fsencode(path), nstep, interval, t_shift, dt, nrec, "\n".join(names).encode(), components.encode(),
+        geo.ctypes.data_as(_f64p), s.ctypes.data_as(_abi.f32p), ns, sp[0] if sp else None, sp[1] if sp else None,
+        nc, cp[0] if cp else None, cp[1] if cp else None, src.ctypes.data_as(_f64p)))
+    return path
+
+
+def read_netcdf(path, name):
+    """One variable of a NetCDF classic file, as float64 of its shape."""
+    nd = C.c_int32(0)
+    n = C.c_int64(0)
+    shape = (C.c_int64 * 8)()
+    _check(_lib().axg_netcdf_read(os.fsencode(path), name.encode(), shape, 8, C.byref(nd), None, 0, C.byref(n)))
+    out = np.zeros(max(n.value, 1), dtype=np.float64)
+    _check(_lib().axg_netcdf_read(os.fsencode(path), name.encode(), shape, 8, C.byref(nd),
+                                  out.ctypes.data_as(_f64p), n.value, C.byref(n)))
+    return out[:n.value].reshape([shape[i] for i in range(min(nd.value, 8))])

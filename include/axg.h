@@ -313,6 +313,12 @@ int axg_wisdom_write(const char *path, int64_t n, const double *rows);
 int axg_wisdom_read(const char *path, double *rows, int64_t cap, int64_t *n);
 int axg_wisdom_get_nu(int64_t n, const double *rows, int32_t num_samples, int64_t nq, const double *sz,
                       int32_t *nu);
+/* The same variable in a NetCDF classic file (64-bit offset), dimensions
+ * "ncdim_<n>" as NetCDF_Writer::defineVariable names them (NuWisdom::writeToFile,
+ * NuWisdom.cpp:19-44, NetCDF_Writer.h:33-47).  The reference creates NetCDF-4
+ * files; nc_open, which NuWisdom::readFromFile calls, detects and reads classic
+ * files too, and axg_wisdom_read takes either format (ascii or classic). */
+int axg_wisdom_write_netcdf(const char *path, int64_t n, const double *rows);
 
 /* One station's ascii file (PointwiseIOAscii.cpp:25-35, 96-102): a line
  * "t c0 .. c(ncol-1)" at full precision for every step index i with
@@ -322,6 +328,25 @@ int axg_wisdom_get_nu(int64_t n, const double *rows, int32_t num_samples, int64_
  * station-interleaved seismograms of axg_get_seismograms). */
 int axg_write_station_ascii(const char *path, int64_t nstep, int32_t interval, double t_shift, double dt,
                             int32_t ncol, const float *trace, int64_t row_stride);
+/* The assembled station file axisem3d_synthetics.nc of PointwiseIONetCDF
+ * (PointwiseIONetCDF.cpp:272-380) in the NetCDF classic format: double
+ * time_points [nrow]; float "NET.STA.<components>" [nrow][3] of each station,
+ * then "NET.STA.RTZ.strain" [nrow][6] and "NET.STA.RTZ.curl" [nrow][3] of the
+ * stations strain_rec / curl_rec name, each with double attributes latitude,
+ * longitude, depth (rec_geo[nrec][3]); global attributes source_latitude,
+ * source_longitude, source_depth (src_geo[3]).  names: nrec "NET.STA" keys
+ * separated by '\n'; disp [nstep][nrec][3], strain [nstep][nstrain][6],
+ * curl [nstep][ncurl][3] as axg_get_seismograms / axg_get_strain_curl return
+ * them; rows at step indices i % interval == 0, t as above. */
+int axg_write_stations_netcdf(const char *path, int64_t nstep, int32_t interval, double t_shift, double dt,
+                              int32_t nrec, const char *names, const char *components, const double *rec_geo,
+                              const float *disp, int32_t nstrain, const int32_t *strain_rec, const float *strain,
+                              int32_t ncurl, const int32_t *curl_rec, const float *curl, const double *src_geo);
+/* One variable of a NetCDF classic file (NetCDF_Reader::read1D / read2D,
+ * NetCDF_Reader.h:50-80): its shape (up to maxdims entries), *ndims, element count *n
+ * and, with values non-null, its values as double (cap elements at most). */
+int axg_netcdf_read(const char *path, const char *name, int64_t *shape, int32_t maxdims, int32_t *ndims,
+                    double *values, int64_t cap, int64_t *n);
 
 /* Features of every GLL point for the learned Nu predictor (no counterpart in
  * the reference; they describe what the reference's NuWisdom learns from):
