@@ -187,6 +187,25 @@ def declare_mesh_api(lib):
                                               C.c_char_p, C.c_char_p, f64p, f32p, C.c_int32, i32p, f32p, C.c_int32,
                                               i32p, f32p, f64p]
     lib.axg_write_stations_netcdf.restype = C.c_int
+    vp = C.c_void_p
+    for name, args in {
+        "axg_exodus_open": [C.c_char_p, C.POINTER(vp)],
+        "axg_exodus_stats": [vp, i64p, C.c_int],
+        "axg_exodus_names": [vp, C.c_int32, C.c_char_p, C.c_int64, i64p],
+        "axg_exodus_global": [vp, C.c_char_p, f64p],
+        "axg_exodus_nodes": [vp, f64p, f64p, f64p],
+        "axg_exodus_quads": [vp, i32p, i32p],
+        "axg_exodus_side_set": [vp, C.c_char_p, i32p],
+        "axg_exodus_elemental": [vp, C.c_char_p, f64p],
+        "axg_exodus_profile": [vp, C.c_char_p, f64p, f64p],
+        "axg_exodus_ellipticity": [vp, f64p, f64p],
+        "axg_exodus_geometry": [vp, i32p, f64p, f64p],
+        "axg_exodus_material": [vp, C.c_char_p, f64p],
+    }.items():
+        getattr(lib, name).argtypes = args
+        getattr(lib, name).restype = C.c_int
+    lib.axg_exodus_free.argtypes = [vp]
+    lib.axg_exodus_free.restype = None
     lib.axg_netcdf_read.argtypes = [C.c_char_p, C.c_char_p, i64p, C.c_int32, i32p, f64p, C.c_int64, i64p]
     lib.axg_netcdf_read.restype = C.c_int
     lib.axg_field_size.argtypes = [C.POINTER(MeshDesc), C.c_int32]

@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 HOST_SOURCES = ["preloop/shell_mesh.cpp", "preloop/nu_wisdom.cpp", "preloop/netcdf_classic.cpp",
-                "host/errors.cpp"]
+                "preloop/exodus.cpp", "host/errors.cpp"]
 # (source, object name, extra flags): the 3-D element launchers are compiled once
 # per size class (modes per lane) so that the classes build in parallel
 HIP_SOURCES = [("axg_domain.hip", "axg_domain", []), ("nu_mlp.hip", "nu_mlp", [])] + [
@@ -49,7 +49,7 @@ def build(verbose=True, jobs=None):
         list(ex.map(run, cmds))
     lib = os.path.join(BUILD, "libaxg.so")
     tmp = lib + ".tmp"
-    run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread", "-ldl"])
     os.replace(tmp, lib)
     # the C++ caller of the Domain / Newmark facade (include/axg_domain.hpp)
     exe = os.path.join(BUILD, "axg_solve")

@@ -1,0 +1,672 @@
+// exodus.cpp -- Exodus mesh model (ExodusModel.cpp) read through the HDF5 C
+// library, and the geometry and nodal material of its quads (Quad.cpp,
+// Material.cpp, the mappings).
+#include "exodus.hpp"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <type_traits>
+
+#include "axg.h"
+#include "mapping.hpp"
+#include "spectral.hpp"
+
+namespace axg {
+void set_last_error(const std::string &msg);
+
+namespace exodus {
+namespace {
+
+// ----------------------------------------------------------------- HDF5 ABI
+// The few HDF5 (1.10+) entry points the reader needs, resolved by name.
+typedef int64_t hid_t;
+typedef int herr_t;
+typedef unsigned long long hsize_t;
+constexpr hid_t kDefault = 0, kAll = 0;
+constexpr int kClassString = 3;
+
+struct H5 {
+    void *so = nullptr;
+    herr_t (*open)();
+    herr_t (*libversion)(unsigned *, unsigned *, unsigned *);
+    herr_t (*eset_auto)(hid_t, void *, void *);
+    hid_t (*fopen)(const char *, unsigned, hid_t);
+    herr_t (*fclose)(hid_t);
+    int (*lexists)(hid_t, const char *, hid_t);
+    hid_t (*dopen)(hid_t, const char *, hid_t);
+    herr_t (*dclose)(hid_t);
+    hid_t (*dspace)(hid_t);
+    hid_t (*dtype)(hid_t);
+    herr_t (*dread)(hid_t, hid_t, hid_t, hid_t, hid_t, void *);
+    int (*sndims)(hid_t);
+    int (*sdims)(hid_t, hsize_t *, hsize_t *);
+    herr_t (*sclose)(hid_t);
+    int (*tclass)(hid_t);
+    size_t (*tsize)(hid_t);
+    herr_t (*tclose)(hid_t);
+    hid_t *nativeDouble = nullptr, *nativeInt = nullptr;
+
+    template <class F>
+    void sym(F &f, const char *name) {
+        f = reinterpret_cast<F>(dlsym(so, name));
+        if (!f) throw std::runtime_error(std::string("ExodusModel || HDF5 library lacks ") + name);
+    }
+    H5() {
+        // AXG_HDF5_LIB names the library; else the usual names and the conda
+        // prefix of this image
+        std::vector<std::string> names;
+        if (const char *e = std::getenv("AXG_HDF5_LIB")) names.push_back(e);
+        for (const char *n : {"libhdf5.so", "libhdf5_serial.so", "/opt/conda/lib/libhdf5.so",
+                              "/usr/lib/x86_64-linux-gnu/hdf5/serial/libhdf5.so"})
+            names.push_back(n);
+        for (const auto &n : names)
+            if ((so = dlopen(n.c_str(), RTLD_NOW | RTLD_LOCAL))) break;
+        if (!so)
+            throw std::runtime_error("ExodusModel || the HDF5 C library (libhdf5.so) is needed to read NetCDF-4 "
+                                     "Exodus files; set AXG_HDF5_LIB to its path.");
+        sym(open, "H5open");
+        sym(libversion, "H5get_libversion");
+        sym(eset_auto, "H5Eset_auto2");
+        sym(fopen, "H5Fopen");
+        sym(fclose, "H5Fclose");
+        sym(lexists, "H5Lexists");
+        sym(dopen, "H5Dopen2");
+        sym(dclose, "H5Dclose");
+        sym(dspace, "H5Dget_space");
+        sym(dtype, "H5Dget_type");
+        sym(dread, "H5Dread");
+        sym(sndims, "H5Sget_simple_extent_ndims");
+        sym(sdims, "H5Sget_simple_extent_dims");
+        sym(sclose, "H5Sclose");
+        sym(tclass, "H5Tget_class");
+        sym(tsize, "H5Tget_size");
+        sym(tclose, "H5Tclose");
+        sym(nativeDouble, "H5T_NATIVE_DOUBLE_g");
+        sym(nativeInt, "H5T_NATIVE_INT_g");
+        open();
+        unsigned maj = 0, min = 0, rel = 0;
+        libversion(&maj, &min, &rel);
+        if (maj != 1 || min < 10)
+            throw std::runtime_error("ExodusModel || HDF5 1.10 or later is needed (hid_t of 64 bits).");
+        eset_auto(0, nullptr, nullptr);
+    }
+};
+
+H5 &h5() {
+    static H5 lib;
+    return lib;
+}
+
+// NetCDF_Reader::read1D / read2D / readString over an HDF5 dataset
+struct Reader {
+    H5 &L;
+    hid_t f = -1;
+    std::string path;
+    explicit Reader(const std::string &p) : L(h5()), path(p) {
+        f = L.fopen(p.c_str(), 0u, kDefault);
+        if (f < 0) throw std::runtime_error("NetCDF_Reader::open || Error opening NetCDF file: || " + p);
+    }
+    ~Reader() {
+        if (f >= 0) L.fclose(f);
+    }
+    bool has(const std::string &name) const { return L.lexists(f, name.c_str(), kDefault) > 0; }
+    hid_t openSet(const std::string &name, std::vector<int64_t> &shape) const {
+        if (!has(name))
+            throw std::runtime_error("NetCDF_Reader::getVariableID || Error inquiring variable ID, variable: " +
+                                     name + " || NetCDF file: " + path);
+        const hid_t d = L.dopen(f, name.c_str(), kDefault);
+        if (d < 0) throw std::runtime_error("NetCDF_Reader || Error opening variable: " + name);
+        const hid_t sp = L.dspace(d);
+        const int nd = L.sndims(sp);
+        std::vector<hsize_t> dims(std::max(nd, 0));
+        if (nd > 0) L.sdims(sp, dims.data(), nullptr);
+        L.sclose(sp);
+        shape.assign(dims.begin(), dims.end());
+        return d;
+    }
+    template <class T>
+    std::vector<T> read(const std::string &name, std::vector<int64_t> *shapeOut = nullptr) const {
+        std::vector<int64_t> shape;
+        const hid_t d = openSet(name, shape);
+        int64_t n = 1;
+        for (int64_t x : shape) n *= x;
+        std::vector<T> v(n);
+        const hid_t mt = std::is_same<T, double>::value ? *L.nativeDouble : *L.nativeInt;
+        const herr_t rc = n ? L.dread(d, mt, kAll, kAll, kDefault, v.data()) : 0;
+        L.dclose(d);
+        if (rc < 0) throw std::runtime_error("NetCDF_Reader || Error reading variable: " + name);
+        if (shapeOut) *shapeOut = shape;
+        return v;
+    }
+    // a [n][len] character array as n strings, cut at the first NUL and trimmed
+    std::vector<std::string> strings(const std::string &name) const {
+        std::vector<int64_t> shape;
+        const hid_t d = openSet(name, shape);
+        const hid_t t = L.dtype(d);
+        if (L.tclass(t) != kClassString || L.tsize(t) != 1 || shape.size() != 2) {
+            L.tclose(t);
+            L.dclose(d);
+            throw std::runtime_error("NetCDF_Reader::readString || not a character array: " + name);
+        }
+        std::vector<char> buf(shape[0] * shape[1] + 1, 0);
+        const herr_t rc = L.dread(d, t, kAll, kAll, kDefault, buf.data());
+        L.tclose(t);
+        L.dclose(d);
+        if (rc < 0) throw std::runtime_error("NetCDF_Reader || Error reading variable: " + name);
+        std::vector<std::string> out;
+        for (int64_t i = 0; i < shape[0]; i++) {
+            const char *p = buf.data() + i * shape[1];
+            std::string s(p, strnlen(p, shape[1]));
+            const size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+            out.push_back(a == std::string::npos ? std::string() : s.substr(a, b - a + 1));
+        }
+        return out;
+    }
+};
+
+std::string trim(const std::string &s) {
+    const size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+bool iequals(const std::string &a, const std::string &b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++)
+        if (std::tolower((unsigned char)a[i]) != std::tolower((unsigned char)b[i])) return false;
+    return true;
+}
+
+int indexOf(const std::vector<std::string> &v, const std::string &x) {
+    return (int)(std::find(v.begin(), v.end(), x) - v.begin());
+}
+
+std::vector<double> elemVar(const Reader &rd, const Model &m, const std::string &name) {
+    const int idx = indexOf(m.elemNamesAll, name);
+    if (idx == (int)m.elemNamesAll.size())
+        throw std::runtime_error("ExodusModel || elemental variable not found: " + name);
+    return rd.read<double>("vals_elem_var" + std::to_string(idx + 1) + "eb1");
+}
+
+}  // namespace
+
+bool Model::cartesian() const {
+    const auto it = records.find("crdsys");
+    if (it == records.end()) throw std::runtime_error("ExodusModel || no crdsys record.");
+    return it->second != "spherical";
+}
+
+int Model::sideSet(const std::string &name, int quad) const {
+    const int i = indexOf(ssNames, name);
+    return i == (int)ssNames.size() ? -1 : ssValues[i][quad];
+}
+
+// ExodusModel::getElementalVariables (ExodusModel.cpp:583-638): a nodal
+// variable NAME_k at node k, or an element variable at the quad's centre,
+// from the axis profile row nearest in radius (in z for Cartesian meshes),
+// the node moved towards the centre by the distance tolerance
+double Model::elemental(const std::string &name, int quad) const {
+    if (name == "element_type") return elemType[quad];
+    if (name == "dt") return elemDt[quad];
+    const bool cart = cartesian();
+    auto coordOf = [&](int node) { return cart ? nz[node] : std::hypot(ns[node], nz[node]); };
+    double cen = 0.;
+    for (int k = 0; k < 4; k++) cen += coordOf(conn[quad][k]);
+    cen /= 4.;
+    std::string base = name;
+    double coord = cen;
+    if (name.size() >= 2 && name[name.size() - 2] == '_') {
+        base = name.substr(0, name.size() - 2);
+        const int node = name.back() - '0';
+        if (node < 0 || node > 3) throw std::runtime_error("ExodusModel || bad nodal variable name: " + name);
+        coord = coordOf(conn[quad][node]);
+        coord += coord < cen ? distTol : -distTol;
+    }
+    const int v = indexOf(axisNames, base);
+    if (v == (int)axisNames.size())
+        throw std::runtime_error("ExodusModel::getElementalVariables || variable not found: " + name);
+    int best = 0;
+    double dmin = DBL_MAX;
+    for (size_t i = 0; i < axisCoords.size(); i++) {
+        const double d = std::abs(axisCoords[i] - coord);
+        if (d < dmin) {
+            dmin = d;
+            best = (int)i;
+        }
+    }
+    return axisValues[v][best];
+}
+
+Model load(const std::string &path) {
+    Reader rd(path);
+    Model m;
+    // ---- readRawData (ExodusModel.cpp:52-268)
+    m.globalNames = rd.strings("name_glo_var");
+    m.globalValues = rd.read<double>("vals_glo_var");
+    m.recordsRaw = rd.strings("info_records");
+    {
+        std::vector<int64_t> sh;
+        const std::vector<int> c = rd.read<int>("connect1", &sh);
+        if (sh.size() != 2 || sh[1] != 4) throw std::runtime_error("ExodusModel || connect1 is not [nquad][4].");
+        m.conn.resize(sh[0]);
+        for (int64_t q = 0; q < sh[0]; q++)
+            for (int k = 0; k < 4; k++) m.conn[q][k] = c[q * 4 + k] - 1;
+    }
+    m.ns = rd.read<double>("coordx");
+    m.nz = rd.read<double>("coordy");
+    if (m.ns.size() != m.nz.size()) throw std::runtime_error("ExodusModel || coordx and coordy differ in size.");
+    for (const auto &q : m.conn)
+        for (int k = 0; k < 4; k++)
+            if (q[k] < 0 || q[k] >= m.numNodes()) throw std::runtime_error("ExodusModel || node tag out of range.");
+    m.distTol = DBL_MAX;
+    for (const auto &q : m.conn)
+        for (int k = 0; k < 4; k++) {
+            const int a = q[k], b = q[(k + 1) % 4];
+            m.distTol = std::min(m.distTol, std::hypot(m.ns[a] - m.ns[b], m.nz[a] - m.nz[b]) / 1000.);
+        }
+    m.ssNames = rd.strings("ss_names");
+    m.ssValues.assign(m.ssNames.size(), std::vector<int>(m.numQuads(), -1));
+    for (size_t i = 0; i < m.ssNames.size(); i++) {
+        const std::vector<int> el = rd.read<int>("elem_ss" + std::to_string(i + 1));
+        const std::vector<int> sd = rd.read<int>("side_ss" + std::to_string(i + 1));
+        if (el.size() != sd.size()) throw std::runtime_error("ExodusModel || side set sizes differ.");
+        for (size_t j = 0; j < el.size(); j++) {
+            if (el[j] < 1 || el[j] > m.numQuads()) throw std::runtime_error("ExodusModel || side set quad out of range.");
+            m.ssValues[i][el[j] - 1] = sd[j] - 1;
+        }
+    }
+    bool cart = false;
+    for (const auto &r : m.recordsRaw) {
+        const size_t eq = r.find('=');
+        if (eq != std::string::npos && iequals(trim(r.substr(0, eq)), "crdsys") &&
+            iequals(trim(r.substr(eq + 1)), "cartesian")) {
+            cart = true;
+            break;
+        }
+    }
+    if (cart) {
+        m.globals.insert({"radius", 6371e3});
+    } else {
+        std::vector<int64_t> sh;
+        const std::vector<double> e = rd.read<double>("ellipticity", &sh);
+        if (sh.size() != 2 || sh[0] != 2) throw std::runtime_error("ExodusModel || ellipticity is not [2][n].");
+        m.ellipKnots.assign(e.begin(), e.begin() + sh[1]);
+        m.ellipCoeffs.assign(e.begin() + sh[1], e.end());
+    }
+    m.elemNamesAll = rd.strings("name_elem_var");
+    std::string ssAxis = cart ? "x0" : "t0";
+    if (!cart && indexOf(m.ssNames, "t1") != (int)m.ssNames.size()) ssAxis = "t1";
+    const int axisIdx = indexOf(m.ssNames, ssAxis);
+    if (axisIdx == (int)m.ssNames.size()) throw std::runtime_error("ExodusModel || no axis side set " + ssAxis);
+    // the depth profile: both ends of every axial edge above -tol, moved
+    // inwards by the tolerance, kept in ascending z (inserted at upper_bound)
+    std::vector<std::pair<int, int>> quadNodes;
+    for (int q = 0; q < m.numQuads(); q++) {
+        const int side = m.ssValues[axisIdx][q];
+        if (side < 0) continue;
+        const int other = side == 3 ? 0 : side + 1;
+        double z1 = m.nz[m.conn[q][side]], z2 = m.nz[m.conn[q][other]];
+        if (std::min(z1, z2) < -m.distTol) continue;
+        const double dir = (z2 - z1) / std::abs(z2 - z1);
+        z1 += dir * m.distTol;
+        z2 -= dir * m.distTol;
+        auto put = [&](double z, int node) {
+            const size_t at = std::upper_bound(m.axisCoords.begin(), m.axisCoords.end(), z) - m.axisCoords.begin();
+            m.axisCoords.insert(m.axisCoords.begin() + at, z);
+            quadNodes.insert(quadNodes.begin() + at, {q, node});
+        };
+        put(z1, side);
+        put(z2, other);
+    }
+    for (const auto &n : m.elemNamesAll) {
+        if (n.size() >= 2 && n[n.size() - 2] == '_') {
+            if (n.back() == '0') m.axisNames.push_back(n.substr(0, n.size() - 2));
+        } else {
+            m.axisNames.push_back(n);
+        }
+    }
+    for (const auto &v : m.axisNames) {
+        std::vector<double> col(m.axisCoords.size());
+        if (indexOf(m.elemNamesAll, v + "_0") != (int)m.elemNamesAll.size()) {
+            std::vector<double> buf[4];
+            for (int k = 0; k < 4; k++) buf[k] = elemVar(rd, m, v + "_" + std::to_string(k));
+            for (size_t i = 0; i < col.size(); i++) col[i] = buf[quadNodes[i].second][quadNodes[i].first];
+        } else {
+            const std::vector<double> buf = elemVar(rd, m, v);
+            for (size_t i = 0; i < col.size(); i++) col[i] = buf[quadNodes[i].first];
+        }
+        m.axisValues.push_back(col);
+    }
+    m.elemType = elemVar(rd, m, "element_type");
+    m.elemDt = elemVar(rd, m, "dt");
+    if ((int)m.elemType.size() != m.numQuads() || (int)m.elemDt.size() != m.numQuads())
+        throw std::runtime_error("ExodusModel || element variables do not match the quads.");
+
+    // ---- formStructured (ExodusModel.cpp:296-355)
+    if (m.globalValues.size() < m.globalNames.size())
+        throw std::runtime_error("ExodusModel || fewer global values than names.");
+    for (size_t i = 0; i < m.globalNames.size(); i++) {
+        const std::string n = m.globalNames[i] == "dt" ? "dt (nPol = 1)" : m.globalNames[i];
+        m.globals.insert({n, m.globalValues[i]});
+    }
+    for (const auto &r : m.recordsRaw) {
+        const size_t eq = r.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string key = trim(r.substr(0, eq));
+        if (key == "crdsys" || key == "model") m.records.insert({key, trim(r.substr(eq + 1))});
+    }
+    m.ssAxis = ssAxis;
+    m.ssSurface = m.cartesian() ? "y1" : "r1";
+    if (m.cartesian()) {
+        const double maxz = *std::max_element(m.nz.begin(), m.nz.end());
+        for (double &z : m.nz) z += m.rOuter() - maxz;
+    }
+
+    // ---- formAuxiliary (ExodusModel.cpp:357-430)
+    std::vector<std::vector<int>> ref(m.numNodes());
+    for (int q = 0; q < m.numQuads(); q++)
+        for (int k = 0; k < 4; k++) ref[m.conn[q][k]].push_back(q);
+    m.gllSpacing.assign(m.numNodes(), 0.);
+    for (int n = 0; n < m.numNodes(); n++)
+        for (int q : ref[n]) {
+            double d = 0.;
+            for (int k = 0; k < 4; k++) {
+                const int a = m.conn[q][k], b = m.conn[q][(k + 1) % 4];
+                d += std::hypot(m.ns[a] - m.ns[b], m.nz[a] - m.nz[b]);
+            }
+            m.gllSpacing[n] += d / 4. / spectral::nPol / ref[n].size();
+        }
+    const int ax = indexOf(m.ssNames, m.ssAxis);
+    for (int q = 0; q < m.numQuads(); q++) {
+        const int side = m.ssValues[ax][q];
+        if (side == 3 || side == -1) continue;
+        const std::array<int, 4> c = m.conn[q];
+        for (int j = 0; j < 4; j++) m.conn[q][j] = c[mapping::period0123(j + side - 3)];
+        for (auto &set : m.ssValues)
+            if (set[q] != -1) set[q] = mapping::period0123(set[q] - side + 3);
+    }
+    std::vector<bool> nearAxis(m.numNodes(), false), onAxis(m.numQuads(), false);
+    for (int q = 0; q < m.numQuads(); q++) {
+        if (m.ssValues[ax][q] == -1) continue;
+        for (int k = 0; k < 4; k++) nearAxis[m.conn[q][k]] = true;
+        onAxis[q] = true;
+    }
+    m.vicinal.assign(m.numQuads(), {-1, -1, -1, -1});
+    for (int q = 0; q < m.numQuads(); q++) {
+        if (onAxis[q]) continue;
+        for (int k = 0; k < 4; k++)
+            if (nearAxis[m.conn[q][k]]) m.vicinal[q][k] = k;
+    }
+    return m;
+}
+
+// Quad::Quad (Quad.cpp:37-110) and Quad::formIntegralFactor (Quad.cpp:527-547)
+QuadGeometry quadGeometry(const Model &m, int quad) {
+    QuadGeometry g;
+    mapping::QuadGeom q;
+    for (int k = 0; k < 4; k++) {
+        q.ns[k] = m.ns[m.conn[quad][k]];
+        q.nz[k] = m.nz[m.conn[quad][k]];
+    }
+    const double tol = m.distTol;
+    const double etype = m.elemType[quad];
+    double r[4];
+    for (int k = 0; k < 4; k++) r[k] = std::hypot(q.ns[k], q.nz[k]);
+    if (etype < .5) {
+        g.type = 0;
+        if (std::abs(r[0] - r[1]) < tol && std::abs(r[2] - r[3]) < tol)
+            g.curvedOuter = r[2] > r[0] ? 2 : 0;
+        else if (std::abs(r[1] - r[2]) < tol && std::abs(r[3] - r[0]) < tol)
+            g.curvedOuter = r[3] > r[1] ? 3 : 1;
+        else
+            throw std::runtime_error("Quad::Quad || Invalid spherical element shape.");
+    } else if (etype < 1.5) {
+        g.type = 1;
+        g.curvedOuter = -1;
+    } else {
+        g.type = 2;
+        if (std::abs(r[0] - r[1]) < tol && r[0] > r[2]) g.curvedOuter = 0;
+        else if (std::abs(r[1] - r[2]) < tol && r[1] > r[3]) g.curvedOuter = 1;
+        else if (std::abs(r[2] - r[3]) < tol && r[2] > r[0]) g.curvedOuter = 2;
+        else if (std::abs(r[3] - r[0]) < tol && r[3] > r[1]) g.curvedOuter = 3;
+        else throw std::runtime_error("Quad::Quad || Invalid semi-spherical element shape.");
+    }
+    q.curvedOuter = g.curvedOuter < 0 ? 2 : g.curvedOuter;
+    g.fluid = m.elemental("fluid", quad) > .5;
+    g.axial = m.sideSet(m.ssAxis, quad);
+    if (g.axial >= 0 && g.type != 1 && (g.axial + g.curvedOuter) % 2 == 0)
+        throw std::runtime_error("Quad::Quad || Conflict in axial setting.");
+    const auto &GLL = spectral::gll();
+    const auto &GLJ = spectral::glj();
+    const auto &bx = g.axial >= 0 ? GLJ : GLL;
+    for (int ipol = 0; ipol < spectral::nP; ipol++)
+        for (int jpol = 0; jpol < spectral::nP; jpol++) {
+            const int k = ipol * spectral::nP + jpol;
+            const double xi = bx.x[ipol], eta = GLL.x[jpol];
+            g.xieta[k][0] = xi;
+            g.xieta[k][1] = eta;
+            double s, z, J[2][2];
+            mapping::mapAny(g.type, q, xi, eta, s, z);
+            mapping::jacobianAny(g.type, q, xi, eta, J);
+            g.crd[k][0] = s;
+            g.crd[k][1] = z;
+            const double detJ = mapping::det2(J), w = bx.w[ipol] * GLL.w[jpol];
+            if (g.axial >= 0)
+                g.ifact[k] = ipol == 0 ? w * J[0][0] * detJ : w * s / (1. + xi) * detJ;
+            else
+                g.ifact[k] = w * s * detJ;
+        }
+    return g;
+}
+
+// Material::Material (Material.cpp:30-90): the four nodal values, bilinear in
+// (xi, eta) (Mapping::interpolate, Mapping.cpp:28-44)
+void nodalMaterial(const Model &m, int quad, const std::string &name, double out[25]) {
+    double v[4];
+    for (int k = 0; k < 4; k++) v[k] = m.elemental(name + "_" + std::to_string(k), quad);
+    const QuadGeometry g = quadGeometry(m, quad);
+    for (int k = 0; k < 25; k++) {
+        double N[4], dN[2][4];
+        mapping::linShape(g.xieta[k][0], g.xieta[k][1], N, dN);
+        out[k] = N[0] * v[0] + N[1] * v[1] + N[2] * v[2] + N[3] * v[3];
+    }
+}
+
+}  // namespace exodus
+}  // namespace axg
+
+// ===================================================================== C ABI
+struct axg_exodus {
+    axg::exodus::Model m;
+};
+
+namespace {
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::exception &e) {
+        axg::set_last_error(e.what());
+        return AXG_EINVAL;
+    }
+}
+
+int nullArg(const char *fn) {
+    axg::set_last_error(std::string(fn) + " || null argument.");
+    return AXG_EINVAL;
+}
+}  // namespace
+
+extern "C" int axg_exodus_open(const char *path, axg_exodus **ex) {
+    if (!path || !ex) return nullArg("axg_exodus_open");
+    *ex = nullptr;
+    return guarded([&] {
+        auto *e = new axg_exodus;
+        try {
+            e->m = axg::exodus::load(path);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *ex = e;
+        return AXG_OK;
+    });
+}
+
+extern "C" void axg_exodus_free(axg_exodus *ex) { delete ex; }
+
+extern "C" int axg_exodus_stats(const axg_exodus *ex, int64_t *stats, int nstats) {
+    if (!ex || !stats) return nullArg("axg_exodus_stats");
+    const auto &m = ex->m;
+    const int64_t v[9] = {m.numQuads(), m.numNodes(), (int64_t)m.globals.size(), (int64_t)m.ssNames.size(),
+                          (int64_t)m.axisCoords.size(), (int64_t)m.axisNames.size(), (int64_t)m.ellipKnots.size(),
+                          m.cartesian() ? 1 : 0, m.hasAttenuation() ? 1 : 0};
+    for (int i = 0; i < nstats; i++) stats[i] = i < 9 ? v[i] : 0;
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_names(const axg_exodus *ex, int32_t kind, char *buf, int64_t cap, int64_t *len) {
+    if (!ex || !len) return nullArg("axg_exodus_names");
+    const auto &m = ex->m;
+    std::string s;
+    auto add = [&](const std::string &x) { s += (s.empty() ? "" : "\n") + x; };
+    if (kind == 0) {
+        for (const auto &g : m.globals) add(g.first);
+    } else if (kind == 1) {
+        for (const auto &n : m.ssNames) add(n);
+    } else if (kind == 2) {
+        for (const auto &n : m.axisNames) add(n);
+    } else if (kind == 3) {
+        for (const auto &r : m.records) add(r.first + "=" + r.second);
+    } else {
+        axg::set_last_error("axg_exodus_names || kind must be 0..3.");
+        return AXG_EINVAL;
+    }
+    *len = (int64_t)s.size();
+    if (buf) {
+        if (cap < *len + 1) {
+            axg::set_last_error("axg_exodus_names || buffer too small.");
+            return AXG_EINVAL;
+        }
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    }
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_global(const axg_exodus *ex, const char *name, double *value) {
+    if (!ex || !name || !value) return nullArg("axg_exodus_global");
+    const auto it = ex->m.globals.find(name);
+    if (it == ex->m.globals.end()) {
+        axg::set_last_error(std::string("ExodusModel || global variable not found: ") + name);
+        return AXG_EINVAL;
+    }
+    *value = it->second;
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_nodes(const axg_exodus *ex, double *sz, double *gll_spacing, double *dist_tol) {
+    if (!ex) return nullArg("axg_exodus_nodes");
+    const auto &m = ex->m;
+    for (int n = 0; n < m.numNodes(); n++) {
+        if (sz) {
+            sz[2 * n] = m.ns[n];
+            sz[2 * n + 1] = m.nz[n];
+        }
+        if (gll_spacing) gll_spacing[n] = m.gllSpacing[n];
+    }
+    if (dist_tol) *dist_tol = m.distTol;
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_quads(const axg_exodus *ex, int32_t *conn, int32_t *vicinal) {
+    if (!ex) return nullArg("axg_exodus_quads");
+    const auto &m = ex->m;
+    for (int q = 0; q < m.numQuads(); q++)
+        for (int k = 0; k < 4; k++) {
+            if (conn) conn[4 * q + k] = m.conn[q][k];
+            if (vicinal) vicinal[4 * q + k] = m.vicinal[q][k];
+        }
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_side_set(const axg_exodus *ex, const char *name, int32_t *side) {
+    if (!ex || !name || !side) return nullArg("axg_exodus_side_set");
+    const auto &m = ex->m;
+    const int i = (int)(std::find(m.ssNames.begin(), m.ssNames.end(), name) - m.ssNames.begin());
+    if (i == (int)m.ssNames.size()) {
+        axg::set_last_error(std::string("ExodusModel || side set not found: ") + name);
+        return AXG_EINVAL;
+    }
+    for (int q = 0; q < m.numQuads(); q++) side[q] = m.ssValues[i][q];
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_elemental(const axg_exodus *ex, const char *name, double *values) {
+    if (!ex || !name || !values) return nullArg("axg_exodus_elemental");
+    return guarded([&] {
+        for (int q = 0; q < ex->m.numQuads(); q++) values[q] = ex->m.elemental(name, q);
+        return AXG_OK;
+    });
+}
+
+extern "C" int axg_exodus_profile(const axg_exodus *ex, const char *name, double *coords, double *values) {
+    if (!ex || !name) return nullArg("axg_exodus_profile");
+    const auto &m = ex->m;
+    const int v = (int)(std::find(m.axisNames.begin(), m.axisNames.end(), name) - m.axisNames.begin());
+    if (v == (int)m.axisNames.size()) {
+        axg::set_last_error(std::string("ExodusModel || profile variable not found: ") + name);
+        return AXG_EINVAL;
+    }
+    for (size_t i = 0; i < m.axisCoords.size(); i++) {
+        if (coords) coords[i] = m.axisCoords[i];
+        if (values) values[i] = m.axisValues[v][i];
+    }
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_ellipticity(const axg_exodus *ex, double *knots, double *coeffs) {
+    if (!ex) return nullArg("axg_exodus_ellipticity");
+    const auto &m = ex->m;
+    for (size_t i = 0; i < m.ellipKnots.size(); i++) {
+        if (knots) knots[i] = m.ellipKnots[i];
+        if (coeffs) coeffs[i] = m.ellipCoeffs[i];
+    }
+    return AXG_OK;
+}
+
+extern "C" int axg_exodus_geometry(const axg_exodus *ex, int32_t *info, double *crd, double *ifact) {
+    if (!ex) return nullArg("axg_exodus_geometry");
+    return guarded([&] {
+        for (int q = 0; q < ex->m.numQuads(); q++) {
+            const axg::exodus::QuadGeometry g = axg::exodus::quadGeometry(ex->m, q);
+            if (info) {
+                info[4 * q] = g.type;
+                info[4 * q + 1] = g.curvedOuter;
+                info[4 * q + 2] = g.axial;
+                info[4 * q + 3] = g.fluid;
+            }
+            for (int k = 0; k < 25; k++) {
+                if (crd) {
+                    crd[(q * 25 + k) * 2] = g.crd[k][0];
+                    crd[(q * 25 + k) * 2 + 1] = g.crd[k][1];
+                }
+                if (ifact) ifact[q * 25 + k] = g.ifact[k];
+            }
+        }
+        return AXG_OK;
+    });
+}
+
+extern "C" int axg_exodus_material(const axg_exodus *ex, const char *name, double *values) {
+    if (!ex || !name || !values) return nullArg("axg_exodus_material");
+    return guarded([&] {
+        for (int q = 0; q < ex->m.numQuads(); q++) axg::exodus::nodalMaterial(ex->m, q, name, values + 25 * q);
+        return AXG_OK;
+    });
+}

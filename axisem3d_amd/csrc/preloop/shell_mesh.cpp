@@ -28,6 +28,7 @@
 // the seeded Gaussian-random-field perturbations and the seeded Nu / Q / moment
 // tensor draws are this project's own synthetic workloads.
 #include "axg.h"
+#include "mapping.hpp"
 #include "nu_wisdom.hpp"
 #include "spectral.hpp"
 
@@ -60,87 +61,7 @@ constexpr double pi = 3.141592653589793238463;
 constexpr double tinyDouble = 1e-10;
 constexpr double tinySingle = 1e-5;
 
-inline int period0123(int p) { return ((p % 4) + 4) % 4; }
-
-// Mapping::sOrthogQ2 (Mapping.cpp:60-65)
-const double kQ2[4][2][2] = {{{-1., 0.}, {0., -1.}},
-                             {{0., -1.}, {1., 0.}},
-                             {{1., 0.}, {0., 1.}},
-                             {{0., 1.}, {-1., 0.}}};
-
-// XMath::makeClose (XMath.cpp:9-16)
-inline void makeClose(double &a, double &b) {
-    if (a - b > pi) b += 2. * pi;
-    if (b - a > pi) a += 2. * pi;
-}
-
-struct QuadGeom {
-    double ns[4], nz[4];  // nodal (s, z)
-    int curvedOuter;
-};
-
-struct Rot {
-    double r0, r3, t0, t1, t2, t3, xi, eta;
-    const double (*Q)[2];
-};
-
-inline Rot rotate(const QuadGeom &q, double xi, double eta) {
-    const double(*Q)[2] = kQ2[q.curvedOuter];
-    double rr[4], tt[4];
-    for (int i = 0; i < 4; i++) {
-        double s2 = Q[0][0] * q.ns[i] + Q[0][1] * q.nz[i];
-        double z2 = Q[1][0] * q.ns[i] + Q[1][1] * q.nz[i];
-        rr[i] = std::sqrt(s2 * s2 + z2 * z2);
-        tt[i] = std::atan2(s2, z2);
-    }
-    int co = q.curvedOuter;
-    Rot r;
-    r.Q = Q;
-    r.r0 = rr[period0123(co - 2)];
-    r.r3 = rr[period0123(co + 1)];
-    r.t0 = tt[period0123(co - 2)];
-    r.t1 = tt[period0123(co - 1)];
-    r.t2 = tt[period0123(co - 0)];
-    r.t3 = tt[period0123(co + 1)];
-    r.xi = Q[0][0] * xi + Q[0][1] * eta;
-    r.eta = Q[1][0] * xi + Q[1][1] * eta;
-    makeClose(r.t2, r.t3);
-    makeClose(r.t0, r.t1);
-    return r;
-}
-
-// SphericalMapping::mapping (SphericalMapping.cpp:9-38)
-inline void sphMapping(const QuadGeom &q, double xi, double eta, double &s, double &z) {
-    Rot R = rotate(q, xi, eta);
-    double a3 = ((1. - R.xi) * R.t3 + (1. + R.xi) * R.t2) / 2.;
-    double a0 = ((1. - R.xi) * R.t0 + (1. + R.xi) * R.t1) / 2.;
-    double s2 = (1. + R.eta) * R.r3 / 2. * std::sin(a3) + (1. - R.eta) * R.r0 / 2. * std::sin(a0);
-    double z2 = (1. + R.eta) * R.r3 / 2. * std::cos(a3) + (1. - R.eta) * R.r0 / 2. * std::cos(a0);
-    // Q2^T * sz2
-    s = R.Q[0][0] * s2 + R.Q[1][0] * z2;
-    z = R.Q[0][1] * s2 + R.Q[1][1] * z2;
-}
-
-// SphericalMapping::jacobian (SphericalMapping.cpp:40-78): J = Q2^T J2 Q2
-inline void sphJacobian(const QuadGeom &q, double xi, double eta, double J[2][2]) {
-    Rot R = rotate(q, xi, eta);
-    double a3 = ((1. - R.xi) * R.t3 + (1. + R.xi) * R.t2) / 2.;
-    double a0 = ((1. - R.xi) * R.t0 + (1. + R.xi) * R.t1) / 2.;
-    double J2[2][2];
-    J2[0][0] = (1. + R.eta) * R.r3 * (R.t2 - R.t3) / 4. * std::cos(a3) +
-               (1. - R.eta) * R.r0 * (R.t1 - R.t0) / 4. * std::cos(a0);
-    J2[0][1] = .5 * (R.r3 * std::sin(a3) - R.r0 * std::sin(a0));
-    J2[1][0] = -(1. + R.eta) * R.r3 * (R.t2 - R.t3) / 4. * std::sin(a3) -
-               (1. - R.eta) * R.r0 * (R.t1 - R.t0) / 4. * std::sin(a0);
-    J2[1][1] = .5 * (R.r3 * std::cos(a3) - R.r0 * std::cos(a0));
-    double T[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) T[i][j] = J2[i][0] * R.Q[0][j] + J2[i][1] * R.Q[1][j];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) J[i][j] = R.Q[0][i] * T[0][j] + R.Q[1][i] * T[1][j];
-}
-
-inline double det2(const double J[2][2]) { return J[0][0] * J[1][1] - J[0][1] * J[1][0]; }
+using namespace axg::mapping;
 
 // Mapping::invMapping (Mapping.cpp:20-31)
 bool invMapping(const QuadGeom &q, double s, double z, double &xi, double &eta) {

@@ -1,0 +1,128 @@
+"""Exodus meshes: the reference's ExodusModel (SOLVER/src/preloop/exodus/
+ExodusModel.cpp) over the library's reader, and the per-quad geometry and nodal
+material of the reference's Quad / Material (Quad.cpp:37-110, 527-547;
+Material.cpp:30-90).  The mesh files the reference ships are NetCDF-4 (HDF5);
+the HDF5 C library is loaded at run time (AXG_HDF5_LIB names it)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+_f64p = C.POINTER(C.c_double)
+
+
+def _lib():
+    from . import lib
+    return lib()
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(_lib().axg_last_error().decode())
+
+
+class ExodusModel:
+    """ExodusModel::initialize of one mesh file."""
+
+    def __init__(self, path):
+        self._h = C.c_void_p()
+        _check(_lib().axg_exodus_open(os.fsencode(path), C.byref(self._h)))
+        st = np.zeros(9, dtype=np.int64)
+        _check(_lib().axg_exodus_stats(self._h, st.ctypes.data_as(_abi.i64p), 9))
+        (self.nquad, self.nnode, _, _, self.nprofile, _, self.nellip, cart, att) = [int(x) for x in st]
+        self.cartesian, self.attenuation = bool(cart), bool(att)
+
+    def close(self):
+        if self._h:
+            _lib().axg_exodus_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _names(self, kind):
+        n = C.c_int64(0)
+        _check(_lib().axg_exodus_names(self._h, kind, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(_lib().axg_exodus_names(self._h, kind, buf, n.value + 1, C.byref(n)))
+        s = buf.value.decode()
+        return s.split("\n") if s else []
+
+    @property
+    def global_names(self):
+        return self._names(0)
+
+    @property
+    def side_set_names(self):
+        return self._names(1)
+
+    @property
+    def profile_names(self):
+        return self._names(2)
+
+    @property
+    def records(self):
+        return dict(x.split("=", 1) for x in self._names(3))
+
+    def global_var(self, name):
+        v = C.c_double(0.)
+        _check(_lib().axg_exodus_global(self._h, name.encode(), C.byref(v)))
+        return v.value
+
+    def nodes(self):
+        """(sz [nnode][2], average GLL spacing [nnode], distance tolerance)"""
+        sz = np.zeros((self.nnode, 2))
+        sp = np.zeros(self.nnode)
+        tol = C.c_double(0.)
+        _check(_lib().axg_exodus_nodes(self._h, sz.ctypes.data_as(_f64p), sp.ctypes.data_as(_f64p), C.byref(tol)))
+        return sz, sp, tol.value
+
+    def quads(self):
+        """(connectivity [nquad][4], vicinal [nquad][4])"""
+        conn = np.zeros((self.nquad, 4), dtype=np.int32)
+        vic = np.zeros((self.nquad, 4), dtype=np.int32)
+        _check(_lib().axg_exodus_quads(self._h, conn.ctypes.data_as(_abi.i32p), vic.ctypes.data_as(_abi.i32p)))
+        return conn, vic
+
+    def side_set(self, name):
+        out = np.zeros(self.nquad, dtype=np.int32)
+        _check(_lib().axg_exodus_side_set(self._h, name.encode(), out.ctypes.data_as(_abi.i32p)))
+        return out
+
+    def elemental(self, name):
+        out = np.zeros(self.nquad)
+        _check(_lib().axg_exodus_elemental(self._h, name.encode(), out.ctypes.data_as(_f64p)))
+        return out
+
+    def profile(self, name):
+        c = np.zeros(self.nprofile)
+        v = np.zeros(self.nprofile)
+        _check(_lib().axg_exodus_profile(self._h, name.encode(), c.ctypes.data_as(_f64p), v.ctypes.data_as(_f64p)))
+        return c, v
+
+    def ellipticity(self):
+        k = np.zeros(self.nellip)
+        c = np.zeros(self.nellip)
+        _check(_lib().axg_exodus_ellipticity(self._h, k.ctypes.data_as(_f64p), c.ctypes.data_as(_f64p)))
+        return k, c
+
+    def geometry(self):
+        """(info [nquad][4]: mapping, curved outer, axial side, fluid;
+        crd [nquad][25][2]; integral factors [nquad][25])"""
+        info = np.zeros((self.nquad, 4), dtype=np.int32)
+        crd = np.zeros((self.nquad, 25, 2))
+        fac = np.zeros((self.nquad, 25))
+        _check(_lib().axg_exodus_geometry(self._h, info.ctypes.data_as(_abi.i32p), crd.ctypes.data_as(_f64p),
+                                          fac.ctypes.data_as(_f64p)))
+        return info, crd, fac
+
+    def material(self, name):
+        """NAME_0..3 interpolated at the GLL points [nquad][25]"""
+        out = np.zeros((self.nquad, 25))
+        _check(_lib().axg_exodus_material(self._h, name.encode(), out.ctypes.data_as(_f64p)))
+        return out

@@ -348,6 +348,45 @@ int axg_write_stations_netcdf(const char *path, int64_t nstep, int32_t interval,
 int axg_netcdf_read(const char *path, const char *name, int64_t *shape, int32_t maxdims, int32_t *ndims,
                     double *values, int64_t cap, int64_t *n);
 
+/* ---- Exodus meshes ----------------------------------------------------------
+ * The reference's ExodusModel (SOLVER/src/preloop/exodus/ExodusModel.cpp):
+ * a NetCDF-4 (HDF5) Exodus file read through the HDF5 C library, loaded at run
+ * time (AXG_HDF5_LIB names it; else libhdf5.so on the loader path or under
+ * /opt/conda/lib), then structured and completed as ExodusModel::initialize
+ * does (readRawData, formStructured, formAuxiliary; ExodusModel.cpp:36-430):
+ * 0-based connectivity with every axial quad turned so that side 3 lies on the
+ * axis, side sets per quad, the depth profile along the axis, the average GLL
+ * spacing per node and the vicinal-axis flags. */
+typedef struct axg_exodus axg_exodus;
+int axg_exodus_open(const char *path, axg_exodus **ex);
+void axg_exodus_free(axg_exodus *ex);
+/* stats: quads, nodes, global variables, side sets, profile rows, profile
+ * variables, ellipticity knots, Cartesian (0/1), attenuation (0/1) */
+int axg_exodus_stats(const axg_exodus *ex, int64_t *stats, int nstats);
+/* names separated by '\n': kind 0 global variables, 1 side sets, 2 profile
+ * variables, 3 records (crdsys, model) as key=value; buf null: length only */
+int axg_exodus_names(const axg_exodus *ex, int32_t kind, char *buf, int64_t cap, int64_t *len);
+int axg_exodus_global(const axg_exodus *ex, const char *name, double *value);
+/* sz [nnode][2], gll_spacing [nnode], the distance tolerance (any may be null) */
+int axg_exodus_nodes(const axg_exodus *ex, double *sz, double *gll_spacing, double *dist_tol);
+/* conn [nquad][4], vicinal [nquad][4] (either may be null) */
+int axg_exodus_quads(const axg_exodus *ex, int32_t *conn, int32_t *vicinal);
+/* side [nquad] of a side set (r1 surface, t0 axis, solid_fluid_boundary, ...) */
+int axg_exodus_side_set(const axg_exodus *ex, const char *name, int32_t *side);
+/* ExodusModel::getElementalVariables of every quad: values [nquad] */
+int axg_exodus_elemental(const axg_exodus *ex, const char *name, double *values);
+/* the depth profile of one variable: coords (z) and values [profile rows] */
+int axg_exodus_profile(const axg_exodus *ex, const char *name, double *coords, double *values);
+int axg_exodus_ellipticity(const axg_exodus *ex, double *knots, double *coeffs);
+/* Quad::Quad and Quad::formIntegralFactor (Quad.cpp:37-110, 527-547):
+ * info [nquad][4] = mapping (0 spherical, 1 linear, 2 semi-spherical), curved
+ * outer side, axial side (-1 if none), fluid; crd [nquad][25][2] GLL (s, z);
+ * ifact [nquad][25] integral factors (any may be null) */
+int axg_exodus_geometry(const axg_exodus *ex, int32_t *info, double *crd, double *ifact);
+/* NAME_0..NAME_3 of every quad interpolated at its GLL points as
+ * Material::Material does (Material.cpp:30-90): values [nquad][25] */
+int axg_exodus_material(const axg_exodus *ex, const char *name, double *values);
+
 /* Features of every GLL point for the learned Nu predictor (no counterpart in
  * the reference; they describe what the reference's NuWisdom learns from):
  * out[npoint][AXG_NFEAT] = radius / outer radius, colatitude / pi, source period

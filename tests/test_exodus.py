@@ -1,0 +1,199 @@
+"""The Exodus front end: ExodusModel (ExodusModel.cpp) on the mesh the
+reference ships (template/input/AxiSEM_prem_ani_one_crust_50.e, a NetCDF-4 /
+HDF5 file kept under tests/golden), and the geometry and nodal material of its
+quads (Quad.cpp:37-110, 527-547; Material.cpp:30-90; the three mappings).
+
+Pins:
+* the raw arrays equal what h5dump, the HDF5 distribution's own dump tool,
+  prints for the same datasets (coordinates exactly, connectivity up to the
+  rotation of the axial quads, which turns side 3 onto the axis);
+* what the geometry fixes: the integral factors of all 2016 quads (spherical,
+  linear and semi-spherical) sum to the volume of the sphere, those of the
+  fluid quads to the outer core's shell; the GLL corners are the nodes; axial
+  quads have nodes 3 and 0 on the axis; the surface side set lies on r = R and
+  the solid-fluid side set on the core radii;
+* what the model fixes: the depth profile and the material of every node of a
+  spherical quad equal PREM (Dziewonski & Anderson 1981, Table 1 polynomials)
+  below a radius of 5590 km, the central cube's nodes PREM at a profile row
+  within one element of them; the mass of the meshed Earth is PREM's
+  5.974e24 kg.
+The Cartesian branch (crdsys = cartesian: axis x0, surface y1, shift to the
+north pole) has no such mesh here and is parity unpinned.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import axisem3d_amd as ax
+
+MESH = os.path.join(os.path.dirname(__file__), "golden", "AxiSEM_prem_ani_one_crust_50.e")
+R = 6371e3
+R_ICB, R_CMB = 1221.5e3, 3480e3
+
+
+@pytest.fixture(scope="module")
+def model():
+    return ax.ExodusModel(MESH)
+
+
+def _h5dump(name, tmp):
+    exe = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if not os.path.exists(exe):
+        pytest.skip("h5dump not available")
+    out = os.path.join(tmp, name + ".txt")
+    subprocess.run([exe, "-d", name, "-y", "-w", "0", "-m", "%.17g", "-o", out, MESH], check=True,
+                   stdout=subprocess.DEVNULL, timeout=60)
+    with open(out) as f:
+        return np.array([float(x) for x in f.read().replace(",", " ").split()])
+
+
+def test_counts_names_records(model):
+    assert (model.nquad, model.nnode) == (2016, 2115)
+    assert not model.cartesian and model.attenuation
+    assert model.side_set_names == ["r1", "solid_fluid_boundary", "t0"]
+    assert model.records == {"crdsys": "spherical", "model": "prem_ani_one_crust"}
+    # the reference renames the global "dt" (ExodusModel::formStructured)
+    assert "dt (nPol = 1)" in model.global_names and "dt" not in model.global_names
+    assert model.global_var("radius") == R
+    assert model.global_var("nr_lin_solids") == 3
+    for v in ("RHO", "VPV", "VPH", "VSV", "VSH", "ETA", "QMU", "QKAPPA", "fluid", "element_type", "dt"):
+        assert v in model.profile_names
+    k, c = model.ellipticity()
+    assert len(k) == 2198 and np.all(np.diff(k) >= 0)
+    with pytest.raises(RuntimeError, match="not found"):
+        model.global_var("nothing")
+    with pytest.raises(RuntimeError, match="not found"):
+        model.side_set("nothing")
+
+
+def test_raw_arrays_against_h5dump(model, tmp_path):
+    sz, spacing, tol = model.nodes()
+    np.testing.assert_array_equal(sz[:, 0], _h5dump("coordx", str(tmp_path)))
+    np.testing.assert_array_equal(sz[:, 1], _h5dump("coordy", str(tmp_path)))
+    raw = _h5dump("connect1", str(tmp_path)).astype(np.int64).reshape(-1, 4) - 1
+    conn, vic = model.quads()
+    axis = model.side_set("t0")
+    for q in range(model.nquad):
+        if axis[q] < 0:
+            np.testing.assert_array_equal(conn[q], raw[q])
+        else:
+            # a cyclic rotation of the file's order
+            assert any((np.roll(raw[q], -k) == conn[q]).all() for k in range(4)), q
+    # distance tolerance: the shortest quad edge / 1000 (ExodusModel.cpp:67-82)
+    e = np.linalg.norm(sz[raw] - sz[np.roll(raw, -1, axis=1)], axis=2)
+    assert tol == pytest.approx(e.min() / 1000., rel=1e-14)
+    # vicinal flags: nodes of non-axial quads that touch an axial quad
+    near = np.zeros(model.nnode, bool)
+    near[conn[axis >= 0].ravel()] = True
+    exp = np.where(near[conn] & (axis[:, None] < 0), np.arange(4)[None, :], -1)
+    np.testing.assert_array_equal(vic, exp)
+    assert (spacing > 0).all()
+
+
+def test_side_sets_and_axis(model):
+    sz, _, tol = model.nodes()
+    conn, _ = model.quads()
+    r = np.hypot(sz[:, 0], sz[:, 1])
+    axis = model.side_set("t0")
+    assert (axis[axis >= 0] == 3).all() and (axis >= 0).sum() == 68
+    q = np.flatnonzero(axis >= 0)
+    assert np.abs(sz[conn[q, 3], 0]).max() < tol and np.abs(sz[conn[q, 0], 0]).max() < tol
+    for name, radii in (("r1", [R]), ("solid_fluid_boundary", [R_ICB, R_CMB])):
+        side = model.side_set(name)
+        qs = np.flatnonzero(side >= 0)
+        assert len(qs) > 0
+        for qq in qs:
+            k = side[qq]
+            for node in (conn[qq, k], conn[qq, (k + 1) % 4]):
+                assert min(abs(r[node] - x) for x in radii) < 1e-6 * R, (name, qq)
+
+
+def test_geometry_closed_forms(model):
+    info, crd, fac = model.geometry()
+    assert np.bincount(info[:, 0]).tolist() == [1696, 192, 128]
+    vol = 2 * np.pi * fac.sum()
+    assert vol == pytest.approx(4. / 3. * np.pi * R ** 3, rel=1e-12)
+    fluid = info[:, 3] == 1
+    vf = 2 * np.pi * fac[fluid].sum()
+    assert vf == pytest.approx(4. / 3. * np.pi * (R_CMB ** 3 - R_ICB ** 3), rel=1e-12)
+    # fluid exactly the quads between the core radii
+    sz, _, _ = model.nodes()
+    conn, _ = model.quads()
+    rc = np.hypot(sz[conn, 0], sz[conn, 1]).mean(axis=1)
+    np.testing.assert_array_equal(fluid, (rc > R_ICB) & (rc < R_CMB))
+    # GLL corners are the nodes: (xi, eta) = (-1,-1), (1,-1), (1,1), (-1,1)
+    for k, ipnt in enumerate((0, 20, 24, 4)):
+        np.testing.assert_allclose(crd[:, ipnt], sz[conn[:, k]], rtol=0, atol=1e-6)
+    # the integral factor of axial points on the axis: s = 0 there
+    ax_q = info[:, 2] >= 0
+    assert np.abs(crd[ax_q][:, :5, 0]).max() < 1e-3
+
+
+def _prem(r):
+    """PREM (Dziewonski & Anderson 1981, Table 1) below r = 5600 km:
+    rho [kg/m3], vp, vs [m/s]."""
+    x = r / R
+    if r <= R_ICB:
+        return (13.0885 - 8.8381 * x ** 2, 11.2622 - 6.3640 * x ** 2, 3.6678 - 4.4475 * x ** 2)
+    if r <= R_CMB:
+        return (12.5815 - 1.2638 * x - 3.6426 * x ** 2 - 5.5281 * x ** 3,
+                11.0487 - 4.0362 * x + 4.8023 * x ** 2 - 13.5732 * x ** 3, 0.)
+    rho = 7.9565 - 6.4761 * x + 5.5283 * x ** 2 - 3.0807 * x ** 3
+    if r <= 3630e3:
+        return (rho, 15.3891 - 5.3181 * x + 5.5242 * x ** 2 - 2.5514 * x ** 3,
+                6.9254 + 1.4672 * x - 2.0834 * x ** 2 + 0.9783 * x ** 3)
+    return (rho, 24.9520 - 40.4673 * x + 51.4832 * x ** 2 - 26.6419 * x ** 3,
+            11.1671 - 13.7818 * x + 17.4575 * x ** 2 - 9.2777 * x ** 3)
+
+
+def test_material_is_prem(model):
+    # the depth profile along the axis (z = r there; the rows sit one distance
+    # tolerance, 24.4 m, inside their nodes, the values are the nodes')
+    for name, col, scale in (("RHO", 0, 1e3), ("VPV", 1, 1e3), ("VSV", 2, 1e3)):
+        z, v = model.profile(name)
+        for zi, vi in zip(z, v):
+            if 0 <= zi < 5590e3:
+                assert vi == pytest.approx(_prem(zi)[col] * scale, rel=1e-5, abs=1e-6), (name, zi)
+    # every node of a spherical quad below 5590 km lies on a radial shell the
+    # axis samples: PREM at its own radius (layer chosen by the quad's centre).
+    # Nodes of the central cube (linear and semi-spherical quads) take the
+    # nearest profile row (ExodusModel::getElementalVariables): PREM at a radius
+    # within one element size of theirs.
+    info, crd, fac = model.geometry()
+    sz, _, _ = model.nodes()
+    conn, _ = model.quads()
+    rn = np.hypot(sz[conn, 0], sz[conn, 1])
+    rc = rn.mean(axis=1)
+    zrow, _ = model.profile("RHO")
+    for name, col in (("RHO", 0), ("VPV", 1), ("VPH", 1), ("VSV", 2), ("VSH", 2)):
+        for k in range(4):
+            got = model.elemental("%s_%d" % (name, k))
+            for q in np.flatnonzero(rn.max(axis=1) < 5590e3):
+                inward = 1. if rn[q, k] < rc[q] else -1.
+                if info[q, 0] == 0:
+                    exp = _prem(rn[q, k] + inward * 100.)[col] * 1e3
+                    assert got[q] == pytest.approx(exp, rel=2e-4, abs=1e-6), (name, k, q)
+                else:
+                    h = np.ptp(rn[q]) + 1e3
+                    rows = zrow[np.abs(zrow - rn[q, k]) <= h]
+                    cand = [_prem(max(z, 0.))[col] * 1e3 for z in rows]
+                    assert min(abs(got[q] - c) for c in cand) <= 1e-5 * abs(got[q]) + 1e-6, (name, k, q)
+    # mass of the meshed Earth, the density bilinear in every quad
+    mass = 2 * np.pi * (model.material("RHO") * fac).sum()
+    assert mass == pytest.approx(5.974e24, rel=1e-3)
+    # quad-wise variables
+    et = model.elemental("element_type")
+    np.testing.assert_array_equal(np.round(et).astype(int), info[:, 0])
+    assert (model.elemental("dt") > 0).all()
+
+
+def test_open_errors(tmp_path):
+    with pytest.raises(RuntimeError, match="Error opening"):
+        ax.ExodusModel(str(tmp_path / "missing.e"))
+    p = str(tmp_path / "classic.nc")
+    ax.write_wisdom(p, np.ones((2, 4)), netcdf=True)
+    with pytest.raises(RuntimeError, match="Error opening"):
+        ax.ExodusModel(p)
