@@ -201,6 +201,7 @@ def declare_mesh_api(lib):
         "axg_exodus_ellipticity": [vp, f64p, f64p],
         "axg_exodus_geometry": [vp, i32p, f64p, f64p],
         "axg_exodus_material": [vp, C.c_char_p, f64p],
+        "axg_exodus_gll": [vp, i32p, i64p],
     }.items():
         getattr(lib, name).argtypes = args
         getattr(lib, name).restype = C.c_int

@@ -465,6 +465,46 @@ QuadGeometry quadGeometry(const Model &m, int quad) {
     return g;
 }
 
+// A point is identified by its node (corners), by its edge and its position
+// counted from the edge's lower node tag (edge interiors), or by its quad
+// (interior); numbering at first sight in the scan gives the numbers of
+// Connectivity::formElemToGLL, whose shared edges run in opposite directions
+// in the two quads of a consistently oriented mesh (get_shared_DOF_quad).
+std::vector<std::array<int, 25>> elemToGLL(const Model &m, int &ngll) {
+    const int P = spectral::nPol;
+    std::map<std::array<int64_t, 3>, int> seen;
+    std::vector<std::array<int, 25>> out(m.numQuads());
+    ngll = 0;
+    for (int q = 0; q < m.numQuads(); q++) {
+        const auto &c = m.conn[q];
+        for (int ipol = 0; ipol <= P; ipol++)
+            for (int jpol = 0; jpol <= P; jpol++) {
+                std::array<int64_t, 3> key;
+                const bool ei = ipol == 0 || ipol == P, ej = jpol == 0 || jpol == P;
+                if (ei && ej) {
+                    // corner: nodes 0 (0,0), 1 (P,0), 2 (P,P), 3 (0,P)
+                    const int k = ipol == 0 ? (jpol == 0 ? 0 : 3) : (jpol == 0 ? 1 : 2);
+                    key = {0, c[k], 0};
+                } else if (ei || ej) {
+                    // edge k from node k to node k+1, position t from node k
+                    int k, t;
+                    if (jpol == 0) { k = 0; t = ipol; }
+                    else if (ipol == P) { k = 1; t = jpol; }
+                    else if (jpol == P) { k = 2; t = P - ipol; }
+                    else { k = 3; t = P - jpol; }
+                    const int a = c[k], b = c[(k + 1) % 4];
+                    key = {1, (int64_t)std::min(a, b) * m.numNodes() + std::max(a, b), a < b ? t : P - t};
+                } else {
+                    key = {2, q, ipol * (P + 1) + jpol};
+                }
+                auto it = seen.find(key);
+                if (it == seen.end()) it = seen.insert({key, ngll++}).first;
+                out[q][ipol * (P + 1) + jpol] = it->second;
+            }
+    }
+    return out;
+}
+
 // Material::Material (Material.cpp:30-90): the four nodal values, bilinear in
 // (xi, eta) (Mapping::interpolate, Mapping.cpp:28-44)
 void nodalMaterial(const Model &m, int quad, const std::string &name, double out[25]) {
@@ -667,6 +707,19 @@ extern "C" int axg_exodus_material(const axg_exodus *ex, const char *name, doubl
     if (!ex || !name || !values) return nullArg("axg_exodus_material");
     return guarded([&] {
         for (int q = 0; q < ex->m.numQuads(); q++) axg::exodus::nodalMaterial(ex->m, q, name, values + 25 * q);
+        return AXG_OK;
+    });
+}
+
+extern "C" int axg_exodus_gll(const axg_exodus *ex, int32_t *elem_gll, int64_t *ngll) {
+    if (!ex || !ngll) return nullArg("axg_exodus_gll");
+    return guarded([&] {
+        int n = 0;
+        const auto e2g = axg::exodus::elemToGLL(ex->m, n);
+        *ngll = n;
+        if (elem_gll)
+            for (size_t q = 0; q < e2g.size(); q++)
+                for (int k = 0; k < 25; k++) elem_gll[q * 25 + k] = e2g[q][k];
         return AXG_OK;
     });
 }

@@ -65,6 +65,11 @@ struct QuadGeometry {
     double xieta[25][2];  // reference coordinates (GLJ in xi on axial quads)
 };
 QuadGeometry quadGeometry(const Model &m, int quad);
+// Connectivity::formElemToGLL (SOLVER/src/preloop/graph/Connectivity.cpp:42-93):
+// global GLL point numbers [quad][ipol * 5 + jpol], new points numbered in the
+// order quads and then (ipol, jpol) are scanned, a point of a lower-numbered
+// quad sharing a node or an edge keeping its number
+std::vector<std::array<int, 25>> elemToGLL(const Model &m, int &ngll);
 // Mapping::interpolate of NAME_0 .. NAME_3 at the GLL points (Material.cpp:30-90)
 void nodalMaterial(const Model &m, int quad, const std::string &name, double out[25]);
 

@@ -126,3 +126,11 @@ class ExodusModel:
         out = np.zeros((self.nquad, 25))
         _check(_lib().axg_exodus_material(self._h, name.encode(), out.ctypes.data_as(_f64p)))
         return out
+
+    def gll(self):
+        """Global GLL point numbers [nquad][25] (Connectivity::formElemToGLL)
+        and their count."""
+        out = np.zeros((self.nquad, 25), dtype=np.int32)
+        n = C.c_int64(0)
+        _check(_lib().axg_exodus_gll(self._h, out.ctypes.data_as(_abi.i32p), C.byref(n)))
+        return out, n.value

@@ -383,6 +383,11 @@ int axg_exodus_ellipticity(const axg_exodus *ex, double *knots, double *coeffs);
  * outer side, axial side (-1 if none), fluid; crd [nquad][25][2] GLL (s, z);
  * ifact [nquad][25] integral factors (any may be null) */
 int axg_exodus_geometry(const axg_exodus *ex, int32_t *info, double *crd, double *ifact);
+/* global GLL point numbers elem_gll [nquad][25] (ipol * 5 + jpol) and their
+ * count, numbered as Connectivity::formElemToGLL does (Connectivity.cpp:42-93:
+ * quads in order, new points in (ipol, jpol) order, shared points keeping the
+ * number of the lowest quad that holds them); elem_gll may be null */
+int axg_exodus_gll(const axg_exodus *ex, int32_t *elem_gll, int64_t *ngll);
 /* NAME_0..NAME_3 of every quad interpolated at its GLL points as
  * Material::Material does (Material.cpp:30-90): values [nquad][25] */
 int axg_exodus_material(const axg_exodus *ex, const char *name, double *values);

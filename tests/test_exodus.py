@@ -190,6 +190,40 @@ def test_material_is_prem(model):
     assert (model.elemental("dt") > 0).all()
 
 
+def test_gll_numbering(model):
+    """Connectivity::formElemToGLL: a conforming quad mesh of order 4 has
+    nnode + 3 nedge + 9 nquad points; every occurrence of a point has the same
+    coordinates in every quad holding it; numbers appear in scan order."""
+    e2g, n = model.gll()
+    conn, _ = model.quads()
+    edges = {tuple(sorted((conn[q, k], conn[q, (k + 1) % 4]))) for q in range(model.nquad) for k in range(4)}
+    assert n == model.nnode + 3 * len(edges) + 9 * model.nquad
+    assert e2g.min() == 0 and e2g.max() == n - 1 and len(np.unique(e2g)) == n
+    np.testing.assert_array_equal(e2g[0], np.arange(25))
+    info, crd, fac = model.geometry()
+    first = np.full((n, 2), np.nan)
+    for q in range(model.nquad):
+        for k in range(25):
+            g = e2g[q, k]
+            if np.isnan(first[g, 0]):
+                first[g] = crd[q, k]
+            else:
+                assert np.abs(first[g] - crd[q, k]).max() < 1e-6 * R, (q, k)
+    # scan order: a quad introduces numbers above all earlier ones
+    top = -1
+    for q in range(model.nquad):
+        new = [g for g in e2g[q] if g > top]
+        if new:
+            assert new == list(range(top + 1, top + 1 + len(new))), q
+            top = max(new)
+    # consistently oriented quads: positive integral factors off the axis
+    off = ~((info[:, 2] >= 0)[:, None] & (np.arange(25) < 5)[None, :])
+    assert (fac[off] > 0).all()
+    # each point's assembled integral factor: the volume again
+    assert 2 * np.pi * np.bincount(e2g.ravel(), fac.ravel()).sum() == pytest.approx(4. / 3. * np.pi * R ** 3,
+                                                                                     rel=1e-12)
+
+
 def test_open_errors(tmp_path):
     with pytest.raises(RuntimeError, match="Error opening"):
         ax.ExodusModel(str(tmp_path / "missing.e"))
