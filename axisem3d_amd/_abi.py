@@ -139,6 +139,7 @@ class ShellConfig(C.Structure):
         ("rec_dump", C.c_int32),
         ("nvol3d", C.c_int32),
         ("vol3d", C.POINTER(Volumetric3D)),
+        ("exodus", C.c_void_p),
     ]
 
 

@@ -522,10 +522,6 @@ void nodalMaterial(const Model &m, int quad, const std::string &name, double out
 }  // namespace axg
 
 // ===================================================================== C ABI
-struct axg_exodus {
-    axg::exodus::Model m;
-};
-
 namespace {
 template <class F>
 int guarded(F &&f) {

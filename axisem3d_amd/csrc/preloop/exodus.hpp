@@ -75,3 +75,7 @@ void nodalMaterial(const Model &m, int quad, const std::string &name, double out
 
 }  // namespace exodus
 }  // namespace axg
+
+struct axg_exodus {
+    axg::exodus::Model m;
+};

@@ -28,6 +28,7 @@ inline void makeClose(double &a, double &b) {
 struct QuadGeom {
     double ns[4], nz[4];  // nodal (s, z)
     int curvedOuter;
+    int type = 0;         // 0 spherical, 1 linear, 2 semi-spherical
 };
 
 struct Rot {

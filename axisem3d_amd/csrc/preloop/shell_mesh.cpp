@@ -28,6 +28,7 @@
 // the seeded Gaussian-random-field perturbations and the seeded Nu / Q / moment
 // tensor draws are this project's own synthetic workloads.
 #include "axg.h"
+#include "exodus.hpp"
 #include "mapping.hpp"
 #include "nu_wisdom.hpp"
 #include "spectral.hpp"
@@ -68,11 +69,11 @@ bool invMapping(const QuadGeom &q, double s, double z, double &xi, double &eta) 
     xi = eta = 0.;
     for (int it = 1; it <= 10; it++) {
         double ms, mz;
-        sphMapping(q, xi, eta, ms, mz);
+        mapAny(q.type, q, xi, eta, ms, mz);
         double ds = s - ms, dz = z - mz;
         if (std::sqrt(ds * ds + dz * dz) < 1e-7) return true;
         double J[2][2];
-        sphJacobian(q, xi, eta, J);
+        jacobianAny(q.type, q, xi, eta, J);
         double d = det2(J);
         xi += (J[1][1] * ds - J[0][1] * dz) / d;
         eta += (-J[1][0] * ds + J[0][0] * dz) / d;
@@ -180,6 +181,8 @@ struct QuadInfo {
     QuadGeom g;
     int nodes[4];
     int it, ir, layer;
+    int id = -1;      // quad of an Exodus mesh
+    int sfSide = -1;  // its side on the solid-fluid boundary side set
     bool axial, fluid, south;
     int gll[nP * nP];          // global point index of (ipol, jpol)
     int vicinal[4];            // ExodusModel::mVicinalAxis
@@ -454,6 +457,76 @@ struct Builder {
 
     explicit Builder(const axg_shell_config &cfg) : c(cfg) {}
 
+    // ---- a mesh read from an Exodus file (ExodusModel) instead of the shell
+    const exodus::Model *exo = nullptr;
+    // vpv, vph, vsv, vsh, rho, eta of every GLL point of every quad
+    std::vector<std::array<std::array<double, 6>, nP * nP>> exoMat;
+
+    // Quad::Quad over the Exodus model (Quad.cpp:37-110): mapping, curved side,
+    // axial side (3 after ExodusModel::formAuxiliary), fluid flag; global point
+    // numbers of Connectivity::formElemToGLL; the average GLL spacing and
+    // vicinal flags of formAuxiliary; the nodal material of Material::Material
+    void buildTopologyExodus() {
+        const exodus::Model &M = *exo;
+        int ng = 0;
+        const auto e2g = exodus::elemToGLL(M, ng);
+        npoint = ng;
+        rnode = {0., M.rOuter()};
+        const int sf = (int)(std::find(M.ssNames.begin(), M.ssNames.end(), "solid_fluid_boundary") - M.ssNames.begin());
+        const bool iso = std::find(M.elemNamesAll.begin(), M.elemNamesAll.end(), "VP_0") != M.elemNamesAll.end();
+        if (!iso && c.material == AXG_MAT_ISOTROPIC)
+            throw std::runtime_error("ExodusModel || the mesh's model is anisotropic: use the transversely "
+                                     "isotropic material.");
+        const auto &GLL = spectral::gll();
+        const auto &GLJ = spectral::glj();
+        quads.resize(M.numQuads());
+        exoMat.resize(M.numQuads());
+        for (int e = 0; e < M.numQuads(); e++) {
+            const exodus::QuadGeometry G = exodus::quadGeometry(M, e);
+            QuadInfo &q = quads[e];
+            q.id = e;
+            q.it = q.ir = -1;
+            q.layer = 0;
+            q.fluid = G.fluid != 0;
+            q.south = false;
+            q.axial = G.axial >= 0;
+            q.g.type = G.type;
+            q.g.curvedOuter = G.curvedOuter < 0 ? 2 : G.curvedOuter;
+            for (int k = 0; k < 4; k++) {
+                q.nodes[k] = M.conn[e][k];
+                q.g.ns[k] = M.ns[M.conn[e][k]];
+                q.g.nz[k] = M.nz[M.conn[e][k]];
+                q.vicinal[k] = M.vicinal[e][k];
+            }
+            q.sfSide = sf < (int)M.ssNames.size() ? M.ssValues[sf][e] : -1;
+            // the side normals below are those of an arc: every side of a
+            // spherical quad, the curved side of a semi-spherical one
+            if (q.sfSide >= 0 && (G.type == 1 || (G.type == 2 && q.sfSide != G.curvedOuter) ||
+                                  (q.axial && q.sfSide % 2 == 1)))
+                throw std::runtime_error("Quad::computeNormal || solid-fluid side that is not an arc.");
+            const auto &bx = q.axial ? GLJ : GLL;
+            for (int i = 0; i < nP; i++) {
+                q.xi[i] = bx.x[i];
+                q.wxi[i] = bx.w[i];
+                q.eta[i] = GLL.x[i];
+                q.weta[i] = GLL.w[i];
+            }
+            for (int k = 0; k < nP * nP; k++) q.gll[k] = e2g[e][k];
+            const char *names[6] = {"VPV", "VPH", "VSV", "VSH", "RHO", "ETA"};
+            const char *namesIso[6] = {"VP", "VP", "VS", "VS", "RHO", nullptr};
+            for (int v = 0; v < 6; v++) {
+                double val[nP * nP];
+                const char *nm = iso ? namesIso[v] : names[v];
+                if (nm) exodus::nodalMaterial(M, e, nm, val);
+                for (int k = 0; k < nP * nP; k++) exoMat[e][k][v] = nm ? val[k] : 1.;
+            }
+        }
+        nodeSpacing = M.gllSpacing;
+        pointElems.assign(npoint, {});
+        for (size_t e = 0; e < quads.size(); e++)
+            for (int k = 0; k < nP * nP; k++) pointElems[quads[e].gll[k]].push_back({(int)e, k});
+    }
+
     int nodeTag(int it, int ir) const { return ir * (NT + 1) + it; }
     int pointTag(int gt, int gr) const { return gr * ngt + gt; }
 
@@ -586,9 +659,9 @@ struct Builder {
                     for (int jpol = 0; jpol < nP; jpol++) {
                         int k = ipol * nP + jpol;
                         double xi = q.xi[ipol], eta = q.eta[jpol];
-                        sphMapping(q.g, xi, eta, q.crd[k][0], q.crd[k][1]);
+                        mapAny(q.g.type, q.g, xi, eta, q.crd[k][0], q.crd[k][1]);
                         if (q.axial && ipol == 0) q.crd[k][0] = 0.;
-                        sphJacobian(q.g, xi, eta, q.J[k]);
+                        jacobianAny(q.g.type, q.g, xi, eta, q.J[k]);
                         double detJ = det2(q.J[k]);
                         double wxw = q.wxi[ipol] * q.weta[jpol];
                         double s = q.crd[k][0];
@@ -817,7 +890,17 @@ struct Builder {
 static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     Builder B(c);
     B.nthread = c.nthread > 0 ? c.nthread : (int)std::max(1u, std::thread::hardware_concurrency());
-    B.buildTopology();
+    if (c.exodus) {
+        // Exodus meshes carry their own model; what the shell generator adds
+        // on top of its layers stays off
+        if (c.undulation != 0 || c.ocean_depth > 0. || c.attenuation != AXG_ATT_NONE)
+            throw std::runtime_error("ExodusModel || undulation, ocean load and attenuation are not available "
+                                     "for Exodus meshes.");
+        B.exo = &c.exodus->m;
+        B.buildTopologyExodus();
+    } else {
+        B.buildTopology();
+    }
     B.buildGeometry();
     B.buildRelabelling();
     // coordinates of the points, with the GLLPoint::setup distance check
@@ -861,8 +944,8 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         std::uniform_real_distribution<double> uq(c.q_min, c.q_max);
         std::uniform_real_distribution<double> u11(-1., 1.);
         for (int e = 0; e < NQ; e++) {
-            const auto &L = B.layers[B.quads[e].layer].p;
-            elemQmu[e] = L.q_mu > 0. ? L.q_mu : uq(rng);
+            const double qmu = B.exo ? 0. : B.layers[B.quads[e].layer].p.q_mu;
+            elemQmu[e] = qmu > 0. ? qmu : uq(rng);
             for (auto &v : elemAnisoDir[e]) v = u11(rng);
         }
     }
@@ -887,10 +970,21 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     if (c.nvol3d < 0 || (c.nvol3d > 0 && !c.vol3d)) throw std::runtime_error("Volumetric3D || bad model list.");
     const bool vol3d = c.nvol3d > 0;
     auto sampleAt = [&](const QuadInfo &q, int k, double phi) {
-        const Layer &L = B.layers[q.layer];
         const double s = q.crd[k][0], z = q.crd[k][1];
         const double r = std::hypot(s, z);
-        const double vp = L.vp(r), vs = L.vs(r), rho = L.rho(r);
+        // the 1-D values: the layer's, or the Exodus mesh's nodal material
+        const std::array<double, 6> *em = B.exo ? &B.exoMat[q.id][k] : nullptr;
+        double vp, vs, rho;
+        if (em) {
+            vp = (*em)[1];
+            vs = (*em)[2];
+            rho = (*em)[4];
+        } else {
+            const Layer &L = B.layers[q.layer];
+            vp = L.vp(r);
+            vs = L.vs(r);
+            rho = L.rho(r);
+        }
         double dvs = 0., dvp = 0.;
         if (threeD && (!q.fluid || c.dv_fluid)) {
             const double x = s * std::cos(phi), y = s * std::sin(phi);
@@ -900,6 +994,12 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         const bool ti = (!q.fluid) && c.material != AXG_MAT_ISOTROPIC;
         auto props = [&](double vpr, double vsr) {
             std::array<double, 6> a;
+            if (em) {
+                // the mesh's polarisations, scaled as vp and vs are
+                const double fp = vpr / vp, fs = vs > 0. ? vsr / vs : 0.;
+                a = {(*em)[0] * fp, (*em)[1] * fp, (*em)[2] * fs, (*em)[3] * fs, rho, q.fluid ? 1. : (*em)[5]};
+                return a;
+            }
             a[0] = ti ? vpr * std::sqrt(c.tiso_phi) : vpr;  // vpv
             a[1] = vpr;                                     // vph
             a[2] = vsr;                                     // vsv
@@ -1385,7 +1485,9 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     // (jpol 0 or nPol) of a quad: spherical sides without undulation, so the
     // normal is radial, times the boundary area of the point
     auto sideNormal = [&](const QuadInfo &q, int side, int ipol, double nv[3]) {
-        const int jside = (side == 0) ? 0 : nPol;
+        // point ipol along the side: sides 0 / 2 at jpol 0 / nPol, sides 1 / 3
+        // at ipol nPol / 0 (then the running index is jpol)
+        const bool alongXi = side == 0 || side == 2;
         double r0 = std::hypot(q.g.ns[side], q.g.nz[side]);
         double t0 = std::atan2(q.g.ns[side], q.g.nz[side]);
         int s1 = period0123(side + 1);
@@ -1394,13 +1496,13 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         double rsf = .5 * (r0 + r1);
         double half_r_dtheta = .5 * rsf * std::abs(t1 - t0);
         double half_r2_dtheta = .5 * rsf * rsf * std::abs(t1 - t0);
-        int k = ipol * nP + jside;
+        const int k = alongXi ? ipol * nP + (side == 0 ? 0 : nPol) : (side == 1 ? nPol : 0) * nP + ipol;
         double s = q.crd[k][0], z = q.crd[k][1];
         double rr = std::hypot(s, z);
         double sint = s / rr, cost = z / rr;
-        double wsf = q.wxi[ipol];  // side 0 / 2: xi weight
+        double wsf = alongXi ? q.wxi[ipol] : q.weta[ipol];
         double fac;
-        if (q.axial) {
+        if (q.axial && alongXi) {
             if (ipol == 0) fac = wsf * q.J[k][0][0] * half_r_dtheta;
             else fac = wsf / (1. + q.xi[ipol]) * sint * half_r2_dtheta;
         } else {
@@ -1426,6 +1528,21 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         std::vector<char> onSF(NPT, 0);
         for (int e = 0; e < NQ; e++) {
             const QuadInfo &q = B.quads[e];
+            if (B.exo) {
+                // the mesh's solid_fluid_boundary side set
+                if (q.sfSide < 0) continue;
+                const int side = q.sfSide;
+                for (int i = 0; i < nP; i++) {
+                    double nv[3];
+                    sideNormal(q, side, i, nv);
+                    const double f = q.fluid ? .5 : -.5;
+                    const int k = (side == 0 || side == 2) ? i * nP + (side == 0 ? 0 : nPol) : (side == 1 ? nPol : 0) * nP + i;
+                    const int p = q.gll[k];
+                    for (int d = 0; d < 3; d++) sfn[p][d] += nv[d] * f;
+                    onSF[p] = 1;
+                }
+                continue;
+            }
             // sides facing an element of the other phase: inner side and outer side
             for (int updown = 0; updown < 2; updown++) {
                 int irn = q.ir + (updown == 0 ? -1 : 1);
@@ -1590,7 +1707,7 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         // relabelling's X on the axis points of a relabelled source element
         double axJ[nP][2][2];
         for (int j = 0; j < nP; j++) {
-            sphJacobian(q.g, spectral::glj().x[0], spectral::gll().x[j], axJ[j]);
+            jacobianAny(q.g.type, q.g, spectral::glj().x[0], spectral::gll().x[j], axJ[j]);
             double d = det2(axJ[j]);
             for (int a = 0; a < 2; a++)
                 for (int bb = 0; bb < 2; bb++) axJ[j][a][bb] /= d;

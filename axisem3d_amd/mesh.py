@@ -65,6 +65,10 @@ def shell_config(which=0, layers=None, **overrides):
             cfg.nvol3d = len(v)
             cfg.vol3d = C.cast(arr, C.POINTER(_abi.Volumetric3D))
             cfg._keep_vol3d = keep_v
+        elif k == "exodus":
+            # an ExodusModel (axisem3d_amd.exodus) in place of the shell
+            cfg.exodus = v._h.value
+            cfg._keep_exodus = v
         elif k == "wisdom":
             # nu_mode 4: Nu from wisdom rows [n][4] (s, z, learned Nu, original Nu)
             keep_nu = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1, 4))

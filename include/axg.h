@@ -291,6 +291,11 @@ typedef struct axg_shell_config {
     int32_t rec_dump;                 /* solid stations dump strain (bit 0) and curl (bit 1) */
     int32_t nvol3d;                   /* 3-D volumetric models, applied in order  */
     const axg_volumetric3d *vol3d;
+    /* a mesh read by axg_exodus_open instead of the shell: its quads, GLL
+     * points, side sets and nodal material (the layers, nelem_theta and the
+     * shell's undulation, ocean and attenuation are not used; setting the last
+     * three is an error) */
+    const struct axg_exodus *exodus;
 } axg_shell_config;
 
 typedef struct axg_mesh axg_mesh;

@@ -1,0 +1,88 @@
+"""The shipped Exodus mesh (AxiSEM_prem_ani_one_crust_50.e, 2016 quads, PREM
+with transverse isotropy) through the builder and the time loop: the quads,
+GLL points, solid-fluid side set and nodal material of the Exodus front end
+become the descriptor the GPU path takes.
+
+Pins: the descriptor's points are the front end's global GLL points; its
+solid-fluid points lie on the inner-core and core-mantle radii and are the
+points of the mesh's solid_fluid_boundary side set; the assembled solid-fluid
+normals of a closed spherical interface sum to zero and their magnitudes to the
+interface's area divided by 2 pi (the normals carry the integration weights);
+on the GPU, seismograms of the mesh over 1200 steps (523 s: the waves reach
+the stations after about 600) equal the oracle's (the reference's time loop
+restated) on the same descriptor.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import axisem3d_amd as ax
+from axisem3d_amd import _abi
+from helpers import rel_l2
+
+MESH = os.path.join(os.path.dirname(__file__), "golden", "AxiSEM_prem_ani_one_crust_50.e")
+R, R_ICB, R_CMB = 6371e3, 1221.5e3, 3480e3
+
+
+def _model():
+    try:
+        return ax.ExodusModel(MESH)
+    except RuntimeError as e:
+        if "HDF5" in str(e):
+            pytest.skip("HDF5 C library not found: %s" % e)
+        raise
+
+
+def _mesh(ex, nstep, nu=2):
+    return ax.ShellMesh(ax.shell_config(0, exodus=ex, nu_mode=0, nu_const=nu, dv_amp=0.0, nstep=nstep,
+                                        material=_abi.MAT_TRANSVERSELY_ISO))
+
+
+def test_exodus_descriptor():
+    ex = _model()
+    m = _mesh(ex, 10)
+    e2g, ngll = ex.gll()
+    assert m.d.npoint == ngll and m.d.nelem == ex.nquad
+    sz = m.array("point_sz").reshape(-1, 2)
+    info, crd, fac = ex.geometry()
+    np.testing.assert_allclose(sz[e2g.ravel()], crd.reshape(-1, 2), rtol=0, atol=1e-3)
+    # solid-fluid points: the side set's points, on the core radii
+    sfp = np.sort(m.array("sf_point"))
+    side = ex.side_set("solid_fluid_boundary")
+    exp = set()
+    for q in np.flatnonzero(side >= 0):
+        k = side[q]
+        idx = {0: [i * 5 for i in range(5)], 2: [i * 5 + 4 for i in range(5)], 1: [20 + i for i in range(5)],
+               3: list(range(5))}[k]
+        exp.update(e2g[q, idx].tolist())
+    np.testing.assert_array_equal(sfp, sorted(exp))
+    r = np.hypot(sz[sfp, 0], sz[sfp, 1])
+    assert np.minimum(np.abs(r - R_ICB), np.abs(r - R_CMB)).max() < 1.
+    # the normals (s, phi, z) with the weights of a surface integral over the
+    # meridian, times s: sum |n| = area / 2 pi of both spheres; sum n = 0 by
+    # symmetry of a closed sphere about the equator in z, and sum n_s > 0
+    nrm = m.array("sf_normal").reshape(-1, 6)[:, :3].astype(np.float64)
+    mag = np.linalg.norm(nrm, axis=1)
+    area = 4 * np.pi * (R_ICB ** 2 + R_CMB ** 2) / (2 * np.pi)
+    assert mag.sum() == pytest.approx(area, rel=1e-3)
+    assert abs(nrm[:, 2].sum()) < 1e-6 * mag.sum()
+    assert m.d.dt > 0 and m.d.src_elem >= 0
+
+
+@pytest.mark.gpu
+def test_exodus_mesh_seismograms_gpu():
+    from oracle.refcpu import RefDomain
+    print("case shipped Exodus mesh, TI PREM, Nu 2, 1200 steps", flush=True)
+    ex = _model()
+    m = _mesh(ex, 1200)
+    g = ax.Domain(m)
+    g.resetZero()
+    g.run(1, 1200, 100, use_graph=True)
+    sg = g.seismograms()
+    g.close()
+    r = RefDomain(m, double=False)
+    r.run(1, 1200, 100)
+    sr = r.seismograms()
+    assert np.isfinite(sg).all() and np.abs(sr).max() > 0
+    assert rel_l2(sg, sr) < 1e-4
