@@ -183,6 +183,7 @@ struct QuadInfo {
     int it, ir, layer;
     int id = -1;      // quad of an Exodus mesh
     int sfSide = -1;  // its side on the solid-fluid boundary side set
+    int surfSide = -1;  // its side on the surface side set
     bool axial, fluid, south;
     int gll[nP * nP];          // global point index of (ipol, jpol)
     int vicinal[4];            // ExodusModel::mVicinalAxis
@@ -499,6 +500,10 @@ struct Builder {
                 q.vicinal[k] = M.vicinal[e][k];
             }
             q.sfSide = sf < (int)M.ssNames.size() ? M.ssValues[sf][e] : -1;
+            q.surfSide = M.sideSet(M.ssSurface, e);
+            if (q.surfSide >= 0 && (G.type == 1 || (G.type == 2 && q.surfSide != G.curvedOuter) ||
+                                    (q.axial && q.surfSide % 2 == 1)))
+                throw std::runtime_error("Quad::computeNormal || surface side that is not an arc.");
             // the side normals below are those of an arc: every side of a
             // spherical quad, the curved side of a semi-spherical one
             if (q.sfSide >= 0 && (G.type == 1 || (G.type == 2 && q.sfSide != G.curvedOuter) ||
@@ -893,9 +898,9 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
     if (c.exodus) {
         // Exodus meshes carry their own model; what the shell generator adds
         // on top of its layers stays off
-        if (c.undulation != 0 || c.ocean_depth > 0. || c.attenuation != AXG_ATT_NONE)
-            throw std::runtime_error("ExodusModel || undulation, ocean load and attenuation are not available "
-                                     "for Exodus meshes.");
+        if (c.undulation != 0 || c.attenuation != AXG_ATT_NONE)
+            throw std::runtime_error("ExodusModel || undulation and attenuation are not available for Exodus "
+                                     "meshes.");
         B.exo = &c.exodus->m;
         B.buildTopologyExodus();
     } else {
@@ -1604,13 +1609,15 @@ static void buildShell(const axg_shell_config &c, axg_mesh &m) {
         std::vector<char> onSurf(NPT, 0);
         for (int e = 0; e < NQ; e++) {
             const QuadInfo &q = B.quads[e];
-            if (q.ir != B.NR - 1 || q.fluid) continue;
-            const int side = q.south ? 0 : 2;  // outer side
-            const int jside = (side == 0) ? 0 : nPol;
+            // the outer side: of the top row of the shell, or on the Exodus
+            // mesh's surface side set (r1)
+            if (q.fluid || (B.exo ? q.surfSide < 0 : q.ir != B.NR - 1)) continue;
+            const int side = B.exo ? q.surfSide : (q.south ? 0 : 2);
             for (int ipol = 0; ipol < nP; ipol++) {
                 double nv[3];
                 sideNormal(q, side, ipol, nv);
-                int p = q.gll[ipol * nP + jside];
+                const int k = (side == 0 || side == 2) ? ipol * nP + (side == 0 ? 0 : nPol) : (side == 1 ? nPol : 0) * nP + ipol;
+                int p = q.gll[k];
                 for (int d = 0; d < 3; d++) surf[p][d] += nv[d];
                 onSurf[p] = 1;
             }

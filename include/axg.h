@@ -292,9 +292,9 @@ typedef struct axg_shell_config {
     int32_t nvol3d;                   /* 3-D volumetric models, applied in order  */
     const axg_volumetric3d *vol3d;
     /* a mesh read by axg_exodus_open instead of the shell: its quads, GLL
-     * points, side sets and nodal material (the layers, nelem_theta and the
-     * shell's undulation, ocean and attenuation are not used; setting the last
-     * three is an error) */
+     * points, side sets and nodal material (the layers and nelem_theta are not
+     * used; the ocean rests on the r1 side set; undulation and attenuation are
+     * not available and setting them is an error) */
     const struct axg_exodus *exodus;
 } axg_shell_config;
 

@@ -34,9 +34,9 @@ def _model():
         raise
 
 
-def _mesh(ex, nstep, nu=2):
+def _mesh(ex, nstep, nu=2, **kw):
     return ax.ShellMesh(ax.shell_config(0, exodus=ex, nu_mode=0, nu_const=nu, dv_amp=0.0, nstep=nstep,
-                                        material=_abi.MAT_TRANSVERSELY_ISO))
+                                        material=_abi.MAT_TRANSVERSELY_ISO, **kw))
 
 
 def test_exodus_descriptor():
@@ -70,12 +70,38 @@ def test_exodus_descriptor():
     assert m.d.dt > 0 and m.d.src_elem >= 0
 
 
-@pytest.mark.gpu
-def test_exodus_mesh_seismograms_gpu():
-    from oracle.refcpu import RefDomain
-    print("case shipped Exodus mesh, TI PREM, Nu 2, 1200 steps", flush=True)
+def test_exodus_ocean_rows():
+    """The ocean rests on the r1 side set: one MassOcean1D row per solid
+    surface point; the rows' water masses are 1027 depth |n| with |n| summing
+    to the surface's area / 2 pi, so their sum is 1027 depth 2 R^2."""
     ex = _model()
-    m = _mesh(ex, 1200)
+    m = _mesh(ex, 10, ocean_depth=3000.)
+    off = m.array("ocean_off")
+    rows = m.array("ocean").reshape(-1, 4).astype(np.float64)
+    sz = m.array("point_sz").reshape(-1, 2)
+    pts = np.flatnonzero(np.diff(off) > 0)
+    assert (np.diff(off)[pts] == 1).all()
+    np.testing.assert_allclose(np.hypot(sz[pts, 0], sz[pts, 1]), R, rtol=1e-9)
+    e2g, _ = ex.gll()
+    side = ex.side_set("r1")
+    surf = set()
+    for q in np.flatnonzero(side >= 0):
+        idx = {0: [i * 5 for i in range(5)], 2: [i * 5 + 4 for i in range(5)], 1: [20 + i for i in range(5)],
+               3: list(range(5))}[side[q]]
+        surf.update(e2g[q, idx].tolist())
+    assert set(pts.tolist()) == surf
+    # rows (1 / (m + mo), 1 / m, sin theta, cos theta): mo = 1 / row0 - 1 / row1
+    mo = 1. / rows[:, 0] - 1. / rows[:, 1]
+    assert mo.sum() == pytest.approx(1027. * 3000. * 2 * R ** 2, rel=2e-3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ocean", [0., 3000.])
+def test_exodus_mesh_seismograms_gpu(ocean):
+    from oracle.refcpu import RefDomain
+    print("case shipped Exodus mesh, TI PREM, Nu 2, 1200 steps, ocean %g m" % ocean, flush=True)
+    ex = _model()
+    m = _mesh(ex, 1200, ocean_depth=ocean)
     g = ax.Domain(m)
     g.resetZero()
     g.run(1, 1200, 100, use_graph=True)
