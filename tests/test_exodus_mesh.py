@@ -35,8 +35,9 @@ def _model():
 
 
 def _mesh(ex, nstep, nu=2, **kw):
-    return ax.ShellMesh(ax.shell_config(0, exodus=ex, nu_mode=0, nu_const=nu, dv_amp=0.0, nstep=nstep,
-                                        material=_abi.MAT_TRANSVERSELY_ISO, **kw))
+    cfg = dict(exodus=ex, nu_mode=0, nu_const=nu, dv_amp=0.0, nstep=nstep, material=_abi.MAT_TRANSVERSELY_ISO)
+    cfg.update(kw)
+    return ax.ShellMesh(ax.shell_config(0, **cfg))
 
 
 def test_exodus_descriptor():
@@ -95,13 +96,24 @@ def test_exodus_ocean_rows():
     assert mo.sum() == pytest.approx(1027. * 3000. * 2 * R ** 2, rel=2e-3)
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("ocean", [0., 3000.])
-def test_exodus_mesh_seismograms_gpu(ocean):
-    from oracle.refcpu import RefDomain
-    print("case shipped Exodus mesh, TI PREM, Nu 2, 1200 steps, ocean %g m" % ocean, flush=True)
+def test_exodus_empirical_nr():
+    """The empirical Nr field (EmpNrField) on the mesh's points: Nr from the
+    floor (3) upwards, varying over the mesh."""
     ex = _model()
-    m = _mesh(ex, 1200, ocean_depth=ocean)
+    m = _mesh(ex, 10, nu=10, nu_mode=5)
+    nr = m.array("point_nr")
+    assert nr.min() >= 3 and nr.max() > 10
+    assert len(np.unique(nr)) > 5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["plain", "ocean", "empirical_nr"])
+def test_exodus_mesh_seismograms_gpu(case):
+    from oracle.refcpu import RefDomain
+    print("case shipped Exodus mesh, TI PREM, 1200 steps, %s" % case, flush=True)
+    ex = _model()
+    kw = {"plain": {}, "ocean": {"ocean_depth": 3000.}, "empirical_nr": {"nu_mode": 5, "nu": 10}}[case]
+    m = _mesh(ex, 1200, **kw)
     g = ax.Domain(m)
     g.resetZero()
     g.run(1, 1200, 100, use_graph=True)
